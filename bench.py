@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Headline benchmark: the radix-partitioned equi-join with fused count + sum.
+
+Workload (one MI355X): R = 2^24 rows of (int64 unique shuffled key, int64
+payload U[0,2^31)), S = 2^28 int64 foreign keys drawn uniformly (or Zipf(s)
+with --zipf) over R's keys, 2-pass radix partitioning with 14 total bits,
+output = match count + int64 sum of R.payload.  Inputs are generated in HBM
+before the clock starts.  A step is one whole join (histogram, both
+partition passes of both relations, the per-partition LDS join).
+
+Prints one JSON line.  `value` = (|R| + |S|) / step time.  `roofline` is the
+dominant kernel's algorithmic bytes / its average launch time (HIP events on
+the stream the kernels run on) against 8 TB/s.  `cpu_baseline` is the
+reference algorithm restated in C (chained buckets, bucket spinlocks, the
+queryhashjoin plan shape) on this host's cores, on the same inputs.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+METRIC = "join tuples/s + % of 8 TB/s HBM roofline; served p50/p99 vs plain per-query GPU"
+
+
+def kernel_bytes(nR: int, nS: int, esR: int = 16, esS: int = 8, kb: int = 8) -> dict:
+    """Algorithmic HBM bytes of one launch of each kernel of the radix join."""
+    return {
+        "hist_build": nR * kb,                 # read the build key column
+        "hist_probe": nS * kb,                 # read the probe key column
+        "scatter1_build": 2 * nR * esR,        # read (key, payload) columns, write rows
+        "scatter1_probe": 2 * nS * esS,
+        "scatter2_build": 2 * nR * esR,        # read pass-1 rows, write pass-2 rows
+        "scatter2_probe": 2 * nS * esS,
+        "join_lds": nR * esR + nS * esS + 16,  # read both partitioned relations once
+    }
+
+
+def cpu_baseline(args, nR, nS, gpu_result):
+    from oracle import oracle as O
+
+    cores = int(os.environ.get("PYTHIA_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    t0 = time.time()
+    rk, rp, sk = O.gen_c2(nR, nS, args.zipf, seed=args.seed)
+    gen_s = time.time() - t0
+    # reference knobs for C2: modulo hash, 2^24 buckets, 4 tuples per bucket
+    secs, cs = O.baseline_join(8, rk, rp, sk, 1 << 24, 4, cores)
+    match = (cs.count == gpu_result.matches and cs.sum_build == gpu_result.sum_build)
+    return {
+        "value": (nR + nS) / secs,
+        "unit": "tuples/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"the full workload (R 2^{nR.bit_length()-1} x S 2^{nS.bit_length()-1}), one run of "
+                  f"{secs:.2f} s; inputs generated outside the clock in {gen_s:.1f} s",
+        "result_matches_gpu": bool(match),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log-build", type=int, default=24)
+    ap.add_argument("--log-probe", type=int, default=28)
+    ap.add_argument("--bits", type=int, default=14)
+    ap.add_argument("--zipf", type=float, default=0.0)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_c2_uniform.json"),
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary (bytes per launch per kernel)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box in this setup has one MI355X, so --gpus must be 1 "
+              "(the path shards by radix partition; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import pythia_amd as pa
+    from oracle import oracle as O  # only for the Zipf CDF table of the input generator
+
+    nR, nS = 1 << args.log_build, 1 << args.log_probe
+    ctx = pa.Context(0)
+    rk, rp, sk = ctx.alloc(nR * 8), ctx.alloc(nR * 8), ctx.alloc(nS * 8)
+    ctx.gen("C2_RKEY", nR, rk, seed=args.seed)
+    ctx.gen("C2_RPAY", nR, rp, seed=args.seed)
+    if args.zipf > 0:
+        cdf = ctx.upload(O.zipf_cdf(nR, args.zipf))
+        ctx.gen("C2_SFK_ZIPF", nS, sk, seed=args.seed, n_build=nR, cdf=cdf)
+    else:
+        ctx.gen("C2_SFK_UNIFORM", nS, sk, seed=args.seed, n_build=nR)
+    ctx.sync()
+    plan = pa.JoinPlan(ctx, 8, nR, nS, radix_bits=args.bits, algo=pa.ALGO_RADIX)
+
+    for _ in range(args.warmup):
+        res = plan.count_sum(rk, rp, nR, sk, None, nS)
+    ctx.sync()
+    print(f"# warmup done: matches={res.matches} sum={res.sum_build}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.enqueue(rk, rp, nR, sk, None, nS)
+    res = plan.result()
+    ctx.sync()
+    t = time.perf_counter() - t0
+    ms_step = 1e3 * t / args.steps
+
+    # per-kernel HIP-event timing over a second set of steps
+    plan.timing(True)
+    for _ in range(args.steps):
+        plan.enqueue(rk, rp, nR, sk, None, nS)
+    plan.result()
+    times = plan.times()
+    plan.timing(False)
+    kb = kernel_bytes(nR, nS)
+    per_kernel = {k: {"launches": n, "avg_ms": ms / max(n, 1)} for k, (n, ms) in times.items()}
+    dom = max((k for k in per_kernel if k in kb), key=lambda k: per_kernel[k]["avg_ms"])
+    avg_s = per_kernel[dom]["avg_ms"] / 1e3
+    achieved = kb[dom] / avg_s
+    traffic = None
+    if os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            traffic = json.load(f).get(dom)
+    job_bytes = 5 * (nR * 16 + nS * 8) + 16
+    print(f"# step {ms_step:.3f} ms; kernels: " +
+          ", ".join(f"{k}={v['avg_ms']:.3f}ms" for k, v in per_kernel.items()), file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": (nR + nS) / (ms_step / 1e3),
+        "unit": "tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (counter-based SplitMix64 generators, generated in HBM)",
+        "config": {
+            "workload": f"radix-partitioned join + count/sum: R 2^{args.log_build} x (int64 key, int64 payload), "
+                        f"S 2^{args.log_probe} int64 FK {'Zipf s=%g' % args.zipf if args.zipf else 'uniform'}, "
+                        f"2-pass radix {args.bits} bits",
+            "build_rows": nR, "probe_rows": nS, "radix_bits": res.radix_bits, "zipf_s": args.zipf,
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "algorithmic_bytes_per_launch": kb[dom], "avg_launch_ms": per_kernel[dom]["avg_ms"],
+        },
+        "job_roofline": {"algorithmic_bytes": job_bytes, "achieved_GBps": job_bytes / (ms_step / 1e3) / 1e9,
+                         "frac": job_bytes / (ms_step / 1e3) / HBM_PEAK},
+        "kernels": per_kernel,
+        "result": {"matches": res.matches, "sum_build": res.sum_build, "lds_passes": res.lds_passes},
+    }
+    if not args.no_cpu_baseline:
+        plan.close()
+        for b in (rk, rp, sk):
+            b.free()
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(args, nR, nS, res)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * pgpu.h - C ABI of libpgpu.so, the MI355X (gfx950) hash-join and
+ * hash-aggregation engine behind the `gpuhashjoin` / `gpuaggregate_sum`
+ * plan operators.
+ *
+ * Rules of the boundary:
+ *   - plain C types only: opaque handles, pointers, sizes; no C++ types,
+ *     exceptions or STL cross it;
+ *   - every call returns an int status (PGPU_OK = 0, negative = error); the
+ *     message of the last failing call on the calling thread is returned by
+ *     pgpu_last_error();
+ *   - pointers named d_* are device (HBM) pointers, obtained from pgpu_alloc;
+ *     every other pointer is host memory owned by the caller;
+ *   - calls on distinct handles may run concurrently from different host
+ *     threads; calls on one handle are serialised by its caller.
+ *
+ * The reference engine (sblanas/pythia) has no foreign-function interface:
+ * its hash join and aggregation are C++ classes called through the Operator
+ * virtuals (operators/operators.h:79-186).  Each declaration below names the
+ * reference code it stands in for.
+ */
+#ifndef PGPU_H
+#define PGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes                                                        */
+/* ------------------------------------------------------------------ */
+#define PGPU_OK 0
+#define PGPU_EINVAL -1       /* bad argument                                  */
+#define PGPU_ENOMEM -2       /* device allocation failed                      */
+#define PGPU_EBUDGET -3      /* HBM budget of the context exceeded            */
+#define PGPU_EDEVICE -4      /* HIP runtime error                             */
+#define PGPU_EOVERFLOW -5    /* a device-side capacity check tripped          */
+#define PGPU_ESTATE -6       /* call out of lifecycle order                   */
+#define PGPU_EUNSUPPORTED -7 /* column type / width not handled               */
+
+/* ------------------------------------------------------------------ */
+/* Context: one device, one HBM budget, the streams work is issued on   */
+/* Replaces numaallocate_onnode / numaallocate_local and their byte     */
+/* accounting (util/numaallocate.cpp:60-181, :402-457).                 */
+/* ------------------------------------------------------------------ */
+typedef struct pgpu_ctx pgpu_ctx;
+
+const char* pgpu_version(void);
+const char* pgpu_last_error(void);
+int pgpu_device_count(int* n);
+/* hbm_budget_bytes = 0 means "all free device memory". */
+int pgpu_ctx_create(int device, uint64_t hbm_budget_bytes, pgpu_ctx** out);
+int pgpu_ctx_destroy(pgpu_ctx* ctx);
+/* The hipStream_t the context issues work on (for events / interop). */
+int pgpu_ctx_stream(pgpu_ctx* ctx, void** hip_stream);
+int pgpu_alloc(pgpu_ctx* ctx, uint64_t bytes, void** d_ptr);
+int pgpu_free(pgpu_ctx* ctx, void* d_ptr);
+int pgpu_hbm_usage(pgpu_ctx* ctx, uint64_t* used, uint64_t* budget, uint64_t* peak);
+int pgpu_h2d(pgpu_ctx* ctx, void* d_dst, const void* src, uint64_t bytes);
+int pgpu_d2h(pgpu_ctx* ctx, void* dst, const void* d_src, uint64_t bytes);
+int pgpu_sync(pgpu_ctx* ctx);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic inputs generated in HBM (the formulas of pgpu_gen.h).      */
+/* No reference counterpart: the reference reads text/binary files      */
+/* (operators/loaders/table.cpp); these fill device columns directly.   */
+/* ------------------------------------------------------------------ */
+enum pgpu_gen_column {
+    PGPU_GEN_C1_RKEY = 1,  /* int32 */
+    PGPU_GEN_C1_RPAY,      /* int32 */
+    PGPU_GEN_C1_SFK,       /* int32 */
+    PGPU_GEN_C1_SPAY,      /* int32 */
+    PGPU_GEN_C2_RKEY,      /* int64, sub = table id */
+    PGPU_GEN_C2_RPAY,      /* int64 */
+    PGPU_GEN_C2_SFK_UNIFORM, /* int64 */
+    PGPU_GEN_C2_SFK_ZIPF,  /* int64, needs d_cdf */
+    PGPU_GEN_C3_D_ORDERKEY, /* int32 */
+    PGPU_GEN_C3_D_CUSTKEY, /* int32 */
+    PGPU_GEN_C3_F_ORDERKEY, /* int32 */
+    PGPU_GEN_C3_F_PARTKEY, /* int32 */
+    PGPU_GEN_C3_F_QUANTITY, /* int32 */
+    PGPU_GEN_C3_F_PRICE,   /* double */
+    PGPU_GEN_C4_RKEY,      /* int32, sub = join id */
+    PGPU_GEN_C4_RPAY,      /* int32 */
+    PGPU_GEN_C4_SFK        /* int32 */
+};
+
+typedef struct pgpu_gen_spec {
+    uint64_t seed;      /* base seed (42)                                 */
+    uint64_t sub;       /* table / join index within a family             */
+    uint64_t n_build;   /* build cardinality the column refers to         */
+    uint64_t offset;    /* first row index generated                      */
+    const double* d_cdf;/* Zipf CDF in HBM (n_build doubles) or NULL      */
+} pgpu_gen_spec;
+
+int pgpu_gen(pgpu_ctx* ctx, int column, const pgpu_gen_spec* spec, uint64_t n, void* d_out);
+
+/* ------------------------------------------------------------------ */
+/* Equi-join with fused count + sum consumer, device-resident columns.  */
+/* Stands in for HashJoinOp::scanStart (build, operators/join.cpp:      */
+/* 396-464, :641-667) + HashJoinOp::getNext (probe, :503-562) feeding   */
+/* AggregateCount / AggregateSum over the join output                   */
+/* (operators/aggregatecount.cpp:37-52, operators/aggregatesum.cpp:     */
+/* 37-107) with no group-by (`fields = ()`).                            */
+/* ------------------------------------------------------------------ */
+enum pgpu_join_algo {
+    PGPU_ALGO_AUTO = 0,
+    PGPU_ALGO_RADIX = 1,  /* 2-pass radix partitioning + LDS tables         */
+    PGPU_ALGO_GLOBAL = 2  /* one open-addressing table in HBM (no partitions) */
+};
+
+typedef struct pgpu_join_cfg {
+    uint32_t key_bytes;     /* 4 (int32) or 8 (int64); payloads have the same width */
+    uint32_t radix_bits;    /* total radix bits (2..14) over both passes; 0 = choose */
+    uint32_t probe_payload; /* 1: probe rows carry a payload summed per match        */
+    uint32_t algo;          /* enum pgpu_join_algo                                   */
+} pgpu_join_cfg;
+
+typedef struct pgpu_join_result {
+    uint64_t matches;   /* |{(r, s) : r.key == s.key}|                         */
+    uint64_t sum_build; /* sum of r.payload over the matches, modulo 2^64      */
+    uint64_t sum_probe; /* sum of s.payload over the matches, modulo 2^64      */
+    uint32_t radix_bits;/* bits actually used (0 for the global table)         */
+    uint32_t lds_passes;/* max build pieces one partition needed (1 = fit LDS) */
+} pgpu_join_result;
+
+typedef struct pgpu_join_plan pgpu_join_plan;
+
+/* Allocates every buffer a join of up to max_build x max_probe rows needs,
+ * so later calls allocate nothing. */
+int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_build,
+                          uint64_t max_probe, pgpu_join_plan** out);
+int pgpu_join_plan_destroy(pgpu_join_plan* plan);
+/* Enqueue the whole join on the context stream; returns without waiting. */
+int pgpu_join_count_sum_async(pgpu_join_plan* plan, const void* d_rkey, const void* d_rpay,
+                              uint64_t n_build, const void* d_skey, const void* d_spay,
+                              uint64_t n_probe);
+/* Wait for the last enqueued join and copy its result. */
+int pgpu_join_plan_result(pgpu_join_plan* plan, pgpu_join_result* out);
+/* async + result. */
+int pgpu_join_count_sum(pgpu_join_plan* plan, const void* d_rkey, const void* d_rpay,
+                        uint64_t n_build, const void* d_skey, const void* d_spay,
+                        uint64_t n_probe, pgpu_join_result* out);
+
+/* Per-kernel timing with HIP events on the stream the kernels run on.
+ * enable != 0 starts recording (and clears totals); pgpu_join_plan_times
+ * returns, per kernel, its name, launch count and summed milliseconds. */
+typedef struct pgpu_kernel_time {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+} pgpu_kernel_time;
+int pgpu_join_plan_timing(pgpu_join_plan* plan, int enable);
+int pgpu_join_plan_times(pgpu_join_plan* plan, pgpu_kernel_time* out, int cap, int* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGPU_H */

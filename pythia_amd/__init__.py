@@ -1,0 +1,139 @@
+"""pythia_amd: MI355X-native hash join / hash aggregation for the Pythia
+query engine's `gpuhashjoin` / `gpuaggregate_sum` operators.
+
+The compute lives in libpgpu.so (hand-written gfx950 HIP kernels behind the C
+ABI of include/pgpu.h).  This module is a thin Python handle over that ABI
+used by tests and the benchmark; the C++ operator host in host/ uses the same
+ABI directly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import ALGO_AUTO, ALGO_GLOBAL, ALGO_RADIX, GEN, PgpuError, call, lib  # noqa: F401
+
+__all__ = ["Context", "DeviceBuffer", "JoinPlan", "device_count", "PgpuError", "GEN",
+           "ALGO_AUTO", "ALGO_RADIX", "ALGO_GLOBAL"]
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = lib().pgpu_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class DeviceBuffer:
+    """A budgeted HBM allocation owned by a Context."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        call("pgpu_alloc", ctx.handle, self.nbytes, C.byref(p))
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr:
+            call("pgpu_free", self.ctx.handle, C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def upload(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        call("pgpu_h2d", self.ctx.handle, C.c_void_p(self.ptr), arr.ctypes.data_as(C.c_void_p), arr.nbytes)
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        call("pgpu_d2h", self.ctx.handle, out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes)
+        return out
+
+
+class Context:
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        call("pgpu_ctx_create", device, hbm_budget, C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().pgpu_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, arr: np.ndarray) -> DeviceBuffer:
+        b = DeviceBuffer(self, max(arr.nbytes, 1))
+        b.upload(arr)
+        return b
+
+    def sync(self):
+        call("pgpu_sync", self.handle)
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        call("pgpu_ctx_stream", self.handle, C.byref(s))
+        return s.value
+
+    def hbm_usage(self):
+        u, b, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        call("pgpu_hbm_usage", self.handle, C.byref(u), C.byref(b), C.byref(p))
+        return u.value, b.value, p.value
+
+    def gen(self, column: str, n: int, out: DeviceBuffer, seed: int = 42, sub: int = 0, n_build: int = 0,
+            offset: int = 0, cdf: DeviceBuffer | None = None):
+        spec = _lib.GenSpec(seed, sub, n_build, offset, C.c_void_p(cdf.ptr) if cdf else None)
+        call("pgpu_gen", self.handle, GEN[column], C.byref(spec), n, C.c_void_p(out.ptr))
+
+
+class JoinPlan:
+    """Preallocated join (count + sum) over device-resident columns."""
+
+    def __init__(self, ctx: Context, key_bytes: int, max_build: int, max_probe: int, radix_bits: int = 0,
+                 probe_payload: bool = False, algo: int = ALGO_AUTO):
+        self.ctx = ctx
+        cfg = _lib.JoinCfg(key_bytes, radix_bits, 1 if probe_payload else 0, algo)
+        h = C.c_void_p()
+        call("pgpu_join_plan_create", ctx.handle, C.byref(cfg), max_build, max_probe, C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().pgpu_join_plan_destroy(self.handle)
+            self.handle = None
+
+    @staticmethod
+    def _p(b):
+        return C.c_void_p(b.ptr if b is not None else None)
+
+    def enqueue(self, rkey, rpay, n_build, skey, spay, n_probe):
+        call("pgpu_join_count_sum_async", self.handle, self._p(rkey), self._p(rpay), n_build, self._p(skey),
+             self._p(spay), n_probe)
+
+    def result(self):
+        r = _lib.JoinResult()
+        call("pgpu_join_plan_result", self.handle, C.byref(r))
+        return r
+
+    def count_sum(self, rkey, rpay, n_build, skey, spay, n_probe):
+        self.enqueue(rkey, rpay, n_build, skey, spay, n_probe)
+        return self.result()
+
+    def timing(self, on: bool):
+        call("pgpu_join_plan_timing", self.handle, 1 if on else 0)
+
+    def times(self):
+        arr = (_lib.KernelTime * 64)()
+        n = C.c_int()
+        call("pgpu_join_plan_times", self.handle, arr, 64, C.byref(n))
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(min(n.value, 64))}

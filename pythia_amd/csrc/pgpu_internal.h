@@ -1,0 +1,100 @@
+// Internal declarations shared by the libpgpu translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/pgpu.h"
+
+namespace pgpu {
+
+// Records a message as the calling thread's last error and returns `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define PGPU_HIP(expr)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::pgpu::fail(PGPU_EDEVICE, "%s failed: %s (%s:%d)", #expr,            \
+                                hipGetErrorString(e_), __FILE__, __LINE__);              \
+    } while (0)
+
+#define PGPU_TRY(expr)                                                                   \
+    do {                                                                                 \
+        int rc_ = (expr);                                                                \
+        if (rc_ != PGPU_OK)                                                              \
+            return rc_;                                                                  \
+    } while (0)
+
+#define PGPU_LAUNCH_CHECK()                                                              \
+    do {                                                                                 \
+        hipError_t e_ = hipGetLastError();                                               \
+        if (e_ != hipSuccess)                                                            \
+            return ::pgpu::fail(PGPU_EDEVICE, "kernel launch failed: %s (%s:%d)",        \
+                                hipGetErrorString(e_), __FILE__, __LINE__);              \
+    } while (0)
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+inline uint32_t ilog2_ceil(uint64_t n) {
+    uint32_t k = 0;
+    while (k < 63 && (1ULL << k) < n) ++k;
+    return k;
+}
+
+// Per-kernel event timing on one stream.  Event pairs are recorded around
+// launches while enabled and folded into totals when harvested.
+class KernelTimer {
+  public:
+    explicit KernelTimer(hipStream_t s) : stream_(s) {}
+    ~KernelTimer();
+    void enable(bool on);
+    bool enabled() const { return on_; }
+    // Returns an index to pass to stop().  -1 when disabled.
+    int start(const char* name);
+    void stop(int idx);
+    int harvest();  // synchronises pending events into totals
+    struct Total {
+        std::string name;
+        uint64_t launches = 0;
+        double ms = 0;
+    };
+    const std::vector<Total>& totals() const { return totals_; }
+
+  private:
+    struct Pending {
+        int slot;
+        hipEvent_t a, b;
+    };
+    hipStream_t stream_;
+    bool on_ = false;
+    std::vector<Total> totals_;
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    hipEvent_t get_event();
+};
+
+}  // namespace pgpu
+
+struct pgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    uint64_t peak = 0;
+    std::mutex mu;
+    std::unordered_map<void*, uint64_t> allocs;
+};
+
+namespace pgpu {
+// Budgeted device allocation on behalf of library objects.
+int ctx_alloc(pgpu_ctx* ctx, uint64_t bytes, void** p);
+int ctx_free(pgpu_ctx* ctx, void* p);
+// Makes ctx->device current on the calling thread.
+int ctx_activate(pgpu_ctx* ctx);
+}  // namespace pgpu

@@ -1,0 +1,964 @@
+// Equi-join with a fused count + sum consumer on gfx950.
+//
+// Reference path replaced: HashJoinOp build (operators/join.cpp:396-464,
+// :641-667) inserting every build tuple under a bucket spinlock into a
+// chained-bucket HashTable (util/hashtable.cpp:230-273), then the probe loop
+// (operators/join.cpp:503-562) walking the bucket chain and comparing keys
+// through Comparator::eval, feeding AggregateCount / AggregateSum
+// (operators/aggregatecount.cpp:37-52, operators/aggregatesum.cpp:81-107).
+//
+// Two device algorithms produce the same (count, sum) result:
+//
+// RADIX (2-pass radix partitioning, the MSD split of
+// ParameterizedModuloValueHasher::generate(2), hash.cpp:237-267: pass 1 takes
+// the high B1 = B/2 bits of the B-bit radix value, pass 2 the low B2 = B-B1):
+//   k_hist      one read of each key column; per block of `blk` rows an LDS
+//               histogram over all 2^B final partitions, written to H[blk][2^B]
+//   k_colscan   exclusive scan of H down the blocks; per-bin totals
+//   k_binscan   exclusive scan of the totals -> base[] (partition starts)
+//   k_off1      pass-1 write offsets off1[b][p1] = base(p1) + sum over p2 of H
+//   k_scatter1  block b streams its rows, reorders each tile by p1 in LDS and
+//               writes contiguous digit runs at its exact offsets (no atomics)
+//   k_scatter2  pass-2 block (p1, b) = the rows block b wrote to p1; it
+//               reorders them by p2 and writes runs at base(p1,p2)+H[b][p1,p2]
+//   k_icount /  probe items: each final partition's probe rows are cut into
+//   k_binscan   items of at most M rows, so one hot (Zipf) partition is spread
+//               over many workgroups
+//   k_join_cs   per item: the partition's build rows go into an LDS table
+//               (open addressing, duplicates folded into (count, sum)), then
+//               the item's probe rows are streamed from HBM and looked up
+// Build partitions larger than the LDS table are handled in LDS-sized pieces
+// (each piece re-probes the item's rows), so no build size overflows.
+//
+// GLOBAL (one open-addressing table in HBM, the "plain" per-query path):
+//   k_gclear, k_gbuild (CAS insert), k_gprobe.
+#include <algorithm>
+#include <cstring>
+
+#include "pgpu_internal.h"
+
+namespace {
+
+constexpr uint64_t kFib = 0x9E3779B97F4A7C15ull;
+constexpr int kMaxBits = 14;
+constexpr int kScatterThreads = 512;
+constexpr int kJoinThreads = 256;
+constexpr int kJoinCap = 2048;            // LDS table slots
+constexpr int kJoinPiece = 1536;          // build rows per LDS piece (load <= 0.75)
+constexpr uint32_t kItemRows = 32768;     // probe rows per join item
+constexpr int kTileBytes = 65536;         // LDS staging tile of the scatters
+
+template <typename KT>
+__device__ __forceinline__ uint64_t hmix(KT k) {
+    return (uint64_t)(int64_t)k * kFib;
+}
+
+template <typename KT>
+struct KeyEmpty;
+template <>
+struct KeyEmpty<int32_t> {
+    static constexpr int32_t v = INT32_MIN;
+};
+template <>
+struct KeyEmpty<int64_t> {
+    static constexpr int64_t v = INT64_MIN;
+};
+
+template <typename KT, bool HV>
+struct Elem;
+template <typename KT>
+struct Elem<KT, true> {
+    KT k;
+    KT v;
+};
+template <typename KT>
+struct Elem<KT, false> {
+    KT k;
+};
+
+// ---------------------------------------------------------------- helpers
+// Exclusive scan of in[0..n) (n <= 256) by one wavefront; out[n] = total.
+__device__ __forceinline__ void wave_excl_scan256(const uint32_t* in, uint32_t* out, int n) {
+    const int lane = threadIdx.x & 63;
+    uint32_t a[4];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int idx = lane * 4 + i;
+        a[i] = idx < n ? in[idx] : 0u;
+        s += a[i];
+    }
+    uint32_t x = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    uint32_t ex = x - s;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int idx = lane * 4 + i;
+        if (idx < n) out[idx] = ex;
+        ex += a[i];
+    }
+    if (lane == 63) out[n] = x;
+}
+
+// Exclusive scan across a 1024-thread block of one uint64 per thread.
+__device__ __forceinline__ uint64_t block_excl_scan1024(uint64_t v, uint64_t* total) {
+    __shared__ uint64_t ws[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        uint64_t t = lane < 16 ? ws[lane] : 0;
+        uint64_t inc = t;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            uint64_t y = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += y;
+        }
+        if (lane < 16) ws[lane] = inc - t;
+        if (lane == 15 && total) *total = inc;
+    }
+    __syncthreads();
+    return ws[w] + (x - v);
+}
+
+// ---------------------------------------------------------------- radix kernels
+template <typename KT>
+__global__ __launch_bounds__(512) void k_hist(const KT* __restrict__ keys, uint64_t n, uint64_t blk, int B,
+                                              uint32_t* __restrict__ H) {
+    extern __shared__ uint32_t lh[];
+    const uint32_t nb = 1u << B;
+    for (uint32_t i = threadIdx.x; i < nb; i += 512) lh[i] = 0;
+    __syncthreads();
+    const uint64_t s = (uint64_t)blockIdx.x * blk;
+    const uint64_t e = min(n, s + blk);
+    constexpr int U = 8;
+    uint64_t i = s + threadIdx.x;
+    for (; i + (U - 1) * 512 < e; i += U * 512) {
+        KT k[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) k[u] = keys[i + u * 512];
+#pragma unroll
+        for (int u = 0; u < U; ++u) atomicAdd(&lh[(uint32_t)(hmix(k[u]) >> (64 - B))], 1u);
+    }
+    for (; i < e; i += 512) atomicAdd(&lh[(uint32_t)(hmix(keys[i]) >> (64 - B))], 1u);
+    __syncthreads();
+    uint32_t* Hb = H + (uint64_t)blockIdx.x * nb;
+    for (uint32_t j = threadIdx.x; j < nb; j += 512) Hb[j] = lh[j];
+}
+
+// One workgroup per 64 bins; its 8 wavefronts scan 8 contiguous segments of
+// blocks, then each bin's column becomes its exclusive prefix over blocks.
+__global__ __launch_bounds__(512) void k_colscan(uint32_t* __restrict__ H, uint32_t nblk, uint32_t nb,
+                                                 uint32_t* __restrict__ tot) {
+    __shared__ uint32_t part[8][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t bin = blockIdx.x * 64 + lane;
+    const uint32_t seg = (nblk + 7) / 8;
+    const uint32_t b0 = min(nblk, w * seg), b1 = min(nblk, b0 + seg);
+    uint32_t s = 0;
+    if (bin < nb)
+        for (uint32_t b = b0; b < b1; ++b) s += H[(uint64_t)b * nb + bin];
+    part[w][lane] = s;
+    __syncthreads();
+    uint32_t run = 0;
+    for (int k = 0; k < w; ++k) run += part[k][lane];
+    if (bin < nb) {
+        for (uint32_t b = b0; b < b1; ++b) {
+            const uint32_t h = H[(uint64_t)b * nb + bin];
+            H[(uint64_t)b * nb + bin] = run;
+            run += h;
+        }
+        if (w == 7) tot[bin] = run;
+    }
+}
+
+// base[i] = sum of in[0..i) for i <= n, single workgroup.
+__global__ __launch_bounds__(1024) void k_binscan(const uint32_t* __restrict__ in, uint32_t n,
+                                                  uint64_t* __restrict__ base) {
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t t0 = threadIdx.x * per;
+    uint64_t s = 0;
+    for (uint32_t i = 0; i < per; ++i)
+        if (t0 + i < n) s += in[t0 + i];
+    uint64_t total = 0;
+    __shared__ uint64_t tot_sh;
+    uint64_t ex = block_excl_scan1024(s, &tot_sh);
+    (void)total;
+    for (uint32_t i = 0; i < per; ++i) {
+        if (t0 + i < n) {
+            base[t0 + i] = ex;
+            ex += in[t0 + i];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) base[n] = tot_sh;
+}
+
+// off1[b][p1] = base[p1 << B2] + sum_p2 Hs[b][p1, p2] for b < nblk;
+// off1[nblk][p1] = base[(p1 + 1) << B2] (end of partition p1).
+__global__ __launch_bounds__(256) void k_off1(const uint32_t* __restrict__ Hs, const uint64_t* __restrict__ base,
+                                              uint32_t nblk, int B, int B2, uint64_t* __restrict__ off1) {
+    const uint32_t nb1 = 1u << (B - B2), nb2 = 1u << B2, nb = 1u << B;
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (uint64_t)(nblk + 1) * nb1) return;
+    const uint32_t b = (uint32_t)(t / nb1), p1 = (uint32_t)(t % nb1);
+    if (b == nblk) {
+        off1[t] = base[(uint64_t)(p1 + 1) << B2];
+        return;
+    }
+    const uint32_t* row = Hs + (uint64_t)b * nb + ((uint64_t)p1 << B2);
+    uint64_t s = 0;
+    for (uint32_t j = 0; j < nb2; ++j) s += row[j];
+    off1[t] = base[(uint64_t)p1 << B2] + s;
+}
+
+template <typename KT, bool HV, int IPT>
+__global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restrict__ keys, const KT* __restrict__ vals,
+                                                              uint64_t n, uint64_t blk, int B, int B2,
+                                                              const uint64_t* __restrict__ off1,
+                                                              Elem<KT, HV>* __restrict__ out) {
+    using E = Elem<KT, HV>;
+    constexpr int T = kScatterThreads * IPT;
+    __shared__ E stage[T];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t lofs[257];
+    __shared__ uint64_t run[256];
+    const int tid = threadIdx.x;
+    const int nb1 = 1 << (B - B2);
+    for (int d = tid; d < nb1; d += kScatterThreads) {
+        run[d] = off1[(uint64_t)blockIdx.x * nb1 + d];
+        hist[d] = 0;
+    }
+    __syncthreads();
+    const uint64_t s = (uint64_t)blockIdx.x * blk;
+    const uint64_t e = min(n, s + blk);
+    for (uint64_t t0 = s; t0 < e; t0 += T) {
+        const uint32_t cnt = (uint32_t)min<uint64_t>(T, e - t0);
+        E el[IPT];
+        uint32_t dg[IPT], rk[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const uint32_t idx = tid + k * kScatterThreads;
+            if (idx < cnt) {
+                el[k].k = keys[t0 + idx];
+                if constexpr (HV) el[k].v = vals[t0 + idx];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const uint32_t idx = tid + k * kScatterThreads;
+            if (idx < cnt) {
+                dg[k] = (uint32_t)(hmix(el[k].k) >> (64 - B)) >> B2;
+                rk[k] = atomicAdd(&hist[dg[k]], 1u);
+            }
+        }
+        __syncthreads();
+        if (tid < 64) wave_excl_scan256(hist, lofs, nb1);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const uint32_t idx = tid + k * kScatterThreads;
+            if (idx < cnt) stage[lofs[dg[k]] + rk[k]] = el[k];
+        }
+        __syncthreads();
+        for (uint32_t j = tid; j < cnt; j += kScatterThreads) {
+            const E x = stage[j];
+            const uint32_t d = (uint32_t)(hmix(x.k) >> (64 - B)) >> B2;
+            out[run[d] + (j - lofs[d])] = x;
+        }
+        __syncthreads();
+        for (int d = tid; d < nb1; d += kScatterThreads) {
+            run[d] += hist[d];
+            hist[d] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename KT, bool HV, int IPT>
+__global__ __launch_bounds__(kScatterThreads) void k_scatter2(const Elem<KT, HV>* __restrict__ in, uint32_t nblk, int B,
+                                                              int B2, const uint64_t* __restrict__ off1,
+                                                              const uint32_t* __restrict__ Hs,
+                                                              const uint64_t* __restrict__ base,
+                                                              Elem<KT, HV>* __restrict__ out) {
+    using E = Elem<KT, HV>;
+    constexpr int T = kScatterThreads * IPT;
+    __shared__ E stage[T];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t lofs[257];
+    __shared__ uint64_t run[256];
+    const int tid = threadIdx.x;
+    const uint32_t nb1 = 1u << (B - B2), nb2 = 1u << B2, nb = 1u << B;
+    const uint32_t p1 = blockIdx.x / nblk, b = blockIdx.x % nblk;
+    const uint64_t s = off1[(uint64_t)b * nb1 + p1];
+    const uint64_t e = off1[(uint64_t)(b + 1) * nb1 + p1];
+    if (s >= e) return;
+    const uint32_t mask2 = nb2 - 1;
+    for (uint32_t d = tid; d < nb2; d += kScatterThreads) {
+        const uint64_t pp = ((uint64_t)p1 << B2) + d;
+        run[d] = base[pp] + Hs[(uint64_t)b * nb + pp];
+        hist[d] = 0;
+    }
+    __syncthreads();
+    for (uint64_t t0 = s; t0 < e; t0 += T) {
+        const uint32_t cnt = (uint32_t)min<uint64_t>(T, e - t0);
+        E el[IPT];
+        uint32_t dg[IPT], rk[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const uint32_t idx = tid + k * kScatterThreads;
+            if (idx < cnt) el[k] = in[t0 + idx];
+        }
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const uint32_t idx = tid + k * kScatterThreads;
+            if (idx < cnt) {
+                dg[k] = (uint32_t)(hmix(el[k].k) >> (64 - B)) & mask2;
+                rk[k] = atomicAdd(&hist[dg[k]], 1u);
+            }
+        }
+        __syncthreads();
+        if (tid < 64) wave_excl_scan256(hist, lofs, (int)nb2);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const uint32_t idx = tid + k * kScatterThreads;
+            if (idx < cnt) stage[lofs[dg[k]] + rk[k]] = el[k];
+        }
+        __syncthreads();
+        for (uint32_t j = tid; j < cnt; j += kScatterThreads) {
+            const E x = stage[j];
+            const uint32_t d = (uint32_t)(hmix(x.k) >> (64 - B)) & mask2;
+            out[run[d] + (j - lofs[d])] = x;
+        }
+        __syncthreads();
+        for (uint32_t d = tid; d < nb2; d += kScatterThreads) {
+            run[d] += hist[d];
+            hist[d] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_icount(const uint32_t* __restrict__ totS, const uint32_t* __restrict__ totR,
+                                                uint32_t nb, uint32_t M, uint32_t* __restrict__ icount) {
+    const uint32_t pp = blockIdx.x * 256 + threadIdx.x;
+    if (pp >= nb) return;
+    icount[pp] = totR[pp] ? (totS[pp] + M - 1) / M : 0u;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+template <typename KT, bool HVS>
+__global__ __launch_bounds__(kJoinThreads) void k_join_cs(
+    const Elem<KT, true>* __restrict__ R2, const uint64_t* __restrict__ baseR, const uint32_t* __restrict__ totR,
+    const Elem<KT, HVS>* __restrict__ S2, const uint64_t* __restrict__ baseS, const uint32_t* __restrict__ totS,
+    const uint64_t* __restrict__ ioff, uint32_t nb, uint32_t M, int B, unsigned long long* __restrict__ result) {
+    constexpr int CAP = kJoinCap;
+    constexpr int L = 11;  // log2(CAP)
+    static_assert((1 << L) == CAP, "CAP");
+    constexpr KT EMPTY = KeyEmpty<KT>::v;
+    __shared__ KT tk[CAP];
+    __shared__ uint32_t tc[CAP];
+    __shared__ unsigned long long ts[CAP];
+    __shared__ unsigned long long sp_cnt, sp_sum;
+    __shared__ uint32_t s_pp;
+    __shared__ unsigned long long red[3][kJoinThreads / 64];
+    const int tid = threadIdx.x;
+    const uint64_t item = blockIdx.x;
+    if (item >= ioff[nb]) return;
+    if (tid == 0) {
+        uint32_t lo = 0, hi = nb - 1;  // largest pp with ioff[pp] <= item
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (ioff[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        s_pp = lo;
+    }
+    __syncthreads();
+    const uint32_t pp = s_pp;
+    const uint64_t sub = item - ioff[pp];
+    const uint64_t sB = baseS[pp] + sub * M;
+    const uint64_t sE = min(baseS[pp] + totS[pp], sB + M);
+    const uint64_t rB = baseR[pp];
+    const uint32_t nR = totR[pp];
+    const int hs = 64 - B - L;  // slot bits sit just below the radix bits
+    unsigned long long acc_c = 0, acc_b = 0, acc_p = 0;
+
+    for (uint32_t p0 = 0; p0 < nR; p0 += kJoinPiece) {
+        for (int i = tid; i < CAP; i += kJoinThreads) {
+            tk[i] = EMPTY;
+            tc[i] = 0;
+            ts[i] = 0;
+        }
+        if (tid == 0) sp_cnt = sp_sum = 0;
+        __syncthreads();
+        const uint32_t pn = min<uint32_t>(kJoinPiece, nR - p0);
+        for (uint32_t i = tid; i < pn; i += kJoinThreads) {
+            const Elem<KT, true> r = R2[rB + p0 + i];
+            if (r.k == EMPTY) {
+                atomicAdd(&sp_cnt, 1ull);
+                atomicAdd(&sp_sum, (unsigned long long)(int64_t)r.v);
+                continue;
+            }
+            uint32_t slot = (uint32_t)(hmix(r.k) >> hs) & (CAP - 1);
+            while (true) {
+                KT prev;
+                if constexpr (sizeof(KT) == 8)
+                    prev = (KT)atomicCAS((unsigned long long*)&tk[slot], (unsigned long long)EMPTY,
+                                         (unsigned long long)r.k);
+                else
+                    prev = (KT)atomicCAS((unsigned int*)&tk[slot], (unsigned int)EMPTY, (unsigned int)r.k);
+                if (prev == EMPTY || prev == r.k) {
+                    atomicAdd(&tc[slot], 1u);
+                    atomicAdd(&ts[slot], (unsigned long long)(int64_t)r.v);
+                    break;
+                }
+                slot = (slot + 1) & (CAP - 1);
+            }
+        }
+        __syncthreads();
+        constexpr int U = 8;
+        uint64_t j = sB + tid;
+        auto probe = [&](const Elem<KT, HVS>& x) {
+            unsigned long long c = 0, sb = 0;
+            if (x.k == EMPTY) {
+                c = sp_cnt;
+                sb = sp_sum;
+            } else {
+                uint32_t slot = (uint32_t)(hmix(x.k) >> hs) & (CAP - 1);
+                while (true) {
+                    const KT t = tk[slot];
+                    if (t == x.k) {
+                        c = tc[slot];
+                        sb = ts[slot];
+                        break;
+                    }
+                    if (t == EMPTY) break;
+                    slot = (slot + 1) & (CAP - 1);
+                }
+            }
+            acc_c += c;
+            acc_b += sb;
+            if constexpr (HVS) acc_p += c * (unsigned long long)(int64_t)x.v;
+        };
+        for (; j + (U - 1) * kJoinThreads < sE; j += U * kJoinThreads) {
+            Elem<KT, HVS> x[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) x[u] = S2[j + u * kJoinThreads];
+#pragma unroll
+            for (int u = 0; u < U; ++u) probe(x[u]);
+        }
+        for (; j < sE; j += kJoinThreads) probe(S2[j]);
+        __syncthreads();
+    }
+    acc_c = wave_sum(acc_c);
+    acc_b = wave_sum(acc_b);
+    acc_p = wave_sum(acc_p);
+    const int lane = tid & 63, w = tid >> 6;
+    if (lane == 0) {
+        red[0][w] = acc_c;
+        red[1][w] = acc_b;
+        red[2][w] = acc_p;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long c = 0, b = 0, p = 0;
+        for (int k = 0; k < kJoinThreads / 64; ++k) {
+            c += red[0][k];
+            b += red[1][k];
+            p += red[2][k];
+        }
+        if (c) {
+            atomicAdd(&result[0], c);
+            atomicAdd(&result[1], b);
+            atomicAdd(&result[2], p);
+        }
+        const unsigned long long pieces = (nR + kJoinPiece - 1) / kJoinPiece;
+        atomicMax(&result[3], pieces);
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------- global table
+template <typename KT>
+struct GSlot {
+    KT k;
+    uint32_t c;
+    unsigned long long s;
+};
+
+template <typename KT>
+__global__ __launch_bounds__(256) void k_gclear(GSlot<KT>* __restrict__ t, uint64_t cap,
+                                                unsigned long long* __restrict__ special) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < cap; i += stride) {
+        t[i].k = KeyEmpty<KT>::v;
+        t[i].c = 0;
+        t[i].s = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 2) special[threadIdx.x] = 0;
+}
+
+template <typename KT>
+__global__ __launch_bounds__(256) void k_gbuild(const KT* __restrict__ rk, const KT* __restrict__ rv, uint64_t n,
+                                                GSlot<KT>* __restrict__ t, int logcap,
+                                                unsigned long long* __restrict__ special) {
+    constexpr KT EMPTY = KeyEmpty<KT>::v;
+    const uint64_t mask = (1ull << logcap) - 1;
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const KT k = rk[i];
+        const unsigned long long v = (unsigned long long)(int64_t)rv[i];
+        if (k == EMPTY) {
+            atomicAdd(&special[0], 1ull);
+            atomicAdd(&special[1], v);
+            continue;
+        }
+        uint64_t slot = (hmix(k) >> (64 - logcap)) & mask;
+        while (true) {
+            KT prev;
+            if constexpr (sizeof(KT) == 8)
+                prev = (KT)atomicCAS((unsigned long long*)&t[slot].k, (unsigned long long)EMPTY, (unsigned long long)k);
+            else
+                prev = (KT)atomicCAS((unsigned int*)&t[slot].k, (unsigned int)EMPTY, (unsigned int)k);
+            if (prev == EMPTY || prev == k) {
+                atomicAdd(&t[slot].c, 1u);
+                atomicAdd(&t[slot].s, v);
+                break;
+            }
+            slot = (slot + 1) & mask;
+        }
+    }
+}
+
+template <typename KT, bool HVS>
+__global__ __launch_bounds__(256) void k_gprobe(const KT* __restrict__ sk, const KT* __restrict__ sv, uint64_t n,
+                                                const GSlot<KT>* __restrict__ t, int logcap,
+                                                const unsigned long long* __restrict__ special,
+                                                unsigned long long* __restrict__ result) {
+    constexpr KT EMPTY = KeyEmpty<KT>::v;
+    const uint64_t mask = (1ull << logcap) - 1;
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    unsigned long long acc_c = 0, acc_b = 0, acc_p = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const KT k = sk[i];
+        unsigned long long c = 0, sb = 0;
+        if (k == EMPTY) {
+            c = special[0];
+            sb = special[1];
+        } else {
+            uint64_t slot = (hmix(k) >> (64 - logcap)) & mask;
+            while (true) {
+                const KT tk = t[slot].k;
+                if (tk == k) {
+                    c = t[slot].c;
+                    sb = t[slot].s;
+                    break;
+                }
+                if (tk == EMPTY) break;
+                slot = (slot + 1) & mask;
+            }
+        }
+        acc_c += c;
+        acc_b += sb;
+        if constexpr (HVS) acc_p += c * (unsigned long long)(int64_t)sv[i];
+    }
+    __shared__ unsigned long long red[3][4];
+    acc_c = wave_sum(acc_c);
+    acc_b = wave_sum(acc_b);
+    acc_p = wave_sum(acc_p);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][w] = acc_c;
+        red[1][w] = acc_b;
+        red[2][w] = acc_p;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        unsigned long long b = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        unsigned long long p = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+        if (c) {
+            atomicAdd(&result[0], c);
+            atomicAdd(&result[1], b);
+            atomicAdd(&result[2], p);
+        }
+    }
+}
+
+}  // namespace
+
+// ====================================================================== plan
+struct pgpu_join_plan {
+    pgpu_ctx* ctx = nullptr;
+    pgpu_join_cfg cfg{};
+    uint64_t maxR = 0, maxS = 0;
+    int bits = 0;           // fixed bits from cfg (0 = choose per call)
+    uint32_t nbmax = 0;     // 2^maxbits allocated
+    uint32_t nblkR = 0, nblkS = 0;  // allocated block rows
+    void *R1 = nullptr, *R2 = nullptr, *S1 = nullptr, *S2 = nullptr;
+    uint32_t *HR = nullptr, *HS = nullptr, *totR = nullptr, *totS = nullptr, *icount = nullptr;
+    uint64_t *baseR = nullptr, *baseS = nullptr, *off1R = nullptr, *off1S = nullptr, *ioff = nullptr;
+    unsigned long long* result = nullptr;  // [4]: count, sum_build, sum_probe, max pieces
+    unsigned long long* h_result = nullptr; // pinned
+    void* gtable = nullptr;
+    uint64_t gcap = 0;
+    int glog = 0;
+    unsigned long long* gspecial = nullptr;
+    uint32_t last_bits = 0;
+    int last_algo = 0;
+    bool pending = false;
+    pgpu::KernelTimer* timer = nullptr;
+    std::vector<void*> owned;
+};
+
+namespace {
+
+using pgpu::ceil_div;
+
+int plan_alloc(pgpu_join_plan* p, uint64_t bytes, void** out) {
+    PGPU_TRY(pgpu::ctx_alloc(p->ctx, bytes, out));
+    p->owned.push_back(*out);
+    return PGPU_OK;
+}
+
+// Elements per LDS staging tile for an element of `esz` bytes.
+inline uint64_t tile_elems(uint64_t esz) { return kTileBytes / esz; }
+
+// Rows per histogram/scatter block: at most 512 blocks, tile-aligned.
+inline uint64_t block_rows(uint64_t n, uint64_t esz) {
+    const uint64_t T = tile_elems(esz);
+    uint64_t blk = ceil_div(std::max<uint64_t>(n, 1), 512);
+    blk = ceil_div(blk, T) * T;
+    return std::max<uint64_t>(blk, T);
+}
+
+inline int choose_bits(uint64_t nR) {
+    // ~1024 build rows per final partition fits the 2048-slot LDS table.
+    int b = (int)pgpu::ilog2_ceil(ceil_div(std::max<uint64_t>(nR, 1), 1024));
+    return std::max(2, std::min(kMaxBits, b));
+}
+
+struct Timed {
+    pgpu::KernelTimer* t;
+    int idx;
+    Timed(pgpu::KernelTimer* t_, const char* name) : t(t_), idx(t_->start(name)) {}
+    ~Timed() { t->stop(idx); }
+};
+
+template <typename KT, bool HV>
+int run_partition(pgpu_join_plan* p, const KT* keys, const KT* vals, uint64_t n, int B, uint32_t* H, uint32_t* tot,
+                  uint64_t* base, uint64_t* off1, Elem<KT, HV>* P1, Elem<KT, HV>* P2, const char* tag) {
+    using E = Elem<KT, HV>;
+    hipStream_t st = p->ctx->stream;
+    const int B2 = B - B / 2;  // generate(2): pass 1 takes floor(B/2) high bits
+    const uint32_t nb = 1u << B, nb1 = 1u << (B - B2);
+    const uint64_t blk = block_rows(n, sizeof(E));
+    const uint32_t nblk = (uint32_t)ceil_div(std::max<uint64_t>(n, 1), blk);
+    constexpr int IPT = kTileBytes / (int)sizeof(E) / kScatterThreads;
+    char name[48];
+    {
+        snprintf(name, sizeof name, "hist_%s", tag);
+        Timed tm(p->timer, name);
+        hipLaunchKernelGGL(k_hist<KT>, dim3(nblk), dim3(512), nb * sizeof(uint32_t), st, keys, n, blk, B, H);
+        PGPU_LAUNCH_CHECK();
+    }
+    {
+        snprintf(name, sizeof name, "scan_%s", tag);
+        Timed tm(p->timer, name);
+        hipLaunchKernelGGL(k_colscan, dim3((unsigned)ceil_div(nb, 64)), dim3(512), 0, st, H, nblk, nb, tot);
+        PGPU_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_binscan, dim3(1), dim3(1024), 0, st, tot, nb, base);
+        PGPU_LAUNCH_CHECK();
+        const uint64_t nt = (uint64_t)(nblk + 1) * nb1;
+        hipLaunchKernelGGL(k_off1, dim3((unsigned)ceil_div(nt, 256)), dim3(256), 0, st, H, base, nblk, B, B2, off1);
+        PGPU_LAUNCH_CHECK();
+    }
+    {
+        snprintf(name, sizeof name, "scatter1_%s", tag);
+        Timed tm(p->timer, name);
+        hipLaunchKernelGGL((k_scatter1<KT, HV, IPT>), dim3(nblk), dim3(kScatterThreads), 0, st, keys, vals, n, blk,
+                           B, B2, off1, P1);
+        PGPU_LAUNCH_CHECK();
+    }
+    {
+        snprintf(name, sizeof name, "scatter2_%s", tag);
+        Timed tm(p->timer, name);
+        hipLaunchKernelGGL((k_scatter2<KT, HV, IPT>), dim3(nblk * nb1), dim3(kScatterThreads), 0, st, P1, nblk, B,
+                           B2, off1, H, base, P2);
+        PGPU_LAUNCH_CHECK();
+    }
+    return PGPU_OK;
+}
+
+template <typename KT, bool HVS>
+int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const KT* sk, const KT* sv, uint64_t nS,
+              int B) {
+    hipStream_t st = p->ctx->stream;
+    const uint32_t nb = 1u << B;
+    PGPU_TRY((run_partition<KT, true>(p, rk, rv, nR, B, p->HR, p->totR, p->baseR, p->off1R,
+                                      (Elem<KT, true>*)p->R1, (Elem<KT, true>*)p->R2, "build")));
+    PGPU_TRY((run_partition<KT, HVS>(p, sk, sv, nS, B, p->HS, p->totS, p->baseS, p->off1S,
+                                     (Elem<KT, HVS>*)p->S1, (Elem<KT, HVS>*)p->S2, "probe")));
+    {
+        Timed tm(p->timer, "join_plan");
+        hipLaunchKernelGGL(k_icount, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, p->totS, p->totR, nb,
+                           kItemRows, p->icount);
+        PGPU_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_binscan, dim3(1), dim3(1024), 0, st, p->icount, nb, p->ioff);
+        PGPU_LAUNCH_CHECK();
+    }
+    const uint64_t max_items = nb + ceil_div(nS, kItemRows) + 1;
+    {
+        Timed tm(p->timer, "join_lds");
+        hipLaunchKernelGGL((k_join_cs<KT, HVS>), dim3((unsigned)max_items), dim3(kJoinThreads), 0, st,
+                           (const Elem<KT, true>*)p->R2, p->baseR, p->totR, (const Elem<KT, HVS>*)p->S2, p->baseS,
+                           p->totS, p->ioff, nb, kItemRows, B, p->result);
+        PGPU_LAUNCH_CHECK();
+    }
+    return PGPU_OK;
+}
+
+template <typename KT, bool HVS>
+int run_global(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const KT* sk, const KT* sv, uint64_t nS) {
+    hipStream_t st = p->ctx->stream;
+    const int logcap = (int)pgpu::ilog2_ceil(std::max<uint64_t>(2 * nR, 2));
+    const uint64_t cap = 1ull << logcap;
+    if (cap > p->gcap) return pgpu::fail(PGPU_EINVAL, "global table: %llu slots needed, plan holds %llu",
+                                         (unsigned long long)cap, (unsigned long long)p->gcap);
+    auto* t = (GSlot<KT>*)p->gtable;
+    {
+        Timed tm(p->timer, "gclear");
+        hipLaunchKernelGGL(k_gclear<KT>, dim3((unsigned)std::min<uint64_t>(ceil_div(cap, 256), 4096)), dim3(256), 0,
+                           st, t, cap, p->gspecial);
+        PGPU_LAUNCH_CHECK();
+    }
+    if (nR) {
+        Timed tm(p->timer, "gbuild");
+        hipLaunchKernelGGL(k_gbuild<KT>, dim3((unsigned)std::min<uint64_t>(ceil_div(nR, 256), 8192)), dim3(256), 0,
+                           st, rk, rv, nR, t, logcap, p->gspecial);
+        PGPU_LAUNCH_CHECK();
+    }
+    if (nS) {
+        Timed tm(p->timer, "gprobe");
+        hipLaunchKernelGGL((k_gprobe<KT, HVS>), dim3((unsigned)std::min<uint64_t>(ceil_div(nS, 256), 8192)),
+                           dim3(256), 0, st, sk, sv, nS, t, logcap, p->gspecial, p->result);
+        PGPU_LAUNCH_CHECK();
+    }
+    return PGPU_OK;
+}
+
+}  // namespace
+
+using namespace pgpu;
+
+extern "C" {
+
+int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_build, uint64_t max_probe,
+                          pgpu_join_plan** out) {
+    if (!ctx || !cfg || !out) return fail(PGPU_EINVAL, "pgpu_join_plan_create: null argument");
+    *out = nullptr;
+    if (cfg->key_bytes != 4 && cfg->key_bytes != 8)
+        return fail(PGPU_EUNSUPPORTED, "join keys must be 4 or 8 bytes (got %u)", cfg->key_bytes);
+    if (cfg->radix_bits != 0 && (cfg->radix_bits < 2 || cfg->radix_bits > kMaxBits))
+        return fail(PGPU_EINVAL, "radix_bits must be 0 or in [2, %d] (got %u)", kMaxBits, cfg->radix_bits);
+    if (cfg->algo > PGPU_ALGO_GLOBAL) return fail(PGPU_EINVAL, "unknown join algorithm %u", cfg->algo);
+    if (max_build >= (1ull << 32) || max_probe >= (1ull << 40))
+        return fail(PGPU_EINVAL, "relation too large (build < 2^32, probe < 2^40 rows)");
+    PGPU_TRY(ctx_activate(ctx));
+    auto* p = new pgpu_join_plan();
+    p->ctx = ctx;
+    p->cfg = *cfg;
+    p->maxR = max_build;
+    p->maxS = max_probe;
+    p->bits = (int)cfg->radix_bits;
+    p->timer = new KernelTimer(ctx->stream);
+    auto cleanup = [&](int rc) {
+        pgpu_join_plan_destroy(p);
+        return rc;
+    };
+    const uint64_t kb = cfg->key_bytes;
+    const uint64_t esR = 2 * kb, esS = cfg->probe_payload ? 2 * kb : kb;
+    const bool want_radix = cfg->algo != PGPU_ALGO_GLOBAL;
+    const bool want_global = cfg->algo != PGPU_ALGO_RADIX;
+    int rc;
+    if ((rc = plan_alloc(p, 4 * sizeof(unsigned long long), (void**)&p->result)) != PGPU_OK) return cleanup(rc);
+    if (hipHostMalloc((void**)&p->h_result, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(PGPU_ENOMEM, "hipHostMalloc for the result failed"));
+    if (want_radix) {
+        const int maxbits = p->bits ? p->bits : kMaxBits;
+        p->nbmax = 1u << maxbits;
+        p->nblkR = (uint32_t)ceil_div(std::max<uint64_t>(max_build, 1), block_rows(max_build, esR));
+        p->nblkS = (uint32_t)ceil_div(std::max<uint64_t>(max_probe, 1), block_rows(max_probe, esS));
+        // block_rows() never yields more than 512 blocks; size for that bound
+        // so smaller calls (which may pick a different block size) fit too.
+        p->nblkR = std::max<uint32_t>(p->nblkR, 512);
+        p->nblkS = std::max<uint32_t>(p->nblkS, 512);
+        const uint64_t nb = p->nbmax;
+        struct {
+            void** ptr;
+            uint64_t bytes;
+        } a[] = {
+            {&p->R1, std::max<uint64_t>(max_build, 1) * esR},
+            {&p->R2, std::max<uint64_t>(max_build, 1) * esR},
+            {&p->S1, std::max<uint64_t>(max_probe, 1) * esS},
+            {&p->S2, std::max<uint64_t>(max_probe, 1) * esS},
+            {(void**)&p->HR, (uint64_t)p->nblkR * nb * 4},
+            {(void**)&p->HS, (uint64_t)p->nblkS * nb * 4},
+            {(void**)&p->totR, nb * 4},
+            {(void**)&p->totS, nb * 4},
+            {(void**)&p->icount, nb * 4},
+            {(void**)&p->baseR, (nb + 1) * 8},
+            {(void**)&p->baseS, (nb + 1) * 8},
+            {(void**)&p->ioff, (nb + 1) * 8},
+            {(void**)&p->off1R, (uint64_t)(p->nblkR + 1) * 256 * 8},
+            {(void**)&p->off1S, (uint64_t)(p->nblkS + 1) * 256 * 8},
+        };
+        for (auto& x : a)
+            if ((rc = plan_alloc(p, x.bytes, x.ptr)) != PGPU_OK) return cleanup(rc);
+    }
+    if (want_global) {
+        p->glog = (int)ilog2_ceil(std::max<uint64_t>(2 * max_build, 2));
+        p->gcap = 1ull << p->glog;
+        const uint64_t slot = kb == 8 ? sizeof(GSlot<int64_t>) : sizeof(GSlot<int32_t>);
+        if ((rc = plan_alloc(p, p->gcap * slot, &p->gtable)) != PGPU_OK) return cleanup(rc);
+        if ((rc = plan_alloc(p, 2 * sizeof(unsigned long long), (void**)&p->gspecial)) != PGPU_OK)
+            return cleanup(rc);
+    }
+    *out = p;
+    return PGPU_OK;
+}
+
+int pgpu_join_plan_destroy(pgpu_join_plan* p) {
+    if (!p) return PGPU_OK;
+    if (p->ctx) {
+        (void)hipSetDevice(p->ctx->device);
+        (void)hipStreamSynchronize(p->ctx->stream);
+    }
+    delete p->timer;
+    for (void* x : p->owned) (void)ctx_free(p->ctx, x);
+    if (p->h_result) (void)hipHostFree(p->h_result);
+    delete p;
+    return PGPU_OK;
+}
+
+int pgpu_join_count_sum_async(pgpu_join_plan* p, const void* d_rkey, const void* d_rpay, uint64_t nR,
+                              const void* d_skey, const void* d_spay, uint64_t nS) {
+    if (!p) return fail(PGPU_EINVAL, "pgpu_join_count_sum_async: null plan");
+    if (nR > p->maxR || nS > p->maxS)
+        return fail(PGPU_EINVAL, "join of %llu x %llu rows exceeds the plan's %llu x %llu",
+                    (unsigned long long)nR, (unsigned long long)nS, (unsigned long long)p->maxR,
+                    (unsigned long long)p->maxS);
+    if ((nR && (!d_rkey || !d_rpay)) || (nS && !d_skey) || (nS && p->cfg.probe_payload && !d_spay))
+        return fail(PGPU_EINVAL, "pgpu_join_count_sum_async: null column");
+    PGPU_TRY(ctx_activate(p->ctx));
+    hipStream_t st = p->ctx->stream;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p->result, 4);
+    PGPU_LAUNCH_CHECK();
+    int algo = (int)p->cfg.algo;
+    if (algo == PGPU_ALGO_AUTO) algo = (nR <= (1u << 16) || !p->R1) ? PGPU_ALGO_GLOBAL : PGPU_ALGO_RADIX;
+    if (algo == PGPU_ALGO_GLOBAL && !p->gtable) algo = PGPU_ALGO_RADIX;
+    if (algo == PGPU_ALGO_RADIX && !p->R1) algo = PGPU_ALGO_GLOBAL;
+    p->last_algo = algo;
+    int rc = PGPU_OK;
+    const bool hv = p->cfg.probe_payload != 0;
+    if (nR && nS) {
+        if (algo == PGPU_ALGO_RADIX) {
+            const int B = p->bits ? p->bits : std::min(choose_bits(nR), (int)ilog2_ceil(p->nbmax));
+            p->last_bits = (uint32_t)B;
+            if (p->cfg.key_bytes == 8)
+                rc = hv ? run_radix<int64_t, true>(p, (const int64_t*)d_rkey, (const int64_t*)d_rpay, nR,
+                                                   (const int64_t*)d_skey, (const int64_t*)d_spay, nS, B)
+                        : run_radix<int64_t, false>(p, (const int64_t*)d_rkey, (const int64_t*)d_rpay, nR,
+                                                    (const int64_t*)d_skey, nullptr, nS, B);
+            else
+                rc = hv ? run_radix<int32_t, true>(p, (const int32_t*)d_rkey, (const int32_t*)d_rpay, nR,
+                                                   (const int32_t*)d_skey, (const int32_t*)d_spay, nS, B)
+                        : run_radix<int32_t, false>(p, (const int32_t*)d_rkey, (const int32_t*)d_rpay, nR,
+                                                    (const int32_t*)d_skey, nullptr, nS, B);
+        } else {
+            p->last_bits = 0;
+            if (p->cfg.key_bytes == 8)
+                rc = hv ? run_global<int64_t, true>(p, (const int64_t*)d_rkey, (const int64_t*)d_rpay, nR,
+                                                    (const int64_t*)d_skey, (const int64_t*)d_spay, nS)
+                        : run_global<int64_t, false>(p, (const int64_t*)d_rkey, (const int64_t*)d_rpay, nR,
+                                                     (const int64_t*)d_skey, nullptr, nS);
+            else
+                rc = hv ? run_global<int32_t, true>(p, (const int32_t*)d_rkey, (const int32_t*)d_rpay, nR,
+                                                    (const int32_t*)d_skey, (const int32_t*)d_spay, nS)
+                        : run_global<int32_t, false>(p, (const int32_t*)d_rkey, (const int32_t*)d_rpay, nR,
+                                                     (const int32_t*)d_skey, nullptr, nS);
+        }
+    }
+    if (rc != PGPU_OK) return rc;
+    PGPU_HIP(hipMemcpyAsync(p->h_result, p->result, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    p->pending = true;
+    return PGPU_OK;
+}
+
+int pgpu_join_plan_result(pgpu_join_plan* p, pgpu_join_result* out) {
+    if (!p || !out) return fail(PGPU_EINVAL, "pgpu_join_plan_result: null argument");
+    if (!p->pending) return fail(PGPU_ESTATE, "pgpu_join_plan_result: no join enqueued");
+    PGPU_TRY(ctx_activate(p->ctx));
+    PGPU_HIP(hipStreamSynchronize(p->ctx->stream));
+    p->pending = false;
+    memset(out, 0, sizeof *out);
+    out->matches = p->h_result[0];
+    out->sum_build = p->h_result[1];
+    out->sum_probe = p->h_result[2];
+    out->radix_bits = p->last_bits;
+    out->lds_passes = (uint32_t)p->h_result[3];
+    return PGPU_OK;
+}
+
+int pgpu_join_count_sum(pgpu_join_plan* p, const void* d_rkey, const void* d_rpay, uint64_t nR, const void* d_skey,
+                        const void* d_spay, uint64_t nS, pgpu_join_result* out) {
+    PGPU_TRY(pgpu_join_count_sum_async(p, d_rkey, d_rpay, nR, d_skey, d_spay, nS));
+    return pgpu_join_plan_result(p, out);
+}
+
+int pgpu_join_plan_timing(pgpu_join_plan* p, int enable) {
+    if (!p) return fail(PGPU_EINVAL, "pgpu_join_plan_timing: null plan");
+    PGPU_TRY(ctx_activate(p->ctx));
+    p->timer->enable(enable != 0);
+    return PGPU_OK;
+}
+
+int pgpu_join_plan_times(pgpu_join_plan* p, pgpu_kernel_time* out, int cap, int* n) {
+    if (!p || !n || (cap > 0 && !out)) return fail(PGPU_EINVAL, "pgpu_join_plan_times: null argument");
+    PGPU_TRY(ctx_activate(p->ctx));
+    p->timer->harvest();
+    const auto& t = p->timer->totals();
+    *n = (int)t.size();
+    for (int i = 0; i < (int)t.size() && i < cap; ++i) {
+        memset(&out[i], 0, sizeof out[i]);
+        snprintf(out[i].name, sizeof out[i].name, "%s", t[i].name.c_str());
+        out[i].launches = t[i].launches;
+        out[i].total_ms = t[i].ms;
+    }
+    return PGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,246 @@
+// Context, budgeted HBM allocation, copies, errors and kernel timing.
+//
+// The reference allocates every operator buffer through
+// numaallocate_onnode/numaallocate_local with per-tag byte accounting
+// (util/numaallocate.cpp:60-181, :402-457).  Here device memory is the
+// resource: every allocation is charged against the context's HBM budget,
+// and an allocation that would exceed it fails with PGPU_EBUDGET instead of
+// blocking, so admission control can act on it.
+#include <cstdio>
+#include <cstring>
+
+#include "pgpu_internal.h"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+namespace pgpu {
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int ctx_activate(pgpu_ctx* ctx) {
+    PGPU_HIP(hipSetDevice(ctx->device));
+    return PGPU_OK;
+}
+
+int ctx_alloc(pgpu_ctx* ctx, uint64_t bytes, void** p) {
+    if (!ctx || !p) return fail(PGPU_EINVAL, "ctx_alloc: null argument");
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~uint64_t(255);
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (ctx->used + bytes > ctx->budget)
+            return fail(PGPU_EBUDGET,
+                        "HBM budget exceeded: %llu B in use + %llu B requested > %llu B budget",
+                        (unsigned long long)ctx->used, (unsigned long long)bytes,
+                        (unsigned long long)ctx->budget);
+        ctx->used += bytes;  // reserve before the (slow) allocation
+        if (ctx->used > ctx->peak) ctx->peak = ctx->used;
+    }
+    PGPU_TRY(ctx_activate(ctx));
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->used -= bytes;
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(PGPU_ENOMEM, "hipMalloc(%llu) failed: %s", (unsigned long long)bytes,
+                    hipGetErrorString(e));
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->allocs[*p] = bytes;
+    return PGPU_OK;
+}
+
+int ctx_free(pgpu_ctx* ctx, void* p) {
+    if (!p) return PGPU_OK;
+    uint64_t bytes = 0;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto it = ctx->allocs.find(p);
+        if (it == ctx->allocs.end())
+            return fail(PGPU_EINVAL, "pgpu_free: %p was not allocated by this context", p);
+        bytes = it->second;
+        ctx->allocs.erase(it);
+        ctx->used -= bytes;
+    }
+    PGPU_TRY(ctx_activate(ctx));
+    PGPU_HIP(hipFree(p));
+    return PGPU_OK;
+}
+
+// ---------------------------------------------------------------- timer
+KernelTimer::~KernelTimer() {
+    for (auto& p : pending_) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool_) (void)hipEventDestroy(e);
+}
+
+hipEvent_t KernelTimer::get_event() {
+    if (!pool_.empty()) {
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void KernelTimer::enable(bool on) {
+    harvest();
+    on_ = on;
+    if (on) totals_.clear();
+}
+
+int KernelTimer::start(const char* name) {
+    if (!on_) return -1;
+    int slot = -1;
+    for (size_t i = 0; i < totals_.size(); ++i)
+        if (totals_[i].name == name) slot = (int)i;
+    if (slot < 0) {
+        totals_.push_back(Total{name, 0, 0.0});
+        slot = (int)totals_.size() - 1;
+    }
+    Pending p{slot, get_event(), get_event()};
+    if (!p.a || !p.b) return -1;
+    (void)hipEventRecord(p.a, stream_);
+    pending_.push_back(p);
+    return (int)pending_.size() - 1;
+}
+
+void KernelTimer::stop(int idx) {
+    if (idx < 0 || idx >= (int)pending_.size()) return;
+    (void)hipEventRecord(pending_[idx].b, stream_);
+}
+
+int KernelTimer::harvest() {
+    for (auto& p : pending_) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            totals_[p.slot].launches += 1;
+            totals_[p.slot].ms += ms;
+        }
+        pool_.push_back(p.a);
+        pool_.push_back(p.b);
+    }
+    pending_.clear();
+    return PGPU_OK;
+}
+
+}  // namespace pgpu
+
+using namespace pgpu;
+
+extern "C" {
+
+const char* pgpu_version(void) { return "pgpu 0.1 gfx950"; }
+
+const char* pgpu_last_error(void) { return g_last_error.c_str(); }
+
+int pgpu_device_count(int* n) {
+    if (!n) return fail(PGPU_EINVAL, "pgpu_device_count: null argument");
+    *n = 0;
+    hipError_t e = hipGetDeviceCount(n);
+    if (e != hipSuccess) {
+        *n = 0;
+        (void)hipGetLastError();
+        return fail(PGPU_EDEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    return PGPU_OK;
+}
+
+int pgpu_ctx_create(int device, uint64_t hbm_budget_bytes, pgpu_ctx** out) {
+    if (!out) return fail(PGPU_EINVAL, "pgpu_ctx_create: null out");
+    *out = nullptr;
+    int n = 0;
+    PGPU_TRY(pgpu_device_count(&n));
+    if (device < 0 || device >= n)
+        return fail(PGPU_EINVAL, "pgpu_ctx_create: device %d out of range (%d devices)", device, n);
+    PGPU_HIP(hipSetDevice(device));
+    size_t freeb = 0, totalb = 0;
+    PGPU_HIP(hipMemGetInfo(&freeb, &totalb));
+    auto* ctx = new pgpu_ctx();
+    ctx->device = device;
+    ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)freeb;
+    // Default priority stream; never blocks on the legacy null stream.
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(PGPU_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return PGPU_OK;
+}
+
+int pgpu_ctx_destroy(pgpu_ctx* ctx) {
+    if (!ctx) return PGPU_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
+    ctx->allocs.clear();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PGPU_OK;
+}
+
+int pgpu_ctx_stream(pgpu_ctx* ctx, void** s) {
+    if (!ctx || !s) return fail(PGPU_EINVAL, "pgpu_ctx_stream: null argument");
+    *s = (void*)ctx->stream;
+    return PGPU_OK;
+}
+
+int pgpu_alloc(pgpu_ctx* ctx, uint64_t bytes, void** d_ptr) { return ctx_alloc(ctx, bytes, d_ptr); }
+
+int pgpu_free(pgpu_ctx* ctx, void* d_ptr) {
+    if (!ctx) return fail(PGPU_EINVAL, "pgpu_free: null ctx");
+    return ctx_free(ctx, d_ptr);
+}
+
+int pgpu_hbm_usage(pgpu_ctx* ctx, uint64_t* used, uint64_t* budget, uint64_t* peak) {
+    if (!ctx) return fail(PGPU_EINVAL, "pgpu_hbm_usage: null ctx");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (used) *used = ctx->used;
+    if (budget) *budget = ctx->budget;
+    if (peak) *peak = ctx->peak;
+    return PGPU_OK;
+}
+
+int pgpu_h2d(pgpu_ctx* ctx, void* d_dst, const void* src, uint64_t bytes) {
+    if (!ctx || (!d_dst && bytes) || (!src && bytes)) return fail(PGPU_EINVAL, "pgpu_h2d: null argument");
+    if (!bytes) return PGPU_OK;
+    PGPU_TRY(ctx_activate(ctx));
+    PGPU_HIP(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PGPU_HIP(hipStreamSynchronize(ctx->stream));
+    return PGPU_OK;
+}
+
+int pgpu_d2h(pgpu_ctx* ctx, void* dst, const void* d_src, uint64_t bytes) {
+    if (!ctx || (!dst && bytes) || (!d_src && bytes)) return fail(PGPU_EINVAL, "pgpu_d2h: null argument");
+    if (!bytes) return PGPU_OK;
+    PGPU_TRY(ctx_activate(ctx));
+    PGPU_HIP(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PGPU_HIP(hipStreamSynchronize(ctx->stream));
+    return PGPU_OK;
+}
+
+int pgpu_sync(pgpu_ctx* ctx) {
+    if (!ctx) return fail(PGPU_EINVAL, "pgpu_sync: null ctx");
+    PGPU_TRY(ctx_activate(ctx));
+    PGPU_HIP(hipStreamSynchronize(ctx->stream));
+    return PGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,23 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device; runs the HIP path")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    import pythia_amd as pa
+
+    if pa.device_count() < 1:
+        pytest.fail("GPU test selected but no HIP device is visible")
+    c = pa.Context(0)
+    yield c
+    c.close()

@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports every function include/*.h declares.
+No compute calls here: this runs without a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("pgpu.h",)]
+LIB = os.path.join(ROOT, "pythia_amd", "libpgpu.so")
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*(pgpu_[a-z_0-9]+)\s*\(", text, flags=re.M)))
+
+
+def nm_exports(path):
+    import subprocess
+
+    out = subprocess.run(["nm", "-D", "--defined-only", path], stdout=subprocess.PIPE, text=True, check=True).stdout
+    return {l.split()[-1] for l in out.splitlines() if l.strip()}
+
+
+@pytest.mark.parametrize("header", HEADERS)
+def test_library_exports_every_declared_symbol(header):
+    assert os.path.exists(LIB), "libpgpu.so not built"
+    C.CDLL(LIB)  # loads (HIP runtime initialises lazily; no device needed)
+    names = declared_functions(header)
+    assert len(names) >= 20
+    missing = [n for n in names if n not in nm_exports(LIB)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_exports_are_c_linkage():
+    exports = nm_exports(LIB)
+    assert "pgpu_join_count_sum" in exports  # unmangled
+    assert not any(e.startswith("_Z") and "pgpu_join" in e and "plan" in e and e.endswith("create") for e in exports)
+
+
+def test_error_path_without_device():
+    """Argument errors are reported through the status code and
+    pgpu_last_error() before any device work."""
+    l = C.CDLL(LIB)
+    l.pgpu_last_error.restype = C.c_char_p
+    rc = l.pgpu_join_plan_create(None, None, C.c_uint64(0), C.c_uint64(0), None)
+    assert rc == -1
+    assert b"null" in l.pgpu_last_error()

@@ -1,0 +1,207 @@
+"""GPU parity of the fused join + count/sum path (libpgpu.so through its C ABI).
+
+Every expected value comes from the oracle: the chained-bucket restatement of
+HashJoinOp (oracle.hashjoin_count_sum) on small inputs, and the generators'
+closed forms (exact int64 sums) up to the full C2 shape.  Integer results
+must be bit-exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pythia_amd as pa
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def _dev(ctx, arr):
+    return ctx.upload(np.ascontiguousarray(arr))
+
+
+@pytest.mark.parametrize("column,dtype,kind", [
+    ("C1_RKEY", np.int32, "c1"), ("C1_RPAY", np.int32, "c1"), ("C1_SFK", np.int32, "c1"),
+    ("C1_SPAY", np.int32, "c1"), ("C2_RKEY", np.int64, "c2"), ("C2_RPAY", np.int64, "c2"),
+    ("C2_SFK_UNIFORM", np.int64, "c2"), ("C2_SFK_ZIPF", np.int64, "c2z"),
+    ("C3_D_ORDERKEY", np.int32, "c3"), ("C3_D_CUSTKEY", np.int32, "c3"), ("C3_F_ORDERKEY", np.int32, "c3"),
+    ("C3_F_PRICE", np.float64, "c3")])
+def test_device_generators_match_host(ctx, column, dtype, kind):
+    nR, nS = 1 << 14, 1 << 16
+    if kind == "c1":
+        rk, rp, sk, sp = O.gen_c1(nR, nS)
+        exp = {"C1_RKEY": rk, "C1_RPAY": rp, "C1_SFK": sk, "C1_SPAY": sp}[column]
+    elif kind.startswith("c2"):
+        z = 1.0 if kind == "c2z" else 0.0
+        rk, rp, sk = O.gen_c2(nR, nS, z)
+        exp = {"C2_RKEY": rk, "C2_RPAY": rp}.get(column, sk)
+    else:
+        dk, dc, fk, fp = O.gen_c3(nR, nS, norders=nR)
+        exp = {"C3_D_ORDERKEY": dk, "C3_D_CUSTKEY": dc, "C3_F_ORDERKEY": fk, "C3_F_PRICE": fp}[column]
+    n = len(exp)
+    out = ctx.alloc(n * np.dtype(dtype).itemsize)
+    cdf = _dev(ctx, O.zipf_cdf(nR, 1.0)) if kind == "c2z" else None
+    ctx.gen(column, n, out, n_build=nR, cdf=cdf)
+    got = out.download(dtype, n)
+    assert np.array_equal(got, exp)
+    out.free()
+    if cdf:
+        cdf.free()
+
+
+def _c2_device(ctx, nR, nS, zipf=0.0):
+    rk, rp, sk = ctx.alloc(nR * 8), ctx.alloc(nR * 8), ctx.alloc(nS * 8)
+    ctx.gen("C2_RKEY", nR, rk)
+    ctx.gen("C2_RPAY", nR, rp)
+    cdf = None
+    if zipf:
+        cdf = _dev(ctx, O.zipf_cdf(nR, zipf))
+        ctx.gen("C2_SFK_ZIPF", nS, sk, n_build=nR, cdf=cdf)
+    else:
+        ctx.gen("C2_SFK_UNIFORM", nS, sk, n_build=nR)
+    ctx.sync()
+    if cdf:
+        cdf.free()
+    return rk, rp, sk
+
+
+@pytest.mark.parametrize("bits", [2, 5, 10, 14])
+@pytest.mark.parametrize("zipf", [0.0, 1.0])
+def test_c2_radix_matches_closed_form(ctx, bits, zipf):
+    nR, nS = 1 << 16, 1 << 21
+    rk, rp, sk = _c2_device(ctx, nR, nS, zipf)
+    plan = pa.JoinPlan(ctx, 8, nR, nS, radix_bits=bits, algo=pa.ALGO_RADIX)
+    r = plan.count_sum(rk, rp, nR, sk, None, nS)
+    ref = O.c2_closed(nR, nS, zipf, threads=THREADS)
+    assert (r.matches, r.sum_build) == (ref.count, ref.sum_build)
+    assert r.radix_bits == bits
+    if bits == 2:
+        assert r.lds_passes > 1  # 16K build rows per partition: LDS pieces
+    plan.close()
+    for b in (rk, rp, sk):
+        b.free()
+
+
+def test_c2_global_matches_closed_form(ctx):
+    nR, nS = 1 << 18, 1 << 21
+    rk, rp, sk = _c2_device(ctx, nR, nS, 0.5)
+    plan = pa.JoinPlan(ctx, 8, nR, nS, algo=pa.ALGO_GLOBAL)
+    r = plan.count_sum(rk, rp, nR, sk, None, nS)
+    ref = O.c2_closed(nR, nS, 0.5, threads=THREADS)
+    assert (r.matches, r.sum_build) == (ref.count, ref.sum_build)
+    plan.close()
+    for b in (rk, rp, sk):
+        b.free()
+
+
+@pytest.mark.parametrize("algo", [pa.ALGO_RADIX, pa.ALGO_GLOBAL])
+def test_c1_int32_with_probe_payload(ctx, algo):
+    nR, nS = 1 << 20, 1 << 24  # the C1 shape
+    bufs = [ctx.alloc(nR * 4), ctx.alloc(nR * 4), ctx.alloc(nS * 4), ctx.alloc(nS * 4)]
+    for b, col, n in zip(bufs, ["C1_RKEY", "C1_RPAY", "C1_SFK", "C1_SPAY"], [nR, nR, nS, nS]):
+        ctx.gen(col, n, b, n_build=nR)
+    plan = pa.JoinPlan(ctx, 4, nR, nS, probe_payload=True, algo=algo)
+    r = plan.count_sum(bufs[0], bufs[1], nR, bufs[2], bufs[3], nS)
+    ref = O.c1_closed(nR, nS, threads=THREADS)
+    assert (r.matches, r.sum_build, r.sum_probe) == (ref.count, ref.sum_build, ref.sum_probe)
+    plan.close()
+    for b in bufs:
+        b.free()
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("algo,bits", [(pa.ALGO_RADIX, 3), (pa.ALGO_RADIX, 9), (pa.ALGO_GLOBAL, 0)])
+def test_random_keys_vs_oracle_join(ctx, kb, algo, bits):
+    """Duplicates on both sides, negative keys, the minimum integer (the
+    table's empty marker) as a real key, and non-matching probes."""
+    rng = np.random.default_rng(kb * 100 + bits)
+    dt = np.int32 if kb == 4 else np.int64
+    lo = np.iinfo(dt).min
+    nR, nS = 9000, 50000
+    bk = rng.integers(-3000, 3000, nR).astype(dt)
+    bk[:7] = lo
+    bp = rng.integers(np.iinfo(dt).min // 4, np.iinfo(dt).max // 4, nR).astype(dt)
+    pk = rng.integers(-3500, 3500, nS).astype(dt)
+    pk[:11] = lo
+    pp = rng.integers(np.iinfo(dt).min // 4, np.iinfo(dt).max // 4, nS).astype(dt)
+    ct = "int" if kb == 4 else "long"
+    bs, ps = O.schema((ct,), (ct,)), O.schema((ct,), (ct,))
+    d = O.join_desc(bs, ps, 0, 0, ["B$1", "P$1"], buckets=1 << 10, tuplesperbucket=4, threads=4)
+    csb, _ = O.hashjoin_count_sum(d, O.rows(bs, bk, bp), O.rows(ps, pk, pp), 0)
+    csp, _ = O.hashjoin_count_sum(d, O.rows(bs, bk, bp), O.rows(ps, pk, pp), 1)
+    plan = pa.JoinPlan(ctx, kb, nR, nS, radix_bits=bits, probe_payload=True, algo=algo)
+    bufs = [_dev(ctx, a) for a in (bk, bp, pk, pp)]
+    r = plan.count_sum(bufs[0], bufs[1], nR, bufs[2], bufs[3], nS)
+    assert r.matches == csb.count
+    mask = (1 << (8 * kb)) - 1  # the reference accumulates in the column type
+    assert r.sum_build & mask == csb.isum & mask
+    assert r.sum_probe & mask == csp.isum & mask
+    # exact 64-bit sums from numpy
+    bsum = {}
+    for k, v in zip(bk.tolist(), bp.tolist()):
+        c, s = bsum.get(k, (0, 0))
+        bsum[k] = (c + 1, s + v)
+    ex_b = sum(bsum.get(k, (0, 0))[1] for k in pk.tolist()) % 2**64
+    ex_p = sum(bsum.get(k, (0, 0))[0] * v for k, v in zip(pk.tolist(), pp.tolist())) % 2**64
+    assert (r.sum_build, r.sum_probe) == (ex_b, ex_p)
+    plan.close()
+    for b in bufs:
+        b.free()
+
+
+@pytest.mark.parametrize("nR,nS", [(0, 1000), (1000, 0), (0, 0), (1, 1)])
+def test_empty_and_tiny_inputs(ctx, nR, nS):
+    bk = np.arange(nR, dtype=np.int64)
+    pk = np.zeros(nS, dtype=np.int64)
+    plan = pa.JoinPlan(ctx, 8, max(nR, 1), max(nS, 1), algo=pa.ALGO_RADIX)
+    bufs = [_dev(ctx, a) for a in (bk, bk * 3, pk)]
+    r = plan.count_sum(bufs[0], bufs[1], nR, bufs[2], None, nS)
+    exp = nS if (nR >= 1 and nS >= 1) else 0
+    assert r.matches == exp and r.sum_build == 0
+    plan.close()
+    for b in bufs:
+        b.free()
+
+
+def test_plan_rejects_oversize_and_bad_config(ctx):
+    with pytest.raises(pa.PgpuError):
+        pa.JoinPlan(ctx, 3, 10, 10)
+    with pytest.raises(pa.PgpuError):
+        pa.JoinPlan(ctx, 8, 10, 10, radix_bits=15)
+    plan = pa.JoinPlan(ctx, 8, 10, 10)
+    b = ctx.alloc(8 * 100)
+    with pytest.raises(pa.PgpuError):
+        plan.count_sum(b, b, 11, b, None, 5)
+    plan.close()
+    b.free()
+
+
+def test_hbm_budget_is_enforced():
+    ctx = pa.Context(0, hbm_budget=64 << 20)
+    b = ctx.alloc(32 << 20)
+    with pytest.raises(pa.PgpuError) as e:
+        ctx.alloc(40 << 20)
+    assert e.value.code == -3
+    with pytest.raises(pa.PgpuError):
+        pa.JoinPlan(ctx, 8, 1 << 22, 1 << 22)
+    b.free()
+    used, budget, peak = ctx.hbm_usage()
+    assert used == 0 and budget == 64 << 20 and peak >= 32 << 20
+    ctx.close()
+
+
+@pytest.mark.parametrize("zipf", [0.0, 1.0])
+def test_c2_full_shape(ctx, zipf):
+    """C2 at its full size: R 2^24 x (int64, int64), S 2^28 int64, 14 radix
+    bits; count and int64 sum bit-exact against the closed form."""
+    nR, nS = 1 << 24, 1 << 28
+    rk, rp, sk = _c2_device(ctx, nR, nS, zipf)
+    plan = pa.JoinPlan(ctx, 8, nR, nS, radix_bits=14, algo=pa.ALGO_RADIX)
+    r = plan.count_sum(rk, rp, nR, sk, None, nS)
+    ref = O.c2_closed(nR, nS, zipf, threads=THREADS)
+    assert (r.matches, r.sum_build) == (ref.count, ref.sum_build)
+    assert r.lds_passes == 1
+    plan.close()
+    for b in (rk, rp, sk):
+        b.free()
