@@ -155,6 +155,85 @@ typedef struct pgpu_kernel_time {
 int pgpu_join_plan_timing(pgpu_join_plan* plan, int enable);
 int pgpu_join_plan_times(pgpu_join_plan* plan, pgpu_kernel_time* out, int cap, int* n);
 
+/* ------------------------------------------------------------------ */
+/* Row-format hash join with materialised output (gpuhashjoin).         */
+/* Replaces HashJoinOp's build (buildFromPage into util/hashtable,       */
+/* operators/join.cpp:641-667) and probe/projection loop                 */
+/* (getNext + constructOutputTuple, operators/join.cpp:475-562).         */
+/* Build rows are the reference's `sbuild` layout (join key then the     */
+/* projected build columns, packed); probe rows are the probe child's    */
+/* tuples.  Output rows are the projection, packed, in the output        */
+/* schema's order.  Keys are signed integers of 4 or 8 bytes compared by */
+/* value (an int key equals a long key of the same value).               */
+/* ------------------------------------------------------------------ */
+typedef struct pgpu_proj_col {
+    uint32_t side;       /* 0: build (sbuild) row, 1: probe row */
+    uint32_t src_offset; /* byte offset in that row              */
+    uint32_t bytes;      /* column width                         */
+} pgpu_proj_col;
+
+#define PGPU_MAX_PROJ 32
+
+typedef struct pgpu_hj_desc {
+    uint32_t build_row_bytes, build_key_offset, build_key_bytes;
+    uint32_t probe_row_bytes, probe_key_offset, probe_key_bytes;
+    uint32_t nproj;
+    pgpu_proj_col proj[PGPU_MAX_PROJ];
+    uint32_t slots;      /* independent probe streams (worker threads), >= 1 */
+} pgpu_hj_desc;
+
+typedef struct pgpu_hj_stats {
+    uint64_t build_rows, probe_rows, output_rows;
+    uint64_t table_slots;
+    uint64_t h2d_bytes, d2h_bytes;
+    uint64_t probe_reruns;   /* output buffer regrowths */
+} pgpu_hj_stats;
+
+typedef struct pgpu_hj pgpu_hj;
+int pgpu_hj_create(pgpu_ctx* ctx, const pgpu_hj_desc* desc, pgpu_hj** out);
+int pgpu_hj_destroy(pgpu_hj* h);
+/* Thread-safe: any worker may append build rows (copied before return). */
+int pgpu_hj_build_append(pgpu_hj* h, const void* rows, uint64_t n);
+/* Uploads the staged build rows and builds the device table. */
+int pgpu_hj_build_finish(pgpu_hj* h);
+/* Joins n probe rows on stream `slot`; the output stays in HBM and is read
+ * with pgpu_hj_pull.  *n_out receives the number of output rows. */
+int pgpu_hj_probe(pgpu_hj* h, uint32_t slot, const void* rows, uint64_t n, uint64_t* n_out);
+/* Copies up to cap_rows further output rows of `slot` into dst. */
+int pgpu_hj_pull(pgpu_hj* h, uint32_t slot, void* dst, uint64_t cap_rows, uint64_t* n_rows);
+int pgpu_hj_stats_get(pgpu_hj* h, pgpu_hj_stats* out);
+
+/* ------------------------------------------------------------------ */
+/* Row-format hash aggregation (gpuaggregate_sum / gpuaggregate_count). */
+/* Replaces GenericAggregate::remember + AggregateSum/AggregateCount    */
+/* fold (operators/genericaggregate.cpp:240-282, aggregatesum.cpp:      */
+/* 53-107, aggregatecount.cpp:44-52) and the group emission of          */
+/* GenericAggregate::getNext (:419-487).  Group keys compare bytewise;   */
+/* the accumulator has the summed column's type (int wraps mod 2^32,    */
+/* long mod 2^64, dec is an FP64 sum); count is a long.                 */
+/* ------------------------------------------------------------------ */
+enum pgpu_agg_kind { PGPU_AGG_SUM_I32 = 1, PGPU_AGG_SUM_I64, PGPU_AGG_SUM_F64, PGPU_AGG_COUNT };
+
+typedef struct pgpu_agg_desc {
+    uint32_t row_bytes;
+    uint32_t nkeys;            /* 0 = one group (no GROUP BY)      */
+    uint32_t key_offset[8];
+    uint32_t key_bytes[8];     /* total key bytes <= 64            */
+    uint32_t kind;             /* enum pgpu_agg_kind               */
+    uint32_t val_offset;       /* summed column (unused for count) */
+    uint64_t groups_hint;      /* initial table sizing, 0 = 64K    */
+} pgpu_agg_desc;
+
+typedef struct pgpu_agg pgpu_agg;
+int pgpu_agg_create(pgpu_ctx* ctx, const pgpu_agg_desc* desc, pgpu_agg** out);
+int pgpu_agg_destroy(pgpu_agg* a);
+/* Thread-safe; rows are copied to HBM and folded before return. */
+int pgpu_agg_append(pgpu_agg* a, const void* rows, uint64_t n);
+/* Compacts the groups; output rows = key columns then the accumulator. */
+int pgpu_agg_finish(pgpu_agg* a, uint64_t* n_groups, uint32_t* out_row_bytes);
+/* Copies output rows [first, first + n) into dst. */
+int pgpu_agg_pull(pgpu_agg* a, uint64_t first, uint64_t n, void* dst);
+
 #ifdef __cplusplus
 }
 #endif

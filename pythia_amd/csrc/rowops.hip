@@ -1,0 +1,667 @@
+// Row-format join and aggregation behind the gpuhashjoin / gpuaggregate_*
+// operators, where tuples arrive and leave as the engine's packed pages.
+//
+// Join (replaces HashJoinOp build + probe + constructOutputTuple,
+// operators/join.cpp:475-667): the build rows (sbuild layout) are uploaded
+// once; an open-addressing multimap in HBM holds (key, row index) per build
+// row.  A probe batch is uploaded, each thread counts its matches, the
+// workgroup scans the counts, reserves one contiguous output range with a
+// single atomic, and writes the projected rows there (output compaction
+// without a second pass).  When the output buffer is too small the batch is
+// re-run into a larger one.
+//
+// Aggregation (replaces GenericAggregate::remember and the AggregateSum /
+// AggregateCount folds, operators/genericaggregate.cpp:240-282): an
+// open-addressing table in HBM keyed by the packed group-key bytes; a slot is
+// claimed with a state CAS (empty -> writing -> ready) and folded with
+// hardware atomics in the accumulator's type.  The table grows (rehash) when
+// half full.  finish() compacts the ready slots into output rows.
+#include <algorithm>
+#include <cstring>
+
+#include "pgpu_internal.h"
+
+namespace {
+
+constexpr uint32_t kEmptyIdx = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z;
+}
+
+__device__ __forceinline__ int64_t load_key(const char* p, uint32_t bytes) {
+    if (bytes == 4) {
+        int32_t v;
+        __builtin_memcpy(&v, p, 4);
+        return v;
+    }
+    int64_t v;
+    __builtin_memcpy(&v, p, 8);
+    return v;
+}
+
+struct JSlot {
+    int64_t key;
+    uint32_t idx;
+    uint32_t pad;
+};
+
+struct JoinDev {
+    pgpu_hj_desc d;
+    uint32_t out_row_bytes;
+    uint32_t dst_off[PGPU_MAX_PROJ];
+};
+
+__global__ __launch_bounds__(256) void k_jclear(JSlot* t, uint64_t cap) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < cap; i += stride) {
+        t[i].key = 0;
+        t[i].idx = kEmptyIdx;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_jbuild(const char* __restrict__ rows, uint64_t n, uint32_t row_bytes,
+                                                uint32_t key_off, uint32_t key_bytes, JSlot* __restrict__ t,
+                                                int logcap) {
+    const uint64_t mask = (1ull << logcap) - 1;
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const int64_t k = load_key(rows + i * row_bytes + key_off, key_bytes);
+        uint64_t s = mix64((uint64_t)k) & mask;
+        while (atomicCAS(&t[s].idx, kEmptyIdx, (uint32_t)i) != kEmptyIdx) s = (s + 1) & mask;
+        t[s].key = k;
+    }
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_scan256(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t ws[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int k = 0; k < w; ++k) pre += ws[k];
+    *total = ws[0] + ws[1] + ws[2] + ws[3];
+    __syncthreads();
+    return pre + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_jprobe(JoinDev jd, const char* __restrict__ build,
+                                                const JSlot* __restrict__ t, int logcap,
+                                                const char* __restrict__ probe, uint64_t n, char* __restrict__ out,
+                                                uint64_t cap_rows, unsigned long long* __restrict__ cursor) {
+    const pgpu_hj_desc& d = jd.d;
+    const uint64_t mask = (1ull << logcap) - 1;
+    __shared__ unsigned long long s_base;
+    // grid-stride over whole blocks so every thread reaches the scans
+    for (uint64_t b0 = (uint64_t)blockIdx.x * 256; b0 < n; b0 += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = b0 + threadIdx.x;
+        const bool live = i < n;
+        const char* prow = probe + (live ? i : 0) * d.probe_row_bytes;
+        const int64_t k = live ? load_key(prow + d.probe_key_offset, d.probe_key_bytes) : 0;
+        uint32_t cnt = 0;
+        const uint64_t h0 = mix64((uint64_t)k) & mask;
+        if (live) {
+            for (uint64_t s = h0; t[s].idx != kEmptyIdx; s = (s + 1) & mask) cnt += t[s].key == k;
+        }
+        uint32_t total;
+        const uint32_t off = block_scan256(cnt, &total);
+        if (threadIdx.x == 0) s_base = total ? atomicAdd(cursor, (unsigned long long)total) : 0ull;
+        __syncthreads();
+        uint64_t pos = s_base + off;
+        if (live && cnt) {
+            for (uint64_t s = h0; t[s].idx != kEmptyIdx; s = (s + 1) & mask) {
+                if (t[s].key != k) continue;
+                if (pos < cap_rows) {
+                    char* o = out + pos * jd.out_row_bytes;
+                    const char* brow = build + (uint64_t)t[s].idx * d.build_row_bytes;
+                    for (uint32_t c = 0; c < d.nproj; ++c) {
+                        const char* src = (d.proj[c].side ? prow : brow) + d.proj[c].src_offset;
+                        char* dst = o + jd.dst_off[c];
+                        for (uint32_t b = 0; b < d.proj[c].bytes; ++b) dst[b] = src[b];
+                    }
+                }
+                ++pos;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- aggregation
+constexpr int kMaxKeyWords = 8;  // 64 key bytes
+
+struct AggDev {
+    uint32_t row_bytes, nkeys, kind, val_offset;
+    uint32_t key_offset[8], key_bytes[8];
+    uint32_t key_words;  // ceil(total key bytes / 8)
+    uint32_t key_total;
+    uint32_t slot_words; // 1 (state) + key_words + 1 (acc)
+};
+
+__device__ __forceinline__ void gather_key(const AggDev& a, const char* row, uint64_t* kw) {
+    char buf[kMaxKeyWords * 8];
+#pragma unroll
+    for (int w = 0; w < kMaxKeyWords; ++w) kw[w] = 0;
+    uint32_t o = 0;
+    for (uint32_t c = 0; c < a.nkeys; ++c) {
+        for (uint32_t b = 0; b < a.key_bytes[c]; ++b) buf[o + b] = row[a.key_offset[c] + b];
+        o += a.key_bytes[c];
+    }
+    for (uint32_t b = o; b < a.key_words * 8; ++b) buf[b] = 0;
+    for (uint32_t w = 0; w < a.key_words; ++w) __builtin_memcpy(&kw[w], buf + 8 * w, 8);
+}
+
+__device__ __forceinline__ void fold(const AggDev& a, unsigned long long* acc, const char* row) {
+    switch (a.kind) {
+        case PGPU_AGG_SUM_I32: {
+            int32_t v;
+            __builtin_memcpy(&v, row + a.val_offset, 4);
+            atomicAdd((unsigned int*)acc, (unsigned int)v);  // low word of the little-endian slot
+            break;
+        }
+        case PGPU_AGG_SUM_I64: {
+            int64_t v;
+            __builtin_memcpy(&v, row + a.val_offset, 8);
+            atomicAdd(acc, (unsigned long long)v);
+            break;
+        }
+        case PGPU_AGG_SUM_F64: {
+            double v;
+            __builtin_memcpy(&v, row + a.val_offset, 8);
+            atomicAdd((double*)acc, v);
+            break;
+        }
+        default:
+            atomicAdd(acc, 1ull);
+    }
+}
+
+// Insert-or-fold of `row` (or of an existing slot's key + partial when
+// `partial` is set, used by rehash).  Slot layout in 64-bit words:
+// [state][key words...][acc]; state 0 empty, 1 being written, 2 ready.
+__device__ void agg_insert(const AggDev& a, unsigned long long* t, uint64_t mask, const uint64_t* kw,
+                           const char* row, const unsigned long long* partial,
+                           unsigned long long* __restrict__ ngroups) {
+    uint64_t h = 0x9E3779B97F4A7C15ull;
+    for (uint32_t w = 0; w < a.key_words; ++w) h = mix64(h ^ kw[w]);
+    uint64_t s = h & mask;
+    while (true) {
+        unsigned long long* slot = t + s * a.slot_words;
+        unsigned long long st = __hip_atomic_load(slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (st == 0) {
+            if (atomicCAS(slot, 0ull, 1ull) == 0ull) {
+                for (uint32_t w = 0; w < a.key_words; ++w) slot[1 + w] = kw[w];
+                __threadfence();
+                __hip_atomic_store(slot, 2ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                atomicAdd(ngroups, 1ull);
+                st = 2;
+            } else {
+                continue;  // lost the race: re-read the same slot
+            }
+        } else if (st == 1) {
+            continue;  // the winner stores the key in the same step
+        }
+        bool eq = true;
+        for (uint32_t w = 0; w < a.key_words && eq; ++w) eq = slot[1 + w] == kw[w];
+        if (eq) {
+            unsigned long long* acc = slot + 1 + a.key_words;
+            if (partial) {
+                if (a.kind == PGPU_AGG_SUM_F64)
+                    atomicAdd((double*)acc, __longlong_as_double((long long)*partial));
+                else if (a.kind == PGPU_AGG_SUM_I32)
+                    atomicAdd((unsigned int*)acc, (unsigned int)*partial);
+                else
+                    atomicAdd(acc, *partial);
+            } else {
+                fold(a, acc, row);
+            }
+            return;
+        }
+        s = (s + 1) & mask;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_agg_append(AggDev a, const char* __restrict__ rows, uint64_t n,
+                                                    unsigned long long* __restrict__ t, uint64_t mask,
+                                                    unsigned long long* __restrict__ ngroups) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const char* row = rows + i * a.row_bytes;
+        uint64_t kw[kMaxKeyWords];
+        gather_key(a, row, kw);
+        agg_insert(a, t, mask, kw, row, nullptr, ngroups);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_agg_rehash(AggDev a, const unsigned long long* __restrict__ old,
+                                                    uint64_t oldcap, unsigned long long* __restrict__ t,
+                                                    uint64_t mask, unsigned long long* __restrict__ ngroups) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < oldcap; i += stride) {
+        const unsigned long long* slot = old + i * a.slot_words;
+        if (slot[0] != 2) continue;
+        uint64_t kw[kMaxKeyWords];
+        for (int w = 0; w < kMaxKeyWords; ++w) kw[w] = w < (int)a.key_words ? slot[1 + w] : 0;
+        agg_insert(a, t, mask, kw, nullptr, slot + 1 + a.key_words, ngroups);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_agg_compact(AggDev a, const unsigned long long* __restrict__ t,
+                                                     uint64_t cap, char* __restrict__ out, uint32_t out_row_bytes,
+                                                     uint32_t acc_bytes, unsigned long long* __restrict__ cursor) {
+    __shared__ unsigned long long s_base;
+    for (uint64_t b0 = (uint64_t)blockIdx.x * 256; b0 < cap; b0 += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = b0 + threadIdx.x;
+        const unsigned long long* slot = t + (i < cap ? i : 0) * a.slot_words;
+        const uint32_t live = (i < cap && slot[0] == 2) ? 1u : 0u;
+        uint32_t total;
+        const uint32_t off = block_scan256(live, &total);
+        if (threadIdx.x == 0) s_base = total ? atomicAdd(cursor, (unsigned long long)total) : 0ull;
+        __syncthreads();
+        if (live) {
+            char* o = out + (s_base + off) * out_row_bytes;
+            const char* kb = (const char*)(slot + 1);
+            for (uint32_t b = 0; b < a.key_total; ++b) o[b] = kb[b];
+            const char* ab = (const char*)(slot + 1 + a.key_words);
+            for (uint32_t b = 0; b < acc_bytes; ++b) o[a.key_total + b] = ab[b];
+        }
+        __syncthreads();
+    }
+}
+
+inline unsigned grid_for(uint64_t n) { return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(pgpu::ceil_div(n, 256), 8192)); }
+
+}  // namespace
+
+// ====================================================================== join
+struct pgpu_hj {
+    pgpu_ctx* ctx = nullptr;
+    JoinDev jd{};
+    std::mutex mu;
+    std::vector<char> staged;
+    uint64_t nbuild = 0;
+    bool built = false;
+    char* d_build = nullptr;
+    JSlot* d_table = nullptr;
+    int logcap = 0;
+    struct Slot {
+        hipStream_t st = nullptr;
+        char* d_in = nullptr;
+        uint64_t in_cap = 0;
+        char* d_out = nullptr;
+        uint64_t out_cap = 0;  // rows
+        unsigned long long* d_cursor = nullptr;
+        unsigned long long* h_cursor = nullptr;
+        uint64_t nout = 0, pulled = 0;
+    };
+    std::vector<Slot> slots;
+    pgpu_hj_stats stats{};
+};
+
+using namespace pgpu;
+
+namespace {
+int slot_reserve(pgpu_ctx* ctx, char** p, uint64_t* cap, uint64_t need) {
+    if (*cap >= need && *p) return PGPU_OK;
+    if (*p) PGPU_TRY(ctx_free(ctx, *p));
+    *p = nullptr;
+    *cap = 0;
+    uint64_t c = std::max<uint64_t>(need, 4096);
+    PGPU_TRY(ctx_alloc(ctx, c, (void**)p));
+    *cap = c;
+    return PGPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pgpu_hj_create(pgpu_ctx* ctx, const pgpu_hj_desc* d, pgpu_hj** out) {
+    if (!ctx || !d || !out) return fail(PGPU_EINVAL, "pgpu_hj_create: null argument");
+    *out = nullptr;
+    if ((d->build_key_bytes != 4 && d->build_key_bytes != 8) || (d->probe_key_bytes != 4 && d->probe_key_bytes != 8))
+        return fail(PGPU_EUNSUPPORTED, "join keys must be 4- or 8-byte integers");
+    if (d->build_key_offset + d->build_key_bytes > d->build_row_bytes ||
+        d->probe_key_offset + d->probe_key_bytes > d->probe_row_bytes)
+        return fail(PGPU_EINVAL, "join key outside its row");
+    if (d->nproj > PGPU_MAX_PROJ || d->slots == 0 || d->slots > 1024)
+        return fail(PGPU_EINVAL, "bad projection count or slot count");
+    auto* h = new pgpu_hj();
+    h->ctx = ctx;
+    h->jd.d = *d;
+    uint32_t o = 0;
+    for (uint32_t c = 0; c < d->nproj; ++c) {
+        const auto& p = d->proj[c];
+        const uint32_t lim = p.side ? d->probe_row_bytes : d->build_row_bytes;
+        if (p.side > 1 || p.src_offset + p.bytes > lim) {
+            delete h;
+            return fail(PGPU_EINVAL, "projected column %u outside its row", c);
+        }
+        h->jd.dst_off[c] = o;
+        o += p.bytes;
+    }
+    h->jd.out_row_bytes = o;
+    h->slots.resize(d->slots);
+    *out = h;
+    return PGPU_OK;
+}
+
+int pgpu_hj_destroy(pgpu_hj* h) {
+    if (!h) return PGPU_OK;
+    (void)ctx_activate(h->ctx);
+    for (auto& s : h->slots) {
+        if (s.st) {
+            (void)hipStreamSynchronize(s.st);
+            (void)hipStreamDestroy(s.st);
+        }
+        if (s.d_in) (void)ctx_free(h->ctx, s.d_in);
+        if (s.d_out) (void)ctx_free(h->ctx, s.d_out);
+        if (s.d_cursor) (void)ctx_free(h->ctx, s.d_cursor);
+        if (s.h_cursor) (void)hipHostFree(s.h_cursor);
+    }
+    if (h->d_build) (void)ctx_free(h->ctx, h->d_build);
+    if (h->d_table) (void)ctx_free(h->ctx, h->d_table);
+    delete h;
+    return PGPU_OK;
+}
+
+int pgpu_hj_build_append(pgpu_hj* h, const void* rows, uint64_t n) {
+    if (!h || (n && !rows)) return fail(PGPU_EINVAL, "pgpu_hj_build_append: null argument");
+    std::lock_guard<std::mutex> g(h->mu);
+    if (h->built) return fail(PGPU_ESTATE, "pgpu_hj_build_append after build_finish");
+    const uint64_t b = n * h->jd.d.build_row_bytes;
+    h->staged.insert(h->staged.end(), (const char*)rows, (const char*)rows + b);
+    h->nbuild += n;
+    return PGPU_OK;
+}
+
+int pgpu_hj_build_finish(pgpu_hj* h) {
+    if (!h) return fail(PGPU_EINVAL, "pgpu_hj_build_finish: null handle");
+    std::lock_guard<std::mutex> g(h->mu);
+    if (h->built) return PGPU_OK;
+    PGPU_TRY(ctx_activate(h->ctx));
+    if (h->nbuild >= kEmptyIdx) return fail(PGPU_EINVAL, "build side exceeds 2^32-1 rows");
+    hipStream_t st = h->ctx->stream;
+    const uint64_t bytes = h->staged.size();
+    PGPU_TRY(ctx_alloc(h->ctx, std::max<uint64_t>(bytes, 8), (void**)&h->d_build));
+    if (bytes) PGPU_HIP(hipMemcpyAsync(h->d_build, h->staged.data(), bytes, hipMemcpyHostToDevice, st));
+    h->logcap = (int)ilog2_ceil(std::max<uint64_t>(2 * h->nbuild, 16));
+    const uint64_t cap = 1ull << h->logcap;
+    PGPU_TRY(ctx_alloc(h->ctx, cap * sizeof(JSlot), (void**)&h->d_table));
+    hipLaunchKernelGGL(k_jclear, dim3(grid_for(cap)), dim3(256), 0, st, h->d_table, cap);
+    PGPU_LAUNCH_CHECK();
+    if (h->nbuild) {
+        hipLaunchKernelGGL(k_jbuild, dim3(grid_for(h->nbuild)), dim3(256), 0, st, h->d_build, h->nbuild,
+                           h->jd.d.build_row_bytes, h->jd.d.build_key_offset, h->jd.d.build_key_bytes,
+                           h->d_table, h->logcap);
+        PGPU_LAUNCH_CHECK();
+    }
+    PGPU_HIP(hipStreamSynchronize(st));
+    h->stats.build_rows = h->nbuild;
+    h->stats.table_slots = cap;
+    h->stats.h2d_bytes += bytes;
+    std::vector<char>().swap(h->staged);
+    h->built = true;
+    return PGPU_OK;
+}
+
+int pgpu_hj_probe(pgpu_hj* h, uint32_t slot, const void* rows, uint64_t n, uint64_t* n_out) {
+    if (!h || !n_out || (n && !rows)) return fail(PGPU_EINVAL, "pgpu_hj_probe: null argument");
+    if (slot >= h->slots.size()) return fail(PGPU_EINVAL, "pgpu_hj_probe: slot %u out of range", slot);
+    if (!h->built) return fail(PGPU_ESTATE, "pgpu_hj_probe before build_finish");
+    PGPU_TRY(ctx_activate(h->ctx));
+    auto& s = h->slots[slot];
+    if (!s.st) PGPU_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+    if (!s.d_cursor) {
+        PGPU_TRY(ctx_alloc(h->ctx, 8, (void**)&s.d_cursor));
+        PGPU_HIP(hipHostMalloc((void**)&s.h_cursor, 8, hipHostMallocDefault));
+    }
+    s.nout = s.pulled = 0;
+    *n_out = 0;
+    if (n == 0 || h->nbuild == 0) {
+        h->stats.probe_rows += n;
+        return PGPU_OK;
+    }
+    const uint32_t prb = h->jd.d.probe_row_bytes, orb = std::max<uint32_t>(h->jd.out_row_bytes, 1);
+    PGPU_TRY(slot_reserve(h->ctx, &s.d_in, &s.in_cap, n * prb));
+    PGPU_HIP(hipMemcpyAsync(s.d_in, rows, n * prb, hipMemcpyHostToDevice, s.st));
+    uint64_t cap_rows = s.out_cap / orb;
+    if (cap_rows < n) {
+        PGPU_TRY(slot_reserve(h->ctx, &s.d_out, &s.out_cap, n * orb));
+        cap_rows = s.out_cap / orb;
+    }
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, s.st, s.d_cursor, 1, 0ull);
+        PGPU_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_jprobe, dim3(grid_for(n)), dim3(256), 0, s.st, h->jd, h->d_build, h->d_table,
+                           h->logcap, s.d_in, n, s.d_out, cap_rows, s.d_cursor);
+        PGPU_LAUNCH_CHECK();
+        PGPU_HIP(hipMemcpyAsync(s.h_cursor, s.d_cursor, 8, hipMemcpyDeviceToHost, s.st));
+        PGPU_HIP(hipStreamSynchronize(s.st));
+        const uint64_t total = *s.h_cursor;
+        if (total <= cap_rows) {
+            s.nout = total;
+            break;
+        }
+        // Output larger than the buffer: grow to the exact size and re-run.
+        PGPU_TRY(slot_reserve(h->ctx, &s.d_out, &s.out_cap, total * orb));
+        cap_rows = s.out_cap / orb;
+        h->stats.probe_reruns++;
+        if (attempt == 1) return fail(PGPU_EOVERFLOW, "probe output did not fit after regrowth");
+    }
+    h->stats.probe_rows += n;
+    h->stats.output_rows += s.nout;
+    h->stats.h2d_bytes += n * prb;
+    *n_out = s.nout;
+    return PGPU_OK;
+}
+
+int pgpu_hj_pull(pgpu_hj* h, uint32_t slot, void* dst, uint64_t cap_rows, uint64_t* n_rows) {
+    if (!h || !n_rows || (cap_rows && !dst)) return fail(PGPU_EINVAL, "pgpu_hj_pull: null argument");
+    if (slot >= h->slots.size()) return fail(PGPU_EINVAL, "pgpu_hj_pull: slot out of range");
+    auto& s = h->slots[slot];
+    const uint64_t k = std::min(cap_rows, s.nout - s.pulled);
+    *n_rows = k;
+    if (!k) return PGPU_OK;
+    PGPU_TRY(ctx_activate(h->ctx));
+    const uint64_t orb = h->jd.out_row_bytes;
+    PGPU_HIP(hipMemcpyAsync(dst, s.d_out + s.pulled * orb, k * orb, hipMemcpyDeviceToHost, s.st));
+    PGPU_HIP(hipStreamSynchronize(s.st));
+    s.pulled += k;
+    h->stats.d2h_bytes += k * orb;
+    return PGPU_OK;
+}
+
+int pgpu_hj_stats_get(pgpu_hj* h, pgpu_hj_stats* out) {
+    if (!h || !out) return fail(PGPU_EINVAL, "pgpu_hj_stats_get: null argument");
+    *out = h->stats;
+    return PGPU_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== aggregation
+struct pgpu_agg {
+    pgpu_ctx* ctx = nullptr;
+    AggDev a{};
+    std::mutex mu;
+    unsigned long long* d_table = nullptr;
+    uint64_t cap = 0;
+    unsigned long long* d_ngroups = nullptr;
+    unsigned long long* h_ngroups = nullptr;
+    char* d_in = nullptr;
+    uint64_t in_cap = 0;
+    char* d_out = nullptr;
+    uint64_t ngroups_out = 0;
+    uint32_t out_row_bytes = 0, acc_bytes = 0;
+    bool finished = false;
+};
+
+namespace {
+int agg_alloc_table(pgpu_agg* g, uint64_t cap, unsigned long long** t) {
+    const uint64_t words = cap * g->a.slot_words;
+    PGPU_TRY(ctx_alloc(g->ctx, words * 8, (void**)t));
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(words)), dim3(256), 0, g->ctx->stream, *t, words, 0ull);
+    PGPU_LAUNCH_CHECK();
+    return PGPU_OK;
+}
+
+int agg_grow(pgpu_agg* g, uint64_t newcap) {
+    unsigned long long* nt = nullptr;
+    PGPU_TRY(agg_alloc_table(g, newcap, &nt));
+    hipStream_t st = g->ctx->stream;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, g->d_ngroups, 1, 0ull);
+    PGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_agg_rehash, dim3(grid_for(g->cap)), dim3(256), 0, st, g->a, g->d_table, g->cap, nt,
+                       newcap - 1, g->d_ngroups);
+    PGPU_LAUNCH_CHECK();
+    PGPU_HIP(hipStreamSynchronize(st));
+    PGPU_TRY(ctx_free(g->ctx, g->d_table));
+    g->d_table = nt;
+    g->cap = newcap;
+    return PGPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pgpu_agg_create(pgpu_ctx* ctx, const pgpu_agg_desc* d, pgpu_agg** out) {
+    if (!ctx || !d || !out) return fail(PGPU_EINVAL, "pgpu_agg_create: null argument");
+    *out = nullptr;
+    if (d->nkeys > 8) return fail(PGPU_EINVAL, "at most 8 group columns");
+    uint32_t kt = 0;
+    for (uint32_t c = 0; c < d->nkeys; ++c) {
+        if (d->key_offset[c] + d->key_bytes[c] > d->row_bytes) return fail(PGPU_EINVAL, "group column outside the row");
+        kt += d->key_bytes[c];
+    }
+    if (kt > kMaxKeyWords * 8) return fail(PGPU_EUNSUPPORTED, "group key wider than 64 bytes");
+    if (d->kind < PGPU_AGG_SUM_I32 || d->kind > PGPU_AGG_COUNT) return fail(PGPU_EINVAL, "unknown aggregate kind");
+    const uint32_t vb = d->kind == PGPU_AGG_SUM_I32 ? 4 : 8;
+    if (d->kind != PGPU_AGG_COUNT && d->val_offset + vb > d->row_bytes)
+        return fail(PGPU_EINVAL, "summed column outside the row");
+    PGPU_TRY(ctx_activate(ctx));
+    auto* g = new pgpu_agg();
+    g->ctx = ctx;
+    g->a.row_bytes = d->row_bytes;
+    g->a.nkeys = d->nkeys;
+    g->a.kind = d->kind;
+    g->a.val_offset = d->val_offset;
+    for (uint32_t c = 0; c < d->nkeys; ++c) {
+        g->a.key_offset[c] = d->key_offset[c];
+        g->a.key_bytes[c] = d->key_bytes[c];
+    }
+    g->a.key_total = kt;
+    g->a.key_words = (kt + 7) / 8;
+    g->a.slot_words = 2 + g->a.key_words;
+    g->acc_bytes = vb;
+    g->out_row_bytes = kt + vb;
+    g->cap = 1ull << ilog2_ceil(std::max<uint64_t>(2 * (d->groups_hint ? d->groups_hint : 32768), 64));
+    int rc = agg_alloc_table(g, g->cap, &g->d_table);
+    if (rc == PGPU_OK) rc = ctx_alloc(ctx, 8, (void**)&g->d_ngroups);
+    if (rc == PGPU_OK && hipHostMalloc((void**)&g->h_ngroups, 8, hipHostMallocDefault) != hipSuccess)
+        rc = fail(PGPU_ENOMEM, "hipHostMalloc failed");
+    if (rc == PGPU_OK) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, ctx->stream, g->d_ngroups, 1, 0ull);
+        if (hipGetLastError() != hipSuccess) rc = fail(PGPU_EDEVICE, "launch failed");
+    }
+    if (rc != PGPU_OK) {
+        pgpu_agg_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return PGPU_OK;
+}
+
+int pgpu_agg_destroy(pgpu_agg* g) {
+    if (!g) return PGPU_OK;
+    (void)ctx_activate(g->ctx);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->d_table) (void)ctx_free(g->ctx, g->d_table);
+    if (g->d_ngroups) (void)ctx_free(g->ctx, g->d_ngroups);
+    if (g->h_ngroups) (void)hipHostFree(g->h_ngroups);
+    if (g->d_in) (void)ctx_free(g->ctx, g->d_in);
+    if (g->d_out) (void)ctx_free(g->ctx, g->d_out);
+    delete g;
+    return PGPU_OK;
+}
+
+int pgpu_agg_append(pgpu_agg* g, const void* rows, uint64_t n) {
+    if (!g || (n && !rows)) return fail(PGPU_EINVAL, "pgpu_agg_append: null argument");
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (g->finished) return fail(PGPU_ESTATE, "pgpu_agg_append after finish");
+    if (!n) return PGPU_OK;
+    PGPU_TRY(ctx_activate(g->ctx));
+    hipStream_t st = g->ctx->stream;
+    // Keep the table at most half full even if every row is a new group.
+    PGPU_HIP(hipMemcpyAsync(g->h_ngroups, g->d_ngroups, 8, hipMemcpyDeviceToHost, st));
+    PGPU_HIP(hipStreamSynchronize(st));
+    uint64_t need = *g->h_ngroups + n;
+    if (need > g->cap / 2) {
+        uint64_t nc = g->cap;
+        while (need > nc / 2) nc *= 2;
+        // grow lazily: never beyond what the distinct keys seen so far can
+        // require, bounded by rows appended
+        PGPU_TRY(agg_grow(g, nc));
+    }
+    PGPU_TRY(slot_reserve(g->ctx, &g->d_in, &g->in_cap, n * g->a.row_bytes));
+    PGPU_HIP(hipMemcpyAsync(g->d_in, rows, n * g->a.row_bytes, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_agg_append, dim3(grid_for(n)), dim3(256), 0, st, g->a, g->d_in, n, g->d_table, g->cap - 1,
+                       g->d_ngroups);
+    PGPU_LAUNCH_CHECK();
+    PGPU_HIP(hipStreamSynchronize(st));
+    return PGPU_OK;
+}
+
+int pgpu_agg_finish(pgpu_agg* g, uint64_t* n_groups, uint32_t* out_row_bytes) {
+    if (!g || !n_groups) return fail(PGPU_EINVAL, "pgpu_agg_finish: null argument");
+    std::lock_guard<std::mutex> lk(g->mu);
+    PGPU_TRY(ctx_activate(g->ctx));
+    hipStream_t st = g->ctx->stream;
+    if (!g->finished) {
+        PGPU_HIP(hipMemcpyAsync(g->h_ngroups, g->d_ngroups, 8, hipMemcpyDeviceToHost, st));
+        PGPU_HIP(hipStreamSynchronize(st));
+        const uint64_t ng = *g->h_ngroups;
+        PGPU_TRY(ctx_alloc(g->ctx, std::max<uint64_t>(ng * g->out_row_bytes, 8), (void**)&g->d_out));
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, g->d_ngroups, 1, 0ull);
+        PGPU_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_agg_compact, dim3(grid_for(g->cap)), dim3(256), 0, st, g->a, g->d_table, g->cap,
+                           g->d_out, g->out_row_bytes, g->acc_bytes, g->d_ngroups);
+        PGPU_LAUNCH_CHECK();
+        PGPU_HIP(hipMemcpyAsync(g->h_ngroups, g->d_ngroups, 8, hipMemcpyDeviceToHost, st));
+        PGPU_HIP(hipStreamSynchronize(st));
+        if (*g->h_ngroups != ng) return fail(PGPU_EOVERFLOW, "group count changed during compaction");
+        g->ngroups_out = ng;
+        g->finished = true;
+    }
+    *n_groups = g->ngroups_out;
+    if (out_row_bytes) *out_row_bytes = g->out_row_bytes;
+    return PGPU_OK;
+}
+
+int pgpu_agg_pull(pgpu_agg* g, uint64_t first, uint64_t n, void* dst) {
+    if (!g || (n && !dst)) return fail(PGPU_EINVAL, "pgpu_agg_pull: null argument");
+    if (!g->finished) return fail(PGPU_ESTATE, "pgpu_agg_pull before finish");
+    if (first + n > g->ngroups_out) return fail(PGPU_EINVAL, "pgpu_agg_pull: range beyond the groups");
+    if (!n) return PGPU_OK;
+    PGPU_TRY(ctx_activate(g->ctx));
+    PGPU_HIP(hipMemcpy(dst, g->d_out + first * g->out_row_bytes, n * g->out_row_bytes, hipMemcpyDeviceToHost));
+    return PGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,519 @@
+// Host side of the GPU plan operators.  Everything device-related goes
+// through pgpu.h; this file only adapts the Operator/Page protocol.
+#include "gpu_operators.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <set>
+#include <sstream>
+
+#include "registry.h"
+
+using pythia::check;
+
+void GpuKnobs::read(libconfig::Setting& node) {
+    if (!node.exists("gpu")) return;
+    libconfig::Setting& g = node["gpu"];
+    g.lookupValue("device", device);
+    long long b = 0;
+    if (g.lookupValue("hbmbudget", b)) hbmbudget = (uint64_t)b;
+    long long pb = 0;
+    if (g.lookupValue("probebatch", pb) && pb > 0) probebatch = (uint64_t)pb;
+}
+
+// The checks TupleHasher::create applies to a hash node
+// (hash.cpp:99-234), in the same order, so plans the reference rejects are
+// rejected here too.  The GPU operators place tuples with their own hash;
+// the join and aggregation results do not depend on it.
+static void validate_hash(const libconfig::Setting& node, const Schema& s, int fieldmin, int fieldmax) {
+    std::string fn = node["fn"];
+    if (fn == "alwayszero") return;
+    int buckets = node["buckets"];
+    if (buckets == 0) throw MissingParameterException("Number of hash buckets cannot be zero.");
+    if (fn == "bytes" || fn == "tpchq1magic") return;
+    if (fieldmin != fieldmax) throw IllegalSchemaDeclarationException();
+    if (fieldmin < 0 || fieldmin >= (int)s.columns()) throw InvalidParameter("hash field out of range");
+    switch (s.getColumnType(fieldmin)) {
+        case CT_INTEGER:
+        case CT_LONG:
+        case CT_DATE: break;
+        default: throw IllegalSchemaDeclarationException();
+    }
+    static const std::set<std::string> known = {"modulo", "range",        "exactrange", "parammodulo",
+                                                "knuth",  "tpchorderkey", "willis"};
+    if (!known.count(fn)) throw UnknownHashException();
+    if (fn == "range" || fn == "exactrange") {
+        (void)(int)node["range"][0];
+        (void)(int)node["range"][1];
+    }
+}
+
+static bool is_int_key(ColumnType t) { return t == CT_INTEGER || t == CT_LONG || t == CT_DATE; }
+
+// createProjectionVector (operators/join.cpp:141-182): "B$i" / "P$j".
+static std::vector<GpuHashJoinOp::JoinPrjT> parse_projection(const libconfig::Setting& line) {
+    std::vector<GpuHashJoinOp::JoinPrjT> ret;
+    for (int i = 0; i < line.getLength(); ++i) {
+        std::string s = line[i];
+        size_t l = s.find('$');
+        if (l == std::string::npos || l == 0) throw InvalidParameter("projection entry `" + s + "'");
+        std::istringstream ss(s.substr(l + 1));
+        unsigned int attr;
+        ss >> attr;
+        if (!ss) throw InvalidParameter("projection entry `" + s + "'");
+        switch (s.at(l - 1)) {
+            case 'B': ret.push_back({GpuHashJoinOp::BuildSide, attr}); break;
+            case 'P': ret.push_back({GpuHashJoinOp::ProbeSide, attr}); break;
+            default: throw InvalidParameter("projection entry `" + s + "'");
+        }
+    }
+    return ret;
+}
+
+// ====================================================================== join
+void GpuHashJoinOp::init(libconfig::Config& root, libconfig::Setting& node) {
+    Operator::init(root, node);
+    projection = parse_projection(node["projection"]);
+    joinattr1 = (int)node["buildjattr"];
+    joinattr2 = (int)node["probejattr"];
+    Schema& bs = buildOp->getOutSchema();
+    Schema& ps = probeOp->getOutSchema();
+    if (joinattr1 >= bs.columns() || joinattr2 >= ps.columns()) throw InvalidParameter("join attribute out of range");
+
+    // thread -> (group, position), group -> leader, size, barrier
+    threadgroups.assign(MAX_THREADS, (unsigned short)-1);
+    threadposingrp.assign(MAX_THREADS, (unsigned short)-1);
+    libconfig::Setting& part = node["threadgroups"];
+    for (int i = 0; i < part.getLength(); ++i) {
+        for (int j = 0; j < part[i].getLength(); ++j) {
+            int tid = part[i][j];
+            if (tid < 0 || tid >= MAX_THREADS || threadgroups[tid] != (unsigned short)-1)
+                throw InvalidParameter("thread " + std::to_string(tid) + " in two thread groups or out of range");
+            threadgroups[tid] = (unsigned short)i;
+            threadposingrp[tid] = (unsigned short)j;
+        }
+        groupleader.push_back((unsigned short)(int)part[i][0]);
+        groupsize.push_back((unsigned short)part[i].getLength());
+        barriers.push_back(new Barrier());
+        barriers.back()->init(part[i].getLength());
+    }
+
+    for (auto& p : projection) {
+        if (p.first == BuildSide) {
+            if (p.second >= bs.columns()) throw InvalidParameter("projected build column out of range");
+            schema.add(bs.get(p.second));
+        } else {
+            if (p.second >= ps.columns()) throw InvalidParameter("projected probe column out of range");
+            schema.add(ps.get(p.second));
+        }
+    }
+    // sbuild = join key + projected build columns (operators/join.cpp:278-286)
+    sbuild.add(bs.get(joinattr1));
+    for (auto& p : projection)
+        if (p.first == BuildSide) sbuild.add(bs.get(p.second));
+
+    if (node["hash"].exists("field")) throw InvalidParameter("hash node must not name `field' for a join");
+    validate_hash(node["hash"], bs, (int)joinattr1, (int)joinattr1);
+    validate_hash(node["hash"], ps, (int)joinattr2, (int)joinattr2);
+    (void)(int)node["tuplesperbucket"];
+    std::string policy = node["allocpolicy"];
+    if (policy == "striped")
+        fprintf(stderr, " ** NUMA POLICY WARNING: gpuhashjoin keeps its table in HBM; allocpolicy is ignored.\n");
+    if (!is_int_key(bs.getColumnType(joinattr1)) || !is_int_key(ps.getColumnType(joinattr2)))
+        throw InvalidParameter("gpuhashjoin joins on int, long or date keys");
+    knobs.read(node);
+    tables.assign(groupleader.size(), nullptr);
+    state.assign(MAX_THREADS, ThreadState());
+}
+
+void GpuHashJoinOp::fail_group(unsigned short g) { barriers.at(g)->break_(); }
+
+void GpuHashJoinOp::threadInit(unsigned short tid) {
+    if (tid >= MAX_THREADS || threadgroups[tid] == (unsigned short)-1)
+        throw InvalidParameter("thread " + std::to_string(tid) + " is in no thread group");
+    const unsigned short g = threadgroups[tid];
+    if (groupleader[g] == tid) {
+        try {
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!ctx) ctx = pythia::acquire_device(knobs.device, knobs.hbmbudget);
+            }
+            pgpu_hj_desc d{};
+            Schema& ps = probeOp->getOutSchema();
+            d.build_row_bytes = sbuild.getTupleSize();
+            d.build_key_offset = 0;
+            d.build_key_bytes = sbuild.getColumnWidth(0);
+            d.probe_row_bytes = ps.getTupleSize();
+            d.probe_key_offset = ps.getColumnOffset(joinattr2);
+            d.probe_key_bytes = ps.getColumnWidth(joinattr2);
+            d.nproj = (uint32_t)projection.size();
+            if (d.nproj > PGPU_MAX_PROJ) throw InvalidParameter("too many projected columns");
+            unsigned buildcol = 1;
+            for (uint32_t j = 0; j < d.nproj; ++j) {
+                if (projection[j].first == BuildSide) {
+                    d.proj[j] = {0, sbuild.getColumnOffset(buildcol), sbuild.getColumnWidth(buildcol)};
+                    ++buildcol;
+                } else {
+                    d.proj[j] = {1, ps.getColumnOffset(projection[j].second), ps.getColumnWidth(projection[j].second)};
+                }
+            }
+            d.slots = MAX_THREADS;
+            check(pgpu_hj_create(ctx, &d, &tables[g]), "pgpu_hj_create");
+        } catch (...) {
+            fail_group(g);
+            throw;
+        }
+    }
+    if (!barriers[g]->Arrive()) throw QueryExecutionError("gpuhashjoin: thread group failed during threadInit");
+    if (buffsize < schema.getTupleSize()) throw InvalidParameter("buffsize smaller than one output tuple");
+    state[tid] = ThreadState();
+    state[tid].out = new Page(buffsize, schema.getTupleSize());
+}
+
+Operator::ResultCode GpuHashJoinOp::scanStart(unsigned short tid, Page* idx, Schema& idxschema) {
+    const unsigned short g = threadgroups[tid];
+    Schema& bs = buildOp->getOutSchema();
+    if (buildOp->scanStart(tid, idx, idxschema) == Error) {
+        fail_group(g);
+        return Error;
+    }
+    // Build: project each build tuple into sbuild and stage it (buildFromPage,
+    // operators/join.cpp:641-667); the library copies before returning.
+    const unsigned sb = sbuild.getTupleSize();
+    std::vector<unsigned> src_off, width;
+    src_off.push_back(bs.getColumnOffset(joinattr1));
+    width.push_back(bs.getColumnWidth(joinattr1));
+    for (auto& p : projection)
+        if (p.first == BuildSide) {
+            src_off.push_back(bs.getColumnOffset(p.second));
+            width.push_back(bs.getColumnWidth(p.second));
+        }
+    std::vector<char> rows;
+    GetNextResultT r{Ready, nullptr};
+    try {
+        while (r.first == Ready) {
+            r = buildOp->getNext(tid);
+            if (r.first == Error) break;
+            const uint64_t n = r.second->getNumTuples();
+            const size_t base = rows.size();
+            rows.resize(base + n * sb);
+            char* dst = rows.data() + base;
+            const char* src = r.second->data();
+            const unsigned ts = bs.getTupleSize();
+            for (uint64_t i = 0; i < n; ++i, src += ts, dst += sb)
+                for (size_t c = 0, o = 0; c < src_off.size(); o += width[c], ++c) memcpy(dst + o, src + src_off[c], width[c]);
+            if (rows.size() >= (32u << 20)) {
+                check(pgpu_hj_build_append(tables[g], rows.data(), rows.size() / sb), "pgpu_hj_build_append");
+                rows.clear();
+            }
+        }
+        if (r.first != Error && !rows.empty())
+            check(pgpu_hj_build_append(tables[g], rows.data(), rows.size() / sb), "pgpu_hj_build_append");
+    } catch (...) {
+        fail_group(g);
+        throw;
+    }
+    if (r.first == Error || buildOp->scanStop(tid) == Error) {
+        fail_group(g);
+        return Error;
+    }
+    // Every thread of the group has staged its build tuples (join.cpp:434).
+    if (!barriers[g]->Arrive()) return Error;
+    if (groupleader[g] == tid) {
+        int rc = pgpu_hj_build_finish(tables[g]);
+        if (rc != PGPU_OK) {
+            fail_group(g);
+            check(rc, "pgpu_hj_build_finish");
+        }
+    }
+    if (!barriers[g]->Arrive()) return Error;
+
+    ResultCode rc = probeOp->scanStart(tid, idx, idxschema);
+    if (rc == Error) return Error;
+    ThreadState& st = state[tid];
+    Page* keep = st.out;
+    st = ThreadState();
+    st.out = keep;
+    // The first probe tuple is read here; an empty probe side is Finished
+    // (operators/join.cpp:444-459).
+    if (!stageProbe(tid)) {
+        st.done = true;
+        return Finished;
+    }
+    return rc;
+}
+
+bool GpuHashJoinOp::stageProbe(unsigned short tid) {
+    ThreadState& st = state[tid];
+    st.probe.clear();
+    st.nprobe = 0;
+    const unsigned ts = probeOp->getOutSchema().getTupleSize();
+    while (!st.depleted && st.nprobe < knobs.probebatch) {
+        GetNextResultT r = probeOp->getNext(tid);
+        // readNextTupleFromProbe throws on a child error (operators/join.cpp:584-586).
+        if (r.first == Error) throw QueryExecutionError("gpuhashjoin: probe input returned Error");
+        const uint64_t n = r.second->getNumTuples();
+        st.probe.insert(st.probe.end(), r.second->data(), r.second->data() + n * ts);
+        st.nprobe += n;
+        if (r.first == Finished) st.depleted = true;
+    }
+    return st.nprobe > 0;
+}
+
+Operator::GetNextResultT GpuHashJoinOp::getNext(unsigned short tid) {
+    ThreadState& st = state[tid];
+    Page* out = st.out;
+    out->clear();
+    if (st.done) return {Finished, out};
+    const unsigned ts = schema.getTupleSize();
+    const uint64_t cap = out->capacity() / ts;
+    pgpu_hj* t = tables[threadgroups[tid]];
+    while (true) {
+        if (st.pending) {
+            const uint64_t used = out->getNumTuples();
+            uint64_t got = 0;
+            check(pgpu_hj_pull(t, tid, out->data() + used * ts, std::min(cap - used, st.pending), &got),
+                  "pgpu_hj_pull");
+            out->setUsed((used + got) * ts);
+            st.pending -= got;
+            if (out->getNumTuples() == cap) return {Ready, out};
+            continue;
+        }
+        if (st.nprobe) {
+            uint64_t nout = 0;
+            check(pgpu_hj_probe(t, tid, st.probe.data(), st.nprobe, &nout), "pgpu_hj_probe");
+            st.pending = nout;
+            st.nprobe = 0;
+            continue;
+        }
+        if (!stageProbe(tid)) {
+            st.done = true;
+            return {Finished, out};
+        }
+    }
+}
+
+Operator::ResultCode GpuHashJoinOp::scanStop(unsigned short tid) { return probeOp->scanStop(tid); }
+
+void GpuHashJoinOp::threadClose(unsigned short tid) {
+    delete state[tid].out;
+    state[tid].out = nullptr;
+    const unsigned short g = threadgroups[tid];
+    barriers[g]->Arrive();  // a broken barrier returns at once
+    if (groupleader[g] == tid) {
+        pgpu_hj_destroy(tables[g]);
+        tables[g] = nullptr;
+    }
+}
+
+void GpuHashJoinOp::destroy() {
+    for (auto*& t : tables) {
+        if (t) pgpu_hj_destroy(t);
+        t = nullptr;
+    }
+    for (auto* b : barriers) delete b;
+    barriers.clear();
+    if (ctx) pythia::release_device(ctx);
+    ctx = nullptr;
+}
+
+pgpu_hj_stats GpuHashJoinOp::stats(unsigned short group) {
+    pgpu_hj_stats s{};
+    if (group < tables.size() && tables[group]) pgpu_hj_stats_get(tables[group], &s);
+    return s;
+}
+
+// ====================================================================== aggregation
+void GpuAggregate::init(libconfig::Config& root, libconfig::Setting& cfg) {
+    Operator::init(root, cfg);
+    if (cfg.exists("fields")) {
+        libconfig::Setting& f = cfg["fields"];
+        for (int i = 0; i < f.getLength(); ++i) aggfields.push_back((unsigned short)(int)f[i]);
+    } else {
+        aggfields.push_back((unsigned short)(int)cfg["field"]);
+    }
+    if (aggfields.empty() && !cfg.exists("hash")) {
+        libconfig::Setting& h = cfg.add("hash", libconfig::Setting::TypeGroup);
+        h.add("fn", libconfig::Setting::TypeString) = "alwayszero";
+    }
+    Schema& in = nextOp->getOutSchema();
+    Schema uds = foldinit(cfg);
+    for (auto f : aggfields) {
+        if (f >= in.columns()) throw InvalidParameter("group column out of range");
+        schema.add(in.get(f));
+    }
+    for (unsigned i = 0; i < uds.columns(); ++i) schema.add(uds.get(i));
+    if (cfg.exists("presorted")) throw NotYetImplemented();
+    // the hash node names its own field / fieldrange (hash.cpp:120-137)
+    libconfig::Setting& h = cfg["hash"];
+    std::string fn = h["fn"];
+    int fmin = 0, fmax = 0;
+    if (fn != "alwayszero") {
+        if (h.exists("fieldrange")) {
+            fmin = h["fieldrange"][0];
+            fmax = h["fieldrange"][1];
+        } else {
+            fmin = fmax = (int)h["field"];
+        }
+    }
+    validate_hash(h, in, fmin, fmax);
+    if (cfg.exists("global")) {
+        global_ = true;
+        threads_ = (int)cfg["threads"];
+        barrier_.init(threads_);
+    }
+    knobs.read(cfg);
+    state.assign(MAX_THREADS, ThreadState());
+}
+
+pgpu_agg* GpuAggregate::newTable() {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!ctx) ctx = pythia::acquire_device(knobs.device, knobs.hbmbudget);
+    }
+    Schema& in = nextOp->getOutSchema();
+    pgpu_agg_desc d{};
+    d.row_bytes = in.getTupleSize();
+    d.nkeys = (uint32_t)aggfields.size();
+    if (d.nkeys > 8) throw InvalidParameter("at most 8 group columns on the GPU");
+    for (uint32_t i = 0; i < d.nkeys; ++i) {
+        d.key_offset[i] = in.getColumnOffset(aggfields[i]);
+        d.key_bytes[i] = in.getColumnWidth(aggfields[i]);
+    }
+    d.kind = kind();
+    d.val_offset = valOffset();
+    pgpu_agg* a = nullptr;
+    check(pgpu_agg_create(ctx, &d, &a), "pgpu_agg_create");
+    return a;
+}
+
+void GpuAggregate::threadInit(unsigned short tid) {
+    if (buffsize < schema.getTupleSize()) throw InvalidParameter("buffsize smaller than one output tuple");
+    state[tid] = ThreadState();
+    state[tid].out = new Page(buffsize, schema.getTupleSize());
+    if (!global_) {
+        state[tid].agg = newTable();
+        return;
+    }
+    try {
+        std::lock_guard<std::mutex> lk(init_mu);  // newTable() takes `mu`
+        if (!shared_) shared_ = newTable();
+    } catch (...) {
+        barrier_.break_();
+        throw;
+    }
+    if (!barrier_.Arrive()) throw QueryExecutionError("gpuaggregate: a thread failed during threadInit");
+}
+
+Operator::ResultCode GpuAggregate::scanStart(unsigned short tid, Page* idx, Schema& idxschema) {
+    ThreadState& st = state[tid];
+    ResultCode rc = nextOp->scanStart(tid, idx, idxschema);
+    if (rc != Ready) {
+        if (global_) barrier_.break_();
+        return rc;
+    }
+    pgpu_agg* t = global_ ? shared_ : st.agg;
+    const unsigned ts = nextOp->getOutSchema().getTupleSize();
+    std::vector<char> rows;
+    GetNextResultT r{Ready, nullptr};
+    try {
+        do {
+            r = nextOp->getNext(tid);
+            if (r.first == Error) break;
+            const uint64_t n = r.second->getNumTuples();
+            rows.insert(rows.end(), r.second->data(), r.second->data() + n * ts);
+            if (rows.size() >= (64u << 20)) {
+                check(pgpu_agg_append(t, rows.data(), rows.size() / ts), "pgpu_agg_append");
+                rows.clear();
+            }
+        } while (r.first == Ready);
+        if (r.first != Error && !rows.empty())
+            check(pgpu_agg_append(t, rows.data(), rows.size() / ts), "pgpu_agg_append");
+    } catch (...) {
+        if (global_) barrier_.break_();
+        throw;
+    }
+    rc = nextOp->scanStop(tid);
+    if (r.first == Error) {
+        if (global_) barrier_.break_();
+        return Error;
+    }
+    uint64_t ng = 0;
+    if (global_) {
+        if (!barrier_.Arrive()) return Error;  // every thread has folded its input
+        check(pgpu_agg_finish(t, &ng, nullptr), "pgpu_agg_finish");  // idempotent
+        // each group is emitted by exactly one thread (operators/genericaggregate.cpp:324-412)
+        st.first = (uint64_t)tid * ng / threads_;
+        st.end = (uint64_t)(tid + 1) * ng / threads_;
+    } else {
+        check(pgpu_agg_finish(t, &ng, nullptr), "pgpu_agg_finish");
+        st.first = 0;
+        st.end = ng;
+    }
+    st.pos = st.first;
+    st.done = false;
+    return rc;
+}
+
+Operator::GetNextResultT GpuAggregate::getNext(unsigned short tid) {
+    ThreadState& st = state[tid];
+    Page* out = st.out;
+    out->clear();
+    if (st.pos >= st.end) return {Finished, out};
+    const unsigned ts = schema.getTupleSize();
+    const uint64_t k = std::min<uint64_t>(out->capacity() / ts, st.end - st.pos);
+    check(pgpu_agg_pull(global_ ? shared_ : st.agg, st.pos, k, out->data()), "pgpu_agg_pull");
+    out->setUsed(k * ts);
+    st.pos += k;
+    return {st.pos >= st.end ? Finished : Ready, out};
+}
+
+void GpuAggregate::threadClose(unsigned short tid) {
+    delete state[tid].out;
+    state[tid].out = nullptr;
+    if (!global_) {
+        pgpu_agg_destroy(state[tid].agg);
+        state[tid].agg = nullptr;
+        return;
+    }
+    barrier_.Arrive();
+    std::lock_guard<std::mutex> lk(mu);
+    if (shared_) {
+        pgpu_agg_destroy(shared_);
+        shared_ = nullptr;
+    }
+}
+
+void GpuAggregate::destroy() {
+    for (auto& s : state)
+        if (s.agg) {
+            pgpu_agg_destroy(s.agg);
+            s.agg = nullptr;
+        }
+    if (shared_) pgpu_agg_destroy(shared_);
+    shared_ = nullptr;
+    if (ctx) pythia::release_device(ctx);
+    ctx = nullptr;
+}
+
+Schema GpuAggregateSum::foldinit(libconfig::Setting& node) {
+    unsigned sumfield = (int)node["sumfield"];
+    Schema& in = nextOp->getOutSchema();
+    if (sumfield >= in.columns()) throw InvalidParameter("sumfield out of range");
+    Schema s;
+    switch (in.getColumnType(sumfield)) {
+        case CT_INTEGER: kind_ = PGPU_AGG_SUM_I32; break;
+        case CT_LONG: kind_ = PGPU_AGG_SUM_I64; break;
+        case CT_DECIMAL: kind_ = PGPU_AGG_SUM_F64; break;
+        default: throw InvalidParameter("aggregate_sum sums int, long or dec columns");
+    }
+    valoff_ = in.getColumnOffset(sumfield);
+    s.add(in.getColumnType(sumfield));
+    return s;
+}
+
+Schema GpuAggregateCount::foldinit(libconfig::Setting&) {
+    Schema s;
+    s.add(CT_LONG);
+    return s;
+}

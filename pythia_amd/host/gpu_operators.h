@@ -1,0 +1,123 @@
+// `gpuhashjoin`, `gpuaggregate_sum` and `gpuaggregate_count`: drop-ins for
+// Pythia's `hashjoin`, `aggregate_sum` and `aggregate_count` plan types.
+// They read exactly the reference operators' configuration keys and produce
+// the same output schema and Page protocol; the join / aggregation itself
+// runs on the GPU through the C ABI of pgpu.h (nothing else crosses to the
+// device).  Optional extra keys, ignored by the CPU types, live in a `gpu`
+// group: device, hbmbudget (bytes), probebatch (probe rows per device call).
+#pragma once
+
+#include <mutex>
+#include <vector>
+
+#include "operators.h"
+#include "pgpu.h"
+
+struct GpuKnobs {
+    int device = 0;
+    uint64_t hbmbudget = 0;
+    uint64_t probebatch = 1 << 20;
+    void read(libconfig::Setting& node);
+};
+
+class GpuHashJoinOp : public DualInputOp {
+  public:
+    enum JoinSrcT { BuildSide, ProbeSide };
+    typedef std::pair<JoinSrcT, unsigned int> JoinPrjT;
+
+    void accept(Visitor* v) override { v->visit(static_cast<DualInputOp*>(this)); }
+    // Keys as HashJoinOp::init / JoinOp::init (operators/join.cpp:184-366):
+    // hash{fn, buckets, ...}, tuplesperbucket, buildjattr, probejattr,
+    // projection, threadgroups, allocpolicy, stripeon.
+    void init(libconfig::Config& root, libconfig::Setting& node) override;
+    void threadInit(unsigned short threadid) override;
+    ResultCode scanStart(unsigned short threadid, Page* indexdatapage, Schema& indexdataschema) override;
+    GetNextResultT getNext(unsigned short threadid) override;
+    ResultCode scanStop(unsigned short threadid) override;
+    void threadClose(unsigned short threadid) override;
+    void destroy() override;
+
+    const Schema& buildSchema() const { return sbuild; }
+    pgpu_hj_stats stats(unsigned short group);
+
+  private:
+    struct ThreadState {
+        Page* out = nullptr;
+        std::vector<char> probe;   // staged probe rows (child pages are transient)
+        uint64_t nprobe = 0;
+        uint64_t pending = 0;      // device output rows not yet pulled
+        bool depleted = false;     // probe child returned Finished
+        bool done = false;
+    };
+    bool stageProbe(unsigned short tid);  // false when nothing more to stage
+    void fail_group(unsigned short group);
+
+    std::vector<JoinPrjT> projection;
+    unsigned int joinattr1 = 0, joinattr2 = 0;
+    std::vector<unsigned short> threadgroups, threadposingrp, groupleader, groupsize;
+    std::vector<Barrier*> barriers;
+    Schema sbuild;
+    std::vector<pgpu_hj*> tables;
+    std::vector<ThreadState> state;
+    GpuKnobs knobs;
+    pgpu_ctx* ctx = nullptr;
+    std::mutex mu;
+};
+
+class GpuAggregate : public SingleInputOp {
+  public:
+    void accept(Visitor* v) override { v->visit(static_cast<SingleInputOp*>(this)); }
+    // Keys as GenericAggregate::init (operators/genericaggregate.cpp:51-173):
+    // field | fields, hash{...}, global + threads, presorted (not implemented).
+    void init(libconfig::Config& root, libconfig::Setting& node) override;
+    void threadInit(unsigned short threadid) override;
+    ResultCode scanStart(unsigned short threadid, Page* indexdatapage, Schema& indexdataschema) override;
+    GetNextResultT getNext(unsigned short threadid) override;
+    ResultCode scanStop(unsigned short) override { return Ready; }
+    void threadClose(unsigned short threadid) override;
+    void destroy() override;
+
+  protected:
+    // The aggregate's own columns (AggregateSum / AggregateCount foldinit).
+    virtual Schema foldinit(libconfig::Setting& node) = 0;
+    virtual uint32_t kind() const = 0;
+    virtual uint32_t valOffset() const { return 0; }
+
+    std::vector<unsigned short> aggfields;
+
+  private:
+    struct ThreadState {
+        Page* out = nullptr;
+        pgpu_agg* agg = nullptr;  // ThreadLocal table
+        uint64_t first = 0, end = 0, pos = 0;
+        bool done = false;
+    };
+    pgpu_agg* newTable();
+    bool global_ = false;
+    int threads_ = 0;
+    pgpu_agg* shared_ = nullptr;
+    uint64_t shared_groups_ = 0;
+    Barrier barrier_;
+    std::vector<ThreadState> state;
+    GpuKnobs knobs;
+    pgpu_ctx* ctx = nullptr;
+    std::mutex mu, init_mu;
+};
+
+class GpuAggregateSum : public GpuAggregate {
+  protected:
+    // sumfield must be int, long or dec; the accumulator has its type
+    // (operators/aggregatesum.cpp:37-51).
+    Schema foldinit(libconfig::Setting& node) override;
+    uint32_t kind() const override { return kind_; }
+    uint32_t valOffset() const override { return valoff_; }
+
+  private:
+    uint32_t kind_ = 0, valoff_ = 0;
+};
+
+class GpuAggregateCount : public GpuAggregate {
+  protected:
+    Schema foldinit(libconfig::Setting& node) override;  // one CT_LONG count
+    uint32_t kind() const override { return PGPU_AGG_COUNT; }
+};

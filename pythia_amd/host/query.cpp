@@ -1,0 +1,164 @@
+#include "query.h"
+
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "gpu_operators.h"
+#include "pythia_host.h"
+#include "registry.h"
+
+namespace {
+
+// sanitycheck (query.cpp:37-60): the branch exists, names a node, and that
+// node has a type.
+void sanitycheck(libconfig::Config& cfg, libconfig::Setting& node, const std::string& branch) {
+    if (!node.exists(branch)) throw MissingParameterException("Cannot find `" + branch + "' attribute in query subtree.");
+    if (!node[branch].exists("name")) throw MissingParameterException("Cannot find `name' attribute in query subtree.");
+    std::string name = node[branch]["name"];
+    if (!cfg.getRoot().exists(name)) throw MissingParameterException("Cannot find description for node `" + name + "'.");
+    if (!cfg.getRoot()[name].exists("type"))
+        throw MissingParameterException("Cannot find mandatory `type' parameter in description for node `" + name +
+                                        "'.");
+}
+
+// constructsubtree (query.cpp:62-233): pick the class by type string, wire the
+// children, then init this node.
+void constructsubtree(libconfig::Config& cfg, libconfig::Setting& node, Operator** root) {
+    std::string name = node["name"];
+    std::string type = cfg.getRoot()[name]["type"];
+    if (type == "scan" || type == "parallelscan" || type == "partitionedscan") {
+        *root = new ParallelScanOp();
+    } else if (type == "memscan") {
+        *root = new MemoryScanOp();
+    } else if (type == "gpuhashjoin") {
+        auto* op = new GpuHashJoinOp();
+        *root = op;
+        sanitycheck(cfg, node, "build");
+        constructsubtree(cfg, node["build"], &op->buildOp);
+        sanitycheck(cfg, node, "probe");
+        constructsubtree(cfg, node["probe"], &op->probeOp);
+    } else if (type == "hashjoin" || type == "aggregate_sum" || type == "aggregate_count") {
+        throw MissingParameterException("`" + type + "' is the reference engine's CPU operator; this engine runs `gpu" +
+                                        type + "'.");
+    } else {
+        SingleInputOp* op = nullptr;
+        if (type == "gpuaggregate_sum")
+            op = new GpuAggregateSum();
+        else if (type == "gpuaggregate_count")
+            op = new GpuAggregateCount();
+        else if (type == "merge")
+            op = new MergeOp();
+        else if (type == "checker_callstate")
+            op = new CallStateChecker();
+        else
+            throw MissingParameterException("`" + type + "' is neither a built-in nor a user-defined type.");
+        *root = op;
+        sanitycheck(cfg, node, "input");
+        constructsubtree(cfg, node["input"], &op->nextOp);
+    }
+    (*root)->init(cfg, cfg.lookup(name));
+}
+
+thread_local std::string g_err;
+
+int set_err(const std::string& m) {
+    g_err = m;
+    return -1;
+}
+
+}  // namespace
+
+void Query::create(libconfig::Config& cfg) {
+    if (!cfg.getRoot().exists("treeroot")) throw MissingParameterException("Cannot find `treeroot' in the plan.");
+    libconfig::Setting& tr = cfg.getRoot()["treeroot"];
+    if (!tr.exists("name")) throw MissingParameterException("Cannot find `name' attribute in query subtree.");
+    constructsubtree(cfg, tr, &tree);
+}
+
+static std::string describe(const Schema& s) {
+    std::string d;
+    for (unsigned i = 0; i < s.columns(); ++i) {
+        if (i) d += ",";
+        switch (s.getColumnType(i)) {
+            case CT_INTEGER: d += "int"; break;
+            case CT_LONG: d += "long"; break;
+            case CT_DECIMAL: d += "dec"; break;
+            case CT_DATE: d += "date"; break;
+            case CT_CHAR: d += "char(" + std::to_string(s.getColumnWidth(i) - 1) + ")"; break;
+            default: d += "?";
+        }
+    }
+    return d;
+}
+
+extern "C" {
+
+const char* pythia_last_error(void) { return g_err.c_str(); }
+
+int pythia_register_table(const char* name, const void* rows, uint64_t nrows, uint32_t row_bytes) {
+    if (!name || (nrows && !rows) || !row_bytes) return set_err("pythia_register_table: bad argument");
+    pythia::register_table(name, rows, nrows, row_bytes);
+    return 0;
+}
+
+int pythia_clear_tables(void) {
+    pythia::clear_tables();
+    return 0;
+}
+
+// The executequery driver loop (drivers/executequery.cpp:32-101): create,
+// threadInit, scanStart, getNext until Finished, scanStop, threadClose,
+// destroy; every output row is collected.
+int pythia_run_query(const char* plan_text, pythia_result* out) {
+    if (!plan_text || !out) return set_err("pythia_run_query: null argument");
+    memset(out, 0, sizeof *out);
+    Query q;
+    int rc = 0;
+    std::string rows;
+    try {
+        libconfig::Config cfg;
+        cfg.readString(plan_text);
+        q.create(cfg);
+        snprintf(out->schema, sizeof out->schema, "%s", describe(q.getOutSchema()).c_str());
+        out->row_bytes = q.getOutSchema().getTupleSize();
+        q.threadInit();
+        auto t0 = std::chrono::steady_clock::now();
+        if (q.scanStart() == Operator::Error) throw QueryExecutionError("scanStart returned Error");
+        Operator::GetNextResultT r{Operator::Ready, nullptr};
+        uint64_t calls = 0;
+        while (r.first == Operator::Ready) {
+            r = q.getNext();
+            ++calls;
+            if (r.first == Operator::Error) throw QueryExecutionError("getNext returned Error");
+            rows.append(r.second->data(), r.second->getUsedSpace());
+        }
+        if (q.scanStop() == Operator::Error) throw QueryExecutionError("scanStop returned Error");
+        auto t1 = std::chrono::steady_clock::now();
+        out->seconds = std::chrono::duration<double>(t1 - t0).count();
+        out->getnext_calls = calls;
+        q.threadClose();
+    } catch (const std::exception& e) {
+        rc = set_err(e.what());
+    }
+    if (q.tree) {
+        try {
+            q.destroy();
+        } catch (const std::exception& e) {
+            if (!rc) rc = set_err(e.what());
+        }
+    }
+    if (rc) return rc;
+    out->nrows = out->row_bytes ? rows.size() / out->row_bytes : 0;
+    out->rows = malloc(rows.size() ? rows.size() : 1);
+    memcpy(out->rows, rows.data(), rows.size());
+    return 0;
+}
+
+void pythia_free_result(pythia_result* r) {
+    if (r && r->rows) free(r->rows);
+    if (r) r->rows = nullptr;
+}
+
+}  // extern "C"
