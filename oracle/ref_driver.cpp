@@ -1,0 +1,212 @@
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/ref.mk against the reference
+// engine's headers and objects (compiled from /root/reference); writes only
+// oracle/_ref/refjoin.
+//
+// Runs the reference's own operators on generated inputs with the plan shape
+// of unit_tests/queryhashjoin.cpp: an in-memory page source for each side ->
+// HashJoinOp (one thread group of T threads, modulo hash) -> partial
+// AggregateSum (or AggregateCount) per worker -> MergeOp(T); the consumer
+// folds the per-worker partials in the accumulator's type, as the final
+// aggregate of that plan does.  (A worker whose first probe read finds the
+// shared scan already drained returns Finished from HashJoinOp::scanStart,
+// operators/join.cpp:451-459; the merge then reports Finished from
+// scanStart, so a final aggregate above it would skip its input.)  Timed from scanStart to scanStop with CLOCK_MONOTONIC as
+// drivers/executequery.cpp:38-59 does (with rdtsc there).
+//
+// usage: refjoin <c1|c2> <log2 R> <log2 S> <threads> <zipf s> <buckets> <tuplesperbucket> <sum|count|probesum>
+// prints one JSON line {"count" or "sum": ..., "seconds": ..., ...}
+#include <pthread.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <string>
+#include <vector>
+
+#include "operators/operators.h"
+#include "query.h"
+#include "visitors/allvisitors.h"
+#include "pgpu_gen.h"
+
+using namespace libconfig;
+
+// Hands out buffsize-sized overlay pages of a packed relation to any thread
+// through a shared atomic cursor (the Table::atomicReadNext pattern).
+class MemScan : public ZeroInputOp {
+  public:
+    MemScan(char* rows, uint64_t n, Schema s) : rows_(rows), n_(n), s_(s), cursor_(0) {}
+    virtual void init(Config& root, Setting& node) {
+        Operator::init(root, node);
+        schema = s_;
+        rpp_ = buffsize / schema.getTupleSize();
+        if (rpp_ == 0) rpp_ = 1;
+        for (int i = 0; i < MAX_THREADS; ++i) pages_.push_back(NULL);
+        empty_ = new Page(rows_, 0, rows_, schema.getTupleSize());
+    }
+    virtual GetNextResultT getNext(unsigned short tid) {
+        delete pages_[tid];
+        pages_[tid] = NULL;
+        uint64_t start = __sync_fetch_and_add(&cursor_, rpp_);
+        if (start >= n_) return std::make_pair(Finished, empty_);
+        uint64_t k = n_ - start < rpp_ ? n_ - start : rpp_;
+        const unsigned ts = schema.getTupleSize();
+        pages_[tid] = new Page(rows_ + start * ts, k * ts, NULL, ts);
+        return std::make_pair(start + k >= n_ ? Finished : Ready, pages_[tid]);
+    }
+
+  private:
+    char* rows_;
+    uint64_t n_;
+    Schema s_;
+    volatile uint64_t cursor_;
+    uint64_t rpp_;
+    std::vector<Page*> pages_;
+    Page* empty_;
+};
+
+static double now() {
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + ts.tv_nsec * 1e-9;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 9) {
+        fprintf(stderr, "usage: refjoin <c1|c2> <log2R> <log2S> <threads> <zipf> <buckets> <tpb> <sum|count|probesum>\n");
+        return 2;
+    }
+    const std::string wl = argv[1];
+    const uint64_t nR = 1ULL << atoi(argv[2]), nS = 1ULL << atoi(argv[3]);
+    const int T = atoi(argv[4]);
+    const double zipf = atof(argv[5]);
+    const int buckets = atoi(argv[6]), tpb = atoi(argv[7]);
+    const std::string agg = argv[8];
+    const uint64_t seed = PGPU_DEFAULT_SEED;
+    const bool is32 = wl == "c1";
+    const unsigned kb = is32 ? 4 : 8;
+
+    // build rows (key, payload); probe rows (fk[, payload])
+    const unsigned prow = is32 ? 8 : 8;
+    std::vector<char> R(nR * 2 * kb), S(nS * prow);
+    std::vector<double> cdf;
+    if (zipf > 0) {
+        cdf.resize(nR);
+        pgpu_zipf_cdf_fill(&cdf[0], nR, zipf);
+    }
+    for (uint64_t i = 0; i < nR; ++i) {
+        if (is32) {
+            int32_t k = pgpu_c1_rkey(seed, i, nR), p = pgpu_c1_rpay(seed, i);
+            memcpy(&R[i * 8], &k, 4);
+            memcpy(&R[i * 8 + 4], &p, 4);
+        } else {
+            int64_t k = pgpu_c2_rkey(seed, 0, i), p = pgpu_c2_rpay(seed, 0, i);
+            memcpy(&R[i * 16], &k, 8);
+            memcpy(&R[i * 16 + 8], &p, 8);
+        }
+    }
+    for (uint64_t j = 0; j < nS; ++j) {
+        if (is32) {
+            int32_t k = pgpu_c1_sfk(seed, j, nR), p = pgpu_c1_spay(seed, j);
+            memcpy(&S[j * 8], &k, 4);
+            memcpy(&S[j * 8 + 4], &p, 4);
+        } else {
+            uint64_t row = zipf > 0 ? pgpu_c2_srow_zipf(seed, 0, j, nR, &cdf[0]) : pgpu_c2_srow_uniform(seed, 0, j, nR);
+            int64_t k = pgpu_c2_rkey(seed, 0, row);
+            memcpy(&S[j * 8], &k, 8);
+        }
+    }
+
+    Schema rs, ss;
+    rs.add(is32 ? CT_INTEGER : CT_LONG);
+    rs.add(is32 ? CT_INTEGER : CT_LONG);
+    ss.add(is32 ? CT_INTEGER : CT_LONG);
+    if (is32) ss.add(CT_INTEGER);
+
+    Config cfg;
+    cfg.getRoot().add("path", Setting::TypeString) = "./";
+    // Many probe pages per worker: a worker whose first probe read finds the
+    // shared scan drained makes HashJoinOp::scanStart return Finished
+    // (operators/join.cpp:451-459), which ends the plan early.
+    uint64_t bs = nS * prow / (64 * (uint64_t)T);
+    if (bs > (1u << 20)) bs = 1u << 20;
+    if (bs < 4096) bs = 4096;
+    bs &= ~(uint64_t)15;
+    cfg.getRoot().add("buffsize", Setting::TypeInt) = (int)bs;
+    Setting& jn = cfg.getRoot().add("join", Setting::TypeGroup);
+    Setting& hn = jn.add("hash", Setting::TypeGroup);
+    hn.add("fn", Setting::TypeString) = "modulo";
+    hn.add("buckets", Setting::TypeInt) = buckets;
+    Setting& tg = jn.add("threadgroups", Setting::TypeList);
+    Setting& g0 = tg.add(Setting::TypeArray);
+    for (int i = 0; i < T; ++i) g0.add(Setting::TypeInt) = i;
+    jn.add("tuplesperbucket", Setting::TypeInt) = tpb;
+    jn.add("allocpolicy", Setting::TypeString) = "local";
+    jn.add("buildjattr", Setting::TypeInt) = 0;
+    jn.add("probejattr", Setting::TypeInt) = 0;
+    Setting& pj = jn.add("projection", Setting::TypeList);
+    pj.add(Setting::TypeString) = agg == "probesum" ? "P$1" : "B$1";
+    Setting& an = cfg.getRoot().add("agg", Setting::TypeGroup);
+    an.add("fields", Setting::TypeList);
+    an.add("sumfield", Setting::TypeInt) = 0;
+    Setting& mn = cfg.getRoot().add("merge", Setting::TypeGroup);
+    mn.add("threads", Setting::TypeInt) = T;
+    Setting& sc = cfg.getRoot().add("scan", Setting::TypeGroup);
+
+    MemScan scanR(&R[0], nR, rs), scanS(&S[0], nS, ss);
+    HashJoinOp join;
+    GenericAggregate* partial;
+    if (agg == "count")
+        partial = new AggregateCount();
+    else
+        partial = new AggregateSum();
+    MergeOp merge;
+    Query q;
+    q.tree = &merge;
+    merge.nextOp = partial;
+    partial->nextOp = &join;
+    join.buildOp = &scanR;
+    join.probeOp = &scanS;
+    scanR.init(cfg, sc);
+    scanS.init(cfg, sc);
+    join.init(cfg, jn);
+    partial->init(cfg, an);
+    merge.init(cfg, mn);
+
+    q.threadInit();
+    double t0 = now();
+    if (q.scanStart() == Operator::Error) {
+        fprintf(stderr, "scanStart failed\n");
+        return 1;
+    }
+    Operator::GetNextResultT r;
+    r.first = Operator::Ready;
+    long long value = 0;
+    int32_t value32 = 0;
+    while (r.first == Operator::Ready) {
+        r = q.getNext();
+        if (r.first == Operator::Error) {
+            fprintf(stderr, "getNext failed\n");
+            return 1;
+        }
+        void* t;
+        Operator::Page::Iterator it = r.second->createIterator();
+        while ((t = it.next())) {
+            if (is32 && agg != "count")
+                value32 = (int32_t)((uint32_t)value32 + (uint32_t)q.getOutSchema().asInt(t, 0));
+            else
+                value = (long long)((unsigned long long)value + (unsigned long long)q.getOutSchema().asLong(t, 0));
+        }
+    }
+    if (is32 && agg != "count") value = value32;
+    q.scanStop();
+    double t1 = now();
+    q.threadClose();
+    printf("{\"workload\": \"%s\", \"build\": %llu, \"probe\": %llu, \"threads\": %d, \"agg\": \"%s\", \"value\": %lld, "
+           "\"seconds\": %.6f, \"spills\": %lu}\n",
+           wl.c_str(), (unsigned long long)nR, (unsigned long long)nS, T, agg.c_str(), value, t1 - t0,
+           0UL);
+    q.destroynofree();
+    return 0;
+}
