@@ -234,6 +234,31 @@ int pgpu_agg_finish(pgpu_agg* a, uint64_t* n_groups, uint32_t* out_row_bytes);
 /* Copies output rows [first, first + n) into dst. */
 int pgpu_agg_pull(pgpu_agg* a, uint64_t first, uint64_t n, void* dst);
 
+/* ------------------------------------------------------------------ */
+/* Join + group-by FP64 sum over device-resident columns (C3 shape):    */
+/*   SUM(F.val) GROUP BY D.grp over F JOIN D ON F.key = D.key            */
+/* Stands in for the plan hashjoin(build D, probe F, projection          */
+/* [B$grp, P$val]) -> aggregate_sum(field 0, sumfield 1): HashJoinOp's   */
+/* probe loop (operators/join.cpp:503-562) feeding                       */
+/* GenericAggregate::remember / AggregateSum::fold                       */
+/* (operators/genericaggregate.cpp:240-282, aggregatesum.cpp:81-107).    */
+/* Keys and groups are int32; sums are FP64 (order of additions is not   */
+/* fixed: results agree with exact sums within rounding).                */
+/* ------------------------------------------------------------------ */
+typedef struct pgpu_groupjoin pgpu_groupjoin;
+int pgpu_groupjoin_create(pgpu_ctx* ctx, uint64_t max_dim, uint64_t max_fact, uint64_t max_groups,
+                          pgpu_groupjoin** out);
+int pgpu_groupjoin_destroy(pgpu_groupjoin* p);
+/* Runs the whole query on the context stream and waits; *n_groups receives
+ * the number of result groups, *n_joined (optional) the joined fact rows. */
+int pgpu_groupjoin_run(pgpu_groupjoin* p, const int32_t* d_dkey, const int32_t* d_dgrp, uint64_t n_dim,
+                       const int32_t* d_fkey, const double* d_fval, uint64_t n_fact, uint64_t* n_groups,
+                       uint64_t* n_joined);
+/* Copies the first n result groups (any of the outputs may be NULL). */
+int pgpu_groupjoin_fetch(pgpu_groupjoin* p, int32_t* keys, double* sums, uint64_t* counts, uint64_t n);
+int pgpu_groupjoin_timing(pgpu_groupjoin* p, int enable);
+int pgpu_groupjoin_times(pgpu_groupjoin* p, pgpu_kernel_time* out, int cap, int* n);
+
 #ifdef __cplusplus
 }
 #endif

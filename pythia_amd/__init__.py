@@ -15,7 +15,7 @@ import numpy as np
 from . import _lib
 from ._lib import ALGO_AUTO, ALGO_GLOBAL, ALGO_RADIX, GEN, PgpuError, call, lib  # noqa: F401
 
-__all__ = ["Context", "DeviceBuffer", "JoinPlan", "device_count", "PgpuError", "GEN",
+__all__ = ["Context", "DeviceBuffer", "JoinPlan", "GroupJoin", "device_count", "PgpuError", "GEN",
            "ALGO_AUTO", "ALGO_RADIX", "ALGO_GLOBAL"]
 
 
@@ -137,3 +137,43 @@ class JoinPlan:
         n = C.c_int()
         call("pgpu_join_plan_times", self.handle, arr, 64, C.byref(n))
         return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(min(n.value, 64))}
+
+
+def _times(fn, handle):
+    arr = (_lib.KernelTime * 64)()
+    n = C.c_int()
+    call(fn, handle, arr, 64, C.byref(n))
+    return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(min(n.value, 64))}
+
+
+class GroupJoin:
+    """SUM(F.val) GROUP BY D.grp over F JOIN D ON key (int32 keys / groups, FP64 sums)."""
+
+    def __init__(self, ctx: Context, max_dim: int, max_fact: int, max_groups: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        call("pgpu_groupjoin_create", ctx.handle, max_dim, max_fact, max_groups, C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().pgpu_groupjoin_destroy(self.handle)
+            self.handle = None
+
+    def run(self, dkey, dgrp, n_dim, fkey, fval, n_fact):
+        ng, nj = C.c_uint64(), C.c_uint64()
+        call("pgpu_groupjoin_run", self.handle, C.c_void_p(dkey.ptr), C.c_void_p(dgrp.ptr), n_dim,
+             C.c_void_p(fkey.ptr), C.c_void_p(fval.ptr), n_fact, C.byref(ng), C.byref(nj))
+        return ng.value, nj.value
+
+    def fetch(self, n):
+        k, s, c = np.empty(n, np.int32), np.empty(n, np.float64), np.empty(n, np.uint64)
+        call("pgpu_groupjoin_fetch", self.handle, k.ctypes.data_as(C.c_void_p), s.ctypes.data_as(C.c_void_p),
+             c.ctypes.data_as(C.c_void_p), n)
+        return k, s, c
+
+    def timing(self, on: bool):
+        call("pgpu_groupjoin_timing", self.handle, 1 if on else 0)
+
+    def times(self):
+        return _times("pgpu_groupjoin_times", self.handle)

@@ -78,6 +78,13 @@ _PROTOS = {
                                       C.c_void_p, C.c_uint64, C.POINTER(JoinResult)]),
     "pgpu_join_plan_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "pgpu_join_plan_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),
+    "pgpu_groupjoin_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pgpu_groupjoin_destroy": (C.c_int, [C.c_void_p]),
+    "pgpu_groupjoin_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                     C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "pgpu_groupjoin_fetch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "pgpu_groupjoin_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "pgpu_groupjoin_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),
 }
 
 

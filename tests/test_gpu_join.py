@@ -205,3 +205,61 @@ def test_c2_full_shape(ctx, zipf):
     plan.close()
     for b in (rk, rp, sk):
         b.free()
+
+
+def _c3_device(ctx, nD, nF, norders):
+    dk, dc, fk, fp = ctx.alloc(nD * 4), ctx.alloc(nD * 4), ctx.alloc(nF * 4), ctx.alloc(nF * 8)
+    ctx.gen("C3_D_ORDERKEY", nD, dk, n_build=nD)
+    ctx.gen("C3_D_CUSTKEY", nD, dc)
+    ctx.gen("C3_F_ORDERKEY", nF, fk, n_build=norders)
+    ctx.gen("C3_F_PRICE", nF, fp)
+    ctx.sync()
+    return dk, dc, fk, fp
+
+
+@pytest.mark.parametrize("nD,nF", [(20000, 100000), (15_000_000, 60_000_000)])
+def test_c3_groupjoin_matches_exact_cents(ctx, nD, nF):
+    """sum(price) group by custkey over F join D on orderkey: every group
+    within 1e-12 relative of the exact integer-cent sum."""
+    bufs = _c3_device(ctx, nD, nF, nD)
+    gj = pa.GroupJoin(ctx, nD, nF, 1_500_000)
+    ng, joined = gj.run(*bufs[:2], nD, *bufs[2:], nF)
+    _, cents = O.c3_closed(nD, nF, norders=nD, threads=THREADS)
+    assert joined == nF
+    assert ng == np.count_nonzero(cents)
+    k, s, c = gj.fetch(ng)
+    exact = cents[k] / 100.0
+    assert np.all(np.abs(s - exact) <= 1e-12 * np.abs(exact))
+    assert len(np.unique(k)) == ng
+    gj.close()
+    for b in bufs:
+        b.free()
+
+
+def test_groupjoin_duplicates_and_sentinel_keys(ctx):
+    """Duplicate dimension keys (each match contributes), the INT32_MIN
+    marker as a real key and group, unmatched fact rows."""
+    rng = np.random.default_rng(11)
+    lo = np.iinfo(np.int32).min
+    dk = rng.integers(-50, 50, 300).astype(np.int32)
+    dk[:3] = lo
+    dg = rng.integers(-5, 5, 300).astype(np.int32)
+    dg[5:9] = lo
+    fk = rng.integers(-60, 60, 5000).astype(np.int32)
+    fk[:4] = lo
+    fv = rng.random(5000)
+    exp = {}
+    for k, v in zip(fk.tolist(), fv.tolist()):
+        for kk, gg in zip(dk.tolist(), dg.tolist()):
+            if kk == k:
+                exp[gg] = exp.get(gg, 0.0) + v
+    bufs = [_dev(ctx, a) for a in (dk, dg, fk, fv)]
+    gj = pa.GroupJoin(ctx, 300, 5000, 100)
+    ng, joined = gj.run(bufs[0], bufs[1], 300, bufs[2], bufs[3], 5000)
+    k, s, c = gj.fetch(ng)
+    assert dict(zip(k.tolist(), s.tolist())).keys() == exp.keys()
+    for kk, ss in zip(k.tolist(), s.tolist()):
+        assert abs(ss - exp[kk]) <= 1e-12 * max(1.0, abs(exp[kk]))
+    gj.close()
+    for b in bufs:
+        b.free()
