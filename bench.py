@@ -45,9 +45,29 @@ def kernel_bytes(nR: int, nS: int, esR: int = 16, esS: int = 8, kb: int = 8) -> 
 
 
 def cpu_baseline(args, nR, nS, gpu_result):
-    from oracle import oracle as O
+    """The reference's own HashJoinOp -> AggregateSum -> MergeOp plan
+    (compiled from its sources into oracle/_ref/refjoin) when that binary is
+    present, else the C restatement in oracle/; same inputs, this host's
+    cores, the reference's C2 knobs (modulo, 2^24 buckets, 4 per bucket)."""
+    import subprocess
 
     cores = int(os.environ.get("PYTHIA_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    ref = os.path.join(ROOT, "oracle", "_ref", "refjoin")
+    if os.path.exists(ref):
+        out = subprocess.run([ref, "c2", str(args.log_build), str(args.log_probe), str(cores), str(args.zipf),
+                              str(1 << 24), "4", "sum"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                             timeout=900, cwd="/tmp")
+        if out.returncode == 0:
+            r = json.loads(out.stdout)
+            return {
+                "value": (nR + nS) / r["seconds"], "unit": "tuples/s", "cores": cores, "kind": "reference",
+                "sample": f"the full workload through the reference's hashjoin -> aggregate_sum -> merge({cores}) "
+                          f"plan, one run of {r['seconds']:.2f} s (inputs generated outside the clock)",
+                "result_matches_gpu": bool(r["value"] % 2**64 == gpu_result.sum_build),
+            }
+        print(f"# reference baseline failed: {out.stderr[-300:]}", file=sys.stderr)
+    from oracle import oracle as O
+
     t0 = time.time()
     rk, rp, sk = O.gen_c2(nR, nS, args.zipf, seed=args.seed)
     gen_s = time.time() - t0

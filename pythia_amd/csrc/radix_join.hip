@@ -42,11 +42,14 @@ namespace {
 constexpr uint64_t kFib = 0x9E3779B97F4A7C15ull;
 constexpr int kMaxBits = 14;
 constexpr int kScatterThreads = 512;
-constexpr int kJoinThreads = 256;
-constexpr int kJoinCap = 2048;            // LDS table slots
-constexpr int kJoinPiece = 1536;          // build rows per LDS piece (load <= 0.75)
+constexpr int kJoinThreads = 512;
+constexpr int kJoinCap = 4096;            // LDS table slots
+constexpr int kJoinPiece = 2048;          // build rows per LDS piece (load <= 0.5)
 constexpr uint32_t kItemRows = 32768;     // probe rows per join item
 constexpr int kTileBytes = 65536;         // LDS staging tile of the scatters
+constexpr int kPartSlots = 2048;          // partial-result slots of the join kernels
+static_assert(kJoinPiece < 65536, "LDS match counters are 16 bits wide");
+static_assert(2 * kJoinPiece <= kJoinCap, "LDS table load factor above 0.5");
 
 template <typename KT>
 __device__ __forceinline__ uint64_t hmix(KT k) {
@@ -242,10 +245,8 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
     __syncthreads();
     const uint64_t s = (uint64_t)blockIdx.x * blk;
     const uint64_t e = min(n, s + blk);
-    for (uint64_t t0 = s; t0 < e; t0 += T) {
-        const uint32_t cnt = (uint32_t)min<uint64_t>(T, e - t0);
-        E el[IPT];
-        uint32_t dg[IPT], rk[IPT];
+    E el[IPT];
+    auto load = [&](uint64_t t0, uint32_t cnt) {
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
             const uint32_t idx = tid + k * kScatterThreads;
@@ -254,6 +255,11 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
                 if constexpr (HV) el[k].v = vals[t0 + idx];
             }
         }
+    };
+    uint32_t cnt = (uint32_t)min<uint64_t>(T, e > s ? e - s : 0);
+    if (s < e) load(s, cnt);
+    for (uint64_t t0 = s; t0 < e;) {
+        uint32_t dg[IPT], rk[IPT];
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
             const uint32_t idx = tid + k * kScatterThreads;
@@ -271,6 +277,11 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
             if (idx < cnt) stage[lofs[dg[k]] + rk[k]] = el[k];
         }
         __syncthreads();
+        // The tile now lives in LDS: start loading the next one into the
+        // registers before writing this one out.
+        const uint64_t next = t0 + cnt;
+        const uint32_t ncnt = next < e ? (uint32_t)min<uint64_t>(T, e - next) : 0u;
+        if (ncnt) load(next, ncnt);
         for (uint32_t j = tid; j < cnt; j += kScatterThreads) {
             const E x = stage[j];
             const uint32_t d = (uint32_t)(hmix(x.k) >> (64 - B)) >> B2;
@@ -282,12 +293,20 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
             hist[d] = 0;
         }
         __syncthreads();
+        t0 = next;
+        cnt = ncnt;
     }
 }
 
+// A pass-2 unit is partition p1's rows from pass-1 blocks [b0, b0+G): they
+// are contiguous in the pass-1 output (block b wrote p1's rows at off1[b][p1]),
+// and the units' output runs for (p1, p2) are contiguous too, starting at
+// base(p1,p2) + H[b0][p1,p2].  G > 1 keeps units near a tile when blocks
+// are small.
 template <typename KT, bool HV, int IPT>
-__global__ __launch_bounds__(kScatterThreads) void k_scatter2(const Elem<KT, HV>* __restrict__ in, uint32_t nblk, int B,
-                                                              int B2, const uint64_t* __restrict__ off1,
+__global__ __launch_bounds__(kScatterThreads) void k_scatter2(const Elem<KT, HV>* __restrict__ in, uint32_t nblk,
+                                                              uint32_t G, int B, int B2,
+                                                              const uint64_t* __restrict__ off1,
                                                               const uint32_t* __restrict__ Hs,
                                                               const uint64_t* __restrict__ base,
                                                               Elem<KT, HV>* __restrict__ out) {
@@ -299,26 +318,30 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter2(const Elem<KT, HV>
     __shared__ uint64_t run[256];
     const int tid = threadIdx.x;
     const uint32_t nb1 = 1u << (B - B2), nb2 = 1u << B2, nb = 1u << B;
-    const uint32_t p1 = blockIdx.x / nblk, b = blockIdx.x % nblk;
-    const uint64_t s = off1[(uint64_t)b * nb1 + p1];
-    const uint64_t e = off1[(uint64_t)(b + 1) * nb1 + p1];
+    const uint32_t ngrp = (nblk + G - 1) / G;
+    const uint32_t p1 = blockIdx.x / ngrp, b0 = (blockIdx.x % ngrp) * G, b1 = min(nblk, b0 + G);
+    const uint64_t s = off1[(uint64_t)b0 * nb1 + p1];
+    const uint64_t e = off1[(uint64_t)b1 * nb1 + p1];
     if (s >= e) return;
     const uint32_t mask2 = nb2 - 1;
     for (uint32_t d = tid; d < nb2; d += kScatterThreads) {
         const uint64_t pp = ((uint64_t)p1 << B2) + d;
-        run[d] = base[pp] + Hs[(uint64_t)b * nb + pp];
+        run[d] = base[pp] + Hs[(uint64_t)b0 * nb + pp];
         hist[d] = 0;
     }
     __syncthreads();
-    for (uint64_t t0 = s; t0 < e; t0 += T) {
-        const uint32_t cnt = (uint32_t)min<uint64_t>(T, e - t0);
-        E el[IPT];
-        uint32_t dg[IPT], rk[IPT];
+    E el[IPT];
+    auto load = [&](uint64_t t0, uint32_t cnt) {
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
             const uint32_t idx = tid + k * kScatterThreads;
             if (idx < cnt) el[k] = in[t0 + idx];
         }
+    };
+    uint32_t cnt = (uint32_t)min<uint64_t>(T, e - s);
+    load(s, cnt);
+    for (uint64_t t0 = s; t0 < e;) {
+        uint32_t dg[IPT], rk[IPT];
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
             const uint32_t idx = tid + k * kScatterThreads;
@@ -336,6 +359,9 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter2(const Elem<KT, HV>
             if (idx < cnt) stage[lofs[dg[k]] + rk[k]] = el[k];
         }
         __syncthreads();
+        const uint64_t next = t0 + cnt;
+        const uint32_t ncnt = next < e ? (uint32_t)min<uint64_t>(T, e - next) : 0u;
+        if (ncnt) load(next, ncnt);
         for (uint32_t j = tid; j < cnt; j += kScatterThreads) {
             const E x = stage[j];
             const uint32_t d = (uint32_t)(hmix(x.k) >> (64 - B)) & mask2;
@@ -347,6 +373,8 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter2(const Elem<KT, HV>
             hist[d] = 0;
         }
         __syncthreads();
+        t0 = next;
+        cnt = ncnt;
     }
 }
 
@@ -357,6 +385,49 @@ __global__ __launch_bounds__(256) void k_icount(const uint32_t* __restrict__ tot
     icount[pp] = totR[pp] ? (totS[pp] + M - 1) / M : 0u;
 }
 
+// items[ioff[pp] + i] = (pp, i): the join workgroup reads its partition and
+// sub-range with one load.
+__global__ __launch_bounds__(256) void k_items(const uint32_t* __restrict__ icount, const uint64_t* __restrict__ ioff,
+                                               uint32_t nb, uint2* __restrict__ items) {
+    const uint32_t pp = blockIdx.x * 256 + threadIdx.x;
+    if (pp >= nb) return;
+    const uint64_t o = ioff[pp];
+    for (uint32_t i = 0; i < icount[pp]; ++i) items[o + i] = make_uint2(pp, i);
+}
+
+// Folds the per-slot partial results into result[0..3].
+__global__ __launch_bounds__(256) void k_reduce_partials(const unsigned long long* __restrict__ part, int nslots,
+                                                         unsigned long long* __restrict__ result) {
+    unsigned long long c = 0, b = 0, p = 0, m = 0;
+    for (int i = threadIdx.x; i < nslots; i += 256) {
+        c += part[4 * i];
+        b += part[4 * i + 1];
+        p += part[4 * i + 2];
+        m = max(m, part[4 * i + 3]);
+    }
+    __shared__ unsigned long long red[4][256];
+    red[0][threadIdx.x] = c;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = p;
+    red[3][threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            red[2][threadIdx.x] += red[2][threadIdx.x + w];
+            red[3][threadIdx.x] = max(red[3][threadIdx.x], red[3][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        result[0] = red[0][0];
+        result[1] = red[1][0];
+        result[2] = red[2][0];
+        result[3] = red[3][0] > 1 ? red[3][0] : 1;  // LDS passes per partition
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -364,38 +435,36 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// LDS table slot: key and folded payload sum read with one 16-byte access.
+struct __align__(16) LSlot {
+    long long key;
+    unsigned long long sum;
+};
+
 template <typename KT, bool HVS>
 __global__ __launch_bounds__(kJoinThreads) void k_join_cs(
     const Elem<KT, true>* __restrict__ R2, const uint64_t* __restrict__ baseR, const uint32_t* __restrict__ totR,
     const Elem<KT, HVS>* __restrict__ S2, const uint64_t* __restrict__ baseS, const uint32_t* __restrict__ totS,
-    const uint64_t* __restrict__ ioff, uint32_t nb, uint32_t M, int B, unsigned long long* __restrict__ result) {
+    const uint64_t* __restrict__ ioff, const uint2* __restrict__ items, uint32_t nb, uint32_t M, int B,
+    unsigned long long* __restrict__ part) {
     constexpr int CAP = kJoinCap;
-    constexpr int L = 11;  // log2(CAP)
+    constexpr int L = 12;  // log2(CAP)
     static_assert((1 << L) == CAP, "CAP");
-    constexpr KT EMPTY = KeyEmpty<KT>::v;
-    __shared__ KT tk[CAP];
-    __shared__ uint32_t tc[CAP];
-    __shared__ unsigned long long ts[CAP];
+    // Keys are held sign-extended to 64 bits; only an int64 key equal to the
+    // empty marker needs the side accumulator.
+    constexpr long long EMPTY = INT64_MIN;
+    __shared__ LSlot tab[CAP];
+    // match counts, two 16-bit counters per word (a piece holds at most
+    // kJoinPiece < 2^16 rows, so a counter cannot carry into its neighbour)
+    __shared__ uint32_t tc[CAP / 2];
     __shared__ unsigned long long sp_cnt, sp_sum;
-    __shared__ uint32_t s_pp;
     __shared__ unsigned long long red[3][kJoinThreads / 64];
     const int tid = threadIdx.x;
     const uint64_t item = blockIdx.x;
     if (item >= ioff[nb]) return;
-    if (tid == 0) {
-        uint32_t lo = 0, hi = nb - 1;  // largest pp with ioff[pp] <= item
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (ioff[mid] <= item)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        s_pp = lo;
-    }
-    __syncthreads();
-    const uint32_t pp = s_pp;
-    const uint64_t sub = item - ioff[pp];
+    const uint2 it = items[item];
+    const uint32_t pp = it.x;
+    const uint64_t sub = it.y;
     const uint64_t sB = baseS[pp] + sub * M;
     const uint64_t sE = min(baseS[pp] + totS[pp], sB + M);
     const uint64_t rB = baseR[pp];
@@ -405,54 +474,52 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_cs(
 
     for (uint32_t p0 = 0; p0 < nR; p0 += kJoinPiece) {
         for (int i = tid; i < CAP; i += kJoinThreads) {
-            tk[i] = EMPTY;
-            tc[i] = 0;
-            ts[i] = 0;
+            tab[i].key = EMPTY;
+            tab[i].sum = 0;
+            if (i < CAP / 2) tc[i] = 0;
         }
         if (tid == 0) sp_cnt = sp_sum = 0;
         __syncthreads();
         const uint32_t pn = min<uint32_t>(kJoinPiece, nR - p0);
         for (uint32_t i = tid; i < pn; i += kJoinThreads) {
             const Elem<KT, true> r = R2[rB + p0 + i];
-            if (r.k == EMPTY) {
+            const long long k = (long long)r.k;
+            if (k == EMPTY) {
                 atomicAdd(&sp_cnt, 1ull);
                 atomicAdd(&sp_sum, (unsigned long long)(int64_t)r.v);
                 continue;
             }
             uint32_t slot = (uint32_t)(hmix(r.k) >> hs) & (CAP - 1);
             while (true) {
-                KT prev;
-                if constexpr (sizeof(KT) == 8)
-                    prev = (KT)atomicCAS((unsigned long long*)&tk[slot], (unsigned long long)EMPTY,
-                                         (unsigned long long)r.k);
-                else
-                    prev = (KT)atomicCAS((unsigned int*)&tk[slot], (unsigned int)EMPTY, (unsigned int)r.k);
-                if (prev == EMPTY || prev == r.k) {
-                    atomicAdd(&tc[slot], 1u);
-                    atomicAdd(&ts[slot], (unsigned long long)(int64_t)r.v);
+                const long long prev = (long long)atomicCAS((unsigned long long*)&tab[slot].key,
+                                                            (unsigned long long)EMPTY, (unsigned long long)k);
+                if (prev == EMPTY || prev == k) {
+                    atomicAdd(&tc[slot >> 1], 1u << (16 * (slot & 1)));
+                    atomicAdd(&tab[slot].sum, (unsigned long long)(int64_t)r.v);
                     break;
                 }
                 slot = (slot + 1) & (CAP - 1);
             }
         }
         __syncthreads();
-        constexpr int U = 8;
+        constexpr int U = HVS ? 8 : 16;
         uint64_t j = sB + tid;
         auto probe = [&](const Elem<KT, HVS>& x) {
+            const long long k = (long long)x.k;
             unsigned long long c = 0, sb = 0;
-            if (x.k == EMPTY) {
+            if (k == EMPTY) {
                 c = sp_cnt;
                 sb = sp_sum;
             } else {
                 uint32_t slot = (uint32_t)(hmix(x.k) >> hs) & (CAP - 1);
                 while (true) {
-                    const KT t = tk[slot];
-                    if (t == x.k) {
-                        c = tc[slot];
-                        sb = ts[slot];
+                    const LSlot t = tab[slot];
+                    if (t.key == k) {
+                        c = (tc[slot >> 1] >> (16 * (slot & 1))) & 0xFFFFu;
+                        sb = t.sum;
                         break;
                     }
-                    if (t == EMPTY) break;
+                    if (t.key == EMPTY) break;
                     slot = (slot + 1) & (CAP - 1);
                 }
             }
@@ -487,13 +554,15 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_cs(
             b += red[1][k];
             p += red[2][k];
         }
+        // spread over kPartSlots addresses: no same-address atomic chain
+        unsigned long long* slot = part + 4 * (item % kPartSlots);
         if (c) {
-            atomicAdd(&result[0], c);
-            atomicAdd(&result[1], b);
-            atomicAdd(&result[2], p);
+            atomicAdd(&slot[0], c);
+            atomicAdd(&slot[1], b);
+            atomicAdd(&slot[2], p);
         }
         const unsigned long long pieces = (nR + kJoinPiece - 1) / kJoinPiece;
-        atomicMax(&result[3], pieces);
+        if (pieces > 1) atomicMax(&slot[3], pieces);
     }
 }
 
@@ -600,12 +669,18 @@ __global__ __launch_bounds__(256) void k_gprobe(const KT* __restrict__ sk, const
         unsigned long long c = red[0][0] + red[0][1] + red[0][2] + red[0][3];
         unsigned long long b = red[1][0] + red[1][1] + red[1][2] + red[1][3];
         unsigned long long p = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+        unsigned long long* slot = result + 4 * (blockIdx.x % kPartSlots);
         if (c) {
-            atomicAdd(&result[0], c);
-            atomicAdd(&result[1], b);
-            atomicAdd(&result[2], p);
+            atomicAdd(&slot[0], c);
+            atomicAdd(&slot[1], b);
+            atomicAdd(&slot[2], p);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_fill64(unsigned long long* __restrict__ p, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) p[i] = 0;
 }
 
 }  // namespace
@@ -622,6 +697,9 @@ struct pgpu_join_plan {
     uint32_t *HR = nullptr, *HS = nullptr, *totR = nullptr, *totS = nullptr, *icount = nullptr;
     uint64_t *baseR = nullptr, *baseS = nullptr, *off1R = nullptr, *off1S = nullptr, *ioff = nullptr;
     unsigned long long* result = nullptr;  // [4]: count, sum_build, sum_probe, max pieces
+    unsigned long long* part = nullptr;    // [kPartSlots][4] per-workgroup partials
+    uint2* items = nullptr;                // (partition, sub-range) per join item
+    uint64_t max_items = 0;
     unsigned long long* h_result = nullptr; // pinned
     void* gtable = nullptr;
     uint64_t gcap = 0;
@@ -706,7 +784,11 @@ int run_partition(pgpu_join_plan* p, const KT* keys, const KT* vals, uint64_t n,
     {
         snprintf(name, sizeof name, "scatter2_%s", tag);
         Timed tm(p->timer, name);
-        hipLaunchKernelGGL((k_scatter2<KT, HV, IPT>), dim3(nblk * nb1), dim3(kScatterThreads), 0, st, P1, nblk, B,
+        // group pass-1 blocks so a pass-2 unit averages about two tiles
+        const uint64_t T = tile_elems(sizeof(E));
+        const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nblk, 2 * T * nb1 / blk));
+        const uint32_t ngrp = (nblk + G - 1) / G;
+        hipLaunchKernelGGL((k_scatter2<KT, HV, IPT>), dim3(ngrp * nb1), dim3(kScatterThreads), 0, st, P1, nblk, G, B,
                            B2, off1, H, base, P2);
         PGPU_LAUNCH_CHECK();
     }
@@ -729,13 +811,17 @@ int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const 
         PGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_binscan, dim3(1), dim3(1024), 0, st, p->icount, nb, p->ioff);
         PGPU_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_items, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, p->icount, p->ioff, nb,
+                           p->items);
+        PGPU_LAUNCH_CHECK();
     }
     const uint64_t max_items = nb + ceil_div(nS, kItemRows) + 1;
+    if (max_items > p->max_items) return pgpu::fail(PGPU_EINVAL, "join items exceed the plan");
     {
         Timed tm(p->timer, "join_lds");
         hipLaunchKernelGGL((k_join_cs<KT, HVS>), dim3((unsigned)max_items), dim3(kJoinThreads), 0, st,
                            (const Elem<KT, true>*)p->R2, p->baseR, p->totR, (const Elem<KT, HVS>*)p->S2, p->baseS,
-                           p->totS, p->ioff, nb, kItemRows, B, p->result);
+                           p->totS, p->ioff, p->items, nb, kItemRows, B, p->part);
         PGPU_LAUNCH_CHECK();
     }
     return PGPU_OK;
@@ -764,7 +850,7 @@ int run_global(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const
     if (nS) {
         Timed tm(p->timer, "gprobe");
         hipLaunchKernelGGL((k_gprobe<KT, HVS>), dim3((unsigned)std::min<uint64_t>(ceil_div(nS, 256), 8192)),
-                           dim3(256), 0, st, sk, sv, nS, t, logcap, p->gspecial, p->result);
+                           dim3(256), 0, st, sk, sv, nS, t, logcap, p->gspecial, p->part);
         PGPU_LAUNCH_CHECK();
     }
     return PGPU_OK;
@@ -805,6 +891,8 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
     const bool want_global = cfg->algo != PGPU_ALGO_RADIX;
     int rc;
     if ((rc = plan_alloc(p, 4 * sizeof(unsigned long long), (void**)&p->result)) != PGPU_OK) return cleanup(rc);
+    if ((rc = plan_alloc(p, 4 * kPartSlots * sizeof(unsigned long long), (void**)&p->part)) != PGPU_OK)
+        return cleanup(rc);
     if (hipHostMalloc((void**)&p->h_result, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(PGPU_ENOMEM, "hipHostMalloc for the result failed"));
     if (want_radix) {
@@ -817,6 +905,7 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
         p->nblkR = std::max<uint32_t>(p->nblkR, 512);
         p->nblkS = std::max<uint32_t>(p->nblkS, 512);
         const uint64_t nb = p->nbmax;
+        p->max_items = nb + ceil_div(std::max<uint64_t>(max_probe, 1), kItemRows) + 1;
         struct {
             void** ptr;
             uint64_t bytes;
@@ -833,6 +922,7 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
             {(void**)&p->baseR, (nb + 1) * 8},
             {(void**)&p->baseS, (nb + 1) * 8},
             {(void**)&p->ioff, (nb + 1) * 8},
+            {(void**)&p->items, (nb + ceil_div(std::max<uint64_t>(max_probe, 1), kItemRows) + 1) * sizeof(uint2)},
             {(void**)&p->off1R, (uint64_t)(p->nblkR + 1) * 256 * 8},
             {(void**)&p->off1S, (uint64_t)(p->nblkS + 1) * 256 * 8},
         };
@@ -875,7 +965,7 @@ int pgpu_join_count_sum_async(pgpu_join_plan* p, const void* d_rkey, const void*
         return fail(PGPU_EINVAL, "pgpu_join_count_sum_async: null column");
     PGPU_TRY(ctx_activate(p->ctx));
     hipStream_t st = p->ctx->stream;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p->result, 4);
+    hipLaunchKernelGGL(k_fill64, dim3(8), dim3(256), 0, st, p->part, (uint64_t)4 * kPartSlots);
     PGPU_LAUNCH_CHECK();
     int algo = (int)p->cfg.algo;
     if (algo == PGPU_ALGO_AUTO) algo = (nR <= (1u << 16) || !p->R1) ? PGPU_ALGO_GLOBAL : PGPU_ALGO_RADIX;
@@ -913,6 +1003,8 @@ int pgpu_join_count_sum_async(pgpu_join_plan* p, const void* d_rkey, const void*
         }
     }
     if (rc != PGPU_OK) return rc;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, p->part, kPartSlots, p->result);
+    PGPU_LAUNCH_CHECK();
     PGPU_HIP(hipMemcpyAsync(p->h_result, p->result, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     p->pending = true;
     return PGPU_OK;

@@ -113,7 +113,7 @@ def test_empty_probe_is_finished(tmp_path):
 
 
 def agg_plan(tmp_path, f, schema, kind, threads, buckets, fields="field = 0;", sumfield=1, global_=True,
-             hashfield="field = 0;", fn="modulo", two_level=False):
+             hashfield="field = 0;", fn="modulo", two_level=False, buffsize=16):
     g = f"global = true; threads = {threads};" if global_ else ""
     sf = f"sumfield = {sumfield};" if kind == "gpuaggregate_sum" else ""
     if two_level:
@@ -122,7 +122,7 @@ def agg_plan(tmp_path, f, schema, kind, threads, buckets, fields="field = 0;", s
     else:
         tree = 'name = "merge"; input = { name = "chk"; input = { name = "agg"; input = { name = "scan"; }; }; };'
     return textwrap.dedent(f"""
-        path = "{tmp_path}/"; buffsize = 16;
+        path = "{tmp_path}/"; buffsize = {buffsize};
         treeroot = {{ {tree} }};
         merge = {{ type = "merge"; threads = {threads}; }};
         chk = {{ type = "checker_callstate"; }};
@@ -170,7 +170,7 @@ def test_composite_key_count(tmp_path):
     a, b = rng.integers(0, 6, 5000), rng.integers(0, 9, 5000)
     f = write_text(tmp_path, "c.tbl", zip(a, b, np.zeros(5000, int)))
     plan = agg_plan(tmp_path, f, '"int", "long", "long"', "gpuaggregate_count", 3, 128, fields="fields = ( 0, 1 );",
-                    hashfield="fieldrange = ( 0, 1 );", fn="bytes")
+                    hashfield="fieldrange = ( 0, 1 );", fn="bytes", buffsize=64)
     out, meta = P.run(plan)
     assert meta["schema"] == "int,long,long"
     got = {(int(x), int(y)): int(c) for x, y, c in zip(out["c0"], out["c1"], out["c2"])}
