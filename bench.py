@@ -183,6 +183,9 @@ def main():
         "kernels": per_kernel,
         "result": {"matches": res.matches, "sum_build": res.sum_build, "lds_passes": res.lds_passes},
     }
+    rd, cp = ctx.membw(1 << 30, 10)
+    line["hbm_calibration"] = {"read_GBps": rd, "copy_GBps": cp,
+                               "note": "streaming uint4 read / copy kernels on this device, same run"}
     if not args.no_cpu_baseline:
         plan.close()
         for b in (rk, rp, sk):

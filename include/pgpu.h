@@ -61,6 +61,9 @@ int pgpu_hbm_usage(pgpu_ctx* ctx, uint64_t* used, uint64_t* budget, uint64_t* pe
 int pgpu_h2d(pgpu_ctx* ctx, void* d_dst, const void* src, uint64_t bytes);
 int pgpu_d2h(pgpu_ctx* ctx, void* dst, const void* d_src, uint64_t bytes);
 int pgpu_sync(pgpu_ctx* ctx);
+/* Roofline calibration (no reference counterpart): sustained HBM GB/s of a
+ * streaming read and of a read + write copy over `bytes`, `iters` times. */
+int pgpu_membw(pgpu_ctx* ctx, uint64_t bytes, int iters, double* read_gbps, double* copy_gbps);
 
 /* ------------------------------------------------------------------ */
 /* Synthetic inputs generated in HBM (the formulas of pgpu_gen.h).      */

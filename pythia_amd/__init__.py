@@ -85,6 +85,12 @@ class Context:
         call("pgpu_ctx_stream", self.handle, C.byref(s))
         return s.value
 
+    def membw(self, nbytes: int = 1 << 30, iters: int = 10):
+        """(read GB/s, copy GB/s) this device sustains for streaming kernels."""
+        r, c = C.c_double(), C.c_double()
+        call("pgpu_membw", self.handle, nbytes, iters, C.byref(r), C.byref(c))
+        return r.value, c.value
+
     def hbm_usage(self):
         u, b, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
         call("pgpu_hbm_usage", self.handle, C.byref(u), C.byref(b), C.byref(p))

@@ -385,14 +385,33 @@ __global__ __launch_bounds__(256) void k_icount(const uint32_t* __restrict__ tot
     icount[pp] = totR[pp] ? (totS[pp] + M - 1) / M : 0u;
 }
 
-// items[ioff[pp] + i] = (pp, i): the join workgroup reads its partition and
-// sub-range with one load.
+// One join work item: a probe sub-range of one final partition and that
+// partition's build range, read by the join workgroup with one 32-byte load.
+struct __align__(16) JoinItem {
+    uint64_t sB;   // first probe row
+    uint64_t rB;   // first build row
+    uint32_t sN;   // probe rows
+    uint32_t nR;   // build rows
+    uint64_t pad;
+};
+
 __global__ __launch_bounds__(256) void k_items(const uint32_t* __restrict__ icount, const uint64_t* __restrict__ ioff,
-                                               uint32_t nb, uint2* __restrict__ items) {
+                                               const uint64_t* __restrict__ baseS, const uint32_t* __restrict__ totS,
+                                               const uint64_t* __restrict__ baseR, const uint32_t* __restrict__ totR,
+                                               uint32_t nb, uint32_t M, JoinItem* __restrict__ items) {
     const uint32_t pp = blockIdx.x * 256 + threadIdx.x;
     if (pp >= nb) return;
     const uint64_t o = ioff[pp];
-    for (uint32_t i = 0; i < icount[pp]; ++i) items[o + i] = make_uint2(pp, i);
+    const uint64_t s0 = baseS[pp], ns = totS[pp];
+    for (uint32_t i = 0; i < icount[pp]; ++i) {
+        JoinItem it;
+        it.sB = s0 + (uint64_t)i * M;
+        it.sN = (uint32_t)min<uint64_t>(M, ns - (uint64_t)i * M);
+        it.rB = baseR[pp];
+        it.nR = totR[pp];
+        it.pad = 0;
+        items[o + i] = it;
+    }
 }
 
 // Folds the per-slot partial results into result[0..3].
@@ -445,7 +464,7 @@ template <typename KT, bool HVS>
 __global__ __launch_bounds__(kJoinThreads) void k_join_cs(
     const Elem<KT, true>* __restrict__ R2, const uint64_t* __restrict__ baseR, const uint32_t* __restrict__ totR,
     const Elem<KT, HVS>* __restrict__ S2, const uint64_t* __restrict__ baseS, const uint32_t* __restrict__ totS,
-    const uint64_t* __restrict__ ioff, const uint2* __restrict__ items, uint32_t nb, uint32_t M, int B,
+    const uint64_t* __restrict__ ioff, const JoinItem* __restrict__ items, uint32_t nb, int B,
     unsigned long long* __restrict__ part) {
     constexpr int CAP = kJoinCap;
     constexpr int L = 12;  // log2(CAP)
@@ -462,15 +481,23 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_cs(
     const int tid = threadIdx.x;
     const uint64_t item = blockIdx.x;
     if (item >= ioff[nb]) return;
-    const uint2 it = items[item];
-    const uint32_t pp = it.x;
-    const uint64_t sub = it.y;
-    const uint64_t sB = baseS[pp] + sub * M;
-    const uint64_t sE = min(baseS[pp] + totS[pp], sB + M);
-    const uint64_t rB = baseR[pp];
-    const uint32_t nR = totR[pp];
+    const JoinItem it = items[item];
+    const uint64_t sB = it.sB, sE = it.sB + it.sN, rB = it.rB;
+    const uint32_t nR = it.nR;
     const int hs = 64 - B - L;  // slot bits sit just below the radix bits
     unsigned long long acc_c = 0, acc_b = 0, acc_p = 0;
+    constexpr int U = HVS ? 8 : 16;
+    Elem<KT, HVS> x[U];
+    auto fetch = [&](uint64_t j) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t idx = j + (uint64_t)u * kJoinThreads;
+            if (idx < sE) x[u] = S2[idx];
+        }
+    };
+    // The first probe batch does not depend on the table: fetch it before
+    // building so both latencies overlap.
+    if (nR) fetch(sB + tid);
 
     for (uint32_t p0 = 0; p0 < nR; p0 += kJoinPiece) {
         for (int i = tid; i < CAP; i += kJoinThreads) {
@@ -502,8 +529,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_cs(
             }
         }
         __syncthreads();
-        constexpr int U = HVS ? 8 : 16;
-        uint64_t j = sB + tid;
+        if (p0) fetch(sB + tid);  // later pieces re-read the item's probe rows
         auto probe = [&](const Elem<KT, HVS>& x) {
             const long long k = (long long)x.k;
             unsigned long long c = 0, sb = 0;
@@ -527,14 +553,13 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_cs(
             acc_b += sb;
             if constexpr (HVS) acc_p += c * (unsigned long long)(int64_t)x.v;
         };
-        for (; j + (U - 1) * kJoinThreads < sE; j += U * kJoinThreads) {
-            Elem<KT, HVS> x[U];
+        for (uint64_t j = sB + tid; j < sE;) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) x[u] = S2[j + u * kJoinThreads];
-#pragma unroll
-            for (int u = 0; u < U; ++u) probe(x[u]);
+            for (int u = 0; u < U; ++u)
+                if (j + (uint64_t)u * kJoinThreads < sE) probe(x[u]);
+            j += (uint64_t)U * kJoinThreads;
+            if (j < sE) fetch(j);
         }
-        for (; j < sE; j += kJoinThreads) probe(S2[j]);
         __syncthreads();
     }
     acc_c = wave_sum(acc_c);
@@ -698,7 +723,7 @@ struct pgpu_join_plan {
     uint64_t *baseR = nullptr, *baseS = nullptr, *off1R = nullptr, *off1S = nullptr, *ioff = nullptr;
     unsigned long long* result = nullptr;  // [4]: count, sum_build, sum_probe, max pieces
     unsigned long long* part = nullptr;    // [kPartSlots][4] per-workgroup partials
-    uint2* items = nullptr;                // (partition, sub-range) per join item
+    JoinItem* items = nullptr;             // one record per join work item
     uint64_t max_items = 0;
     unsigned long long* h_result = nullptr; // pinned
     void* gtable = nullptr;
@@ -811,8 +836,8 @@ int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const 
         PGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_binscan, dim3(1), dim3(1024), 0, st, p->icount, nb, p->ioff);
         PGPU_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_items, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, p->icount, p->ioff, nb,
-                           p->items);
+        hipLaunchKernelGGL(k_items, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, p->icount, p->ioff,
+                           p->baseS, p->totS, p->baseR, p->totR, nb, kItemRows, p->items);
         PGPU_LAUNCH_CHECK();
     }
     const uint64_t max_items = nb + ceil_div(nS, kItemRows) + 1;
@@ -821,7 +846,7 @@ int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const 
         Timed tm(p->timer, "join_lds");
         hipLaunchKernelGGL((k_join_cs<KT, HVS>), dim3((unsigned)max_items), dim3(kJoinThreads), 0, st,
                            (const Elem<KT, true>*)p->R2, p->baseR, p->totR, (const Elem<KT, HVS>*)p->S2, p->baseS,
-                           p->totS, p->ioff, p->items, nb, kItemRows, B, p->part);
+                           p->totS, p->ioff, p->items, nb, B, p->part);
         PGPU_LAUNCH_CHECK();
     }
     return PGPU_OK;
@@ -922,7 +947,7 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
             {(void**)&p->baseR, (nb + 1) * 8},
             {(void**)&p->baseS, (nb + 1) * 8},
             {(void**)&p->ioff, (nb + 1) * 8},
-            {(void**)&p->items, (nb + ceil_div(std::max<uint64_t>(max_probe, 1), kItemRows) + 1) * sizeof(uint2)},
+            {(void**)&p->items, (nb + ceil_div(std::max<uint64_t>(max_probe, 1), kItemRows) + 1) * sizeof(JoinItem)},
             {(void**)&p->off1R, (uint64_t)(p->nblkR + 1) * 256 * 8},
             {(void**)&p->off1S, (uint64_t)(p->nblkS + 1) * 256 * 8},
         };
