@@ -106,7 +106,6 @@ def main():
         sys.exit(2)
 
     import pythia_amd as pa
-    from oracle import oracle as O  # only for the Zipf CDF table of the input generator
 
     nR, nS = 1 << args.log_build, 1 << args.log_probe
     ctx = pa.Context(0)
@@ -114,7 +113,7 @@ def main():
     ctx.gen("C2_RKEY", nR, rk, seed=args.seed)
     ctx.gen("C2_RPAY", nR, rp, seed=args.seed)
     if args.zipf > 0:
-        cdf = ctx.upload(O.zipf_cdf(nR, args.zipf))
+        cdf = ctx.upload(pa.zipf_cdf(nR, args.zipf))
         ctx.gen("C2_SFK_ZIPF", nS, sk, seed=args.seed, n_build=nR, cdf=cdf)
     else:
         ctx.gen("C2_SFK_UNIFORM", nS, sk, seed=args.seed, n_build=nR)

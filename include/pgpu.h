@@ -99,6 +99,12 @@ typedef struct pgpu_gen_spec {
 } pgpu_gen_spec;
 
 int pgpu_gen(pgpu_ctx* ctx, int column, const pgpu_gen_spec* spec, uint64_t n, void* d_out);
+/* Host-side helpers of the same generators: the Zipf(s) CDF over n ranks
+ * (upload it and pass it as d_cdf), and `count` log-uniform sizes
+ * round(2^U(lo, hi)) of family `sub` starting at item `first` (C4 join
+ * sizes use sub 4, C5 table sizes 5 and probe batch sizes 6). */
+int pgpu_zipf_cdf(uint64_t n, double s, double* cdf);
+int pgpu_sizes(uint64_t seed, uint64_t sub, uint64_t first, uint64_t count, double lo, double hi, uint64_t* out);
 
 /* ------------------------------------------------------------------ */
 /* Equi-join with fused count + sum consumer, device-resident columns.  */

@@ -25,6 +25,21 @@ def device_count() -> int:
     return n.value if rc == 0 else 0
 
 
+def zipf_cdf(n: int, s: float) -> np.ndarray:
+    """Zipf(s) CDF over n ranks, as the generators' Zipf column expects it."""
+    out = np.empty(n, np.float64)
+    call("pgpu_zipf_cdf", n, s, out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def sizes(family: int, count: int, lo: float, hi: float, seed: int = 42, first: int = 0) -> np.ndarray:
+    """round(2^U(lo, hi)) sizes of a family (4: batched joins, 5: resident
+    tables, 6: probe batches)."""
+    out = np.empty(count, np.uint64)
+    call("pgpu_sizes", seed, family, first, count, lo, hi, out.ctypes.data_as(C.c_void_p))
+    return out
+
+
 class DeviceBuffer:
     """A budgeted HBM allocation owned by a Context."""
 

@@ -69,6 +69,9 @@ _PROTOS = {
     "pgpu_sync": (C.c_int, [C.c_void_p]),
     "pgpu_membw": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pgpu_gen": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(GenSpec), C.c_uint64, C.c_void_p]),
+    "pgpu_zipf_cdf": (C.c_int, [C.c_uint64, C.c_double, C.c_void_p]),
+    "pgpu_sizes": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double,
+                             C.c_void_p]),
     "pgpu_join_plan_create": (C.c_int, [C.c_void_p, C.POINTER(JoinCfg), C.c_uint64, C.c_uint64,
                                         C.POINTER(C.c_void_p)]),
     "pgpu_join_plan_destroy": (C.c_int, [C.c_void_p]),

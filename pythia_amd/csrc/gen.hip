@@ -97,3 +97,21 @@ extern "C" int pgpu_gen(pgpu_ctx* ctx, int column, const pgpu_gen_spec* spec, ui
             return fail(PGPU_EINVAL, "pgpu_gen: unknown column %d", column);
     }
 }
+
+// Host-only helpers (the formulas they use are not compiled for the device).
+#if !defined(__HIP_DEVICE_COMPILE__)
+extern "C" int pgpu_zipf_cdf(uint64_t n, double s, double* cdf) {
+    using namespace pgpu;
+    if (!cdf || n == 0 || !(s > 0)) return fail(PGPU_EINVAL, "pgpu_zipf_cdf: need n > 0, s > 0 and an output array");
+    pgpu_zipf_cdf_fill(cdf, n, s);
+    return PGPU_OK;
+}
+
+extern "C" int pgpu_sizes(uint64_t seed, uint64_t sub, uint64_t first, uint64_t count, double lo, double hi,
+                          uint64_t* out) {
+    using namespace pgpu;
+    if ((!out && count) || !(lo <= hi) || hi > 40) return fail(PGPU_EINVAL, "pgpu_sizes: bad argument");
+    for (uint64_t i = 0; i < count; ++i) out[i] = pgpu_loguniform_size(seed, sub, first + i, lo, hi);
+    return PGPU_OK;
+}
+#endif

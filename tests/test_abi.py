@@ -52,3 +52,21 @@ def test_error_path_without_device():
     rc = l.pgpu_join_plan_create(None, None, C.c_uint64(0), C.c_uint64(0), None)
     assert rc == -1
     assert b"null" in l.pgpu_last_error()
+
+
+def test_host_generator_helpers_match_oracle():
+    """The library's host-side Zipf CDF is the same array the oracle builds,
+    and log-uniform sizes stay in [2^lo, 2^hi] with a log-mean near the middle."""
+    import numpy as np
+
+    import pythia_amd as pa
+    from oracle import oracle as O
+
+    a, b = pa.zipf_cdf(5000, 1.0), O.zipf_cdf(5000, 1.0)
+    assert np.array_equal(a, b)
+    # closed form: CDF(1) = 1 / H_n for s = 1
+    assert abs(a[0] - 1.0 / np.sum(1.0 / np.arange(1, 5001))) < 1e-15
+    sz = pa.sizes(4, 4096, 10, 22)
+    assert sz.min() >= 1 << 10 and sz.max() <= 1 << 22
+    assert abs(np.log2(sz.astype(np.float64)).mean() - 16.0) < 0.2
+    assert np.array_equal(sz[100:200], pa.sizes(4, 100, 10, 22, first=100))
