@@ -87,7 +87,8 @@ enum pgpu_gen_column {
     PGPU_GEN_C3_F_PRICE,   /* double */
     PGPU_GEN_C4_RKEY,      /* int32, sub = join id */
     PGPU_GEN_C4_RPAY,      /* int32 */
-    PGPU_GEN_C4_SFK        /* int32 */
+    PGPU_GEN_C4_SFK,       /* int32 */
+    PGPU_GEN_C4_SPAY       /* int32 */
 };
 
 typedef struct pgpu_gen_spec {
@@ -163,6 +164,67 @@ typedef struct pgpu_kernel_time {
 } pgpu_kernel_time;
 int pgpu_join_plan_timing(pgpu_join_plan* plan, int enable);
 int pgpu_join_plan_times(pgpu_join_plan* plan, pgpu_kernel_time* out, int cap, int* n);
+
+/* ------------------------------------------------------------------ */
+/* Batched joins: many independent joins (count + sums, as above) of     */
+/* uneven sizes bound into one plan and run as one hipGraph: segmented   */
+/* radix partitioning of every join at once, then one persistent kernel  */
+/* pulling (join, partition) items from a work queue.                    */
+/* Stands in for running the queryhashjoin plan (HashJoinOp ->           */
+/* AggregateCount / AggregateSum, operators/join.cpp:396-562,            */
+/* aggregatesum.cpp:81-107) once per join; the reference has no batched  */
+/* form.                                                                 */
+/* ------------------------------------------------------------------ */
+typedef struct pgpu_batch_join {
+    const void* d_rkey; /* build keys (key_bytes wide)                     */
+    const void* d_rpay; /* build payloads (same width)                     */
+    uint64_t n_build;   /* < 2^32                                          */
+    const void* d_skey; /* probe keys                                      */
+    const void* d_spay; /* probe payloads (read when probe_payload != 0)   */
+    uint64_t n_probe;   /* < 2^32                                          */
+} pgpu_batch_join;
+
+typedef struct pgpu_batch_cfg {
+    uint32_t key_bytes;     /* 4 or 8                                               */
+    uint32_t probe_payload; /* 1: sum probe payloads of matches too                 */
+    uint32_t variant;       /* join workgroup shape: 1 = 4096-slot LDS table x 512   */
+                            /* threads, 2 = 8192 slots x 1024 threads; 0 = 2         */
+    uint32_t part_rows;     /* target build rows per partition; 0 = variant default */
+    uint32_t graph;         /* 1: capture the launch sequence into a hipGraph       */
+} pgpu_batch_cfg;
+
+typedef struct pgpu_batch_info_t {
+    pgpu_batch_cfg cfg;     /* the configuration in force (defaults resolved) */
+    uint32_t items;         /* work items of the persistent join             */
+    uint32_t grid;          /* persistent workgroups                          */
+    uint64_t build_rows, probe_rows, chunks;
+    uint32_t graph;         /* 1 if replayed as a hipGraph                    */
+} pgpu_batch_info_t;
+
+typedef struct pgpu_tune_entry {
+    uint32_t variant, part_rows;
+    double ms;              /* mean batch time of this candidate */
+} pgpu_tune_entry;
+
+typedef struct pgpu_batch_plan pgpu_batch_plan;
+/* The plan keeps the join descriptors' device pointers: the columns must
+ * stay allocated and keep their sizes for the plan's lifetime. */
+int pgpu_batch_plan_create(pgpu_ctx* ctx, const pgpu_batch_cfg* cfg, const pgpu_batch_join* joins, uint32_t n,
+                           pgpu_batch_plan** out);
+int pgpu_batch_plan_destroy(pgpu_batch_plan* plan);
+/* Enqueues the whole batch on the context stream (a graph launch). */
+int pgpu_batch_run_async(pgpu_batch_plan* plan);
+/* Waits for the batch; out (may be NULL) receives n results in join order. */
+int pgpu_batch_wait(pgpu_batch_plan* plan, pgpu_join_result* out);
+int pgpu_batch_info(pgpu_batch_plan* plan, pgpu_batch_info_t* info);
+/* Per-kernel HIP-event timing (runs the launches directly, not the graph). */
+int pgpu_batch_timing(pgpu_batch_plan* plan, int enable);
+int pgpu_batch_times(pgpu_batch_plan* plan, pgpu_kernel_time* out, int cap, int* n);
+/* Autotuner: times every (variant, part_rows) candidate on this batch
+ * (reps graph replays each after a warm-up) and returns the fastest in
+ * *best (key_bytes / probe_payload / graph taken from base). */
+int pgpu_batch_autotune(pgpu_ctx* ctx, const pgpu_batch_cfg* base, const pgpu_batch_join* joins, uint32_t n,
+                        int reps, pgpu_batch_cfg* best, pgpu_tune_entry* entries, int cap, int* n_entries);
 
 /* ------------------------------------------------------------------ */
 /* Row-format hash join with materialised output (gpuhashjoin).         */

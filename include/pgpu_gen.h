@@ -246,6 +246,10 @@ PGPU_HD int32_t pgpu_c4_rpay(uint64_t seed, uint64_t join, uint64_t i)
 {
     return (int32_t)(pgpu_rand(pgpu_stream_key(seed, PGPU_STREAM_RPAY, 16 + join), i) >> 33);
 }
+PGPU_HD int32_t pgpu_c4_spay(uint64_t seed, uint64_t join, uint64_t j)
+{
+    return (int32_t)(pgpu_rand(pgpu_stream_key(seed, PGPU_STREAM_SPAY, 16 + join), j) >> 33);
+}
 /* 10% of probe rows carry a key from [n, n + 2^30), never in the build. */
 PGPU_HD int32_t pgpu_c4_sfk(uint64_t seed, uint64_t join, uint64_t j, uint64_t n)
 {

@@ -1025,9 +1025,59 @@ void orc_c4_closed(uint64_t seed, uint64_t join, uint64_t n, uint64_t nprobe, or
         if (fk < 0 || (uint64_t)fk >= n) continue;
         r.count++;
         r.sum_build += (uint64_t)(int64_t)pgpu_c4_rpay(seed, join, (uint64_t)inv[fk]);
+        r.sum_probe += (uint64_t)(int64_t)pgpu_c4_spay(seed, join, j);
     }
     *out = r;
     free(inv);
+}
+
+void orc_gen_c4(uint64_t seed, uint64_t join, uint64_t n, uint64_t m, int32_t* rkey, int32_t* rpay, int32_t* sfk,
+                int32_t* spay)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        rkey[i] = pgpu_c4_rkey(seed, join, i, n);
+        rpay[i] = pgpu_c4_rpay(seed, join, i);
+    }
+    for (uint64_t j = 0; j < m; ++j) {
+        sfk[j] = pgpu_c4_sfk(seed, join, j, n);
+        spay[j] = pgpu_c4_spay(seed, join, j);
+    }
+}
+
+typedef struct {
+    uint64_t seed, first, count;
+    const uint64_t *n, *m;
+    orc_cs64* out;
+    uint64_t* next;
+    pthread_mutex_t* mu;
+} c4_many_arg;
+
+static void* c4_many_worker(void* p)
+{
+    c4_many_arg* a = (c4_many_arg*)p;
+    for (;;) {
+        pthread_mutex_lock(a->mu);
+        uint64_t i = (*a->next)++;
+        pthread_mutex_unlock(a->mu);
+        if (i >= a->count) break;
+        orc_c4_closed(a->seed, a->first + i, a->n[i], a->m[i], &a->out[i]);
+    }
+    return NULL;
+}
+
+void orc_c4_closed_many(uint64_t seed, uint64_t first, uint64_t count, const uint64_t* n, const uint64_t* m,
+                        int threads, orc_cs64* out)
+{
+    if (threads < 1) threads = 1;
+    pthread_mutex_t mu;
+    pthread_mutex_init(&mu, NULL);
+    uint64_t next = 0;
+    c4_many_arg arg = {seed, first, count, n, m, out, &next, &mu};
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+    for (int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, c4_many_worker, &arg);
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    pthread_mutex_destroy(&mu);
 }
 
 /* ===================================================================== */

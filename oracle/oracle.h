@@ -158,10 +158,17 @@ void orc_c2_closed(uint64_t seed, uint64_t sub, uint64_t nR, uint64_t nS, double
  * cents[ncust + 1]; returns the number of joined fact rows. */
 uint64_t orc_c3_closed(uint64_t seed, uint64_t nD, uint64_t nF, uint64_t norders, uint64_t ncust, int threads,
                        int64_t* cents);
-/* C4: count + sum of R.payload for join `join` of sizes n, 8n. */
+/* C4: count, sum of R.payload and sum of S.payload over the matches of join
+ * `join` (build n rows, probe nprobe rows). */
 void orc_c4_closed(uint64_t seed, uint64_t join, uint64_t n, uint64_t nprobe, orc_cs64* out);
+/* The same for joins first..first+count-1 with build sizes n[i] and probe
+ * sizes m[i], spread over `threads` threads. */
+void orc_c4_closed_many(uint64_t seed, uint64_t first, uint64_t count, const uint64_t* n, const uint64_t* m,
+                        int threads, orc_cs64* out);
 
 /* ---------------- generator wrappers (pgpu_gen.h) ---------------------- */
+void orc_gen_c4(uint64_t seed, uint64_t join, uint64_t n, uint64_t m, int32_t* rkey, int32_t* rpay, int32_t* sfk,
+                int32_t* spay);
 void orc_gen_c1(uint64_t seed, uint64_t nR, uint64_t nS, int32_t* rkey, int32_t* rpay, int32_t* sfk,
                 int32_t* spay);
 void orc_gen_c2(uint64_t seed, uint64_t sub, uint64_t nR, uint64_t nS, double zipf_s, int64_t* rkey,

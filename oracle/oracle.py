@@ -113,10 +113,12 @@ def lib():
             "orc_c3_closed": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int,
                                            P]),
             "orc_c4_closed": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(CS64)]),
+            "orc_c4_closed_many": (None, [C.c_uint64, C.c_uint64, C.c_uint64, P, P, C.c_int, C.POINTER(CS64)]),
             "orc_gen_c1": (None, [C.c_uint64, C.c_uint64, C.c_uint64, P, P, P, P]),
             "orc_gen_c2": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, P, P, P]),
             "orc_gen_c3": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, P, P, P, P]),
             "orc_zipf_cdf": (None, [C.c_uint64, C.c_double, P]),
+            "orc_gen_c4": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, P, P, P, P]),
             "orc_baseline_join": (C.c_double, [C.c_int, P, P, C.c_uint64, P, C.c_uint64, C.c_uint, C.c_uint,
                                                C.c_int, C.POINTER(CS64)]),
         }
@@ -263,6 +265,12 @@ def gen_c3(nD, nF, norders=15_000_000, seed=42):
     return dk, dc, fk, fp
 
 
+def gen_c4(join, n, m, seed=42):
+    rk, rp, sk, sp = (np.empty(x, np.int32) for x in (n, n, m, m))
+    lib().orc_gen_c4(seed, join, n, m, _ptr(rk), _ptr(rp), _ptr(sk), _ptr(sp))
+    return rk, rp, sk, sp
+
+
 def zipf_cdf(n, s):
     cdf = np.empty(n, np.float64)
     lib().orc_zipf_cdf(n, s, _ptr(cdf))
@@ -291,6 +299,15 @@ def c4_closed(join, n, nprobe, seed=42) -> CS64:
     r = CS64()
     lib().orc_c4_closed(seed, join, n, nprobe, C.byref(r))
     return r
+
+
+def c4_closed_many(first, n, m, seed=42, threads=8):
+    """Closed forms of joins first.. with build sizes n[i], probe sizes m[i]."""
+    n = np.ascontiguousarray(n, dtype=np.uint64)
+    m = np.ascontiguousarray(m, dtype=np.uint64)
+    out = (CS64 * len(n))()
+    lib().orc_c4_closed_many(seed, first, len(n), _ptr(n), _ptr(m), threads, out)
+    return list(out)
 
 
 def baseline_join(kb, rk, rp, sk, buckets, tuplesperbucket, threads):

@@ -15,8 +15,8 @@ import numpy as np
 from . import _lib
 from ._lib import ALGO_AUTO, ALGO_GLOBAL, ALGO_RADIX, GEN, PgpuError, call, lib  # noqa: F401
 
-__all__ = ["Context", "DeviceBuffer", "JoinPlan", "GroupJoin", "device_count", "PgpuError", "GEN",
-           "ALGO_AUTO", "ALGO_RADIX", "ALGO_GLOBAL"]
+__all__ = ["Context", "DeviceBuffer", "JoinPlan", "GroupJoin", "BatchPlan", "batch_autotune", "device_count",
+           "PgpuError", "GEN", "ALGO_AUTO", "ALGO_RADIX", "ALGO_GLOBAL"]
 
 
 def device_count() -> int:
@@ -198,3 +198,72 @@ class GroupJoin:
 
     def times(self):
         return _times("pgpu_groupjoin_times", self.handle)
+
+
+def _batch_array(joins):
+    """joins: sequence of (rkey, rpay, n_build, skey, spay, n_probe) with
+    DeviceBuffer / raw int device pointers / None."""
+    def ptr(b):
+        if b is None:
+            return None
+        return b.ptr if isinstance(b, DeviceBuffer) else int(b)
+
+    arr = (_lib.BatchJoin * len(joins))()
+    for i, (rk, rp, nb, sk, sp, ns) in enumerate(joins):
+        arr[i] = _lib.BatchJoin(ptr(rk), ptr(rp), nb, ptr(sk), ptr(sp), ns)
+    return arr
+
+
+class BatchPlan:
+    """Many independent joins bound into one plan (one hipGraph replay per run)."""
+
+    def __init__(self, ctx: Context, joins, key_bytes: int = 4, probe_payload: bool = True, variant: int = 0,
+                 part_rows: int = 0, graph: bool = True):
+        self.ctx = ctx
+        self.n = len(joins)
+        self._arr = _batch_array(joins)
+        cfg = _lib.BatchCfg(key_bytes, 1 if probe_payload else 0, variant, part_rows, 1 if graph else 0)
+        h = C.c_void_p()
+        call("pgpu_batch_plan_create", ctx.handle, C.byref(cfg), self._arr, self.n, C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().pgpu_batch_plan_destroy(self.handle)
+            self.handle = None
+
+    def enqueue(self):
+        call("pgpu_batch_run_async", self.handle)
+
+    def wait(self):
+        out = (_lib.JoinResult * self.n)()
+        call("pgpu_batch_wait", self.handle, out)
+        return list(out)
+
+    def run(self):
+        self.enqueue()
+        return self.wait()
+
+    def info(self):
+        i = _lib.BatchInfo()
+        call("pgpu_batch_info", self.handle, C.byref(i))
+        return i
+
+    def timing(self, on: bool):
+        call("pgpu_batch_timing", self.handle, 1 if on else 0)
+
+    def times(self):
+        return _times("pgpu_batch_times", self.handle)
+
+
+def batch_autotune(ctx: Context, joins, key_bytes: int = 4, probe_payload: bool = True, reps: int = 2):
+    """Times every (variant, part_rows) candidate on this batch; returns
+    (best (variant, part_rows), [(variant, part_rows, ms), ...])."""
+    arr = _batch_array(joins)
+    base = _lib.BatchCfg(key_bytes, 1 if probe_payload else 0, 0, 0, 1)
+    best = _lib.BatchCfg()
+    ent = (_lib.TuneEntry * 16)()
+    n = C.c_int()
+    call("pgpu_batch_autotune", ctx.handle, C.byref(base), arr, len(joins), reps, C.byref(best), ent, 16,
+         C.byref(n))
+    return (best.variant, best.part_rows), [(ent[i].variant, ent[i].part_rows, ent[i].ms) for i in range(n.value)]

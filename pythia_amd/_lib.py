@@ -20,7 +20,7 @@ STATUS = {
 GEN = {name: i for i, name in enumerate([
     "C1_RKEY", "C1_RPAY", "C1_SFK", "C1_SPAY", "C2_RKEY", "C2_RPAY", "C2_SFK_UNIFORM", "C2_SFK_ZIPF",
     "C3_D_ORDERKEY", "C3_D_CUSTKEY", "C3_F_ORDERKEY", "C3_F_PARTKEY", "C3_F_QUANTITY", "C3_F_PRICE",
-    "C4_RKEY", "C4_RPAY", "C4_SFK"], start=1)}
+    "C4_RKEY", "C4_RPAY", "C4_SFK", "C4_SPAY"], start=1)}
 
 ALGO_AUTO, ALGO_RADIX, ALGO_GLOBAL = 0, 1, 2
 
@@ -38,6 +38,25 @@ class JoinCfg(C.Structure):
 class JoinResult(C.Structure):
     _fields_ = [("matches", C.c_uint64), ("sum_build", C.c_uint64), ("sum_probe", C.c_uint64),
                 ("radix_bits", C.c_uint32), ("lds_passes", C.c_uint32)]
+
+
+class BatchJoin(C.Structure):
+    _fields_ = [("d_rkey", C.c_void_p), ("d_rpay", C.c_void_p), ("n_build", C.c_uint64),
+                ("d_skey", C.c_void_p), ("d_spay", C.c_void_p), ("n_probe", C.c_uint64)]
+
+
+class BatchCfg(C.Structure):
+    _fields_ = [("key_bytes", C.c_uint32), ("probe_payload", C.c_uint32), ("variant", C.c_uint32),
+                ("part_rows", C.c_uint32), ("graph", C.c_uint32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("cfg", BatchCfg), ("items", C.c_uint32), ("grid", C.c_uint32), ("build_rows", C.c_uint64),
+                ("probe_rows", C.c_uint64), ("chunks", C.c_uint64), ("graph", C.c_uint32)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("variant", C.c_uint32), ("part_rows", C.c_uint32), ("ms", C.c_double)]
 
 
 class KernelTime(C.Structure):
@@ -82,6 +101,16 @@ _PROTOS = {
                                       C.c_void_p, C.c_uint64, C.POINTER(JoinResult)]),
     "pgpu_join_plan_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "pgpu_join_plan_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),
+    "pgpu_batch_plan_create": (C.c_int, [C.c_void_p, C.POINTER(BatchCfg), C.POINTER(BatchJoin), C.c_uint32,
+                                         C.POINTER(C.c_void_p)]),
+    "pgpu_batch_plan_destroy": (C.c_int, [C.c_void_p]),
+    "pgpu_batch_run_async": (C.c_int, [C.c_void_p]),
+    "pgpu_batch_wait": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pgpu_batch_info": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
+    "pgpu_batch_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "pgpu_batch_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),
+    "pgpu_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(BatchCfg), C.POINTER(BatchJoin), C.c_uint32, C.c_int,
+                                      C.POINTER(BatchCfg), C.POINTER(TuneEntry), C.c_int, C.POINTER(C.c_int)]),
     "pgpu_groupjoin_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pgpu_groupjoin_destroy": (C.c_int, [C.c_void_p]),
     "pgpu_groupjoin_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,

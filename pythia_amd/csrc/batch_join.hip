@@ -1,0 +1,570 @@
+// Batched equi-joins: many independent (build, probe) pairs of uneven sizes
+// joined with fused count + sum consumers in one launch sequence.
+//
+// Reference path replaced, once per join: the queryhashjoin plan shape
+// (unit_tests/queryhashjoin.cpp) of HashJoinOp build + probe
+// (operators/join.cpp:396-464, :503-562, :641-667) over a chained-bucket
+// HashTable (util/hashtable.cpp:230-273), feeding AggregateCount /
+// AggregateSum (operators/aggregatecount.cpp:37-52,
+// operators/aggregatesum.cpp:81-107).  The reference runs one such plan per
+// query; here a whole batch shares every launch:
+//
+//   k_bhist     one block per chunk of kChunkRows rows of any side of any
+//               join: LDS histogram over that join's 2^bits partitions
+//               (bits from the join's build size, one pass, <= 12 bits)
+//   k_bscan     one block per side of a join: exclusive scans of the
+//               histograms -> each chunk's write offsets + partition bounds
+//   k_bscatter  one block per chunk: tiles reordered by partition in LDS and
+//               written as contiguous runs at exact offsets (no atomics)
+//   k_bjoin     persistent: every workgroup pulls (join, partition) items
+//               from one atomic queue, builds the partition's LDS table and
+//               streams its probe rows; per-join results by atomics
+//
+// The batch plan captures the sequence once into a hipGraph and replays it.
+// The autotuner times every (workgroup shape, partition size) candidate on
+// the batch itself and keeps the fastest.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "batch_internal.h"
+#include "join_common.h"
+
+using namespace pgpu::dev;
+using pgpu::batch::Chunk;
+using pgpu::batch::Job;
+using pgpu::batch::kChunkRows;
+using pgpu::batch::kMaxBits;
+using pgpu::batch::kScatterTileBytes;
+using pgpu::batch::Seg;
+
+namespace {
+
+constexpr int kND = 1 << kMaxBits;
+
+template <typename KT>
+__global__ __launch_bounds__(512) void k_bhist(const Seg* __restrict__ segs, const Chunk* __restrict__ chunks,
+                                               uint32_t chunk_base, uint32_t* __restrict__ H) {
+    __shared__ uint32_t h[kND];
+    const uint32_t ci = chunk_base + blockIdx.x;
+    const Chunk c = chunks[ci];
+    const Seg s = segs[c.seg];
+    const uint64_t r0 = c.row0, r1 = min<uint64_t>(s.n, r0 + kChunkRows);
+    const uint32_t nd = 1u << s.bits;
+    uint32_t* out = H + s.hist + (uint64_t)(ci - s.chunk0) * nd;
+    if (s.bits == 0) {
+        if (threadIdx.x == 0) out[0] = (uint32_t)(r1 - r0);
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < nd; i += 512) h[i] = 0;
+    __syncthreads();
+    const KT* key = (const KT*)s.key;
+    const int sh = 64 - (int)s.bits;
+    constexpr int G = 16 / sizeof(KT);
+    uint64_t r = r0;
+    if (((uintptr_t)key & 15) == 0) {
+        const uint64_t ng = (r1 - r0) / G;
+        const uint4* kv = (const uint4*)(key + r0);
+#pragma unroll 4
+        for (uint64_t g = threadIdx.x; g < ng; g += 512) {
+            const uint4 q = kv[g];
+            KT ks[G];
+            memcpy(ks, &q, 16);
+#pragma unroll
+            for (int u = 0; u < G; ++u) atomicAdd(&h[hmix(ks[u]) >> sh], 1u);
+        }
+        r = r0 + ng * G;
+    }
+    for (r += threadIdx.x; r < r1; r += 512) atomicAdd(&h[hmix(key[r]) >> sh], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nd; i += 512) out[i] = h[i];
+}
+
+__global__ __launch_bounds__(1024) void k_bscan(const Seg* __restrict__ segs, uint32_t seg_base,
+                                                uint32_t* __restrict__ H, uint32_t* __restrict__ bnd) {
+    __shared__ uint32_t tot[kND];
+    __shared__ uint32_t ws[1024 / 64 + 1];
+    const Seg s = segs[seg_base + blockIdx.x];
+    const uint32_t nd = 1u << s.bits, nc = s.nchunks;
+    uint32_t* h = H + s.hist;
+    for (uint32_t d = threadIdx.x; d < nd; d += 1024) {
+        uint32_t t = 0;
+#pragma unroll 8
+        for (uint32_t c = 0; c < nc; ++c) t += h[(uint64_t)c * nd + d];
+        tot[d] = t;
+    }
+    __syncthreads();
+    constexpr int PER = kND / 1024;
+    uint32_t loc[PER], sum = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const uint32_t idx = threadIdx.x * PER + u;
+        loc[u] = idx < nd ? tot[idx] : 0u;
+        sum += loc[u];
+    }
+    uint32_t total;
+    uint32_t ex = block_excl_scan<1024>(sum, ws, &total);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const uint32_t idx = threadIdx.x * PER + u;
+        if (idx < nd) tot[idx] = ex;
+        ex += loc[u];
+    }
+    __syncthreads();
+    uint32_t* b = bnd + s.bnd;
+    for (uint32_t d = threadIdx.x; d < nd; d += 1024) {
+        uint32_t run = tot[d];
+        b[d] = run;
+#pragma unroll 8
+        for (uint32_t c = 0; c < nc; ++c) {
+            const uint32_t v = h[(uint64_t)c * nd + d];
+            h[(uint64_t)c * nd + d] = run;
+            run += v;
+        }
+    }
+    if (threadIdx.x == 0) b[nd] = (uint32_t)s.n;
+}
+
+// Loads rows [g*G, g*G+G) of a chunk-relative group into k/v (bounds per row).
+template <typename KT, bool HV>
+__device__ __forceinline__ void load_group(const KT* key, const KT* pay, bool vec, uint64_t row, uint64_t r1, KT* k,
+                                           KT* v) {
+    constexpr int G = 16 / sizeof(KT);
+    if (vec && row + G <= r1) {
+        const uint4 q = *(const uint4*)(key + row);
+        memcpy(k, &q, 16);
+        if constexpr (HV) {
+            if (pay) {
+                const uint4 p = *(const uint4*)(pay + row);
+                memcpy(v, &p, 16);
+            } else {
+#pragma unroll
+                for (int u = 0; u < G; ++u) v[u] = 0;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (row + u < r1) {
+                k[u] = key[row + u];
+                if constexpr (HV) v[u] = pay ? pay[row + u] : (KT)0;
+            }
+        }
+    }
+}
+
+template <typename KT, bool HV>
+__global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs, const Chunk* __restrict__ chunks,
+                                                   uint32_t chunk_base, const uint32_t* __restrict__ H,
+                                                   Row<KT, HV>* __restrict__ out) {
+    using E = Row<KT, HV>;
+    constexpr int T = 1024;
+    constexpr int G = 16 / sizeof(KT);            // rows per 16-byte key load
+    constexpr int TR = kScatterTileBytes / sizeof(E);  // rows per tile
+    constexpr int NG = TR / (T * G);              // row groups per thread per tile
+    static_assert(NG >= 1 && TR % (T * G) == 0, "tile shape");
+    __shared__ E stage[TR];
+    __shared__ uint32_t cnt[kND], start[kND], cur[kND];
+    __shared__ uint32_t ws[T / 64 + 1];
+    const uint32_t ci = chunk_base + blockIdx.x;
+    const Chunk c = chunks[ci];
+    const Seg s = segs[c.seg];
+    const uint64_t r0 = c.row0, r1 = min<uint64_t>(s.n, r0 + kChunkRows);
+    const uint32_t nd = 1u << s.bits;
+    const KT* key = (const KT*)s.key;
+    const KT* pay = (const KT*)s.pay;
+    E* dst = out + s.dst;
+    const uint32_t* h = H + s.hist + (uint64_t)(ci - s.chunk0) * nd;
+    const bool vec = ((uintptr_t)key & 15) == 0 && (!pay || ((uintptr_t)pay & 15) == 0);
+    if (s.bits == 0) {  // one partition: copy into the row layout
+        const uint32_t base = h[0];
+        for (uint64_t g = r0 + (uint64_t)threadIdx.x * G; g < r1; g += (uint64_t)T * G) {
+            KT k[G], v[G];
+            load_group<KT, HV>(key, pay, vec, g, r1, k, v);
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (g + u < r1) {
+                    E e;
+                    e.k = k[u];
+                    if constexpr (HV) e.v = v[u];
+                    dst[base + (g + u - r0)] = e;
+                }
+            }
+        }
+        return;
+    }
+    for (uint32_t d = threadIdx.x; d < nd; d += T) {
+        cur[d] = h[d];
+        cnt[d] = 0;
+    }
+    __syncthreads();
+    const int sh = 64 - (int)s.bits;
+    constexpr int PER = kND / T;
+    for (uint64_t t0 = r0; t0 < r1; t0 += TR) {
+        const uint64_t te = min<uint64_t>(r1, t0 + TR);
+        const uint32_t tn = (uint32_t)(te - t0);
+        KT k[NG][G], v[NG][G];
+        uint32_t rk[NG][G];
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const uint64_t row = t0 + ((uint64_t)q * T + threadIdx.x) * G;
+            load_group<KT, HV>(key, pay, vec, row, te, k[q], v[q]);
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (row + u < te) rk[q][u] = atomicAdd(&cnt[hmix(k[q][u]) >> sh], 1u);
+        }
+        __syncthreads();
+        uint32_t loc[PER], sum = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const uint32_t idx = threadIdx.x * PER + u;
+            loc[u] = idx < nd ? cnt[idx] : 0u;
+            sum += loc[u];
+        }
+        uint32_t total;
+        uint32_t ex = block_excl_scan<T>(sum, ws, &total);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const uint32_t idx = threadIdx.x * PER + u;
+            if (idx < nd) start[idx] = ex;
+            ex += loc[u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const uint64_t row = t0 + ((uint64_t)q * T + threadIdx.x) * G;
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (row + u < te) {
+                    E e;
+                    e.k = k[q][u];
+                    if constexpr (HV) e.v = v[q][u];
+                    stage[start[hmix(e.k) >> sh] + rk[q][u]] = e;
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t pos = threadIdx.x; pos < tn; pos += T) {
+            const E e = stage[pos];
+            const uint32_t d = (uint32_t)(hmix(e.k) >> sh);
+            dst[cur[d] + (pos - start[d])] = e;
+        }
+        __syncthreads();
+        for (uint32_t d = threadIdx.x; d < nd; d += T) {
+            cur[d] += cnt[d];
+            cnt[d] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// LDS table slot: key, match count and folded payload sum in 16 bytes for
+// int32 keys; int64 keys keep their counts in a packed 16-bit array.
+template <typename KT>
+struct LSlot;
+template <>
+struct __align__(16) LSlot<int32_t> {
+    int32_t key;
+    uint32_t cnt;
+    unsigned long long sum;
+};
+template <>
+struct __align__(16) LSlot<int64_t> {
+    long long key;
+    unsigned long long sum;
+};
+
+template <int CAP>
+struct Log2;
+template <>
+struct Log2<4096> {
+    static constexpr int v = 12;
+};
+template <>
+struct Log2<8192> {
+    static constexpr int v = 13;
+};
+
+template <typename KT, bool HVS, int CAP, int T>
+__global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const uint32_t* __restrict__ item_job,
+                                             uint32_t n_items, const Row<KT, true>* __restrict__ R,
+                                             const Row<KT, HVS>* __restrict__ S, const uint32_t* __restrict__ bndR,
+                                             const uint32_t* __restrict__ bndS, uint32_t* __restrict__ queue,
+                                             unsigned long long* __restrict__ res) {
+    constexpr int L = Log2<CAP>::v;
+    constexpr uint32_t PIECE = CAP / 2;  // load factor <= 0.5
+    constexpr bool W = sizeof(KT) == 8;
+    static_assert(PIECE < 65536, "16-bit LDS match counters");
+    constexpr KT EMPTY = KeyEmpty<KT>::v;
+    using SR = Row<KT, HVS>;
+    constexpr int U = sizeof(SR) >= 16 ? 8 : 16;  // probe rows prefetched per thread
+    __shared__ LSlot<KT> tab[CAP];
+    __shared__ uint32_t tc[W ? CAP / 2 : 1];
+    __shared__ unsigned long long sp_cnt, sp_sum;
+    __shared__ unsigned long long red[3][T / 64];
+    __shared__ uint32_t s_item;
+    const int tid = threadIdx.x;
+    while (true) {
+        if (tid == 0) s_item = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= n_items) break;
+        const uint32_t j = item_job[item];
+        const Job jb = jobs[j];
+        const uint32_t d = item - jb.item0;
+        const uint64_t rb = jb.rdst + bndR[jb.rbnd + d];
+        const uint32_t nR = bndR[jb.rbnd + d + 1] - bndR[jb.rbnd + d];
+        const uint64_t sb = jb.sdst + bndS[jb.sbnd + d];
+        const uint64_t se = jb.sdst + bndS[jb.sbnd + d + 1];
+        const int hs = 64 - (int)jb.bits - L;
+        unsigned long long acc_c = 0, acc_b = 0, acc_p = 0;
+        SR x[U];
+        auto fetch = [&](uint64_t q) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t idx = q + (uint64_t)u * T;
+                if (idx < se) x[u] = S[idx];
+            }
+        };
+        const bool work = nR && se > sb;
+        if (work) fetch(sb + tid);
+        for (uint32_t p0 = 0; work && p0 < nR; p0 += PIECE) {
+            for (int i = tid; i < CAP; i += T) {
+                tab[i].key = EMPTY;
+                tab[i].sum = 0;
+                if constexpr (W) {
+                    if (i < CAP / 2) tc[i] = 0;
+                } else {
+                    tab[i].cnt = 0;
+                }
+            }
+            if (tid == 0) sp_cnt = sp_sum = 0;
+            __syncthreads();
+            const uint32_t pn = min(PIECE, nR - p0);
+            for (uint32_t i = tid; i < pn; i += T) {
+                const Row<KT, true> r = R[rb + p0 + i];
+                if (r.k == EMPTY) {
+                    atomicAdd(&sp_cnt, 1ull);
+                    atomicAdd(&sp_sum, (unsigned long long)(int64_t)r.v);
+                    continue;
+                }
+                uint32_t slot = (uint32_t)(hmix(r.k) >> hs) & (CAP - 1);
+                while (true) {
+                    KT prev;
+                    if constexpr (W)
+                        prev = (KT)atomicCAS((unsigned long long*)&tab[slot].key, (unsigned long long)EMPTY,
+                                             (unsigned long long)r.k);
+                    else
+                        prev = (KT)atomicCAS((int*)&tab[slot].key, (int)EMPTY, (int)r.k);
+                    if (prev == EMPTY || prev == r.k) {
+                        if constexpr (W)
+                            atomicAdd(&tc[slot >> 1], 1u << (16 * (slot & 1)));
+                        else
+                            atomicAdd(&tab[slot].cnt, 1u);
+                        atomicAdd(&tab[slot].sum, (unsigned long long)(int64_t)r.v);
+                        break;
+                    }
+                    slot = (slot + 1) & (CAP - 1);
+                }
+            }
+            __syncthreads();
+            if (p0) fetch(sb + tid);
+            auto probe = [&](const SR& e) {
+                unsigned long long c = 0, sbv = 0;
+                if (e.k == EMPTY) {
+                    c = sp_cnt;
+                    sbv = sp_sum;
+                } else {
+                    uint32_t slot = (uint32_t)(hmix(e.k) >> hs) & (CAP - 1);
+                    while (true) {
+                        const LSlot<KT> t = tab[slot];
+                        if (t.key == e.k) {
+                            if constexpr (W)
+                                c = (tc[slot >> 1] >> (16 * (slot & 1))) & 0xFFFFu;
+                            else
+                                c = t.cnt;
+                            sbv = t.sum;
+                            break;
+                        }
+                        if (t.key == EMPTY) break;
+                        slot = (slot + 1) & (CAP - 1);
+                    }
+                }
+                acc_c += c;
+                acc_b += sbv;
+                if constexpr (HVS) acc_p += c * (unsigned long long)(int64_t)e.v;
+            };
+            for (uint64_t q = sb + tid; q < se;) {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (q + (uint64_t)u * T < se) probe(x[u]);
+                q += (uint64_t)U * T;
+                if (q < se) fetch(q);
+            }
+            __syncthreads();
+        }
+        acc_c = wave_sum(acc_c);
+        acc_b = wave_sum(acc_b);
+        acc_p = wave_sum(acc_p);
+        const int lane = tid & 63, w = tid >> 6;
+        if (lane == 0) {
+            red[0][w] = acc_c;
+            red[1][w] = acc_b;
+            red[2][w] = acc_p;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long cc = 0, bb = 0, pp = 0;
+            for (int k = 0; k < T / 64; ++k) {
+                cc += red[0][k];
+                bb += red[1][k];
+                pp += red[2][k];
+            }
+            unsigned long long* o = res + 4 * (uint64_t)j;
+            if (cc) {
+                atomicAdd(&o[0], cc);
+                atomicAdd(&o[1], bb);
+                atomicAdd(&o[2], pp);
+            }
+            const unsigned long long pieces = work ? (nR + PIECE - 1) / PIECE : 0;
+            if (pieces > 1) atomicMax(&o[3], pieces);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bzero(unsigned long long* __restrict__ p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = 0;
+}
+
+template <typename KT, bool HVS, int CAP, int T>
+int launch_join_t(hipStream_t st, int grid, const Job* jobs, const uint32_t* item_job, uint32_t n_items,
+                  const void* R, const void* S, const uint32_t* bndR, const uint32_t* bndS, uint32_t* queue,
+                  unsigned long long* res) {
+    hipLaunchKernelGGL((k_bjoin<KT, HVS, CAP, T>), dim3(grid), dim3(T), 0, st, jobs, item_job, n_items,
+                       (const Row<KT, true>*)R, (const Row<KT, HVS>*)S, bndR, bndS, queue, res);
+    PGPU_LAUNCH_CHECK();
+    return PGPU_OK;
+}
+
+template <typename KT, bool HVS, int CAP, int T>
+int occupancy_t(int* blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, (const void*)k_bjoin<KT, HVS, CAP, T>, T, 0) ==
+                   hipSuccess
+               ? PGPU_OK
+               : pgpu::fail(PGPU_EDEVICE, "occupancy query failed");
+}
+
+}  // namespace
+
+namespace pgpu {
+namespace batch {
+
+void Layout::add(const void* key, const void* pay, uint64_t n, uint32_t bits, uint32_t chunk_base) {
+    Seg s{};
+    s.key = key;
+    s.pay = pay;
+    s.n = n;
+    s.dst = rows;
+    s.hist = hist;
+    s.bnd = bnd;
+    s.bits = bits;
+    s.chunk0 = chunk_base + (uint32_t)chunks.size();
+    s.nchunks = (uint32_t)ceil_div(n, kChunkRows);
+    for (uint32_t c = 0; c < s.nchunks; ++c) chunks.push_back(Chunk{(uint32_t)segs.size(), 0, (uint64_t)c * kChunkRows});
+    rows += n;
+    hist += (uint64_t)s.nchunks << bits;
+    bnd += (1ull << bits) + 1;
+    segs.push_back(s);
+}
+
+int variant_piece(int variant) { return variant == 1 ? 2048 : 4096; }
+uint32_t variant_target(int variant) { return variant == 1 ? 1536 : 3072; }
+
+uint32_t bits_for(uint64_t n, uint32_t target) {
+    if (n <= target) return 0;
+    return std::min<uint32_t>(kMaxBits, ilog2_ceil(ceil_div(n, target)));
+}
+
+int launch_hist(hipStream_t st, int kb, const Seg* segs, const Chunk* chunks, uint32_t chunk_base, uint32_t n,
+                uint32_t* H) {
+    if (!n) return PGPU_OK;
+    if (kb == 8)
+        hipLaunchKernelGGL(k_bhist<int64_t>, dim3(n), dim3(512), 0, st, segs, chunks, chunk_base, H);
+    else
+        hipLaunchKernelGGL(k_bhist<int32_t>, dim3(n), dim3(512), 0, st, segs, chunks, chunk_base, H);
+    PGPU_LAUNCH_CHECK();
+    return PGPU_OK;
+}
+
+int launch_scan(hipStream_t st, const Seg* segs, uint32_t seg_base, uint32_t nsegs, uint32_t* H, uint32_t* bnd) {
+    if (!nsegs) return PGPU_OK;
+    hipLaunchKernelGGL(k_bscan, dim3(nsegs), dim3(1024), 0, st, segs, seg_base, H, bnd);
+    PGPU_LAUNCH_CHECK();
+    return PGPU_OK;
+}
+
+int launch_scatter(hipStream_t st, int kb, bool hv, const Seg* segs, const Chunk* chunks, uint32_t chunk_base,
+                   uint32_t n, const uint32_t* H, void* out) {
+    if (!n) return PGPU_OK;
+    if (kb == 8) {
+        if (hv)
+            hipLaunchKernelGGL((k_bscatter<int64_t, true>), dim3(n), dim3(1024), 0, st, segs, chunks, chunk_base, H,
+                               (Row<int64_t, true>*)out);
+        else
+            hipLaunchKernelGGL((k_bscatter<int64_t, false>), dim3(n), dim3(1024), 0, st, segs, chunks, chunk_base, H,
+                               (Row<int64_t, false>*)out);
+    } else {
+        if (hv)
+            hipLaunchKernelGGL((k_bscatter<int32_t, true>), dim3(n), dim3(1024), 0, st, segs, chunks, chunk_base, H,
+                               (Row<int32_t, true>*)out);
+        else
+            hipLaunchKernelGGL((k_bscatter<int32_t, false>), dim3(n), dim3(1024), 0, st, segs, chunks, chunk_base, H,
+                               (Row<int32_t, false>*)out);
+    }
+    PGPU_LAUNCH_CHECK();
+    return PGPU_OK;
+}
+
+#define PGPU_JOIN_VARIANTS(X)                               \
+    X(int64_t, true, 1, 4096, 512) X(int64_t, true, 2, 8192, 1024)   \
+    X(int64_t, false, 1, 4096, 512) X(int64_t, false, 2, 8192, 1024) \
+    X(int32_t, true, 1, 4096, 512) X(int32_t, true, 2, 8192, 1024)   \
+    X(int32_t, false, 1, 4096, 512) X(int32_t, false, 2, 8192, 1024)
+
+int launch_join(hipStream_t st, int kb, bool hvs, int variant, int grid, const Job* jobs, const uint32_t* item_job,
+                uint32_t n_items, const void* R, const void* S, const uint32_t* bndR, const uint32_t* bndS,
+                uint32_t* queue, unsigned long long* res) {
+    if (!n_items) return PGPU_OK;
+#define X(KT, HV, V, CAP, T)                                                                                      \
+    if ((kb == 8) == (sizeof(KT) == 8) && hvs == HV && variant == V)                                              \
+        return launch_join_t<KT, HV, CAP, T>(st, grid, jobs, item_job, n_items, R, S, bndR, bndS, queue, res);
+    PGPU_JOIN_VARIANTS(X)
+#undef X
+    return fail(PGPU_EINVAL, "unknown join variant %d", variant);
+}
+
+int join_grid(int kb, bool hvs, int variant, int* grid) {
+    int dev = 0, cus = 0, blocks = 0;
+    PGPU_HIP(hipGetDevice(&dev));
+    PGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+#define X(KT, HV, V, CAP, T)                                             \
+    if ((kb == 8) == (sizeof(KT) == 8) && hvs == HV && variant == V) { \
+        PGPU_TRY((occupancy_t<KT, HV, CAP, T>(&blocks)));                \
+        *grid = cus * std::max(blocks, 1);                               \
+        return PGPU_OK;                                                  \
+    }
+    PGPU_JOIN_VARIANTS(X)
+#undef X
+    return fail(PGPU_EINVAL, "unknown join variant %d", variant);
+}
+
+int launch_zero(hipStream_t st, unsigned long long* p, uint64_t n) {
+    if (!n) return PGPU_OK;
+    const unsigned g = (unsigned)std::min<uint64_t>(ceil_div(n, 256), 1024);
+    hipLaunchKernelGGL(k_bzero, dim3(g), dim3(256), 0, st, p, n);
+    PGPU_LAUNCH_CHECK();
+    return PGPU_OK;
+}
+
+}  // namespace batch
+}  // namespace pgpu

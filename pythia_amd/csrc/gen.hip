@@ -47,6 +47,8 @@ __global__ __launch_bounds__(256) void k_gen(pgpu_gen_spec spec, uint64_t n, voi
             ((int32_t*)out)[t] = pgpu_c4_rpay(spec.seed, spec.sub, i);
         } else if constexpr (COL == PGPU_GEN_C4_SFK) {
             ((int32_t*)out)[t] = pgpu_c4_sfk(spec.seed, spec.sub, i, spec.n_build);
+        } else if constexpr (COL == PGPU_GEN_C4_SPAY) {
+            ((int32_t*)out)[t] = pgpu_c4_spay(spec.seed, spec.sub, i);
         }
     }
 }
@@ -92,6 +94,7 @@ extern "C" int pgpu_gen(pgpu_ctx* ctx, int column, const pgpu_gen_spec* spec, ui
         CASE(PGPU_GEN_C4_RKEY)
         CASE(PGPU_GEN_C4_RPAY)
         CASE(PGPU_GEN_C4_SFK)
+        CASE(PGPU_GEN_C4_SPAY)
 #undef CASE
         default:
             return fail(PGPU_EINVAL, "pgpu_gen: unknown column %d", column);

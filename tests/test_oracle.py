@@ -314,6 +314,35 @@ def test_c2_generator_and_closed_form(zipf):
     assert cs.count == ref.count and cs.sum_build == ref.sum_build
 
 
+@pytest.mark.parametrize("join,n", [(0, 1000), (7, 5000), (1023, 1 << 14)])
+def test_c4_generator_and_closed_form(join, n):
+    """Build keys are a permutation of 0..n-1, about 10% of probe keys fall
+    in the disjoint miss range [n, n + 2^30), and the closed form agrees with
+    the chained-bucket restatement of the join and with a direct-address join."""
+    m = 8 * n
+    rk, rp, sk, sp = O.gen_c4(join, n, m)
+    assert np.array_equal(np.sort(rk), np.arange(n))
+    miss = sk >= n
+    assert (sk[miss] < n + 2**30).all() and (sk >= 0).all()
+    assert abs(miss.mean() - 0.1) < 4 * np.sqrt(0.09 / m) + 1e-3
+    ref = O.c4_closed(join, n, m)
+    inv = np.empty(n, np.int64)
+    inv[rk] = np.arange(n)
+    hit = ~miss
+    assert ref.count == int(hit.sum())
+    assert ref.sum_build == int(rp[inv[sk[hit]]].astype(np.int64).sum()) % 2**64
+    assert ref.sum_probe == int(sp[hit].astype(np.int64).sum()) % 2**64
+    bs, ps = O.schema(("int",), ("int",)), O.schema(("int",), ("int",))
+    d = O.join_desc(bs, ps, 0, 0, ["B$1", "P$1"], buckets=1 << 10, tuplesperbucket=4, threads=4)
+    cs_b, _ = O.hashjoin_count_sum(d, O.rows(bs, rk, rp), O.rows(ps, sk, sp), 0)
+    cs_p, _ = O.hashjoin_count_sum(d, O.rows(bs, rk, rp), O.rows(ps, sk, sp), 1)
+    assert cs_b.count == ref.count
+    assert cs_b.isum % 2**32 == ref.sum_build % 2**32
+    assert cs_p.isum % 2**32 == ref.sum_probe % 2**32
+    many = O.c4_closed_many(join, [n], [m], threads=2)
+    assert (many[0].count, many[0].sum_build, many[0].sum_probe) == (ref.count, ref.sum_build, ref.sum_probe)
+
+
 def test_zipf_cdf_shape():
     cdf = O.zipf_cdf(1000, 1.0)
     assert np.all(np.diff(cdf) > 0) and cdf[-1] == 1.0
