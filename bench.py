@@ -1,24 +1,32 @@
 #!/usr/bin/env python3
-"""Headline benchmark: the radix-partitioned equi-join with fused count + sum.
+"""Benchmarks of the MI355X join engine.  Prints one JSON line.
 
-Workload (one MI355X): R = 2^24 rows of (int64 unique shuffled key, int64
+--workload c2 (default, the headline): the radix-partitioned equi-join with
+fused count + sum.  R = 2^24 rows of (int64 unique shuffled key, int64
 payload U[0,2^31)), S = 2^28 int64 foreign keys drawn uniformly (or Zipf(s)
 with --zipf) over R's keys, 2-pass radix partitioning with 14 total bits,
-output = match count + int64 sum of R.payload.  Inputs are generated in HBM
-before the clock starts.  A step is one whole join (histogram, both
-partition passes of both relations, the per-partition LDS join).
+output = match count + int64 sum of R.payload.  A step is one whole join
+(histogram, both partition passes of both relations, the per-partition LDS
+join).  `value` = (|R| + |S|) / step time.
 
-Prints one JSON line.  `value` = (|R| + |S|) / step time.  `roofline` is the
+--workload c4: 1024 independent int32 joins (build sizes round(2^U(10,22)),
+probe 8x with 10% non-matching keys, count + both payload sums), run as one
+autotuned, hipGraph-captured batch plan; a step is the whole batch.
+`value` = sum of 9 n_i / step time; compared against the same 1024 joins run
+one after another through the plain per-join path.
+
+Inputs are generated in HBM before the clock starts.  `roofline` is the
 dominant kernel's algorithmic bytes / its average launch time (HIP events on
 the stream the kernels run on) against 8 TB/s.  `cpu_baseline` is the
-reference algorithm restated in C (chained buckets, bucket spinlocks, the
-queryhashjoin plan shape) on this host's cores, on the same inputs.
+reference's own plan compiled from its sources (oracle/_ref/refjoin) when
+present, else its C restatement, on this host's cores.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -29,9 +37,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
 METRIC = "join tuples/s + % of 8 TB/s HBM roofline; served p50/p99 vs plain per-query GPU"
+REFJOIN = os.path.join(ROOT, "oracle", "_ref", "refjoin")
 
 
-def kernel_bytes(nR: int, nS: int, esR: int = 16, esS: int = 8, kb: int = 8) -> dict:
+def cpu_cores():
+    return int(os.environ.get("PYTHIA_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+
+
+def kernel_bytes_c2(nR: int, nS: int, esR: int = 16, esS: int = 8, kb: int = 8) -> dict:
     """Algorithmic HBM bytes of one launch of each kernel of the radix join."""
     return {
         "hist_build": nR * kb,                 # read the build key column
@@ -44,17 +57,33 @@ def kernel_bytes(nR: int, nS: int, esR: int = 16, esS: int = 8, kb: int = 8) -> 
     }
 
 
-def cpu_baseline(args, nR, nS, gpu_result):
+def roofline(per_kernel: dict, kb: dict, traffic_file: str | None):
+    dom = max((k for k in per_kernel if k in kb), key=lambda k: per_kernel[k]["avg_ms"])
+    avg_s = per_kernel[dom]["avg_ms"] / 1e3
+    achieved = kb[dom] / avg_s
+    traffic = None
+    if traffic_file and os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            traffic = json.load(f).get(dom)
+    return {
+        "bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK, "traffic": traffic,
+        "algorithmic_bytes_per_launch": kb[dom], "avg_launch_ms": per_kernel[dom]["avg_ms"],
+    }
+
+
+def per_kernel_times(times: dict) -> dict:
+    return {k: {"launches": n, "avg_ms": ms / max(n, 1)} for k, (n, ms) in times.items()}
+
+
+def cpu_baseline_c2(args, nR, nS, gpu_result):
     """The reference's own HashJoinOp -> AggregateSum -> MergeOp plan
     (compiled from its sources into oracle/_ref/refjoin) when that binary is
     present, else the C restatement in oracle/; same inputs, this host's
     cores, the reference's C2 knobs (modulo, 2^24 buckets, 4 per bucket)."""
-    import subprocess
-
-    cores = int(os.environ.get("PYTHIA_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-    ref = os.path.join(ROOT, "oracle", "_ref", "refjoin")
-    if os.path.exists(ref):
-        out = subprocess.run([ref, "c2", str(args.log_build), str(args.log_probe), str(cores), str(args.zipf),
+    cores = cpu_cores()
+    if os.path.exists(REFJOIN):
+        out = subprocess.run([REFJOIN, "c2", str(args.log_build), str(args.log_probe), str(cores), str(args.zipf),
                               str(1 << 24), "4", "sum"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
                              timeout=900, cwd="/tmp")
         if out.returncode == 0:
@@ -85,26 +114,7 @@ def cpu_baseline(args, nR, nS, gpu_result):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--log-build", type=int, default=24)
-    ap.add_argument("--log-probe", type=int, default=28)
-    ap.add_argument("--bits", type=int, default=14)
-    ap.add_argument("--zipf", type=float, default=0.0)
-    ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_c2_uniform.json"),
-                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary (bytes per launch per kernel)")
-    args = ap.parse_args()
-
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every box in this setup has one MI355X, so --gpus must be 1 "
-              "(the path shards by radix partition; see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
+def run_c2(args):
     import pythia_amd as pa
 
     nR, nS = 1 << args.log_build, 1 << args.log_probe
@@ -138,17 +148,8 @@ def main():
     for _ in range(args.steps):
         plan.enqueue(rk, rp, nR, sk, None, nS)
     plan.result()
-    times = plan.times()
+    per_kernel = per_kernel_times(plan.times())
     plan.timing(False)
-    kb = kernel_bytes(nR, nS)
-    per_kernel = {k: {"launches": n, "avg_ms": ms / max(n, 1)} for k, (n, ms) in times.items()}
-    dom = max((k for k in per_kernel if k in kb), key=lambda k: per_kernel[k]["avg_ms"])
-    avg_s = per_kernel[dom]["avg_ms"] / 1e3
-    achieved = kb[dom] / avg_s
-    traffic = None
-    if os.path.exists(args.traffic):
-        with open(args.traffic) as f:
-            traffic = json.load(f).get(dom)
     job_bytes = 5 * (nR * 16 + nS * 8) + 16
     print(f"# step {ms_step:.3f} ms; kernels: " +
           ", ".join(f"{k}={v['avg_ms']:.3f}ms" for k, v in per_kernel.items()), file=sys.stderr, flush=True)
@@ -172,13 +173,11 @@ def main():
                         f"2-pass radix {args.bits} bits",
             "build_rows": nR, "probe_rows": nS, "radix_bits": res.radix_bits, "zipf_s": args.zipf,
         },
-        "roofline": {
-            "bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": traffic,
-            "algorithmic_bytes_per_launch": kb[dom], "avg_launch_ms": per_kernel[dom]["avg_ms"],
-        },
+        "roofline": roofline(per_kernel, kernel_bytes_c2(nR, nS), args.traffic or os.path.join(
+            ROOT, "profiles", "traffic_c2_uniform.json")),
         "job_roofline": {"algorithmic_bytes": job_bytes, "achieved_GBps": job_bytes / (ms_step / 1e3) / 1e9,
-                         "frac": job_bytes / (ms_step / 1e3) / HBM_PEAK},
+                         "frac": job_bytes / (ms_step / 1e3) / HBM_PEAK,
+                         "compulsory_bytes": nR * 16 + nS * 8 + 16},
         "kernels": per_kernel,
         "result": {"matches": res.matches, "sum_build": res.sum_build, "lds_passes": res.lds_passes},
     }
@@ -190,7 +189,184 @@ def main():
         for b in (rk, rp, sk):
             b.free()
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(args, nR, nS, res)
+        line["cpu_baseline"] = cpu_baseline_c2(args, nR, nS, res)
+    return line
+
+
+def c4_inputs(ctx, n_joins, seed):
+    """Generates the batched-join family in four HBM arrays (16-byte aligned
+    per join); returns (sizes, joins, buffers)."""
+    import pythia_amd as pa
+
+    sizes = pa.sizes(4, n_joins, 10, 22, seed=seed).astype(np.int64)
+
+    def pad(x):
+        return (x + 3) // 4 * 4
+
+    boff = np.concatenate([[0], np.cumsum([pad(int(n)) for n in sizes])])
+    poff = np.concatenate([[0], np.cumsum([pad(8 * int(n)) for n in sizes])])
+    rk, rp = ctx.alloc(4 * int(boff[-1])), ctx.alloc(4 * int(boff[-1]))
+    sk, sp = ctx.alloc(4 * int(poff[-1])), ctx.alloc(4 * int(poff[-1]))
+    joins = []
+    for j, n in enumerate(int(x) for x in sizes):
+        m = 8 * n
+        a, b = rk.ptr + 4 * int(boff[j]), rp.ptr + 4 * int(boff[j])
+        c, d = sk.ptr + 4 * int(poff[j]), sp.ptr + 4 * int(poff[j])
+        ctx.gen("C4_RKEY", n, a, seed=seed, sub=j, n_build=n)
+        ctx.gen("C4_RPAY", n, b, seed=seed, sub=j)
+        ctx.gen("C4_SFK", m, c, seed=seed, sub=j, n_build=n)
+        ctx.gen("C4_SPAY", m, d, seed=seed, sub=j)
+        joins.append((a, b, n, c, d, m))
+    ctx.sync()
+    return sizes, joins, [rk, rp, sk, sp]
+
+
+def cpu_baseline_c4(args, sizes, res):
+    """The reference's plan run once per join (oracle/_ref/refjoin c4) over a
+    bounded sample of the batch: the first joins up to ~2.5e8 tuples."""
+    cores = cpu_cores()
+    cum = np.cumsum(9 * sizes)
+    count = max(1, min(int(np.searchsorted(cum, 2.5e8)) + 1, len(sizes)))
+    if os.path.exists(REFJOIN):
+        out = subprocess.run([REFJOIN, "c4", "0", str(count), str(cores), "4", "sum"], stdout=subprocess.PIPE,
+                             stderr=subprocess.PIPE, text=True, timeout=900, cwd="/tmp")
+        if out.returncode == 0:
+            r = json.loads(out.stdout)
+            gpu32 = [int(np.array([x.sum_build & 0xFFFFFFFF], np.uint32).view(np.int32)[0]) for x in res[:count]]
+            return {"value": r["tuples"] / r["seconds"], "unit": "tuples/s", "cores": cores, "kind": "reference",
+                    "sample": f"joins 0..{count - 1} of the batch ({r['tuples']} tuples), each through the "
+                              f"reference's hashjoin -> aggregate_sum -> merge({cores}) plan with 2^ceil(log2 n) "
+                              f"modulo buckets; {r['seconds']:.2f} s summed plan time",
+                    "result_matches_gpu": r["values"] == gpu32}
+        print(f"# reference baseline failed: {out.stderr[-300:]}", file=sys.stderr)
+    from oracle import oracle as O
+
+    total, tuples, ok = 0.0, 0, True
+    for j in range(count):
+        n = int(sizes[j])
+        rk, rp, sk, sp = O.gen_c4(j, n, 8 * n)
+        secs, cs = O.baseline_join(4, rk, rp, sk, 1 << max(0, (n - 1).bit_length()), 4, cores)
+        total += secs
+        tuples += 9 * n
+        ok &= cs.count == res[j].matches
+    return {"value": tuples / total, "unit": "tuples/s", "cores": cores, "kind": "port",
+            "sample": f"joins 0..{count - 1} of the batch through the C restatement", "result_matches_gpu": ok}
+
+
+def run_c4(args):
+    import pythia_amd as pa
+
+    ctx = pa.Context(0)
+    sizes, joins, bufs = c4_inputs(ctx, args.joins, args.seed)
+    tuples = int(9 * sizes.sum())
+    print(f"# c4: {len(joins)} joins, build {int(sizes.sum())} rows, probe {int(8 * sizes.sum())} rows",
+          file=sys.stderr, flush=True)
+    tune = None
+    variant, part_rows = 0, 0
+    if not args.no_tune:
+        best, table = pa.batch_autotune(ctx, joins, key_bytes=4, probe_payload=True, reps=2)
+        variant, part_rows = best
+        tune = {"best": {"variant": best[0], "part_rows": best[1]},
+                "candidates": [{"variant": v, "part_rows": r, "ms": ms} for v, r, ms in table]}
+        print(f"# autotune: {tune}", file=sys.stderr, flush=True)
+    plan = pa.BatchPlan(ctx, joins, key_bytes=4, probe_payload=True, variant=variant, part_rows=part_rows,
+                        graph=True)
+    for _ in range(args.warmup):
+        res = plan.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.enqueue()
+    res = plan.wait()
+    ms_step = 1e3 * (time.perf_counter() - t0) / args.steps
+    plan.timing(True)
+    plan.run()
+    per_kernel = per_kernel_times(plan.times())
+    plan.timing(False)
+    info = plan.info()
+    print(f"# batched step {ms_step:.3f} ms; kernels: " +
+          ", ".join(f"{k}={v['avg_ms']:.3f}ms" for k, v in per_kernel.items()), file=sys.stderr, flush=True)
+
+    # the same joins one after another through the plain per-join path
+    nmax = int(sizes.max())
+    plain = pa.JoinPlan(ctx, 4, nmax, 8 * nmax, probe_payload=True, algo=pa.ALGO_AUTO)
+    for j in joins[:8]:
+        plain.count_sum(*j)
+    t0 = time.perf_counter()
+    pres = [plain.count_sum(*j) for j in joins]
+    plain_ms = 1e3 * (time.perf_counter() - t0)
+    same = all((a.matches, a.sum_build, a.sum_probe) == (b.matches, b.sum_build, b.sum_probe)
+               for a, b in zip(res, pres))
+    plain.close()
+    print(f"# plain sequential {plain_ms:.1f} ms, batched {ms_step:.2f} ms, same results: {same}",
+          file=sys.stderr, flush=True)
+
+    nb, ns = int(sizes.sum()), int(8 * sizes.sum())
+    kb = {
+        "batch_hist": 4 * (nb + ns),
+        "batch_scatter_build": 16 * nb,
+        "batch_scatter_probe": 16 * ns,
+        "batch_join": 8 * (nb + ns) + 32 * len(joins),
+    }
+    compulsory = 8 * nb + 8 * ns + 16 * len(joins)
+    line = {
+        "metric": METRIC,
+        "value": tuples / (ms_step / 1e3),
+        "unit": "tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (counter-based SplitMix64 generators, generated in HBM)",
+        "config": {"workload": f"{len(joins)} independent int32 joins, build round(2^U(10,22)) rows of (key, "
+                               f"payload), probe 8x (10% misses), count + payload sums, one batched hipGraph plan",
+                   "joins": len(joins), "build_rows": nb, "probe_rows": ns,
+                   "variant": info.cfg.variant, "part_rows": info.cfg.part_rows, "items": info.items,
+                   "persistent_workgroups": info.grid},
+        "roofline": roofline(per_kernel, kb, args.traffic),
+        "job_roofline": {"compulsory_bytes": compulsory,
+                         "achieved_GBps": compulsory / (ms_step / 1e3) / 1e9,
+                         "frac": compulsory / (ms_step / 1e3) / HBM_PEAK},
+        "batched_vs_plain": {"plain_sequential_ms": plain_ms, "batched_ms": ms_step,
+                             "speedup": plain_ms / ms_step, "same_results": same},
+        "autotune": tune,
+        "kernels": per_kernel,
+    }
+    if not args.no_cpu_baseline:
+        plan.close()
+        for b in bufs:
+            b.free()
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline_c4(args, sizes, res)
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["c2", "c4"], default="c2")
+    ap.add_argument("--log-build", type=int, default=24)
+    ap.add_argument("--log-probe", type=int, default=28)
+    ap.add_argument("--bits", type=int, default=14)
+    ap.add_argument("--zipf", type=float, default=0.0)
+    ap.add_argument("--joins", type=int, default=1024)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=None,
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary (bytes per launch per kernel)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box in this setup has one MI355X, so --gpus must be 1 "
+              "(the path shards by radix partition; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    line = run_c4(args) if args.workload == "c4" else run_c2(args)
     print(json.dumps(line), flush=True)
 
 

@@ -14,7 +14,9 @@
 // drivers/executequery.cpp:38-59 does (with rdtsc there).
 //
 // usage: refjoin <c1|c2> <log2 R> <log2 S> <threads> <zipf s> <buckets> <tuplesperbucket> <sum|count|probesum>
-// prints one JSON line {"count" or "sum": ..., "seconds": ..., ...}
+//        refjoin c4 <first join> <joins> <threads> <tuplesperbucket> <sum|probesum>
+//          (the batched-join family: one plan per join, buckets = 2^ceil(log2 n))
+// prints one JSON line {"value": ..., "seconds": ..., ...}
 #include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -72,57 +74,16 @@ static double now() {
     return ts.tv_sec + ts.tv_nsec * 1e-9;
 }
 
-int main(int argc, char** argv) {
-    if (argc < 9) {
-        fprintf(stderr, "usage: refjoin <c1|c2> <log2R> <log2S> <threads> <zipf> <buckets> <tpb> <sum|count|probesum>\n");
-        return 2;
-    }
-    const std::string wl = argv[1];
-    const uint64_t nR = 1ULL << atoi(argv[2]), nS = 1ULL << atoi(argv[3]);
-    const int T = atoi(argv[4]);
-    const double zipf = atof(argv[5]);
-    const int buckets = atoi(argv[6]), tpb = atoi(argv[7]);
-    const std::string agg = argv[8];
-    const uint64_t seed = PGPU_DEFAULT_SEED;
-    const bool is32 = wl == "c1";
-    const unsigned kb = is32 ? 4 : 8;
-
-    // build rows (key, payload); probe rows (fk[, payload])
-    const unsigned prow = is32 ? 8 : 8;
-    std::vector<char> R(nR * 2 * kb), S(nS * prow);
-    std::vector<double> cdf;
-    if (zipf > 0) {
-        cdf.resize(nR);
-        pgpu_zipf_cdf_fill(&cdf[0], nR, zipf);
-    }
-    for (uint64_t i = 0; i < nR; ++i) {
-        if (is32) {
-            int32_t k = pgpu_c1_rkey(seed, i, nR), p = pgpu_c1_rpay(seed, i);
-            memcpy(&R[i * 8], &k, 4);
-            memcpy(&R[i * 8 + 4], &p, 4);
-        } else {
-            int64_t k = pgpu_c2_rkey(seed, 0, i), p = pgpu_c2_rpay(seed, 0, i);
-            memcpy(&R[i * 16], &k, 8);
-            memcpy(&R[i * 16 + 8], &p, 8);
-        }
-    }
-    for (uint64_t j = 0; j < nS; ++j) {
-        if (is32) {
-            int32_t k = pgpu_c1_sfk(seed, j, nR), p = pgpu_c1_spay(seed, j);
-            memcpy(&S[j * 8], &k, 4);
-            memcpy(&S[j * 8 + 4], &p, 4);
-        } else {
-            uint64_t row = zipf > 0 ? pgpu_c2_srow_zipf(seed, 0, j, nR, &cdf[0]) : pgpu_c2_srow_uniform(seed, 0, j, nR);
-            int64_t k = pgpu_c2_rkey(seed, 0, row);
-            memcpy(&S[j * 8], &k, 8);
-        }
-    }
-
+// Runs the plan once over packed rows; *value gets the folded aggregate (an
+// int32 sum wraps in 32 bits, as AggregateSum's int accumulator does).
+static int run_plan(char* R, uint64_t nR, char* S, uint64_t nS, bool is32, bool probe_pay, int T, int buckets,
+                    int tpb, const std::string& agg, long long* value_out, double* secs) {
     Schema rs, ss;
     rs.add(is32 ? CT_INTEGER : CT_LONG);
     rs.add(is32 ? CT_INTEGER : CT_LONG);
     ss.add(is32 ? CT_INTEGER : CT_LONG);
-    if (is32) ss.add(CT_INTEGER);
+    if (probe_pay) ss.add(is32 ? CT_INTEGER : CT_LONG);
+    const uint64_t prow = ss.getTupleSize();
 
     Config cfg;
     cfg.getRoot().add("path", Setting::TypeString) = "./";
@@ -154,7 +115,7 @@ int main(int argc, char** argv) {
     mn.add("threads", Setting::TypeInt) = T;
     Setting& sc = cfg.getRoot().add("scan", Setting::TypeGroup);
 
-    MemScan scanR(&R[0], nR, rs), scanS(&S[0], nS, ss);
+    MemScan scanR(R, nR, rs), scanS(S, nS, ss);
     HashJoinOp join;
     GenericAggregate* partial;
     if (agg == "count")
@@ -203,10 +164,116 @@ int main(int argc, char** argv) {
     q.scanStop();
     double t1 = now();
     q.threadClose();
-    printf("{\"workload\": \"%s\", \"build\": %llu, \"probe\": %llu, \"threads\": %d, \"agg\": \"%s\", \"value\": %lld, "
-           "\"seconds\": %.6f, \"spills\": %lu}\n",
-           wl.c_str(), (unsigned long long)nR, (unsigned long long)nS, T, agg.c_str(), value, t1 - t0,
-           0UL);
     q.destroynofree();
+    delete partial;
+    *value_out = value;
+    *secs = t1 - t0;
+    return 0;
+}
+
+static void usage() {
+    fprintf(stderr,
+            "usage: refjoin <c1|c2> <log2R> <log2S> <threads> <zipf> <buckets> <tpb> <sum|count|probesum>\n"
+            "       refjoin c4 <first> <joins> <threads> <tpb> <sum|probesum>\n");
+}
+
+// The batched family: joins first..first+count-1, each its own plan; inputs
+// are generated outside the clock; prints the summed plan time and each
+// join's aggregate.
+static int run_c4(int argc, char** argv) {
+    if (argc < 7) {
+        usage();
+        return 2;
+    }
+    const uint64_t first = strtoull(argv[2], NULL, 10), count = strtoull(argv[3], NULL, 10);
+    const int T = atoi(argv[4]), tpb = atoi(argv[5]);
+    const std::string agg = argv[6];
+    const uint64_t seed = PGPU_DEFAULT_SEED;
+    double total = 0;
+    uint64_t tuples = 0;
+    std::string values;
+    for (uint64_t j = first; j < first + count; ++j) {
+        const uint64_t n = pgpu_loguniform_size(seed, 4, j, 10.0, 22.0), m = 8 * n;
+        std::vector<char> R(n * 8), S(m * 8);
+        for (uint64_t i = 0; i < n; ++i) {
+            int32_t k = pgpu_c4_rkey(seed, j, i, n), p = pgpu_c4_rpay(seed, j, i);
+            memcpy(&R[i * 8], &k, 4);
+            memcpy(&R[i * 8 + 4], &p, 4);
+        }
+        for (uint64_t i = 0; i < m; ++i) {
+            int32_t k = pgpu_c4_sfk(seed, j, i, n), p = pgpu_c4_spay(seed, j, i);
+            memcpy(&S[i * 8], &k, 4);
+            memcpy(&S[i * 8 + 4], &p, 4);
+        }
+        int buckets = 1;
+        while ((uint64_t)buckets < n) buckets <<= 1;
+        long long v = 0;
+        double secs = 0;
+        if (run_plan(&R[0], n, &S[0], m, true, true, T, buckets, tpb, agg, &v, &secs)) return 1;
+        total += secs;
+        tuples += n + m;
+        char buf[32];
+        snprintf(buf, sizeof buf, "%s%lld", values.empty() ? "" : ", ", v);
+        values += buf;
+    }
+    printf("{\"workload\": \"c4\", \"first\": %llu, \"joins\": %llu, \"threads\": %d, \"agg\": \"%s\", "
+           "\"tuples\": %llu, \"seconds\": %.6f, \"values\": [%s]}\n",
+           (unsigned long long)first, (unsigned long long)count, T, agg.c_str(), (unsigned long long)tuples, total,
+           values.c_str());
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc >= 2 && std::string(argv[1]) == "c4") return run_c4(argc, argv);
+    if (argc < 9) {
+        usage();
+        return 2;
+    }
+    const std::string wl = argv[1];
+    const uint64_t nR = 1ULL << atoi(argv[2]), nS = 1ULL << atoi(argv[3]);
+    const int T = atoi(argv[4]);
+    const double zipf = atof(argv[5]);
+    const int buckets = atoi(argv[6]), tpb = atoi(argv[7]);
+    const std::string agg = argv[8];
+    const uint64_t seed = PGPU_DEFAULT_SEED;
+    const bool is32 = wl == "c1";
+    const unsigned kb = is32 ? 4 : 8;
+
+    // build rows (key, payload); probe rows (fk[, payload])
+    const unsigned prow = 8;
+    std::vector<char> R(nR * 2 * kb), S(nS * prow);
+    std::vector<double> cdf;
+    if (zipf > 0) {
+        cdf.resize(nR);
+        pgpu_zipf_cdf_fill(&cdf[0], nR, zipf);
+    }
+    for (uint64_t i = 0; i < nR; ++i) {
+        if (is32) {
+            int32_t k = pgpu_c1_rkey(seed, i, nR), p = pgpu_c1_rpay(seed, i);
+            memcpy(&R[i * 8], &k, 4);
+            memcpy(&R[i * 8 + 4], &p, 4);
+        } else {
+            int64_t k = pgpu_c2_rkey(seed, 0, i), p = pgpu_c2_rpay(seed, 0, i);
+            memcpy(&R[i * 16], &k, 8);
+            memcpy(&R[i * 16 + 8], &p, 8);
+        }
+    }
+    for (uint64_t j = 0; j < nS; ++j) {
+        if (is32) {
+            int32_t k = pgpu_c1_sfk(seed, j, nR), p = pgpu_c1_spay(seed, j);
+            memcpy(&S[j * 8], &k, 4);
+            memcpy(&S[j * 8 + 4], &p, 4);
+        } else {
+            uint64_t row = zipf > 0 ? pgpu_c2_srow_zipf(seed, 0, j, nR, &cdf[0]) : pgpu_c2_srow_uniform(seed, 0, j, nR);
+            int64_t k = pgpu_c2_rkey(seed, 0, row);
+            memcpy(&S[j * 8], &k, 8);
+        }
+    }
+    long long value = 0;
+    double secs = 0;
+    if (run_plan(&R[0], nR, &S[0], nS, is32, is32, T, buckets, tpb, agg, &value, &secs)) return 1;
+    printf("{\"workload\": \"%s\", \"build\": %llu, \"probe\": %llu, \"threads\": %d, \"agg\": \"%s\", "
+           "\"value\": %lld, \"seconds\": %.6f}\n",
+           wl.c_str(), (unsigned long long)nR, (unsigned long long)nS, T, agg.c_str(), value, secs);
     return 0;
 }

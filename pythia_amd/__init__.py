@@ -113,8 +113,11 @@ class Context:
 
     def gen(self, column: str, n: int, out: DeviceBuffer, seed: int = 42, sub: int = 0, n_build: int = 0,
             offset: int = 0, cdf: DeviceBuffer | None = None):
+        """Fills n rows of a generated column into `out` (a DeviceBuffer or a
+        raw device address)."""
         spec = _lib.GenSpec(seed, sub, n_build, offset, C.c_void_p(cdf.ptr) if cdf else None)
-        call("pgpu_gen", self.handle, GEN[column], C.byref(spec), n, C.c_void_p(out.ptr))
+        dst = out.ptr if isinstance(out, DeviceBuffer) else int(out)
+        call("pgpu_gen", self.handle, GEN[column], C.byref(spec), n, C.c_void_p(dst))
 
 
 class JoinPlan:
@@ -135,7 +138,9 @@ class JoinPlan:
 
     @staticmethod
     def _p(b):
-        return C.c_void_p(b.ptr if b is not None else None)
+        if b is None:
+            return C.c_void_p(None)
+        return C.c_void_p(b.ptr if isinstance(b, DeviceBuffer) else int(b))
 
     def enqueue(self, rkey, rpay, n_build, skey, spay, n_probe):
         call("pgpu_join_count_sum_async", self.handle, self._p(rkey), self._p(rpay), n_build, self._p(skey),

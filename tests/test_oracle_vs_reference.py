@@ -51,3 +51,24 @@ def test_c2_reference_equals_oracle(zipf, threads):
     rk, rp, sk = O.gen_c2(1 << lr, 1 << ls, zipf)
     secs, cs = O.baseline_join(8, rk, rp, sk, 1 << 12, 4, threads)
     assert (cs.count, cs.sum_build) == (ref.count, ref.sum_build)
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_c4_reference_equals_oracle(threads):
+    """The batched-join family: the reference's plan run once per join gives
+    the closed forms' sums (int32 accumulator) for each join."""
+    import sys
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import pythia_amd as pa
+
+    first, count = 5, 4
+    sizes = pa.sizes(4, count, 10, 22, first=first)
+    ref = O.c4_closed_many(first, sizes, 8 * sizes, threads=4)
+    for agg, field in (("sum", "sum_build"), ("probesum", "sum_probe")):
+        out = subprocess.run([REF, "c4", str(first), str(count), str(threads), "4", agg], stdout=subprocess.PIPE,
+                             stderr=subprocess.PIPE, text=True, timeout=300, cwd="/tmp")
+        assert out.returncode == 0, out.stderr
+        r = json.loads(out.stdout)
+        assert r["tuples"] == int(9 * sizes.sum())
+        assert r["values"] == [i32(getattr(c, field)) for c in ref]
