@@ -227,6 +227,76 @@ int pgpu_batch_autotune(pgpu_ctx* ctx, const pgpu_batch_cfg* base, const pgpu_ba
                         int reps, pgpu_batch_cfg* best, pgpu_tune_entry* entries, int cap, int* n_entries);
 
 /* ------------------------------------------------------------------ */
+/* Serving: join + sum queries against build tables kept resident in    */
+/* HBM, admitted FIFO under an HBM budget and continuously batched onto */
+/* overlapping streams (PGPU_SERVE_BATCHED), or run one at a time       */
+/* through the per-query join plan that rebuilds the table              */
+/* (PGPU_SERVE_PLAIN).  Stands in for executing the queryhashjoin plan  */
+/* per query (drivers/executequery.cpp:38-59 over operators/join.cpp:   */
+/* 396-562 and aggregatesum.cpp:81-107), which rebuilds the HashTable   */
+/* (util/hashtable.cpp:57-97) every time; the reference has no server.  */
+/* Keys and payloads are int64; a query's probe batch is device memory  */
+/* owned by the caller and held until its result is polled.             */
+/* ------------------------------------------------------------------ */
+enum pgpu_serve_mode { PGPU_SERVE_BATCHED = 0, PGPU_SERVE_PLAIN = 1 };
+
+typedef struct pgpu_server_cfg {
+    uint64_t hbm_budget;  /* resident tables + admitted probe batches; 0 = the context's */
+    uint32_t key_bytes;   /* 8                                                          */
+    uint32_t mode;        /* enum pgpu_serve_mode                                       */
+    uint32_t slots;       /* batches in flight at once (streams); 0 = 3                 */
+    uint32_t max_batch;   /* queries per launch; 0 = 256                                */
+    uint32_t max_items;   /* 64K-row work items per launch; 0 = 32768                   */
+    uint32_t pad;
+} pgpu_server_cfg;
+
+typedef struct pgpu_served_result {
+    uint64_t tag;           /* the caller's tag                          */
+    uint32_t table;
+    uint32_t batch_size;    /* queries that shared its launch            */
+    uint64_t m;             /* probe rows                                */
+    uint64_t matches;
+    uint64_t sum_build;     /* sum of build payloads over the matches    */
+    double t_arrival;       /* seconds on the server clock               */
+    double t_admit;
+    double t_done;
+    uint64_t batch;         /* launch sequence number                    */
+} pgpu_served_result;
+
+typedef struct pgpu_server_stats {
+    uint64_t tables, resident_bytes, reserved_bytes, budget;
+    uint64_t submitted, completed, batches, queued;
+} pgpu_server_stats;
+
+typedef struct pgpu_server pgpu_server;
+int pgpu_server_create(pgpu_ctx* ctx, const pgpu_server_cfg* cfg, pgpu_server** out);
+int pgpu_server_destroy(pgpu_server* s);
+/* Registers a table (int64 key / payload columns in HBM, kept allocated by
+ * the caller); batched mode hashes it into a resident table now.  Tables are
+ * registered while no query is outstanding. */
+int pgpu_server_add_table(pgpu_server* s, const void* d_key, const void* d_pay, uint64_t n, uint32_t* table_id);
+/* Plain mode: sizes the per-query join plan for probes up to max_probe rows. */
+int pgpu_server_prepare_plain(pgpu_server* s, uint64_t max_probe);
+/* Queues a query (thread-safe, returns at once).  t_arrival: seconds on the
+ * server clock (pgpu_server_now), 0 = now; latency is t_done - t_arrival. */
+int pgpu_server_submit(pgpu_server* s, uint32_t table, const void* d_probe, uint64_t m, uint64_t tag,
+                       double t_arrival);
+/* Waits until every submitted query has completed. */
+int pgpu_server_drain(pgpu_server* s);
+/* Takes up to cap completed results (completion order). */
+int pgpu_server_poll(pgpu_server* s, pgpu_served_result* out, int cap, int* n);
+int pgpu_server_now(pgpu_server* s, double* t);
+/* Load generator: submits query i at offsets[i] seconds after the call
+ * (busy-waiting for each arrival), drains, and returns result i in out[i]. */
+int pgpu_server_replay(pgpu_server* s, const uint32_t* tables, const void* const* probes, const uint64_t* ms,
+                       const double* offsets, uint64_t n, pgpu_served_result* out);
+int pgpu_server_stats_get(pgpu_server* s, pgpu_server_stats* st);
+/* The C5 query trace: query q's table (uniform over ntables), probe rows
+ * round(2^U(lo, hi)) and Poisson arrival offset (seconds, rate per second). */
+int pgpu_c5_trace(uint64_t seed, uint64_t count, uint32_t ntables, double lo, double hi, double rate,
+                  uint32_t* tables, uint64_t* ms, double* offsets);
+
+/* ------------------------------------------------------------------ */
 /* Row-format hash join with materialised output (gpuhashjoin).         */
 /* Replaces HashJoinOp's build (buildFromPage into util/hashtable,       */
 /* operators/join.cpp:641-667) and probe/projection loop                 */

@@ -289,6 +289,20 @@ static inline uint64_t pgpu_loguniform_size(uint64_t seed, uint64_t sub, uint64_
     double u = pgpu_unit(pgpu_rand(pgpu_stream_key(seed, PGPU_STREAM_SIZES, sub), idx));
     return (uint64_t)llround(exp2(lo + (hi - lo) * u));
 }
+
+/* C5: the table a query probes (uniform over ntables). */
+static inline uint32_t pgpu_c5_table(uint64_t seed, uint64_t q, uint64_t ntables)
+{
+    return (uint32_t)pgpu_uniform(pgpu_rand(pgpu_stream_key(seed, PGPU_STREAM_TABLEID, 0), q), ntables);
+}
+
+/* C5: exponential inter-arrival gap (seconds) before query q at `rate`
+ * queries/s; `sub` separates streams of different rates. */
+static inline double pgpu_c5_gap(uint64_t seed, uint64_t sub, uint64_t q, double rate)
+{
+    double u = pgpu_unit(pgpu_rand(pgpu_stream_key(seed, PGPU_STREAM_ARRIVALS, sub), q));
+    return -log1p(-u) / rate;
+}
 #endif
 
 #endif /* PGPU_GEN_H */

@@ -878,7 +878,7 @@ void orc_gen_c3(uint64_t seed, uint64_t nD, uint64_t nF, uint64_t norders, int32
 }
 
 typedef struct closed_arg {
-    uint64_t seed, sub, nR, lo, hi;
+    uint64_t seed, sub, nR, lo, hi, first; /* rows first + [lo, hi) */
     const double* cdf;
     const int32_t* inv; /* C1: key -> row */
     int kind;
@@ -889,7 +889,7 @@ static void* closed_worker(void* p)
 {
     closed_arg* a = (closed_arg*)p;
     orc_cs64 r = {0, 0, 0};
-    for (uint64_t j = a->lo; j < a->hi; ++j) {
+    for (uint64_t j = a->first + a->lo; j < a->first + a->hi; ++j) {
         if (a->kind == 1) {
             int32_t fk = pgpu_c1_sfk(a->seed, j, a->nR);
             int32_t row = a->inv[fk];
@@ -961,6 +961,18 @@ void orc_c2_closed(uint64_t seed, uint64_t sub, uint64_t nR, uint64_t nS, double
     a.kind = 2;
     run_closed(&a, nS, threads, out);
     free(cdf);
+}
+
+void orc_c5_closed(uint64_t seed, uint64_t table, uint64_t n, uint64_t first, uint64_t m, int threads, orc_cs64* out)
+{
+    closed_arg a;
+    memset(&a, 0, sizeof a);
+    a.seed = seed;
+    a.sub = table;
+    a.nR = n;
+    a.first = first;
+    a.kind = 2;
+    run_closed(&a, m, threads, out);
 }
 
 typedef struct c3_arg {

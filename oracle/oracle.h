@@ -166,6 +166,10 @@ void orc_c4_closed(uint64_t seed, uint64_t join, uint64_t n, uint64_t nprobe, or
 void orc_c4_closed_many(uint64_t seed, uint64_t first, uint64_t count, const uint64_t* n, const uint64_t* m,
                         int threads, orc_cs64* out);
 
+/* C5: probe rows first..first+m-1 of the uniform FK column of table `table`
+ * (n rows): count and sum of R.payload over the matches. */
+void orc_c5_closed(uint64_t seed, uint64_t table, uint64_t n, uint64_t first, uint64_t m, int threads, orc_cs64* out);
+
 /* ---------------- generator wrappers (pgpu_gen.h) ---------------------- */
 void orc_gen_c4(uint64_t seed, uint64_t join, uint64_t n, uint64_t m, int32_t* rkey, int32_t* rpay, int32_t* sfk,
                 int32_t* spay);

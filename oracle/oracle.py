@@ -118,6 +118,8 @@ def lib():
             "orc_gen_c2": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, P, P, P]),
             "orc_gen_c3": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, P, P, P, P]),
             "orc_zipf_cdf": (None, [C.c_uint64, C.c_double, P]),
+            "orc_c5_closed": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int,
+                                     C.POINTER(CS64)]),
             "orc_gen_c4": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, P, P, P, P]),
             "orc_baseline_join": (C.c_double, [C.c_int, P, P, C.c_uint64, P, C.c_uint64, C.c_uint, C.c_uint,
                                                C.c_int, C.POINTER(CS64)]),
@@ -308,6 +310,13 @@ def c4_closed_many(first, n, m, seed=42, threads=8):
     out = (CS64 * len(n))()
     lib().orc_c4_closed_many(seed, first, len(n), _ptr(n), _ptr(m), threads, out)
     return list(out)
+
+
+def c5_closed(table, n, first, m, seed=42, threads=8) -> CS64:
+    """Query over probe rows first.. of table `table`'s uniform FK column."""
+    r = CS64()
+    lib().orc_c5_closed(seed, table, n, first, m, threads, C.byref(r))
+    return r
 
 
 def baseline_join(kb, rk, rp, sk, buckets, tuplesperbucket, threads):

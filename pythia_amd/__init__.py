@@ -15,8 +15,11 @@ import numpy as np
 from . import _lib
 from ._lib import ALGO_AUTO, ALGO_GLOBAL, ALGO_RADIX, GEN, PgpuError, call, lib  # noqa: F401
 
-__all__ = ["Context", "DeviceBuffer", "JoinPlan", "GroupJoin", "BatchPlan", "batch_autotune", "device_count",
-           "PgpuError", "GEN", "ALGO_AUTO", "ALGO_RADIX", "ALGO_GLOBAL"]
+__all__ = ["Context", "DeviceBuffer", "JoinPlan", "GroupJoin", "BatchPlan", "batch_autotune", "Server",
+           "c5_trace", "device_count", "PgpuError", "GEN", "ALGO_AUTO", "ALGO_RADIX", "ALGO_GLOBAL",
+           "SERVE_BATCHED", "SERVE_PLAIN"]
+
+SERVE_BATCHED, SERVE_PLAIN = 0, 1
 
 
 def device_count() -> int:
@@ -272,3 +275,78 @@ def batch_autotune(ctx: Context, joins, key_bytes: int = 4, probe_payload: bool 
     call("pgpu_batch_autotune", ctx.handle, C.byref(base), arr, len(joins), reps, C.byref(best), ent, 16,
          C.byref(n))
     return (best.variant, best.part_rows), [(ent[i].variant, ent[i].part_rows, ent[i].ms) for i in range(n.value)]
+
+
+def _addr(b):
+    if b is None:
+        return None
+    return b.ptr if isinstance(b, DeviceBuffer) else int(b)
+
+
+def c5_trace(count: int, rate: float, ntables: int = 64, lo: float = 14, hi: float = 25, seed: int = 42):
+    """(tables, probe sizes, arrival offsets in seconds) of the served query stream."""
+    t = np.empty(count, np.uint32)
+    m = np.empty(count, np.uint64)
+    off = np.empty(count, np.float64)
+    call("pgpu_c5_trace", seed, count, ntables, lo, hi, rate, t.ctypes.data_as(C.c_void_p),
+         m.ctypes.data_as(C.c_void_p), off.ctypes.data_as(C.c_void_p))
+    return t, m, off
+
+
+class Server:
+    """Join + sum serving engine (resident tables, HBM-budget admission,
+    continuous batching; or plain per-query joins with mode=SERVE_PLAIN)."""
+
+    def __init__(self, ctx: Context, mode: int = SERVE_BATCHED, hbm_budget: int = 0, slots: int = 0,
+                 max_batch: int = 0, max_items: int = 0):
+        self.ctx = ctx
+        cfg = _lib.ServerCfg(hbm_budget, 8, mode, slots, max_batch, max_items, 0)
+        h = C.c_void_p()
+        call("pgpu_server_create", ctx.handle, C.byref(cfg), C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().pgpu_server_destroy(self.handle)
+            self.handle = None
+
+    def add_table(self, key, pay, n: int) -> int:
+        t = C.c_uint32()
+        call("pgpu_server_add_table", self.handle, C.c_void_p(_addr(key)), C.c_void_p(_addr(pay)), n, C.byref(t))
+        return t.value
+
+    def prepare_plain(self, max_probe: int):
+        call("pgpu_server_prepare_plain", self.handle, max_probe)
+
+    def submit(self, table: int, probe, m: int, tag: int = 0, t_arrival: float = 0.0):
+        call("pgpu_server_submit", self.handle, table, C.c_void_p(_addr(probe)), m, tag, t_arrival)
+
+    def drain(self):
+        call("pgpu_server_drain", self.handle)
+
+    def poll(self, cap: int = 4096):
+        out = (_lib.ServedResult * cap)()
+        n = C.c_int()
+        call("pgpu_server_poll", self.handle, out, cap, C.byref(n))
+        return [out[i] for i in range(n.value)]
+
+    def now(self) -> float:
+        t = C.c_double()
+        call("pgpu_server_now", self.handle, C.byref(t))
+        return t.value
+
+    def replay(self, tables, probes, ms, offsets):
+        n = len(tables)
+        t = np.ascontiguousarray(tables, dtype=np.uint32)
+        p = np.ascontiguousarray([_addr(x) for x in probes], dtype=np.uint64)
+        m = np.ascontiguousarray(ms, dtype=np.uint64)
+        o = np.ascontiguousarray(offsets, dtype=np.float64)
+        out = (_lib.ServedResult * max(n, 1))()
+        call("pgpu_server_replay", self.handle, t.ctypes.data_as(C.c_void_p), p.ctypes.data_as(C.c_void_p),
+             m.ctypes.data_as(C.c_void_p), o.ctypes.data_as(C.c_void_p), n, out)
+        return [out[i] for i in range(n)]
+
+    def stats(self):
+        st = _lib.ServerStats()
+        call("pgpu_server_stats_get", self.handle, C.byref(st))
+        return {f: getattr(st, f) for f, _ in st._fields_}

@@ -59,6 +59,22 @@ class TuneEntry(C.Structure):
     _fields_ = [("variant", C.c_uint32), ("part_rows", C.c_uint32), ("ms", C.c_double)]
 
 
+class ServerCfg(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("key_bytes", C.c_uint32), ("mode", C.c_uint32), ("slots", C.c_uint32),
+                ("max_batch", C.c_uint32), ("max_items", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class ServedResult(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("table", C.c_uint32), ("batch_size", C.c_uint32), ("m", C.c_uint64),
+                ("matches", C.c_uint64), ("sum_build", C.c_uint64), ("t_arrival", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("batch", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("tables", "resident_bytes", "reserved_bytes", "budget", "submitted",
+                                          "completed", "batches", "queued")]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -111,6 +127,19 @@ _PROTOS = {
     "pgpu_batch_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),
     "pgpu_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(BatchCfg), C.POINTER(BatchJoin), C.c_uint32, C.c_int,
                                       C.POINTER(BatchCfg), C.POINTER(TuneEntry), C.c_int, C.POINTER(C.c_int)]),
+    "pgpu_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
+    "pgpu_server_destroy": (C.c_int, [C.c_void_p]),
+    "pgpu_server_add_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32)]),
+    "pgpu_server_prepare_plain": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "pgpu_server_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_uint64, C.c_double]),
+    "pgpu_server_drain": (C.c_int, [C.c_void_p]),
+    "pgpu_server_poll": (C.c_int, [C.c_void_p, C.POINTER(ServedResult), C.c_int, C.POINTER(C.c_int)]),
+    "pgpu_server_now": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "pgpu_server_replay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                     C.POINTER(ServedResult)]),
+    "pgpu_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "pgpu_c5_trace": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                C.c_void_p, C.c_void_p, C.c_void_p]),
     "pgpu_groupjoin_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pgpu_groupjoin_destroy": (C.c_int, [C.c_void_p]),
     "pgpu_groupjoin_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,

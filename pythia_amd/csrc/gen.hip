@@ -117,4 +117,19 @@ extern "C" int pgpu_sizes(uint64_t seed, uint64_t sub, uint64_t first, uint64_t 
     for (uint64_t i = 0; i < count; ++i) out[i] = pgpu_loguniform_size(seed, sub, first + i, lo, hi);
     return PGPU_OK;
 }
+
+extern "C" int pgpu_c5_trace(uint64_t seed, uint64_t count, uint32_t ntables, double lo, double hi, double rate,
+                             uint32_t* tables, uint64_t* ms, double* offsets) {
+    using namespace pgpu;
+    if ((count && (!tables || !ms || !offsets)) || ntables == 0 || !(rate > 0) || !(lo <= hi) || hi > 40)
+        return fail(PGPU_EINVAL, "pgpu_c5_trace: bad argument");
+    double t = 0;
+    for (uint64_t q = 0; q < count; ++q) {
+        tables[q] = pgpu_c5_table(seed, q, ntables);
+        ms[q] = pgpu_loguniform_size(seed, 6, q, lo, hi);
+        t += pgpu_c5_gap(seed, (uint64_t)llround(rate), q, rate);
+        offsets[q] = t;
+    }
+    return PGPU_OK;
+}
 #endif

@@ -1,0 +1,162 @@
+"""GPU parity of the serving engine: resident tables, HBM-budget admission and
+continuous batching must not change any query's result.
+
+Expected values come from the oracle's closed forms (exact count and int64
+sum of each query over its table) and from exact numpy joins for tables with
+duplicate keys and the minimum integer as a real key.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pythia_amd as pa
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+NT = 8
+
+
+@pytest.fixture(scope="module")
+def world(ctx):
+    """NT tables (C2 family, sub = table) and 96 queries with uniform FKs."""
+    sizes = pa.sizes(5, NT, 10, 17)
+    tabs = []
+    for t, n in enumerate(int(x) for x in sizes):
+        k, p = ctx.alloc(8 * n), ctx.alloc(8 * n)
+        ctx.gen("C2_RKEY", n, k, sub=t)
+        ctx.gen("C2_RPAY", n, p, sub=t)
+        tabs.append((k, p, n))
+    qt, qm, _ = pa.c5_trace(96, 1000.0, ntables=NT, lo=8, hi=17)
+    probes = []
+    for q, (t, m) in enumerate(zip(qt.tolist(), qm.tolist())):
+        b = ctx.alloc(8 * m)
+        ctx.gen("C2_SFK_UNIFORM", m, b, sub=t, n_build=tabs[t][2], offset=q << 32)
+        probes.append(b)
+    ctx.sync()
+    exp = [O.c5_closed(t, tabs[t][2], q << 32, m, threads=THREADS) for q, (t, m) in
+           enumerate(zip(qt.tolist(), qm.tolist()))]
+    yield tabs, qt, qm, probes, exp
+    for b in probes:
+        b.free()
+    for k, p, _ in tabs:
+        k.free()
+        p.free()
+
+
+def _server(ctx, tabs, **kw):
+    s = pa.Server(ctx, **kw)
+    ids = [s.add_table(k, p, n) for k, p, n in tabs]
+    assert ids == list(range(len(tabs)))
+    return s
+
+
+def test_served_results_alone_and_in_the_mix(ctx, world):
+    """Every query gives its closed-form result when the whole stream arrives
+    at once (continuous batching mixes them into shared launches) and when
+    it runs alone."""
+    tabs, qt, qm, probes, exp = world
+    s = _server(ctx, tabs)
+    mix = s.replay(qt, probes, qm, np.zeros(len(qt)))
+    assert [(r.matches, r.sum_build) for r in mix] == [(e.count, e.sum_build) for e in exp]
+    assert max(r.batch_size for r in mix) > 1  # queries shared launches
+    assert all(r.t_arrival <= r.t_admit <= r.t_done for r in mix)
+    alone = []
+    for q in range(len(qt)):
+        s.submit(int(qt[q]), probes[q], int(qm[q]), tag=q)
+        s.drain()
+        (r,) = s.poll()
+        assert r.batch_size == 1 and r.tag == q
+        alone.append((r.matches, r.sum_build))
+    assert alone == [(r.matches, r.sum_build) for r in mix]
+    st = s.stats()
+    assert st["completed"] == st["submitted"] == 2 * len(qt) and st["reserved_bytes"] == 0
+    s.close()
+
+
+def test_poisson_replay_latencies(ctx, world):
+    tabs, qt, qm, probes, exp = world
+    _, _, off = pa.c5_trace(len(qt), 2000.0, ntables=NT, lo=8, hi=17)
+    s = _server(ctx, tabs, slots=2)
+    res = s.replay(qt, probes, qm, off)
+    assert [(r.matches, r.sum_build) for r in res] == [(e.count, e.sum_build) for e in exp]
+    lat = np.array([r.t_done - r.t_arrival for r in res])
+    assert (lat > 0).all() and np.median(lat) < 0.5
+    s.close()
+
+
+def test_plain_mode_gives_the_same_results(ctx, world):
+    tabs, qt, qm, probes, exp = world
+    s = _server(ctx, tabs, mode=pa.SERVE_PLAIN)
+    s.prepare_plain(int(qm.max()))
+    res = s.replay(qt[:32], probes[:32], qm[:32], np.zeros(32))
+    assert [(r.matches, r.sum_build) for r in res] == [(e.count, e.sum_build) for e in exp[:32]]
+    assert all(r.batch_size == 1 for r in res)
+    s.close()
+
+
+def test_hbm_budget_admission(ctx, world):
+    """A budget that holds the tables plus about one probe batch admits the
+    stream a few queries at a time; results do not change, and a probe that
+    can never fit is refused."""
+    tabs, qt, qm, probes, exp = world
+    probe = pa.Server(ctx)
+    for k, p, n in tabs:
+        probe.add_table(k, p, n)
+    resident = probe.stats()["resident_bytes"]
+    probe.close()
+    budget = resident + int(qm.max()) * 8 + 64
+    s = _server(ctx, tabs, hbm_budget=budget)
+    res = s.replay(qt, probes, qm, np.zeros(len(qt)))
+    assert [(r.matches, r.sum_build) for r in res] == [(e.count, e.sum_build) for e in exp]
+    big = int(qm.max())
+    # admitted batches never held more probe bytes than the budget leaves
+    by_batch = {}
+    for r in res:
+        by_batch.setdefault(r.batch, 0)
+        by_batch[r.batch] += r.m * 8 + 64
+    assert max(by_batch.values()) <= budget - resident
+    with pytest.raises(pa.PgpuError):
+        s.submit(0, probes[0], (budget - resident) // 8 + 1)
+    assert big * 8 + 64 <= budget - resident
+    s.close()
+
+
+def test_duplicates_and_empty_marker_key(ctx):
+    """Tables with repeated keys fold them into (count, sum); the minimum
+    int64 (the table's empty marker) is still a joinable key."""
+    rng = np.random.default_rng(5)
+    lo = np.iinfo(np.int64).min
+    bk = rng.integers(-2000, 2000, 30000).astype(np.int64)
+    bk[:4] = lo
+    bp = rng.integers(-2**40, 2**40, 30000).astype(np.int64)
+    pk = rng.integers(-2500, 2500, 70000).astype(np.int64)
+    pk[:6] = lo
+    k, p, q = ctx.upload(bk), ctx.upload(bp), ctx.upload(pk)
+    s = pa.Server(ctx)
+    t = s.add_table(k, p, len(bk))
+    s.submit(t, q, len(pk))
+    s.submit(t, q, 1000)
+    s.drain()
+    got = sorted((r.m, r.matches, r.sum_build) for r in s.poll())
+    keys, inv = np.unique(bk, return_inverse=True)
+    cnt = np.bincount(inv)
+    sums = np.zeros(len(keys), np.uint64)
+    np.add.at(sums, inv, bp.view(np.uint64))
+
+    def exact(probe):
+        idx = np.searchsorted(keys, probe)
+        ok = idx < len(keys)
+        ok[ok] = keys[idx[ok]] == probe[ok]
+        return int(cnt[idx[ok]].sum()), int(sums[idx[ok]].sum(dtype=np.uint64))
+
+    assert got == sorted([(len(pk),) + exact(pk), (1000,) + exact(pk[:1000])])
+    # the chained-bucket restatement agrees on count and the int64 sum
+    bs, ps = O.schema(("long",), ("long",)), O.schema(("long",))
+    d = O.join_desc(bs, ps, 0, 0, ["B$1"], buckets=1 << 10, tuplesperbucket=4, threads=2)
+    cs, _ = O.hashjoin_count_sum(d, O.rows(bs, bk, bp), O.rows(ps, pk), 0)
+    assert (cs.count, cs.isum % 2**64) == exact(pk)
+    s.close()
+    for b in (k, p, q):
+        b.free()

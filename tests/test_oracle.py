@@ -366,3 +366,17 @@ def test_c3_closed_form_matches_oracle_join_aggregate():
     for c, s in zip(agg["c0"], agg["c1"]):
         exact = cents[c] / 100.0
         assert abs(s - exact) <= 1e-12 * abs(exact)
+
+
+def test_c5_closed_form_is_c2_family_with_offsets():
+    """A served query reads rows first.. of a table's uniform FK column: with
+    first = 0 it is the C2 closed form; with an offset it equals a direct
+    computation over the generated table."""
+    n, m = 1 << 12, 5000
+    a, b = O.c5_closed(3, n, 0, m), O.c2_closed(n, m, sub=3)
+    assert (a.count, a.sum_build) == (b.count, b.sum_build)
+    rk, rp, sk = O.gen_c2(n, (1 << 13) + m, sub=3)
+    pos = {int(k): i for i, k in enumerate(rk)}
+    tail = sk[1 << 13:]
+    c = O.c5_closed(3, n, 1 << 13, m, threads=3)
+    assert c.count == m and c.sum_build == int(sum(int(rp[pos[int(k)]]) for k in tail)) % 2**64
