@@ -442,7 +442,7 @@ def run_c5(args):
     return line
 
 
-def main():def main():
+def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
