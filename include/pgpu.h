@@ -246,8 +246,10 @@ typedef struct pgpu_server_cfg {
     uint32_t mode;        /* enum pgpu_serve_mode                                       */
     uint32_t slots;       /* batches in flight at once (streams); 0 = 3                 */
     uint32_t max_batch;   /* queries per launch; 0 = 256                                */
-    uint32_t max_items;   /* 64K-row work items per launch; 0 = 32768                   */
-    uint32_t pad;
+    uint32_t max_items;   /* 64K-row random-probe work items per launch; 0 = 32768      */
+    uint32_t part_min_rows; /* probe batches of at least this many rows (and a quarter */
+                            /* of their table) take the partitioned path; 0 = 2^20,    */
+                            /* 1 = every batch, UINT32_MAX = none                      */
 } pgpu_server_cfg;
 
 typedef struct pgpu_served_result {
@@ -266,6 +268,7 @@ typedef struct pgpu_served_result {
 typedef struct pgpu_server_stats {
     uint64_t tables, resident_bytes, reserved_bytes, budget;
     uint64_t submitted, completed, batches, queued;
+    uint64_t partitioned;   /* queries that took the partitioned path */
 } pgpu_server_stats;
 
 typedef struct pgpu_server pgpu_server;

@@ -298,9 +298,9 @@ class Server:
     continuous batching; or plain per-query joins with mode=SERVE_PLAIN)."""
 
     def __init__(self, ctx: Context, mode: int = SERVE_BATCHED, hbm_budget: int = 0, slots: int = 0,
-                 max_batch: int = 0, max_items: int = 0):
+                 max_batch: int = 0, max_items: int = 0, part_min_rows: int = 0):
         self.ctx = ctx
-        cfg = _lib.ServerCfg(hbm_budget, 8, mode, slots, max_batch, max_items, 0)
+        cfg = _lib.ServerCfg(hbm_budget, 8, mode, slots, max_batch, max_items, part_min_rows)
         h = C.c_void_p()
         call("pgpu_server_create", ctx.handle, C.byref(cfg), C.byref(h))
         self.handle = h

@@ -61,7 +61,7 @@ class TuneEntry(C.Structure):
 
 class ServerCfg(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("key_bytes", C.c_uint32), ("mode", C.c_uint32), ("slots", C.c_uint32),
-                ("max_batch", C.c_uint32), ("max_items", C.c_uint32), ("pad", C.c_uint32)]
+                ("max_batch", C.c_uint32), ("max_items", C.c_uint32), ("part_min_rows", C.c_uint32)]
 
 
 class ServedResult(C.Structure):
@@ -72,7 +72,7 @@ class ServedResult(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("tables", "resident_bytes", "reserved_bytes", "budget", "submitted",
-                                          "completed", "batches", "queued")]
+                                          "completed", "batches", "queued", "partitioned")]
 
 
 class KernelTime(C.Structure):

@@ -9,16 +9,25 @@
 // analogue of a resident table is HashTable::serialize / deserialize
 // (util/hashtable.cpp:99-201).
 //
-// Served mode: each registered table is hashed once into an open-addressing
-// table in HBM (16-byte {key, folded payload sum} slots at load <= 0.5; a
-// per-slot count array only when the table holds duplicate keys).  A
-// scheduler thread owns a few in-flight slots, each with its own stream:
-// whenever a slot is free it admits the queued queries that fit the HBM
-// budget (FIFO), cuts their probe rows into work items and launches one
-// probe kernel over all of them, so queries that arrive while others run
-// join the next launch and overlap on the device.  Plain mode runs each
-// query alone through the per-query join plan (pgpu_join_count_sum), which
-// builds the table from its columns every time.
+// Served mode keeps every registered table resident in two forms:
+//   - an open-addressing hash table in HBM (16-byte {key, folded payload
+//     sum} slots at load <= 0.5; a per-slot count array is read only when
+//     the table holds duplicate keys), probed directly by small queries;
+//   - its rows radix-partitioned into LDS-sized partitions (the batched
+//     join's layout), joined by large queries after partitioning their
+//     probe rows the same way: streaming instead of random HBM access.
+// A scheduler thread owns a few in-flight slots, each with its own stream
+// and staging.  Whenever a slot is free it admits the queued queries that
+// fit the HBM budget (FIFO) and launches them together: one random-probe
+// kernel over the small queries' row ranges, one segmented partition pass
+// plus one persistent LDS join over the large ones, and a finish kernel
+// that writes the results to pinned host memory.  Queries that arrive
+// while others run join the next launch and overlap on the device.  Work
+// descriptors are read by the kernels from pinned host memory, so no DMA
+// copy sits on a query's path.
+//
+// Plain mode runs each query alone through the per-query join plan
+// (pgpu_join_count_sum), which builds the table from its columns each time.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,17 +38,24 @@
 #include <deque>
 #include <thread>
 
+#include "batch_internal.h"
 #include "join_common.h"
-#include "pgpu_internal.h"
 
 using namespace pgpu;
 using namespace pgpu::dev;
+using pgpu::batch::Chunk;
+using pgpu::batch::Job;
+using pgpu::batch::kChunkRows;
+using pgpu::batch::Seg;
 
 namespace {
 
-constexpr uint64_t kItemRows = 65536;  // probe rows per work item
+constexpr uint64_t kItemRows = 65536;  // probe rows per random-probe work item
 constexpr int kProbeThreads = 256;
-constexpr int kProbeU = 8;  // independent probes in flight per thread
+constexpr int kProbeU = 8;             // independent probes in flight per thread
+constexpr int kVariant = 2;            // LDS join shape of the partitioned path
+constexpr uint32_t kPartTarget = 3584; // build rows per resident partition (one LDS piece
+                                       // holds 4096, so a partition rarely needs two)
 
 struct __align__(16) SSlot {
     long long key;
@@ -58,7 +74,7 @@ struct TDesc {
 struct QItem {
     const long long* probe;
     uint64_t r0, r1;
-    uint32_t q;      // query index within the batch
+    uint32_t q;      // query index within the batch (result slot)
     uint32_t table;
 };
 
@@ -156,10 +172,22 @@ __global__ __launch_bounds__(kProbeThreads) void k_sprobe(const QItem* __restric
             b += red[1][i];
         }
         if (c) {
-            atomicAdd(&res[2 * (uint64_t)it.q], c);
-            atomicAdd(&res[2 * (uint64_t)it.q + 1], b);
+            atomicAdd(&res[4 * (uint64_t)it.q], c);
+            atomicAdd(&res[4 * (uint64_t)it.q + 1], b);
         }
     }
+}
+
+// Copies the batch's (count, sum) pairs to pinned host memory and leaves the
+// device accumulators and the join queue at zero for the slot's next batch.
+__global__ __launch_bounds__(256) void k_sfinish(unsigned long long* __restrict__ res, uint32_t nq,
+                                                 unsigned long long* __restrict__ host, uint32_t* __restrict__ queue) {
+    for (uint32_t i = threadIdx.x; i < nq; i += 256) {
+        host[2 * i] = res[4 * i];
+        host[2 * i + 1] = res[4 * i + 1];
+        res[4 * i] = res[4 * i + 1] = res[4 * i + 2] = res[4 * i + 3] = 0;
+    }
+    if (threadIdx.x == 0) *queue = 0;
 }
 
 __global__ __launch_bounds__(256) void k_szero(unsigned long long* __restrict__ p, uint64_t n) {
@@ -191,15 +219,39 @@ struct Table {
     unsigned long long* special = nullptr;
     uint64_t cap = 0;
     uint64_t bytes = 0;
+    uint32_t bits = 0;      // partition bits of the resident partitioned rows
+    uint64_t rdst = 0;      // first row in the resident partitioned-row arena
+    uint64_t rbnd = 0;      // first bound word in the resident bounds arena
+};
+
+// Per-slot capacities of the partitioned path.
+struct Caps {
+    uint64_t rows = 1ull << 26;      // probe rows partitioned per launch
+    uint64_t hist = 1ull << 23;      // histogram words
+    uint64_t bnd = 1ull << 20;       // bound words
+    uint64_t chunks = 4096;
+    uint64_t pitems = 1ull << 20;    // join work items
+};
+
+// Pinned, device-readable staging of one slot (one allocation).
+struct Staging {
+    QItem* items;
+    Seg* segs;
+    Chunk* chunks;
+    Job* jobs;
+    uint32_t* item_job;
+    unsigned long long* res;  // 2 words per query, written by k_sfinish
 };
 
 struct Slot {
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
-    QItem* h_items = nullptr;           // pinned
-    QItem* d_items = nullptr;
-    unsigned long long* h_res = nullptr;  // pinned
-    unsigned long long* d_res = nullptr;
+    void* host = nullptr;   // the pinned block
+    Staging h{}, d{};       // host / device views of it
+    unsigned long long* d_res = nullptr;  // [max_batch][4] accumulators + the queue word
+    uint32_t* H = nullptr;
+    uint32_t* bnd = nullptr;
+    void* SP = nullptr;
     std::vector<Query*> qs;
     bool busy = false;
 };
@@ -209,9 +261,14 @@ struct Slot {
 struct pgpu_server {
     pgpu_ctx* ctx = nullptr;
     pgpu_server_cfg cfg{};
+    Caps caps;
     std::vector<Table> tables;
     TDesc* d_tabs = nullptr;
-    bool tabs_dirty = true;
+    void* R = nullptr;        // resident partitioned rows (Row<int64, true>)
+    uint64_t R_used = 0, R_cap = 0;
+    uint32_t* RB = nullptr;   // resident partition bounds
+    uint64_t RB_used = 0, RB_cap = 0;
+    int join_grid = 0;
     uint64_t resident = 0;
     uint64_t reserved = 0;  // HBM held by admitted, unfinished queries
     std::vector<Slot> slots;
@@ -221,7 +278,7 @@ struct pgpu_server {
     std::condition_variable cv;
     std::deque<Query*> pending;
     std::deque<Query*> finished;
-    uint64_t submitted = 0, completed = 0, batches = 0;
+    uint64_t submitted = 0, completed = 0, batches = 0, part_queries = 0;
     std::thread th;
     bool stop = false;
     int error = PGPU_OK;
@@ -231,8 +288,6 @@ struct pgpu_server {
 
 namespace {
 
-uint32_t max_items_per_slot(const pgpu_server* s) { return std::max<uint32_t>(s->cfg.max_items, 1); }
-
 void set_error(pgpu_server* s, int rc) {
     std::lock_guard<std::mutex> g(s->mu);
     if (s->error == PGPU_OK) {
@@ -241,14 +296,21 @@ void set_error(pgpu_server* s, int rc) {
     }
 }
 
+bool partitioned(const pgpu_server* s, const Query* q) {
+    const uint64_t pmin = s->cfg.part_min_rows ? s->cfg.part_min_rows : (1u << 20);
+    if (pmin == UINT32_MAX || q->m == 0) return false;
+    const Table& t = s->tables[q->table];
+    return q->m >= pmin && (pmin == 1 || 4 * q->m >= t.n) && q->m <= s->caps.rows;
+}
+
 // Completes the queries of a slot whose event has fired.
 void reap(pgpu_server* s, Slot& sl) {
     const double t = now_s();
     std::lock_guard<std::mutex> g(s->mu);
     for (size_t i = 0; i < sl.qs.size(); ++i) {
         Query* q = sl.qs[i];
-        q->matches = sl.h_res[2 * i];
-        q->sum = sl.h_res[2 * i + 1];
+        q->matches = sl.h.res[2 * i];
+        q->sum = sl.h.res[2 * i + 1];
         q->t_done = t;
         s->reserved -= q->reserve;
         s->finished.push_back(q);
@@ -263,47 +325,98 @@ void reap(pgpu_server* s, Slot& sl) {
 // no lock held; returns true if it launched something.
 bool launch_batch(pgpu_server* s, Slot& sl) {
     std::vector<Query*> qs;
-    uint64_t items = 0;
+    uint64_t items = 0, rows = 0, hist = 0, bnd = 0, chunks = 0, pitems = 0;
     const double t = now_s();
     {
         std::lock_guard<std::mutex> g(s->mu);
         while (!s->pending.empty() && qs.size() < s->cfg.max_batch) {
             Query* q = s->pending.front();
-            const uint64_t qi = std::max<uint64_t>(1, ceil_div(q->m, kItemRows));
-            if (items + qi > max_items_per_slot(s)) break;
             if (s->resident + s->reserved + q->reserve > s->cfg.hbm_budget) break;  // FIFO: the head waits
+            if (partitioned(s, q)) {
+                const uint32_t bits = s->tables[q->table].bits;
+                const uint64_t nc = ceil_div(q->m, kChunkRows);
+                if (rows + q->m > s->caps.rows || hist + (nc << bits) > s->caps.hist ||
+                    bnd + (1ull << bits) + 1 > s->caps.bnd || chunks + nc > s->caps.chunks ||
+                    pitems + (1ull << bits) > s->caps.pitems)
+                    break;
+                rows += q->m;
+                hist += nc << bits;
+                bnd += (1ull << bits) + 1;
+                chunks += nc;
+                pitems += 1ull << bits;
+            } else {
+                const uint64_t qi = ceil_div(q->m, kItemRows);
+                if (items + qi > s->cfg.max_items) break;
+                items += qi;
+            }
             s->pending.pop_front();
             s->reserved += q->reserve;
             q->t_admit = t;
             qs.push_back(q);
-            items += qi;
         }
     }
     if (qs.empty()) return false;
     ctx_activate(s->ctx);
     const uint32_t bid = (uint32_t)s->batches++;
-    uint32_t ni = 0;
+    uint32_t ni = 0, nseg = 0, nchunk = 0, nitem = 0;
+    uint64_t prow = 0, phist = 0, pbnd = 0;
     for (uint32_t i = 0; i < qs.size(); ++i) {
         Query* q = qs[i];
         q->batch = bid;
         q->batch_size = (uint32_t)qs.size();
-        for (uint64_t r = 0; r < q->m; r += kItemRows)
-            sl.h_items[ni++] = QItem{q->probe, r, std::min(q->m, r + kItemRows), i, q->table};
+        if (!partitioned(s, q)) {
+            for (uint64_t r = 0; r < q->m; r += kItemRows)
+                sl.h.items[ni++] = QItem{q->probe, r, std::min(q->m, r + kItemRows), i, q->table};
+            continue;
+        }
+        const Table& tb = s->tables[q->table];
+        Seg sg{};
+        sg.key = q->probe;
+        sg.pay = nullptr;
+        sg.n = q->m;
+        sg.dst = prow;
+        sg.hist = phist;
+        sg.bnd = pbnd;
+        sg.chunk0 = nchunk;
+        sg.nchunks = (uint32_t)ceil_div(q->m, kChunkRows);
+        sg.bits = tb.bits;
+        for (uint32_t c = 0; c < sg.nchunks; ++c) sl.h.chunks[nchunk++] = Chunk{nseg, 0, (uint64_t)c * kChunkRows};
+        Job jb{};
+        jb.rdst = tb.rdst;
+        jb.rbnd = tb.rbnd;
+        jb.sdst = prow;
+        jb.sbnd = pbnd;
+        jb.bits = tb.bits;
+        jb.item0 = nitem;
+        for (uint32_t d = 0; d < (1u << tb.bits); ++d) sl.h.item_job[nitem++] = i;
+        sl.h.jobs[i] = jb;
+        sl.h.segs[nseg++] = sg;
+        prow += q->m;
+        phist += (uint64_t)sg.nchunks << tb.bits;
+        pbnd += (1ull << tb.bits) + 1;
     }
+    std::atomic_thread_fence(std::memory_order_seq_cst);  // staging written before the launches read it
     int rc = PGPU_OK;
-    const uint64_t nres = 2 * qs.size();
-    hipLaunchKernelGGL(k_szero, dim3((unsigned)std::min<uint64_t>(ceil_div(nres, 256), 64)), dim3(256), 0, sl.st,
-                       sl.d_res, nres);
-    if (ni) {
-        if (hipMemcpyAsync(sl.d_items, sl.h_items, ni * sizeof(QItem), hipMemcpyHostToDevice, sl.st) != hipSuccess)
-            rc = fail(PGPU_EDEVICE, "item upload failed");
-        hipLaunchKernelGGL(k_sprobe, dim3(ni), dim3(kProbeThreads), 0, sl.st, sl.d_items, s->d_tabs, sl.d_res);
-    }
+    uint32_t* queue = (uint32_t*)(sl.d_res + 4ull * s->cfg.max_batch);
+    if (ni) hipLaunchKernelGGL(k_sprobe, dim3(ni), dim3(kProbeThreads), 0, sl.st, sl.d.items, s->d_tabs, sl.d_res);
     if (rc == PGPU_OK && hipGetLastError() != hipSuccess) rc = fail(PGPU_EDEVICE, "probe launch failed");
-    if (rc == PGPU_OK &&
-        hipMemcpyAsync(sl.h_res, sl.d_res, nres * 8, hipMemcpyDeviceToHost, sl.st) != hipSuccess)
-        rc = fail(PGPU_EDEVICE, "result copy failed");
+    if (rc == PGPU_OK && nseg) {
+        rc = batch::launch_hist(sl.st, 8, sl.d.segs, sl.d.chunks, 0, nchunk, sl.H);
+        if (rc == PGPU_OK) rc = batch::launch_scan(sl.st, sl.d.segs, 0, nseg, sl.H, sl.bnd);
+        if (rc == PGPU_OK) rc = batch::launch_scatter(sl.st, 8, false, sl.d.segs, sl.d.chunks, 0, nchunk, sl.H, sl.SP);
+        if (rc == PGPU_OK)
+            rc = batch::launch_join(sl.st, 8, false, kVariant, std::min<int>(s->join_grid, (int)nitem), sl.d.jobs,
+                                    sl.d.item_job, nitem, s->R, sl.SP, s->RB, sl.bnd, queue, sl.d_res);
+    }
+    if (rc == PGPU_OK) {
+        hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, sl.st, sl.d_res, (uint32_t)qs.size(), sl.d.res, queue);
+        if (hipGetLastError() != hipSuccess) rc = fail(PGPU_EDEVICE, "finish launch failed");
+    }
     if (rc == PGPU_OK && hipEventRecord(sl.ev, sl.st) != hipSuccess) rc = fail(PGPU_EDEVICE, "event record failed");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->part_queries += nseg;
+    }
     sl.qs = std::move(qs);
     sl.busy = true;
     if (rc != PGPU_OK) set_error(s, rc);
@@ -387,8 +500,97 @@ int upload_tabs(pgpu_server* s) {
     s->d_tabs = nullptr;
     PGPU_TRY(ctx_alloc(s->ctx, std::max<size_t>(h.size(), 1) * sizeof(TDesc), (void**)&s->d_tabs));
     PGPU_HIP(hipMemcpy(s->d_tabs, h.data(), h.size() * sizeof(TDesc), hipMemcpyHostToDevice));
-    s->tabs_dirty = false;
     return PGPU_OK;
+}
+
+// Grows a resident arena to hold `need` elements of `es` bytes, keeping
+// its contents.
+int grow(pgpu_server* s, void** p, uint64_t* cap, uint64_t used, uint64_t need, uint64_t es) {
+    if (need <= *cap) return PGPU_OK;
+    const uint64_t ncap = std::max(need, 2 * *cap);
+    void* np = nullptr;
+    PGPU_TRY(ctx_alloc(s->ctx, ncap * es, &np));
+    if (*p && used) {
+        if (hipMemcpy(np, *p, used * es, hipMemcpyDeviceToDevice) != hipSuccess) {
+            ctx_free(s->ctx, np);
+            return fail(PGPU_EDEVICE, "resident arena copy failed");
+        }
+    }
+    if (*p) PGPU_TRY(ctx_free(s->ctx, *p));
+    s->resident += (ncap - *cap) * es;
+    *p = np;
+    *cap = ncap;
+    return PGPU_OK;
+}
+
+// Partitions a table's rows into the resident arena (the batched join's
+// segmented partition pass over one segment).
+int partition_table(pgpu_server* s, Table& t) {
+    t.bits = batch::bits_for(t.n, kPartTarget);
+    const uint64_t nb = (1ull << t.bits) + 1;
+    PGPU_TRY(grow(s, &s->R, &s->R_cap, s->R_used, s->R_used + std::max<uint64_t>(t.n, 1), 16));
+    PGPU_TRY(grow(s, (void**)&s->RB, &s->RB_cap, s->RB_used, s->RB_used + nb, 4));
+    t.rdst = s->R_used;
+    t.rbnd = s->RB_used;
+    batch::Layout lay;
+    lay.add(t.key, t.pay, t.n, t.bits, 0);
+    Seg sg = lay.segs[0];
+    sg.dst = t.rdst;
+    sg.bnd = t.rbnd;
+    void *d_seg = nullptr, *d_chunks = nullptr, *d_H = nullptr;
+    int rc = ctx_alloc(s->ctx, sizeof(Seg), &d_seg);
+    if (rc == PGPU_OK) rc = ctx_alloc(s->ctx, std::max<size_t>(lay.chunks.size(), 1) * sizeof(Chunk), &d_chunks);
+    if (rc == PGPU_OK) rc = ctx_alloc(s->ctx, std::max<uint64_t>(lay.hist, 1) * 4, &d_H);
+    hipStream_t st = s->ctx->stream;
+    if (rc == PGPU_OK &&
+        (hipMemcpy(d_seg, &sg, sizeof sg, hipMemcpyHostToDevice) != hipSuccess ||
+         (!lay.chunks.empty() &&
+          hipMemcpy(d_chunks, lay.chunks.data(), lay.chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice) !=
+              hipSuccess)))
+        rc = fail(PGPU_EDEVICE, "table partition upload failed");
+    const uint32_t nc = (uint32_t)lay.chunks.size();
+    if (rc == PGPU_OK) rc = batch::launch_hist(st, 8, (Seg*)d_seg, (Chunk*)d_chunks, 0, nc, (uint32_t*)d_H);
+    if (rc == PGPU_OK) rc = batch::launch_scan(st, (Seg*)d_seg, 0, 1, (uint32_t*)d_H, s->RB);
+    if (rc == PGPU_OK)
+        rc = batch::launch_scatter(st, 8, true, (Seg*)d_seg, (Chunk*)d_chunks, 0, nc, (uint32_t*)d_H, s->R);
+    if (rc == PGPU_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(PGPU_EDEVICE, "table partition failed");
+    if (d_seg) ctx_free(s->ctx, d_seg);
+    if (d_chunks) ctx_free(s->ctx, d_chunks);
+    if (d_H) ctx_free(s->ctx, d_H);
+    if (rc != PGPU_OK) return rc;
+    s->R_used += std::max<uint64_t>(t.n, 1);
+    s->RB_used += nb;
+    return PGPU_OK;
+}
+
+template <typename T>
+T* carve(char*& p, uint64_t n) {
+    T* r = (T*)p;
+    p += (n * sizeof(T) + 255) & ~uint64_t(255);
+    return r;
+}
+
+Staging carve_all(char* base, const pgpu_server_cfg& c, const Caps& k) {
+    Staging s;
+    char* p = base;
+    s.items = carve<QItem>(p, c.max_items);
+    s.segs = carve<Seg>(p, c.max_batch);
+    s.chunks = carve<Chunk>(p, k.chunks);
+    s.jobs = carve<Job>(p, c.max_batch);
+    s.item_job = carve<uint32_t>(p, k.pitems);
+    s.res = carve<unsigned long long>(p, 2ull * c.max_batch);
+    return s;
+}
+
+uint64_t staging_bytes(const pgpu_server_cfg& c, const Caps& k) {
+    uint64_t b = 0;
+    b += (c.max_items * sizeof(QItem) + 255) & ~uint64_t(255);
+    b += (c.max_batch * sizeof(Seg) + 255) & ~uint64_t(255);
+    b += (k.chunks * sizeof(Chunk) + 255) & ~uint64_t(255);
+    b += (c.max_batch * sizeof(Job) + 255) & ~uint64_t(255);
+    b += (k.pitems * 4 + 255) & ~uint64_t(255);
+    b += (2ull * c.max_batch * 8 + 255) & ~uint64_t(255);
+    return b;
 }
 
 }  // namespace
@@ -414,17 +616,32 @@ int pgpu_server_create(pgpu_ctx* ctx, const pgpu_server_cfg* cfg, pgpu_server** 
         pgpu_server_destroy(s);
         return rc;
     };
+    if (s->cfg.slots) {
+        int rc = batch::join_grid(8, false, kVariant, &s->join_grid);
+        if (rc != PGPU_OK) return cleanup(rc);
+    }
     s->slots.resize(s->cfg.slots);
+    const uint64_t sbytes = staging_bytes(s->cfg, s->caps);
     for (Slot& sl : s->slots) {
         if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess)
             return cleanup(fail(PGPU_EDEVICE, "stream / event creation failed"));
-        if (hipHostMalloc((void**)&sl.h_items, s->cfg.max_items * sizeof(QItem), hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void**)&sl.h_res, 2ull * s->cfg.max_batch * 8, hipHostMallocDefault) != hipSuccess)
+        if (hipHostMalloc(&sl.host, sbytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
             return cleanup(fail(PGPU_ENOMEM, "pinned staging allocation failed"));
-        int rc = ctx_alloc(ctx, s->cfg.max_items * sizeof(QItem), (void**)&sl.d_items);
-        if (rc == PGPU_OK) rc = ctx_alloc(ctx, 2ull * s->cfg.max_batch * 8, (void**)&sl.d_res);
+        void* dview = nullptr;
+        if (hipHostGetDevicePointer(&dview, sl.host, 0) != hipSuccess)
+            return cleanup(fail(PGPU_EDEVICE, "hipHostGetDevicePointer failed"));
+        sl.h = carve_all((char*)sl.host, s->cfg, s->caps);
+        sl.d = carve_all((char*)dview, s->cfg, s->caps);
+        const uint64_t res_words = 4ull * s->cfg.max_batch + 1;
+        int rc = ctx_alloc(ctx, res_words * 8, (void**)&sl.d_res);
+        if (rc == PGPU_OK) rc = ctx_alloc(ctx, s->caps.hist * 4, (void**)&sl.H);
+        if (rc == PGPU_OK) rc = ctx_alloc(ctx, s->caps.bnd * 4, (void**)&sl.bnd);
+        if (rc == PGPU_OK) rc = ctx_alloc(ctx, s->caps.rows * 8, &sl.SP);
         if (rc != PGPU_OK) return cleanup(rc);
+        s->resident += res_words * 8 + s->caps.hist * 4 + s->caps.bnd * 4 + s->caps.rows * 8;
+        hipLaunchKernelGGL(k_szero, dim3(4), dim3(256), 0, sl.st, sl.d_res, res_words);
+        if (hipStreamSynchronize(sl.st) != hipSuccess) return cleanup(fail(PGPU_EDEVICE, "slot init failed"));
     }
     s->th = std::thread(scheduler, s);
     *out = s;
@@ -446,10 +663,9 @@ int pgpu_server_destroy(pgpu_server* s) {
         if (sl.st) (void)hipStreamSynchronize(sl.st);
         if (sl.ev) (void)hipEventDestroy(sl.ev);
         if (sl.st) (void)hipStreamDestroy(sl.st);
-        if (sl.h_items) (void)hipHostFree(sl.h_items);
-        if (sl.h_res) (void)hipHostFree(sl.h_res);
-        if (sl.d_items) (void)ctx_free(s->ctx, sl.d_items);
-        if (sl.d_res) (void)ctx_free(s->ctx, sl.d_res);
+        if (sl.host) (void)hipHostFree(sl.host);
+        for (void* p : {(void*)sl.d_res, (void*)sl.H, (void*)sl.bnd, sl.SP})
+            if (p) (void)ctx_free(s->ctx, p);
     }
     for (Table& t : s->tables) {
         if (t.slots) (void)ctx_free(s->ctx, t.slots);
@@ -457,6 +673,8 @@ int pgpu_server_destroy(pgpu_server* s) {
         if (t.special) (void)ctx_free(s->ctx, t.special);
     }
     if (s->d_tabs) (void)ctx_free(s->ctx, s->d_tabs);
+    if (s->R) (void)ctx_free(s->ctx, s->R);
+    if (s->RB) (void)ctx_free(s->ctx, s->RB);
     if (s->plain) pgpu_join_plan_destroy(s->plain);
     for (Query* q : s->pending) delete q;
     for (Query* q : s->finished) delete q;
@@ -467,6 +685,7 @@ int pgpu_server_destroy(pgpu_server* s) {
 int pgpu_server_add_table(pgpu_server* s, const void* d_key, const void* d_pay, uint64_t n, uint32_t* table_id) {
     if (!s || !table_id || (n && (!d_key || !d_pay))) return fail(PGPU_EINVAL, "pgpu_server_add_table: null argument");
     if (s->cfg.key_bytes != 8) return fail(PGPU_EUNSUPPORTED, "served tables have int64 keys");
+    if (n >= (1ull << 32)) return fail(PGPU_EINVAL, "a served table holds fewer than 2^32 rows");
     {
         std::lock_guard<std::mutex> g(s->mu);
         if (s->submitted != s->completed || !s->pending.empty())
@@ -480,7 +699,8 @@ int pgpu_server_add_table(pgpu_server* s, const void* d_key, const void* d_pay, 
     if (s->cfg.mode == PGPU_SERVE_BATCHED) {
         t.cap = 1ull << std::max<uint32_t>(4, ilog2_ceil(2 * std::max<uint64_t>(n, 1)));
         t.bytes = t.cap * (sizeof(SSlot) + 4) + 32;
-        if (s->resident + t.bytes > s->cfg.hbm_budget)
+        const uint64_t part_bytes = 16 * std::max<uint64_t>(n, 1) + 4 * ((1ull << batch::bits_for(n, kPartTarget)) + 1);
+        if (s->resident + t.bytes + part_bytes > s->cfg.hbm_budget)
             return fail(PGPU_EBUDGET, "resident table of %llu rows exceeds the server's HBM budget",
                         (unsigned long long)n);
         PGPU_TRY(ctx_alloc(s->ctx, t.cap * sizeof(SSlot), (void**)&t.slots));
@@ -501,11 +721,11 @@ int pgpu_server_add_table(pgpu_server* s, const void* d_key, const void* d_pay, 
                                t.cap - 1, t.special);
         PGPU_LAUNCH_CHECK();
         PGPU_HIP(hipStreamSynchronize(st));
+        s->resident += t.bytes;
+        PGPU_TRY(partition_table(s, t));
     } else {
-        t.bytes = 0;  // plain mode keeps only the columns (rebuilt per query)
-        s->plain_maxR = std::max(s->plain_maxR, n);
+        s->plain_maxR = std::max(s->plain_maxR, n);  // plain mode keeps only the columns
     }
-    s->resident += t.bytes;
     s->tables.push_back(t);
     *table_id = (uint32_t)(s->tables.size() - 1);
     if (s->cfg.mode == PGPU_SERVE_BATCHED) PGPU_TRY(upload_tabs(s));
@@ -530,9 +750,7 @@ int pgpu_server_submit(pgpu_server* s, uint32_t table, const void* d_probe, uint
     if (table >= s->tables.size()) return fail(PGPU_EINVAL, "unknown table %u", table);
     if (s->cfg.mode == PGPU_SERVE_PLAIN && (!s->plain || m > s->plain_maxS))
         return fail(PGPU_ESTATE, "plain mode needs pgpu_server_prepare_plain for probes of this size");
-    if (s->cfg.mode == PGPU_SERVE_BATCHED && ceil_div(m, kItemRows) > max_items_per_slot(s))
-        return fail(PGPU_EINVAL, "probe batch of %llu rows exceeds one launch's %u work items",
-                    (unsigned long long)m, max_items_per_slot(s));
+    if (m >= (1ull << 32)) return fail(PGPU_EINVAL, "a probe batch holds fewer than 2^32 rows");
     auto* q = new Query();
     q->tag = tag;
     q->table = table;
@@ -540,6 +758,11 @@ int pgpu_server_submit(pgpu_server* s, uint32_t table, const void* d_probe, uint
     q->m = m;
     q->reserve = m * 8 + 64;  // the probe batch is held in HBM until its result is out
     q->t_arrival = t_arrival > 0 ? s->t0 + t_arrival : now_s();
+    if (s->cfg.mode == PGPU_SERVE_BATCHED && !partitioned(s, q) && ceil_div(m, kItemRows) > s->cfg.max_items) {
+        delete q;
+        return fail(PGPU_EINVAL, "probe batch of %llu rows exceeds one launch's %u work items",
+                    (unsigned long long)m, s->cfg.max_items);
+    }
     std::lock_guard<std::mutex> g(s->mu);
     if (s->resident + q->reserve > s->cfg.hbm_budget) {
         delete q;
@@ -624,6 +847,7 @@ int pgpu_server_stats_get(pgpu_server* s, pgpu_server_stats* st) {
     st->completed = s->completed;
     st->batches = s->batches;
     st->queued = s->pending.size();
+    st->partitioned = s->part_queries;
     return PGPU_OK;
 }
 

@@ -52,12 +52,14 @@ def _server(ctx, tabs, **kw):
     return s
 
 
-def test_served_results_alone_and_in_the_mix(ctx, world):
+@pytest.mark.parametrize("part_min", [1, 0, 0xFFFFFFFF], ids=["partitioned", "by-size", "random-probe"])
+def test_served_results_alone_and_in_the_mix(ctx, world, part_min):
     """Every query gives its closed-form result when the whole stream arrives
     at once (continuous batching mixes them into shared launches) and when
-    it runs alone."""
+    it runs alone, on the partitioned path, the random-probe path, or as the
+    size rule splits them."""
     tabs, qt, qm, probes, exp = world
-    s = _server(ctx, tabs)
+    s = _server(ctx, tabs, part_min_rows=part_min)
     mix = s.replay(qt, probes, qm, np.zeros(len(qt)))
     assert [(r.matches, r.sum_build) for r in mix] == [(e.count, e.sum_build) for e in exp]
     assert max(r.batch_size for r in mix) > 1  # queries shared launches
@@ -72,13 +74,17 @@ def test_served_results_alone_and_in_the_mix(ctx, world):
     assert alone == [(r.matches, r.sum_build) for r in mix]
     st = s.stats()
     assert st["completed"] == st["submitted"] == 2 * len(qt) and st["reserved_bytes"] == 0
+    if part_min == 1:
+        assert st["partitioned"] == 2 * len(qt)
+    elif part_min == 0xFFFFFFFF:
+        assert st["partitioned"] == 0
     s.close()
 
 
 def test_poisson_replay_latencies(ctx, world):
     tabs, qt, qm, probes, exp = world
     _, _, off = pa.c5_trace(len(qt), 2000.0, ntables=NT, lo=8, hi=17)
-    s = _server(ctx, tabs, slots=2)
+    s = _server(ctx, tabs, slots=2, part_min_rows=4096)
     res = s.replay(qt, probes, qm, off)
     assert [(r.matches, r.sum_build) for r in res] == [(e.count, e.sum_build) for e in exp]
     lat = np.array([r.t_done - r.t_arrival for r in res])
@@ -134,12 +140,17 @@ def test_duplicates_and_empty_marker_key(ctx):
     pk = rng.integers(-2500, 2500, 70000).astype(np.int64)
     pk[:6] = lo
     k, p, q = ctx.upload(bk), ctx.upload(bp), ctx.upload(pk)
-    s = pa.Server(ctx)
-    t = s.add_table(k, p, len(bk))
-    s.submit(t, q, len(pk))
-    s.submit(t, q, 1000)
-    s.drain()
-    got = sorted((r.m, r.matches, r.sum_build) for r in s.poll())
+    got_by_path = []
+    for part_min in (1, 0xFFFFFFFF):
+        s = pa.Server(ctx, part_min_rows=part_min)
+        t = s.add_table(k, p, len(bk))
+        s.submit(t, q, len(pk))
+        s.submit(t, q, 1000)
+        s.drain()
+        got_by_path.append(sorted((r.m, r.matches, r.sum_build) for r in s.poll()))
+        s.close()
+    got = got_by_path[0]
+    assert got_by_path[1] == got
     keys, inv = np.unique(bk, return_inverse=True)
     cnt = np.bincount(inv)
     sums = np.zeros(len(keys), np.uint64)
@@ -157,6 +168,5 @@ def test_duplicates_and_empty_marker_key(ctx):
     d = O.join_desc(bs, ps, 0, 0, ["B$1"], buckets=1 << 10, tuplesperbucket=4, threads=2)
     cs, _ = O.hashjoin_count_sum(d, O.rows(bs, bk, bp), O.rows(ps, pk), 0)
     assert (cs.count, cs.isum % 2**64) == exact(pk)
-    s.close()
     for b in (k, p, q):
         b.free()
