@@ -455,7 +455,8 @@ def main():
     ap.add_argument("--joins", type=int, default=1024)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--queries", type=int, default=1024, help="c5: query pool size")
-    ap.add_argument("--rates", default="200,500,1000,2000", help="c5: Poisson arrival rates (queries/s)")
+    ap.add_argument("--rates", default="200,500,1000,2000,8000",
+                    help="c5: Poisson arrival rates (queries/s); 8000/s overloads the plain path")
     ap.add_argument("--seconds", type=float, default=2.0, help="c5: arrival window per rate")
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -11,7 +11,7 @@
 namespace pgpu {
 namespace batch {
 
-constexpr uint32_t kChunkRows = 131072;  // rows per histogram / scatter block
+constexpr uint32_t kChunkRows = 131072;  // default rows per histogram / scatter block
 constexpr int kMaxBits = 12;             // partition bits of one segment (one pass)
 constexpr int kScatterTileBytes = 65536; // LDS staging per scatter tile
 
@@ -27,7 +27,7 @@ struct Seg {
     uint32_t chunk0;   // first chunk (global index)
     uint32_t nchunks;
     uint32_t bits;
-    uint32_t pad;
+    uint32_t chunk_rows;  // rows per chunk (a multiple of 16)
 };
 
 struct Chunk {
@@ -51,7 +51,8 @@ struct Layout {
     std::vector<Chunk> chunks;
     uint64_t rows = 0, hist = 0, bnd = 0;
     // chunk_base: global index of this layout's first chunk.
-    void add(const void* key, const void* pay, uint64_t n, uint32_t bits, uint32_t chunk_base);
+    void add(const void* key, const void* pay, uint64_t n, uint32_t bits, uint32_t chunk_base,
+             uint32_t chunk_rows = kChunkRows);
 };
 
 // Rows per partition the join kernel variant holds in LDS in one piece.

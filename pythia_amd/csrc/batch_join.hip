@@ -9,7 +9,7 @@
 // operators/aggregatesum.cpp:81-107).  The reference runs one such plan per
 // query; here a whole batch shares every launch:
 //
-//   k_bhist     one block per chunk of kChunkRows rows of any side of any
+//   k_bhist     one block per chunk (the segment's chunk_rows rows) of any side of any
 //               join: LDS histogram over that join's 2^bits partitions
 //               (bits from the join's build size, one pass, <= 12 bits)
 //   k_bscan     one block per side of a join: exclusive scans of the
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(512) void k_bhist(const Seg* __restrict__ segs, con
     const uint32_t ci = chunk_base + blockIdx.x;
     const Chunk c = chunks[ci];
     const Seg s = segs[c.seg];
-    const uint64_t r0 = c.row0, r1 = min<uint64_t>(s.n, r0 + kChunkRows);
+    const uint64_t r0 = c.row0, r1 = min<uint64_t>(s.n, r0 + s.chunk_rows);
     const uint32_t nd = 1u << s.bits;
     uint32_t* out = H + s.hist + (uint64_t)(ci - s.chunk0) * nd;
     if (s.bits == 0) {
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs,
     const uint32_t ci = chunk_base + blockIdx.x;
     const Chunk c = chunks[ci];
     const Seg s = segs[c.seg];
-    const uint64_t r0 = c.row0, r1 = min<uint64_t>(s.n, r0 + kChunkRows);
+    const uint64_t r0 = c.row0, r1 = min<uint64_t>(s.n, r0 + s.chunk_rows);
     const uint32_t nd = 1u << s.bits;
     const KT* key = (const KT*)s.key;
     const KT* pay = (const KT*)s.pay;
@@ -459,7 +459,8 @@ int occupancy_t(int* blocks) {
 namespace pgpu {
 namespace batch {
 
-void Layout::add(const void* key, const void* pay, uint64_t n, uint32_t bits, uint32_t chunk_base) {
+void Layout::add(const void* key, const void* pay, uint64_t n, uint32_t bits, uint32_t chunk_base,
+                 uint32_t chunk_rows) {
     Seg s{};
     s.key = key;
     s.pay = pay;
@@ -468,9 +469,10 @@ void Layout::add(const void* key, const void* pay, uint64_t n, uint32_t bits, ui
     s.hist = hist;
     s.bnd = bnd;
     s.bits = bits;
+    s.chunk_rows = chunk_rows;
     s.chunk0 = chunk_base + (uint32_t)chunks.size();
-    s.nchunks = (uint32_t)ceil_div(n, kChunkRows);
-    for (uint32_t c = 0; c < s.nchunks; ++c) chunks.push_back(Chunk{(uint32_t)segs.size(), 0, (uint64_t)c * kChunkRows});
+    s.nchunks = (uint32_t)ceil_div(n, chunk_rows);
+    for (uint32_t c = 0; c < s.nchunks; ++c) chunks.push_back(Chunk{(uint32_t)segs.size(), 0, (uint64_t)c * chunk_rows});
     rows += n;
     hist += (uint64_t)s.nchunks << bits;
     bnd += (1ull << bits) + 1;

@@ -50,7 +50,7 @@ using pgpu::batch::Seg;
 
 namespace {
 
-constexpr uint64_t kItemRows = 65536;  // probe rows per random-probe work item
+constexpr uint64_t kItemRows = 16384;  // probe rows per random-probe work item
 constexpr int kProbeThreads = 256;
 constexpr int kProbeU = 8;             // independent probes in flight per thread
 constexpr int kVariant = 2;            // LDS join shape of the partitioned path
@@ -227,9 +227,9 @@ struct Table {
 // Per-slot capacities of the partitioned path.
 struct Caps {
     uint64_t rows = 1ull << 26;      // probe rows partitioned per launch
-    uint64_t hist = 1ull << 23;      // histogram words
+    uint64_t hist = 1ull << 25;      // histogram words
     uint64_t bnd = 1ull << 20;       // bound words
-    uint64_t chunks = 4096;
+    uint64_t chunks = 16384;
     uint64_t pitems = 1ull << 20;    // join work items
 };
 
@@ -296,6 +296,13 @@ void set_error(pgpu_server* s, int rc) {
     }
 }
 
+// Chunk size of a probe batch on the partitioned path: about 128 blocks per
+// batch, so one query alone still spreads over the device.
+uint32_t chunk_rows_for(uint64_t m) {
+    const uint64_t c = ceil_div(std::max<uint64_t>(m, 1), 128);
+    return (uint32_t)std::min<uint64_t>(kChunkRows, std::max<uint64_t>(4096, ceil_div(c, 4096) * 4096));
+}
+
 bool partitioned(const pgpu_server* s, const Query* q) {
     const uint64_t pmin = s->cfg.part_min_rows ? s->cfg.part_min_rows : (1u << 20);
     if (pmin == UINT32_MAX || q->m == 0) return false;
@@ -334,7 +341,7 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
             if (s->resident + s->reserved + q->reserve > s->cfg.hbm_budget) break;  // FIFO: the head waits
             if (partitioned(s, q)) {
                 const uint32_t bits = s->tables[q->table].bits;
-                const uint64_t nc = ceil_div(q->m, kChunkRows);
+                const uint64_t nc = ceil_div(q->m, chunk_rows_for(q->m));
                 if (rows + q->m > s->caps.rows || hist + (nc << bits) > s->caps.hist ||
                     bnd + (1ull << bits) + 1 > s->caps.bnd || chunks + nc > s->caps.chunks ||
                     pitems + (1ull << bits) > s->caps.pitems)
@@ -378,9 +385,11 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
         sg.hist = phist;
         sg.bnd = pbnd;
         sg.chunk0 = nchunk;
-        sg.nchunks = (uint32_t)ceil_div(q->m, kChunkRows);
+        sg.chunk_rows = chunk_rows_for(q->m);
+        sg.nchunks = (uint32_t)ceil_div(q->m, sg.chunk_rows);
         sg.bits = tb.bits;
-        for (uint32_t c = 0; c < sg.nchunks; ++c) sl.h.chunks[nchunk++] = Chunk{nseg, 0, (uint64_t)c * kChunkRows};
+        for (uint32_t c = 0; c < sg.nchunks; ++c)
+            sl.h.chunks[nchunk++] = Chunk{nseg, 0, (uint64_t)c * sg.chunk_rows};
         Job jb{};
         jb.rdst = tb.rdst;
         jb.rbnd = tb.rbnd;
