@@ -262,8 +262,8 @@ def run_c4(args):
     print(f"# c4: {len(joins)} joins, build {int(sizes.sum())} rows, probe {int(8 * sizes.sum())} rows",
           file=sys.stderr, flush=True)
     tune = None
-    variant, part_rows = 0, 0
-    if not args.no_tune:
+    variant, part_rows = args.variant, args.part_rows
+    if not args.no_tune and not variant:
         best, table = pa.batch_autotune(ctx, joins, key_bytes=4, probe_payload=True, reps=2)
         variant, part_rows = best
         tune = {"best": {"variant": best[0], "part_rows": best[1]},
@@ -454,6 +454,8 @@ def main():
     ap.add_argument("--zipf", type=float, default=0.0)
     ap.add_argument("--joins", type=int, default=1024)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--variant", type=int, default=0, help="c4: join workgroup shape (skips the autotuner)")
+    ap.add_argument("--part-rows", type=int, default=0, help="c4: build rows per partition with --variant")
     ap.add_argument("--queries", type=int, default=1024, help="c5: query pool size")
     ap.add_argument("--rates", default="200,500,1000,2000,8000",
                     help="c5: Poisson arrival rates (queries/s); 8000/s overloads the plain path")
