@@ -128,9 +128,9 @@ def run_c2(args):
     else:
         ctx.gen("C2_SFK_UNIFORM", nS, sk, seed=args.seed, n_build=nR)
     ctx.sync()
-    plan = pa.JoinPlan(ctx, 8, nR, nS, radix_bits=args.bits, algo=pa.ALGO_RADIX)
+    plan = pa.JoinPlan(ctx, 8, nR, nS, radix_bits=args.bits, algo=pa.ALGO_RADIX, join_variant=args.join_variant)
 
-    for _ in range(args.warmup):
+    for _ in range(max(args.warmup, 1)):
         res = plan.count_sum(rk, rp, nR, sk, None, nS)
     ctx.sync()
     print(f"# warmup done: matches={res.matches} sum={res.sum_build}", file=sys.stderr, flush=True)
@@ -451,6 +451,7 @@ def main():
     ap.add_argument("--log-build", type=int, default=24)
     ap.add_argument("--log-probe", type=int, default=28)
     ap.add_argument("--bits", type=int, default=14)
+    ap.add_argument("--join-variant", type=int, default=0, help="c2: LDS join shape (pgpu_join_cfg.join_variant)")
     ap.add_argument("--zipf", type=float, default=0.0)
     ap.add_argument("--joins", type=int, default=1024)
     ap.add_argument("--no-tune", action="store_true")

@@ -127,9 +127,9 @@ class JoinPlan:
     """Preallocated join (count + sum) over device-resident columns."""
 
     def __init__(self, ctx: Context, key_bytes: int, max_build: int, max_probe: int, radix_bits: int = 0,
-                 probe_payload: bool = False, algo: int = ALGO_AUTO):
+                 probe_payload: bool = False, algo: int = ALGO_AUTO, join_variant: int = 0):
         self.ctx = ctx
-        cfg = _lib.JoinCfg(key_bytes, radix_bits, 1 if probe_payload else 0, algo)
+        cfg = _lib.JoinCfg(key_bytes, radix_bits, 1 if probe_payload else 0, algo, join_variant)
         h = C.c_void_p()
         call("pgpu_join_plan_create", ctx.handle, C.byref(cfg), max_build, max_probe, C.byref(h))
         self.handle = h

@@ -32,7 +32,7 @@ class GenSpec(C.Structure):
 
 class JoinCfg(C.Structure):
     _fields_ = [("key_bytes", C.c_uint32), ("radix_bits", C.c_uint32),
-                ("probe_payload", C.c_uint32), ("algo", C.c_uint32)]
+                ("probe_payload", C.c_uint32), ("algo", C.c_uint32), ("join_variant", C.c_uint32)]
 
 
 class JoinResult(C.Structure):

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(512) void k_bhist(const Seg* __restrict__ segs, con
         const uint4* kv = (const uint4*)(key + r0);
 #pragma unroll 4
         for (uint64_t g = threadIdx.x; g < ng; g += 512) {
-            const uint4 q = kv[g];
+            const uint4 q = nt_load(&kv[g]);
             KT ks[G];
             memcpy(ks, &q, 16);
 #pragma unroll
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs,
         for (uint32_t pos = threadIdx.x; pos < tn; pos += T) {
             const E e = stage[pos];
             const uint32_t d = (uint32_t)(hmix(e.k) >> sh);
-            dst[cur[d] + (pos - start[d])] = e;
+            dst[cur[d] + (pos - start[d])] = e;  // runs: keep L2 write-combining
         }
         __syncthreads();
         for (uint32_t d = threadIdx.x; d < nd; d += T) {
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint64_t idx = q + (uint64_t)u * T;
-                if (idx < se) x[u] = S[idx];
+                if (idx < se) x[u] = nt_load(&S[idx]);
             }
         };
         const bool work = nR && se > sb;
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
             __syncthreads();
             const uint32_t pn = min(PIECE, nR - p0);
             for (uint32_t i = tid; i < pn; i += T) {
-                const Row<KT, true> r = R[rb + p0 + i];
+                const Row<KT, true> r = nt_load(&R[rb + p0 + i]);
                 if (r.k == EMPTY) {
                     atomicAdd(&sp_cnt, 1ull);
                     atomicAdd(&sp_sum, (unsigned long long)(int64_t)r.v);

@@ -43,6 +43,49 @@ struct Row<KT, false> {
     KT k;
 };
 
+// Streaming loads / stores with the non-temporal cache policy: partition
+// passes read every byte once, so keeping the lines in the caches only
+// evicts useful ones (measured on MI355X: streaming read 6.3 -> 7.0 TB/s,
+// copy 5.4 -> 6.0 TB/s).  Stores of short scattered runs must not use it:
+// they lose the L2's merging of partial lines (the radix scatters slowed by
+// 10-50 %).  T is 4, 8 or 16 bytes, naturally aligned.
+template <typename T>
+__device__ __forceinline__ T nt_load(const T* p) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "nt_load width");
+    T r;
+    if constexpr (sizeof(T) == 16) {
+        typedef long long v2 __attribute__((ext_vector_type(2)));
+        const v2 x = __builtin_nontemporal_load((const v2*)p);
+        __builtin_memcpy(&r, &x, 16);
+    } else if constexpr (sizeof(T) == 8) {
+        const long long x = __builtin_nontemporal_load((const long long*)p);
+        __builtin_memcpy(&r, &x, 8);
+    } else {
+        const int x = __builtin_nontemporal_load((const int*)p);
+        __builtin_memcpy(&r, &x, 4);
+    }
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void nt_store(T* p, const T& v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "nt_store width");
+    if constexpr (sizeof(T) == 16) {
+        typedef long long v2 __attribute__((ext_vector_type(2)));
+        v2 x;
+        __builtin_memcpy(&x, &v, 16);
+        __builtin_nontemporal_store(x, (v2*)p);
+    } else if constexpr (sizeof(T) == 8) {
+        long long x;
+        __builtin_memcpy(&x, &v, 8);
+        __builtin_nontemporal_store(x, (long long*)p);
+    } else {
+        int x;
+        __builtin_memcpy(&x, &v, 4);
+        __builtin_nontemporal_store(x, (int*)p);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

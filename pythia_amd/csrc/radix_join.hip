@@ -33,11 +33,16 @@
 // GLOBAL (one open-addressing table in HBM, the "plain" per-query path):
 //   k_gclear, k_gbuild (CAS insert), k_gprobe.
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 
+#include "join_common.h"
 #include "pgpu_internal.h"
 
 namespace {
+
+using pgpu::dev::nt_load;
+using pgpu::dev::nt_store;
 
 constexpr uint64_t kFib = 0x9E3779B97F4A7C15ull;
 constexpr int kMaxBits = 14;
@@ -149,7 +154,7 @@ __global__ __launch_bounds__(512) void k_hist(const KT* __restrict__ keys, uint6
     for (; i + (U - 1) * 512 < e; i += U * 512) {
         KT k[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) k[u] = keys[i + u * 512];
+        for (int u = 0; u < U; ++u) k[u] = nt_load(&keys[i + u * 512]);
 #pragma unroll
         for (int u = 0; u < U; ++u) atomicAdd(&lh[(uint32_t)(hmix(k[u]) >> (64 - B))], 1u);
     }
@@ -460,15 +465,21 @@ struct __align__(16) LSlot {
     unsigned long long sum;
 };
 
-template <typename KT, bool HVS>
-__global__ __launch_bounds__(kJoinThreads) void k_join_cs(
+constexpr int log2i(int v) { return v <= 1 ? 0 : 1 + log2i(v / 2); }
+
+// CAP LDS slots, T threads, PIECE build rows per LDS piece, UB probe rows of
+// 8 B prefetched per thread, WPE waves per SIMD the register budget allows.
+template <typename KT, bool HVS, int CAP, int T, int PIECE, int UB, int WPE>
+__global__ __launch_bounds__(T, WPE) void k_join_cs(
     const Elem<KT, true>* __restrict__ R2, const uint64_t* __restrict__ baseR, const uint32_t* __restrict__ totR,
     const Elem<KT, HVS>* __restrict__ S2, const uint64_t* __restrict__ baseS, const uint32_t* __restrict__ totS,
     const uint64_t* __restrict__ ioff, const JoinItem* __restrict__ items, uint32_t nb, int B,
     unsigned long long* __restrict__ part) {
-    constexpr int CAP = kJoinCap;
-    constexpr int L = 12;  // log2(CAP)
+    constexpr int L = log2i(CAP);
     static_assert((1 << L) == CAP, "CAP");
+    static_assert(PIECE < 65536 && PIECE <= CAP * 5 / 8, "LDS piece");
+    constexpr int kJoinThreads = T;
+    constexpr int kJoinPiece = PIECE;
     // Keys are held sign-extended to 64 bits; only an int64 key equal to the
     // empty marker needs the side accumulator.
     constexpr long long EMPTY = INT64_MIN;
@@ -482,22 +493,51 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_cs(
     const uint64_t item = blockIdx.x;
     if (item >= ioff[nb]) return;
     const JoinItem it = items[item];
-    const uint64_t sB = it.sB, sE = it.sB + it.sN, rB = it.rB;
-    const uint32_t nR = it.nR;
-    const int hs = 64 - B - L;  // slot bits sit just below the radix bits
+    const uint32_t sN = it.sN, nR = it.nR;
+    const Elem<KT, HVS>* __restrict__ S = S2 + it.sB;
+    const Elem<KT, true>* __restrict__ R = R2 + it.rB;
+    // LDS slot = bits [64-B-L, 64-B) of hmix(k): they lie in the high word of
+    // the 64-bit product (B + L <= 32), computed without its low half.
+    const int sh = 32 - B - L;
+    auto slot_of = [&](KT key) -> uint32_t {
+        const uint64_t kk = (uint64_t)(int64_t)key;
+        const uint32_t lo = (uint32_t)kk, hi = (uint32_t)(kk >> 32);
+        const uint32_t h = __umulhi(lo, (uint32_t)kFib) + lo * (uint32_t)(kFib >> 32) + hi * (uint32_t)kFib;
+        return (h >> sh) & (CAP - 1);
+    };
     unsigned long long acc_c = 0, acc_b = 0, acc_p = 0;
-    constexpr int U = HVS ? 8 : 16;
-    Elem<KT, HVS> x[U];
-    auto fetch = [&](uint64_t j) {
+    // Probe rows held in registers per thread (one batch = U x T rows).
+    constexpr int U = HVS ? UB / 2 : UB;
+    constexpr uint32_t BATCH = (uint32_t)U * kJoinThreads;
+    Elem<KT, HVS> x[U] = {};
+    // rows of the batch at `rb` this thread holds: tid + u*T < sN - rb
+    auto rows_in = [&](uint32_t rb) -> uint32_t {
+        const uint32_t r0 = rb + tid;
+        return r0 < sN ? min<uint32_t>(U, (sN - r0 + kJoinThreads - 1) / kJoinThreads) : 0u;
+    };
+    auto fetch = [&](uint32_t rb) {
+        const uint32_t cnt = rows_in(rb);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint64_t idx = j + (uint64_t)u * kJoinThreads;
-            if (idx < sE) x[u] = S2[idx];
+        for (int u = 0; u < U; ++u)
+            if ((uint32_t)u < cnt) x[u] = nt_load(&S[rb + tid + u * kJoinThreads]);
+    };
+    constexpr int RB = (kJoinPiece + kJoinThreads - 1) / kJoinThreads;  // build rows per thread per piece
+    Elem<KT, true> br[RB];
+    auto fetch_build = [&](uint32_t p0) {
+        const uint32_t pn = min<uint32_t>(kJoinPiece, nR - p0);
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            const uint32_t i = tid + b * kJoinThreads;
+            if (i < pn) br[b] = nt_load(&R[p0 + i]);
         }
     };
-    // The first probe batch does not depend on the table: fetch it before
-    // building so both latencies overlap.
-    if (nR) fetch(sB + tid);
+    // Neither the build rows nor the first probe batch depend on the table:
+    // both are in flight while the table is cleared.
+    if (nR) {
+        fetch_build(0);
+        fetch(0);
+    }
+    const unsigned short* tc16 = (const unsigned short*)tc;  // little-endian halves of tc[]
 
     for (uint32_t p0 = 0; p0 < nR; p0 += kJoinPiece) {
         for (int i = tid; i < CAP; i += kJoinThreads) {
@@ -508,15 +548,17 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_cs(
         if (tid == 0) sp_cnt = sp_sum = 0;
         __syncthreads();
         const uint32_t pn = min<uint32_t>(kJoinPiece, nR - p0);
-        for (uint32_t i = tid; i < pn; i += kJoinThreads) {
-            const Elem<KT, true> r = R2[rB + p0 + i];
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            if (tid + b * kJoinThreads >= pn) continue;
+            const Elem<KT, true> r = br[b];
             const long long k = (long long)r.k;
             if (k == EMPTY) {
                 atomicAdd(&sp_cnt, 1ull);
                 atomicAdd(&sp_sum, (unsigned long long)(int64_t)r.v);
                 continue;
             }
-            uint32_t slot = (uint32_t)(hmix(r.k) >> hs) & (CAP - 1);
+            uint32_t slot = slot_of(r.k);
             while (true) {
                 const long long prev = (long long)atomicCAS((unsigned long long*)&tab[slot].key,
                                                             (unsigned long long)EMPTY, (unsigned long long)k);
@@ -528,37 +570,69 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_cs(
                 slot = (slot + 1) & (CAP - 1);
             }
         }
+        if (p0 + kJoinPiece < nR) fetch_build(p0 + kJoinPiece);  // next piece, if any
         __syncthreads();
-        if (p0) fetch(sB + tid);  // later pieces re-read the item's probe rows
-        auto probe = [&](const Elem<KT, HVS>& x) {
-            const long long k = (long long)x.k;
-            unsigned long long c = 0, sb = 0;
-            if (k == EMPTY) {
-                c = sp_cnt;
-                sb = sp_sum;
-            } else {
-                uint32_t slot = (uint32_t)(hmix(x.k) >> hs) & (CAP - 1);
-                while (true) {
-                    const LSlot t = tab[slot];
-                    if (t.key == k) {
-                        c = (tc[slot >> 1] >> (16 * (slot & 1))) & 0xFFFFu;
-                        sb = t.sum;
-                        break;
-                    }
-                    if (t.key == EMPTY) break;
-                    slot = (slot + 1) & (CAP - 1);
-                }
+        if (p0) fetch(0);  // later pieces re-read the item's probe rows
+        const unsigned long long spc = sp_cnt, sps = sp_sum;
+        // One batch: all U slot reads in flight at once, collision chains
+        // walked only by the lanes that need them, selects instead of
+        // branches.  FULL batches (every thread holds U rows) skip the
+        // row-count masks.
+        auto probe_batch = [&](auto full_tag, uint32_t cnt) {
+            constexpr bool FULL = decltype(full_tag)::value;
+            uint32_t slot[U];
+            LSlot t[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                slot[u] = slot_of(x[u].k);
+                t[u] = tab[slot[u]];
             }
-            acc_c += c;
-            acc_b += sb;
-            if constexpr (HVS) acc_p += c * (unsigned long long)(int64_t)x.v;
-        };
-        for (uint64_t j = sB + tid; j < sE;) {
+            bool any = false;
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if (j + (uint64_t)u * kJoinThreads < sE) probe(x[u]);
-            j += (uint64_t)U * kJoinThreads;
-            if (j < sE) fetch(j);
+                any |= (FULL || (uint32_t)u < cnt) && t[u].key != (long long)x[u].k && t[u].key != EMPTY;
+            if (any) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (!FULL && (uint32_t)u >= cnt) continue;
+                    const long long k = (long long)x[u].k;
+                    while (t[u].key != k && t[u].key != EMPTY) {
+                        slot[u] = (slot[u] + 1) & (CAP - 1);
+                        t[u] = tab[slot[u]];
+                    }
+                }
+            }
+            uint32_t bc = 0;  // fits 32 bits: U rows x PIECE matches
+            unsigned long long bs = 0, bp = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const long long k = (long long)x[u].k;
+                const uint32_t raw = tc16[slot[u]];
+                const bool hit = t[u].key == k;
+                uint32_t c = hit ? raw : 0u;
+                unsigned long long sb = hit ? t[u].sum : 0ull;
+                if (k == EMPTY) {  // the empty marker as a real key: side accumulator
+                    c = (uint32_t)spc;
+                    sb = sps;
+                }
+                if (!FULL && (uint32_t)u >= cnt) {
+                    c = 0;
+                    sb = 0;
+                }
+                bc += c;
+                bs += sb;
+                if constexpr (HVS) bp += (unsigned long long)c * (unsigned long long)(int64_t)x[u].v;
+            }
+            acc_c += bc;
+            acc_b += bs;
+            acc_p += bp;
+        };
+        for (uint32_t rb = 0; rb < sN; rb += BATCH) {
+            if (rb + BATCH <= sN)
+                probe_batch(std::true_type{}, (uint32_t)U);
+            else
+                probe_batch(std::false_type{}, rows_in(rb));
+            if (rb + BATCH < sN) fetch(rb + BATCH);
         }
         __syncthreads();
     }
@@ -735,6 +809,7 @@ struct pgpu_join_plan {
     bool pending = false;
     pgpu::KernelTimer* timer = nullptr;
     std::vector<void*> owned;
+    int cus = 0;  // compute units of the device (persistent grid sizing)
 };
 
 namespace {
@@ -844,9 +919,17 @@ int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const 
     if (max_items > p->max_items) return pgpu::fail(PGPU_EINVAL, "join items exceed the plan");
     {
         Timed tm(p->timer, "join_lds");
-        hipLaunchKernelGGL((k_join_cs<KT, HVS>), dim3((unsigned)max_items), dim3(kJoinThreads), 0, st,
-                           (const Elem<KT, true>*)p->R2, p->baseR, p->totR, (const Elem<KT, HVS>*)p->S2, p->baseS,
-                           p->totS, p->ioff, p->items, nb, B, p->part);
+#define PGPU_JOIN(CAP, T, PIECE, UB, WPE)                                                                     \
+    hipLaunchKernelGGL((k_join_cs<KT, HVS, CAP, T, PIECE, UB, WPE>), dim3((unsigned)max_items), dim3(T), 0, st, \
+                       (const Elem<KT, true>*)p->R2, p->baseR, p->totR, (const Elem<KT, HVS>*)p->S2, p->baseS,    \
+                       p->totS, p->ioff, p->items, nb, B, p->part)
+        switch (p->cfg.join_variant) {
+            case 2: PGPU_JOIN(4096, 256, 2048, 32, 2); break;
+            case 3: PGPU_JOIN(2048, 256, 1280, 16, 4); break;
+            case 4: PGPU_JOIN(4096, 512, 2048, 16, 4); break;
+            default: PGPU_JOIN(4096, 512, 2048, 8, 4); break;
+        }
+#undef PGPU_JOIN
         PGPU_LAUNCH_CHECK();
     }
     return PGPU_OK;
@@ -896,6 +979,7 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
     if (cfg->radix_bits != 0 && (cfg->radix_bits < 2 || cfg->radix_bits > kMaxBits))
         return fail(PGPU_EINVAL, "radix_bits must be 0 or in [2, %d] (got %u)", kMaxBits, cfg->radix_bits);
     if (cfg->algo > PGPU_ALGO_GLOBAL) return fail(PGPU_EINVAL, "unknown join algorithm %u", cfg->algo);
+    if (cfg->join_variant > 4) return fail(PGPU_EINVAL, "unknown LDS join variant %u", cfg->join_variant);
     if (max_build >= (1ull << 32) || max_probe >= (1ull << 40))
         return fail(PGPU_EINVAL, "relation too large (build < 2^32, probe < 2^40 rows)");
     PGPU_TRY(ctx_activate(ctx));
@@ -910,6 +994,9 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
         pgpu_join_plan_destroy(p);
         return rc;
     };
+    if (hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+        p->cus < 1)
+        return cleanup(fail(PGPU_EDEVICE, "cannot read the device's compute unit count"));
     const uint64_t kb = cfg->key_bytes;
     const uint64_t esR = 2 * kb, esS = cfg->probe_payload ? 2 * kb : kb;
     const bool want_radix = cfg->algo != PGPU_ALGO_GLOBAL;

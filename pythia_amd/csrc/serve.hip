@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kProbeThreads) void k_sprobe(const QItem* __restric
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint64_t row = r + (uint64_t)u * kProbeThreads;
-            k[u] = row < it.r1 ? it.probe[row] : INT64_MIN;
+            k[u] = row < it.r1 ? nt_load(&it.probe[row]) : INT64_MIN;
             s[u] = hmix(k[u]) >> td.shift;
         }
 #pragma unroll
