@@ -344,6 +344,182 @@ def run_c4(args):
     return line
 
 
+def run_c1(args):
+    """C1 through the operator API: the queryhashjoin plan shape with
+    type = "gpuhashjoin" / "gpuaggregate_sum" / "gpuaggregate_count" over
+    host-resident relations (pages are copied to the device and back), next
+    to the same join over device-resident columns."""
+    import textwrap
+
+    import pythia_amd as pa
+    from pythia_amd import plan as P
+
+    nR, nS = 1 << 20, 1 << 24
+    ctx = pa.Context(0)
+    cols = {}
+    for name, n in (("C1_RKEY", nR), ("C1_RPAY", nR), ("C1_SFK", nS), ("C1_SPAY", nS)):
+        b = ctx.alloc(4 * n)
+        ctx.gen(name, n, b, seed=args.seed, n_build=nR)
+        cols[name] = b
+    ctx.sync()
+    # device-resident columns: the fused count + sum
+    plan = pa.JoinPlan(ctx, 4, nR, nS, probe_payload=True, algo=pa.ALGO_RADIX)
+    for _ in range(max(args.warmup, 1)):
+        dres = plan.count_sum(cols["C1_RKEY"], cols["C1_RPAY"], nR, cols["C1_SFK"], cols["C1_SPAY"], nS)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dres = plan.count_sum(cols["C1_RKEY"], cols["C1_RPAY"], nR, cols["C1_SFK"], cols["C1_SPAY"], nS)
+    dev_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    plan.close()
+    rdt = np.dtype({"names": ["k", "v"], "formats": [np.int32, np.int32], "offsets": [0, 4], "itemsize": 8})
+    R = np.empty(nR, rdt)
+    S = np.empty(nS, rdt)
+    R["k"], R["v"] = cols["C1_RKEY"].download(np.int32, nR), cols["C1_RPAY"].download(np.int32, nR)
+    S["k"], S["v"] = cols["C1_SFK"].download(np.int32, nS), cols["C1_SPAY"].download(np.int32, nS)
+    for b in cols.values():
+        b.free()
+    P.register_table("c1_r", R)
+    P.register_table("c1_s", S)
+    T = cpu_cores()
+    groups = ", ".join(str(i) for i in range(T))
+
+    def text(agg):
+        return textwrap.dedent(f"""
+            path = "./"; buffsize = 1048576;
+            treeroot = {{ name = "merge"; input = {{ name = "agg"; input = {{ name = "join";
+                          build = {{ name = "r"; }}; probe = {{ name = "s"; }}; }}; }}; }};
+            merge = {{ type = "merge"; threads = {T}; }};
+            agg = {{ type = "gpu{agg}"; fields = (); {'sumfield = 0;' if agg == 'aggregate_sum' else ''} }};
+            join = {{ type = "gpuhashjoin"; hash = {{ fn = "modulo"; buckets = 65536; }};
+                      threadgroups = ( [ {groups} ] ); tuplesperbucket = 16; allocpolicy = "local";
+                      buildjattr = 0; probejattr = 0; projection = ( "B$1" ); }};
+            r = {{ type = "memscan"; table = "c1_r"; schema = ( "int", "int" ); }};
+            s = {{ type = "memscan"; table = "c1_s"; schema = ( "int", "int" ); }};
+        """)
+
+    P.run(text("aggregate_sum"))  # warm-up: device context, kernels
+    secs, out = [], None
+    for _ in range(max(1, min(args.steps, 5))):
+        out, meta = P.run(text("aggregate_sum"))
+        secs.append(meta["seconds"])
+    cnt, cmeta = P.run(text("aggregate_count"))
+    op_s = float(np.median(secs))
+    sum32 = int(np.int64(out["c0"].astype(np.int64).sum()).astype(np.int32)) if len(out) else 0
+    count = int(cnt["c0"].sum())
+    P.clear_tables()
+    print(f"# c1 operator API {1e3 * op_s:.1f} ms per query, device columns {dev_ms:.3f} ms", file=sys.stderr,
+          flush=True)
+    line = {
+        "metric": METRIC,
+        "value": (nR + nS) / op_s,
+        "unit": "tuples/s",
+        "n_gpus": 1,
+        "steps": len(secs),
+        "warmup": 1,
+        "ms_per_step": 1e3 * op_s,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (counter-based SplitMix64 generators)",
+        "config": {"workload": f"queryhashjoin plan through the operator API: R 2^20 (int32 key, int32 payload) "
+                               f"x S 2^24, gpuhashjoin -> gpuaggregate_sum / gpuaggregate_count -> merge({T}), "
+                               f"host-resident pages (PCIe-bound)", "threads": T},
+        "device_columns": {"ms_per_join": dev_ms, "tuples_per_s": (nR + nS) / (dev_ms / 1e3),
+                           "matches": dres.matches, "sum_build": dres.sum_build},
+        "result": {"count": count, "sum_int32": sum32,
+                   "matches_device_path": bool(count == dres.matches and
+                                               sum32 == int(np.int64(dres.sum_build % 2**32).astype(np.int32)))},
+    }
+    if not args.no_cpu_baseline and os.path.exists(REFJOIN):
+        out = subprocess.run([REFJOIN, "c1", "20", "24", str(T), "0", "65536", "16", "sum"], stdout=subprocess.PIPE,
+                             stderr=subprocess.PIPE, text=True, timeout=900, cwd="/tmp")
+        if out.returncode == 0:
+            r = json.loads(out.stdout)
+            line["cpu_baseline"] = {"value": (nR + nS) / r["seconds"], "unit": "tuples/s", "cores": T,
+                                    "kind": "reference",
+                                    "sample": f"the full workload through the reference's hashjoin -> aggregate_sum "
+                                              f"-> merge({T}) plan (modulo 2^16 buckets, 16 per bucket), "
+                                              f"{r['seconds']:.3f} s",
+                                    "result_matches_gpu": bool(r["value"] == sum32)}
+    return line
+
+
+def run_c3(args):
+    """C3: SUM(F.price) GROUP BY D.custkey over F JOIN D ON orderkey."""
+    import pythia_amd as pa
+
+    nD, nF = args.c3_dim, args.c3_fact
+    ctx = pa.Context(0)
+    dk, dc, fk, fp = ctx.alloc(nD * 4), ctx.alloc(nD * 4), ctx.alloc(nF * 4), ctx.alloc(nF * 8)
+    ctx.gen("C3_D_ORDERKEY", nD, dk, seed=args.seed, n_build=nD)
+    ctx.gen("C3_D_CUSTKEY", nD, dc, seed=args.seed)
+    ctx.gen("C3_F_ORDERKEY", nF, fk, seed=args.seed, n_build=nD)
+    ctx.gen("C3_F_PRICE", nF, fp, seed=args.seed)
+    ctx.sync()
+    gj = pa.GroupJoin(ctx, nD, nF, 1_500_000)
+    for _ in range(max(args.warmup, 1)):
+        ng, nj = gj.run(dk, dc, nD, fk, fp, nF)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ng, nj = gj.run(dk, dc, nD, fk, fp, nF)
+    ms_step = 1e3 * (time.perf_counter() - t0) / args.steps
+    gj.timing(True)
+    for _ in range(args.steps):
+        gj.run(dk, dc, nD, fk, fp, nF)
+    per_kernel = per_kernel_times(gj.times())
+    gj.timing(False)
+    keys, sums, counts = gj.fetch(ng)
+    total = float(np.sum(sums))
+    print(f"# c3 step {ms_step:.3f} ms, {ng} groups, joined {nj}; kernels: " +
+          ", ".join(f"{k}={v['avg_ms']:.3f}ms" for k, v in per_kernel.items()), file=sys.stderr, flush=True)
+    kb = {"gj_probe": nF * 12, "gj_build": nD * 8, "gj_compact": 1_500_000 * 24}
+    compulsory = nF * 12 + nD * 8 + 1_500_000 * 12
+    line = {
+        "metric": METRIC,
+        "value": (nD + nF) / (ms_step / 1e3),
+        "unit": "tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (counter-based SplitMix64 generators, generated in HBM)",
+        "config": {"workload": f"join + group-by FP64 sum: fact {nF} rows (int32 orderkey, FP64 price), "
+                               f"dimension {nD} rows (int32 orderkey, int32 custkey), group by custkey",
+                   "dimension_rows": nD, "fact_rows": nF, "groups": ng},
+        "roofline": roofline(per_kernel, kb, None),
+        "job_roofline": {"compulsory_bytes": compulsory, "achieved_GBps": compulsory / (ms_step / 1e3) / 1e9,
+                         "frac": compulsory / (ms_step / 1e3) / HBM_PEAK},
+        "kernels": per_kernel,
+        "result": {"groups": ng, "joined": nj, "total": total},
+    }
+    if not args.no_cpu_baseline:
+        gj.close()
+        for b in (dk, dc, fk, fp):
+            b.free()
+        cores = cpu_cores()
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        if os.path.exists(REFJOIN):
+            out = subprocess.run([REFJOIN, "c3", str(nD), str(nF), str(cores)], stdout=subprocess.PIPE,
+                                 stderr=subprocess.PIPE, text=True, timeout=900, cwd="/tmp")
+            if out.returncode == 0:
+                r = json.loads(out.stdout)
+                line["cpu_baseline"] = {
+                    "value": (nD + nF) / r["seconds"], "unit": "tuples/s", "cores": cores, "kind": "reference",
+                    "sample": f"the full workload through the reference's hashjoin -> aggregate_sum(field 0) -> "
+                              f"merge({cores}) plan, one run of {r['seconds']:.2f} s (inputs generated outside "
+                              f"the clock)",
+                    "result_matches_gpu": bool(r["groups"] == ng and abs(r["total"] - total) <= 1e-9 * abs(total)),
+                }
+            else:
+                print(f"# reference baseline failed: {out.stderr[-300:]}", file=sys.stderr)
+    return line
+
+
 def latency_stats(res):
     lat = np.array([r.t_done - r.t_arrival for r in res])
     t0 = min(r.t_arrival for r in res)
@@ -447,7 +623,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["c2", "c4", "c5"], default="c2")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c2")
+    ap.add_argument("--c3-dim", type=int, default=15_000_000)
+    ap.add_argument("--c3-fact", type=int, default=60_000_000)
     ap.add_argument("--log-build", type=int, default=24)
     ap.add_argument("--log-probe", type=int, default=28)
     ap.add_argument("--bits", type=int, default=14)
@@ -471,7 +649,7 @@ def main():
         print("bench.py measures one GPU: every box in this setup has one MI355X, so --gpus must be 1 "
               "(the path shards by radix partition; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    line = {"c2": run_c2, "c4": run_c4, "c5": run_c5}[args.workload](args)
+    line = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
     print(json.dumps(line), flush=True)
 
 

@@ -16,6 +16,9 @@
 // usage: refjoin <c1|c2> <log2 R> <log2 S> <threads> <zipf s> <buckets> <tuplesperbucket> <sum|count|probesum>
 //        refjoin c4 <first join> <joins> <threads> <tuplesperbucket> <sum|probesum>
 //          (the batched-join family: one plan per join, buckets = 2^ceil(log2 n))
+//        refjoin c3 <dimension rows> <fact rows> <threads>
+//          (join + group-by: hashjoin(D, F) -> aggregate_sum(field 0) per worker ->
+//           merge; the consumer folds the per-worker groups)
 // prints one JSON line {"value": ..., "seconds": ..., ...}
 #include <pthread.h>
 #include <stdint.h>
@@ -174,7 +177,8 @@ static int run_plan(char* R, uint64_t nR, char* S, uint64_t nS, bool is32, bool 
 static void usage() {
     fprintf(stderr,
             "usage: refjoin <c1|c2> <log2R> <log2S> <threads> <zipf> <buckets> <tpb> <sum|count|probesum>\n"
-            "       refjoin c4 <first> <joins> <threads> <tpb> <sum|probesum>\n");
+            "       refjoin c4 <first> <joins> <threads> <tpb> <sum|probesum>\n"
+            "       refjoin c3 <dimension rows> <fact rows> <threads>\n");
 }
 
 // The batched family: joins first..first+count-1, each its own plan; inputs
@@ -223,8 +227,131 @@ static int run_c4(int argc, char** argv) {
     return 0;
 }
 
+// C3: SUM(F.price) GROUP BY D.custkey over F JOIN D ON orderkey, with the
+// packed rows the reference would scan (D: int orderkey, int custkey;
+// F: int orderkey, int partkey, int quantity, dec price = 20 bytes).
+static int run_c3(int argc, char** argv) {
+    if (argc < 5) {
+        usage();
+        return 2;
+    }
+    const uint64_t nD = strtoull(argv[2], NULL, 10), nF = strtoull(argv[3], NULL, 10);
+    const int T = atoi(argv[4]);
+    const uint64_t seed = PGPU_DEFAULT_SEED, norders = nD;
+    std::vector<char> D(nD * 8), F(nF * 20);
+    for (uint64_t i = 0; i < nD; ++i) {
+        int32_t k = pgpu_c3_d_orderkey(seed, i, nD), c = pgpu_c3_d_custkey(seed, i, PGPU_C3_CUSTS);
+        memcpy(&D[i * 8], &k, 4);
+        memcpy(&D[i * 8 + 4], &c, 4);
+    }
+    for (uint64_t j = 0; j < nF; ++j) {
+        int32_t k = pgpu_c3_f_orderkey(seed, j, norders), pk = pgpu_c3_f_partkey(seed, j),
+                q = pgpu_c3_f_quantity(seed, j);
+        double pr = pgpu_c3_f_price(seed, j);
+        memcpy(&F[j * 20], &k, 4);
+        memcpy(&F[j * 20 + 4], &pk, 4);
+        memcpy(&F[j * 20 + 8], &q, 4);
+        memcpy(&F[j * 20 + 12], &pr, 8);
+    }
+    Schema ds, fs;
+    ds.add(CT_INTEGER);
+    ds.add(CT_INTEGER);
+    fs.add(CT_INTEGER);
+    fs.add(CT_INTEGER);
+    fs.add(CT_INTEGER);
+    fs.add(CT_DECIMAL);
+    Config cfg;
+    cfg.getRoot().add("path", Setting::TypeString) = "./";
+    uint64_t bs = nF * 20 / (64 * (uint64_t)T);
+    if (bs > (1u << 20)) bs = 1u << 20;
+    if (bs < 4096) bs = 4096;
+    bs &= ~(uint64_t)15;
+    cfg.getRoot().add("buffsize", Setting::TypeInt) = (int)bs;
+    Setting& jn = cfg.getRoot().add("join", Setting::TypeGroup);
+    Setting& hn = jn.add("hash", Setting::TypeGroup);
+    hn.add("fn", Setting::TypeString) = "modulo";
+    hn.add("buckets", Setting::TypeInt) = 1 << 24;
+    Setting& tg = jn.add("threadgroups", Setting::TypeList);
+    Setting& g0 = tg.add(Setting::TypeArray);
+    for (int i = 0; i < T; ++i) g0.add(Setting::TypeInt) = i;
+    jn.add("tuplesperbucket", Setting::TypeInt) = 4;
+    jn.add("allocpolicy", Setting::TypeString) = "local";
+    jn.add("buildjattr", Setting::TypeInt) = 0;
+    jn.add("probejattr", Setting::TypeInt) = 0;
+    Setting& pj = jn.add("projection", Setting::TypeList);
+    pj.add(Setting::TypeString) = "B$1";
+    pj.add(Setting::TypeString) = "P$3";
+    Setting& an = cfg.getRoot().add("agg", Setting::TypeGroup);
+    an.add("field", Setting::TypeInt) = 0;
+    an.add("sumfield", Setting::TypeInt) = 1;
+    Setting& ah = an.add("hash", Setting::TypeGroup);
+    ah.add("fn", Setting::TypeString) = "modulo";
+    ah.add("buckets", Setting::TypeInt) = 1 << 21;
+    ah.add("field", Setting::TypeInt) = 0;
+    Setting& mn = cfg.getRoot().add("merge", Setting::TypeGroup);
+    mn.add("threads", Setting::TypeInt) = T;
+    Setting& sc = cfg.getRoot().add("scan", Setting::TypeGroup);
+
+    MemScan scanD(&D[0], nD, ds), scanF(&F[0], nF, fs);
+    HashJoinOp join;
+    AggregateSum partial;
+    MergeOp merge;
+    Query q;
+    q.tree = &merge;
+    merge.nextOp = &partial;
+    partial.nextOp = &join;
+    join.buildOp = &scanD;
+    join.probeOp = &scanF;
+    scanD.init(cfg, sc);
+    scanF.init(cfg, sc);
+    join.init(cfg, jn);
+    partial.init(cfg, an);
+    merge.init(cfg, mn);
+
+    q.threadInit();
+    std::vector<double> sums(PGPU_C3_CUSTS + 1, 0.0);
+    std::vector<char> seen(PGPU_C3_CUSTS + 1, 0);
+    double t0 = now();
+    if (q.scanStart() == Operator::Error) {
+        fprintf(stderr, "scanStart failed\n");
+        return 1;
+    }
+    Operator::GetNextResultT r;
+    r.first = Operator::Ready;
+    while (r.first == Operator::Ready) {
+        r = q.getNext();
+        if (r.first == Operator::Error) {
+            fprintf(stderr, "getNext failed\n");
+            return 1;
+        }
+        void* t;
+        Operator::Page::Iterator it = r.second->createIterator();
+        while ((t = it.next())) {
+            const int c = q.getOutSchema().asInt(t, 0);
+            if (c < 0 || c > (int)PGPU_C3_CUSTS) continue;
+            sums[c] += q.getOutSchema().asDecimal(t, 1);  // the final aggregate over the partials
+            seen[c] = 1;
+        }
+    }
+    q.scanStop();
+    double t1 = now();
+    q.threadClose();
+    q.destroynofree();
+    uint64_t groups = 0;
+    double total = 0;
+    for (uint64_t c = 0; c <= PGPU_C3_CUSTS; ++c) {
+        groups += seen[c];
+        total += sums[c];
+    }
+    printf("{\"workload\": \"c3\", \"build\": %llu, \"probe\": %llu, \"threads\": %d, \"groups\": %llu, "
+           "\"total\": %.17g, \"seconds\": %.6f}\n",
+           (unsigned long long)nD, (unsigned long long)nF, T, (unsigned long long)groups, total, t1 - t0);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && std::string(argv[1]) == "c4") return run_c4(argc, argv);
+    if (argc >= 2 && std::string(argv[1]) == "c3") return run_c3(argc, argv);
     if (argc < 9) {
         usage();
         return 2;

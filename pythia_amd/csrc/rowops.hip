@@ -245,6 +245,67 @@ __global__ __launch_bounds__(256) void k_agg_append(AggDev a, const char* __rest
     }
 }
 
+// No GROUP BY: one group.  Each workgroup folds its rows in registers and
+// LDS (in the accumulator's type, so an int32 sum still wraps mod 2^32) and
+// folds the partial into the single slot with one atomic, instead of one
+// same-address atomic per row.
+__global__ __launch_bounds__(256) void k_agg_single(AggDev a, const char* __restrict__ rows, uint64_t n,
+                                                    unsigned long long* __restrict__ t, uint64_t mask,
+                                                    unsigned long long* __restrict__ ngroups) {
+    __shared__ unsigned long long red[256 / 64];
+    __shared__ double redf[256 / 64];
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    unsigned long long acc = 0;
+    double accf = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const char* row = rows + i * a.row_bytes;
+        switch (a.kind) {
+            case PGPU_AGG_SUM_I32: {
+                int32_t v;
+                __builtin_memcpy(&v, row + a.val_offset, 4);
+                acc += (unsigned long long)(uint32_t)v;  // low 32 bits are the wrapped int32 sum
+                break;
+            }
+            case PGPU_AGG_SUM_I64: {
+                int64_t v;
+                __builtin_memcpy(&v, row + a.val_offset, 8);
+                acc += (unsigned long long)v;
+                break;
+            }
+            case PGPU_AGG_SUM_F64: {
+                double v;
+                __builtin_memcpy(&v, row + a.val_offset, 8);
+                accf += v;
+                break;
+            }
+            default:
+                acc += 1;
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        acc += __shfl_xor(acc, d, 64);
+        accf += __shfl_xor(accf, d, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[w] = acc;
+        redf[w] = accf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long p = 0;
+        double pf = 0;
+        for (int k = 0; k < 256 / 64; ++k) {
+            p += red[k];
+            pf += redf[k];
+        }
+        if (a.kind == PGPU_AGG_SUM_F64) p = (unsigned long long)__double_as_longlong(pf);
+        uint64_t kw[kMaxKeyWords] = {};
+        agg_insert(a, t, mask, kw, nullptr, &p, ngroups);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_agg_rehash(AggDev a, const unsigned long long* __restrict__ old,
                                                     uint64_t oldcap, unsigned long long* __restrict__ t,
                                                     uint64_t mask, unsigned long long* __restrict__ ngroups) {
@@ -611,7 +672,7 @@ int pgpu_agg_append(pgpu_agg* g, const void* rows, uint64_t n) {
     // Keep the table at most half full even if every row is a new group.
     PGPU_HIP(hipMemcpyAsync(g->h_ngroups, g->d_ngroups, 8, hipMemcpyDeviceToHost, st));
     PGPU_HIP(hipStreamSynchronize(st));
-    uint64_t need = *g->h_ngroups + n;
+    uint64_t need = g->a.nkeys ? *g->h_ngroups + n : 1;  // no GROUP BY: one group
     if (need > g->cap / 2) {
         uint64_t nc = g->cap;
         while (need > nc / 2) nc *= 2;
@@ -621,8 +682,12 @@ int pgpu_agg_append(pgpu_agg* g, const void* rows, uint64_t n) {
     }
     PGPU_TRY(slot_reserve(g->ctx, &g->d_in, &g->in_cap, n * g->a.row_bytes));
     PGPU_HIP(hipMemcpyAsync(g->d_in, rows, n * g->a.row_bytes, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_agg_append, dim3(grid_for(n)), dim3(256), 0, st, g->a, g->d_in, n, g->d_table, g->cap - 1,
-                       g->d_ngroups);
+    if (g->a.nkeys == 0)
+        hipLaunchKernelGGL(k_agg_single, dim3((unsigned)std::min<uint64_t>(grid_for(n), 1024)), dim3(256), 0, st, g->a,
+                           g->d_in, n, g->d_table, g->cap - 1, g->d_ngroups);
+    else
+        hipLaunchKernelGGL(k_agg_append, dim3(grid_for(n)), dim3(256), 0, st, g->a, g->d_in, n, g->d_table,
+                           g->cap - 1, g->d_ngroups);
     PGPU_LAUNCH_CHECK();
     PGPU_HIP(hipStreamSynchronize(st));
     return PGPU_OK;

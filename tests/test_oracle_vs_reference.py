@@ -72,3 +72,19 @@ def test_c4_reference_equals_oracle(threads):
         r = json.loads(out.stdout)
         assert r["tuples"] == int(9 * sizes.sum())
         assert r["values"] == [i32(getattr(c, field)) for c in ref]
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_c3_reference_equals_oracle(threads):
+    """Join + group-by: the reference's hashjoin -> aggregate_sum(field 0)
+    plan finds the closed form's groups, and its FP64 total matches the exact
+    integer-cent total within rounding."""
+    nD, nF = 20000, 100000
+    out = subprocess.run([REF, "c3", str(nD), str(nF), str(threads)], stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True, timeout=300, cwd="/tmp")
+    assert out.returncode == 0, out.stderr
+    r = json.loads(out.stdout)
+    joined, cents = O.c3_closed(nD, nF, norders=nD)
+    assert r["groups"] == int(np.count_nonzero(cents))
+    exact = int(cents.sum()) / 100.0
+    assert abs(r["total"] - exact) <= 1e-12 * exact
