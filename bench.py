@@ -471,9 +471,17 @@ def run_c3(args):
     gj.timing(False)
     keys, sums, counts = gj.fetch(ng)
     total = float(np.sum(sums))
-    print(f"# c3 step {ms_step:.3f} ms, {ng} groups, joined {nj}; kernels: " +
+    path = {pa.GJ_PARTITIONED: "partitioned", pa.GJ_GENERAL: "general"}[gj.path()]
+    print(f"# c3 step {ms_step:.3f} ms ({path} path), {ng} groups, joined {nj}; kernels: " +
           ", ".join(f"{k}={v['avg_ms']:.3f}ms" for k, v in per_kernel.items()), file=sys.stderr, flush=True)
-    kb = {"gj_probe": nF * 12, "gj_build": nD * 8, "gj_compact": 1_500_000 * 24}
+    # algorithmic bytes per timed region: each kernel's own reads + writes of
+    # the columns and partitioned rows it must touch (partitioned path), or
+    # the fused probe's column reads (general path)
+    ncust = 1_500_000
+    kb = {"gj_minmax": nD * 8, "gj_dim_hist": nD * 4, "gj_dim_scatter": nD * 14, "gj_fact_hist": nF * 4,
+          "gj_fact_scatter": nF * 22, "gj_probe_hist": nF * 2 + nD * 6, "gj_probe_scatter": nF * 20 + nD * 6,
+          "gj_agg": nF * 10 + ncust * 12, "gj_compact": ncust * 12 + ng * 20,
+          "gj_probe": nF * 12, "gj_build": nD * 8}
     compulsory = nF * 12 + nD * 8 + 1_500_000 * 12
     line = {
         "metric": METRIC,
@@ -490,7 +498,7 @@ def run_c3(args):
         "data": "synthetic (counter-based SplitMix64 generators, generated in HBM)",
         "config": {"workload": f"join + group-by FP64 sum: fact {nF} rows (int32 orderkey, FP64 price), "
                                f"dimension {nD} rows (int32 orderkey, int32 custkey), group by custkey",
-                   "dimension_rows": nD, "fact_rows": nF, "groups": ng},
+                   "dimension_rows": nD, "fact_rows": nF, "groups": ng, "path": path},
         "roofline": roofline(per_kernel, kb, None),
         "job_roofline": {"compulsory_bytes": compulsory, "achieved_GBps": compulsory / (ms_step / 1e3) / 1e9,
                          "frac": compulsory / (ms_step / 1e3) / HBM_PEAK},

@@ -404,6 +404,13 @@ int pgpu_groupjoin_run(pgpu_groupjoin* p, const int32_t* d_dkey, const int32_t* 
                        uint64_t* n_joined);
 /* Copies the first n result groups (any of the outputs may be NULL). */
 int pgpu_groupjoin_fetch(pgpu_groupjoin* p, int32_t* keys, double* sums, uint64_t* counts, uint64_t n);
+/* The algorithm the last run used: PGPU_GJ_PARTITIONED (unique dimension
+ * keys within a range of 2^25 values, group keys within 2^22: two radix
+ * passes, LDS lookups and LDS sums) or PGPU_GJ_GENERAL (HBM hash tables and
+ * FP64 atomics: any keys, duplicate dimension keys). */
+#define PGPU_GJ_PARTITIONED 1
+#define PGPU_GJ_GENERAL 2
+int pgpu_groupjoin_path(pgpu_groupjoin* p, int* path);
 int pgpu_groupjoin_timing(pgpu_groupjoin* p, int enable);
 int pgpu_groupjoin_times(pgpu_groupjoin* p, pgpu_kernel_time* out, int cap, int* n);
 

@@ -20,6 +20,7 @@ __all__ = ["Context", "DeviceBuffer", "JoinPlan", "GroupJoin", "BatchPlan", "bat
            "SERVE_BATCHED", "SERVE_PLAIN"]
 
 SERVE_BATCHED, SERVE_PLAIN = 0, 1
+GJ_PARTITIONED, GJ_GENERAL = 1, 2
 
 
 def device_count() -> int:
@@ -200,6 +201,12 @@ class GroupJoin:
         call("pgpu_groupjoin_fetch", self.handle, k.ctypes.data_as(C.c_void_p), s.ctypes.data_as(C.c_void_p),
              c.ctypes.data_as(C.c_void_p), n)
         return k, s, c
+
+    def path(self) -> int:
+        """GJ_PARTITIONED or GJ_GENERAL: the algorithm of the last run."""
+        v = C.c_int()
+        call("pgpu_groupjoin_path", self.handle, C.byref(v))
+        return v.value
 
     def timing(self, on: bool):
         call("pgpu_groupjoin_timing", self.handle, 1 if on else 0)

@@ -6,6 +6,7 @@ closed forms (exact int64 sums) up to the full C2 shape.  Integer results
 must be bit-exact.
 """
 import os
+import zlib
 
 import numpy as np
 import pytest
@@ -231,6 +232,8 @@ def test_c3_groupjoin_matches_exact_cents(ctx, nD, nF):
     exact = cents[k] / 100.0
     assert np.all(np.abs(s - exact) <= 1e-12 * np.abs(exact))
     assert len(np.unique(k)) == ng
+    assert int(c.sum()) == nF
+    assert gj.path() == pa.GJ_PARTITIONED
     gj.close()
     for b in bufs:
         b.free()
@@ -257,9 +260,128 @@ def test_groupjoin_duplicates_and_sentinel_keys(ctx):
     gj = pa.GroupJoin(ctx, 300, 5000, 100)
     ng, joined = gj.run(bufs[0], bufs[1], 300, bufs[2], bufs[3], 5000)
     k, s, c = gj.fetch(ng)
+    assert gj.path() == pa.GJ_GENERAL  # duplicate dimension keys
     assert dict(zip(k.tolist(), s.tolist())).keys() == exp.keys()
     for kk, ss in zip(k.tolist(), s.tolist()):
         assert abs(ss - exp[kk]) <= 1e-12 * max(1.0, abs(exp[kk]))
+    gj.close()
+    for b in bufs:
+        b.free()
+
+
+I32_MIN, I32_MAX = np.iinfo(np.int32).min, np.iinfo(np.int32).max
+
+
+def _gj_oracle(dk, dg, fk, fv):
+    """sum(F.val) group by D.grp over F join D on key through the oracle's
+    HashJoinOp + GenericAggregate restatement: (joined rows, {grp: sum})."""
+    ds, fs = O.schema(("int",), ("int",)), O.schema(("int",), ("dec",))
+    d = O.join_desc(ds, fs, 0, 0, ["B$1", "P$1"], buckets=65536, tuplesperbucket=4)
+    joined, _ = O.hashjoin(d, O.rows(ds, dk, dg), O.rows(fs, fk, fv))
+    out = O.aggregate(O.agg_desc(O.join_outschema(d), [0], 1, buckets=65536), joined)
+    return len(joined), dict(zip(out["c0"].tolist(), out["c1"].tolist()))
+
+
+def _gj_device(ctx, dk, dg, fk, fv, max_groups):
+    bufs = [_dev(ctx, a) for a in (dk, dg, fk, fv)]
+    gj = pa.GroupJoin(ctx, len(dk), len(fk), max_groups)
+    try:
+        ng, joined = gj.run(bufs[0], bufs[1], len(dk), bufs[2], bufs[3], len(fk))
+        k, s, c = gj.fetch(ng)
+        return gj.path(), joined, k, s, c
+    finally:
+        gj.close()
+        for b in bufs:
+            b.free()
+
+
+def _gj_case(name, rng):
+    """(dim keys, dim groups, fact keys, fact values, max_groups, expected path)."""
+    if name == "gaps_misses_low":  # keys and groups at the bottom of the int32 range
+        dk = (I32_MIN + rng.choice(100_000, 50_000, replace=False)).astype(np.int32)
+        dg = (I32_MIN + rng.integers(0, 3000, dk.size)).astype(np.int32)
+        fk = (I32_MIN + rng.integers(0, 110_000, 300_000)).astype(np.int32)
+        return dk, dg, fk, rng.random(fk.size) + 0.5, 3000, pa.GJ_PARTITIONED
+    if name == "gaps_misses_high":  # keys and groups at the top, misses below the key range
+        dk = (I32_MAX - rng.choice(100_000, 60_000, replace=False)).astype(np.int32)
+        dg = (I32_MAX - rng.integers(0, 50_000, dk.size)).astype(np.int32)
+        fk = (I32_MAX - rng.integers(0, 120_000, 300_000)).astype(np.int32)
+        return dk, dg, fk, rng.random(fk.size) + 0.5, 50_000, pa.GJ_PARTITIONED
+    if name == "skew":  # one key holds 60 % of the fact rows; 7 groups: multi-item partitions
+        dk = rng.permutation(5000).astype(np.int32)
+        dg = (dk % 7).astype(np.int32)
+        fk = rng.integers(0, 5000, 400_000).astype(np.int32)
+        fk[rng.random(fk.size) < 0.6] = 17
+        return dk, dg, fk, rng.random(fk.size) + 0.5, 7, pa.GJ_PARTITIONED
+    if name == "widest_dense":  # 2^25 key range, 2^22 group range: every partition count at its maximum
+        dk = np.sort(rng.choice(1 << 25, 100_000, replace=False)).astype(np.int32)
+        dk[0], dk[-1] = 0, (1 << 25) - 1
+        dg = rng.integers(0, 1 << 22, dk.size).astype(np.int32)
+        dg[0], dg[-1] = 0, (1 << 22) - 1
+        fk = rng.choice(dk, 400_000).astype(np.int32)
+        miss = rng.random(fk.size) < 0.1
+        fk[miss] = rng.integers(-5, (1 << 25) + 5, int(miss.sum()))
+        return dk, dg, fk, rng.random(fk.size) + 0.5, 100_000, pa.GJ_PARTITIONED
+    if name == "wide_keys":  # key range beyond 2^25: general path
+        dk = np.unique(rng.integers(I32_MIN, I32_MAX, 60_000)).astype(np.int32)[:50_000]
+        dg = rng.integers(0, 1000, dk.size).astype(np.int32)
+        fk = rng.choice(dk, 200_000).astype(np.int32)
+        return dk, dg, fk, rng.random(fk.size) + 0.5, 1000, pa.GJ_GENERAL
+    if name == "wide_groups":  # group range beyond 2^22: general path
+        dk = rng.permutation(20_000).astype(np.int32)
+        dg = rng.integers(I32_MIN + 1, I32_MAX, dk.size).astype(np.int32)
+        fk = rng.integers(0, 20_000, 200_000).astype(np.int32)
+        return dk, dg, fk, rng.random(fk.size) + 0.5, 20_000, pa.GJ_GENERAL
+    if name == "tiny":
+        dk, dg = np.array([5], np.int32), np.array([I32_MIN], np.int32)
+        fk = np.array([5, 4, 5, 6, 5], np.int32)
+        return dk, dg, fk, np.array([1.0, 2.0, 3.0, 4.0, 5.0]), 1, pa.GJ_PARTITIONED
+    raise ValueError(name)
+
+
+@pytest.mark.parametrize("name", ["gaps_misses_low", "gaps_misses_high", "skew", "widest_dense", "wide_keys",
+                                  "wide_groups", "tiny"])
+def test_groupjoin_vs_oracle(ctx, name):
+    """Group join against the oracle's hashjoin -> aggregate_sum: the key /
+    group range edges, misses, skewed partitions split into several work
+    items, the largest partition counts, and the general path's cases."""
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    dk, dg, fk, fv, max_groups, path = _gj_case(name, rng)
+    exp_joined, exp = _gj_oracle(dk, dg, fk, fv)
+    got_path, joined, k, s, c = _gj_device(ctx, dk, dg, fk, fv, max_groups)
+    assert got_path == path
+    assert joined == exp_joined and int(c.sum()) == exp_joined
+    assert len(np.unique(k)) == len(k)
+    assert set(k.tolist()) == set(exp)
+    for kk, ss in zip(k.tolist(), s.tolist()):
+        assert abs(ss - exp[kk]) <= 1e-12 * abs(exp[kk])
+
+
+@pytest.mark.parametrize("dense", [True, False])
+def test_groupjoin_more_groups_than_planned_is_an_error(ctx, dense):
+    """More result groups than max_groups: a clear overflow error on both
+    paths, no write past the outputs."""
+    rng = np.random.default_rng(5)
+    dk = rng.permutation(10_000).astype(np.int32) if dense else \
+        np.unique(rng.integers(I32_MIN, I32_MAX, 12_000)).astype(np.int32)[:10_000]
+    dg = np.arange(dk.size, dtype=np.int32)
+    fk = rng.choice(dk, 50_000).astype(np.int32)
+    bufs = [_dev(ctx, a) for a in (dk, dg, fk, rng.random(fk.size))]
+    gj = pa.GroupJoin(ctx, dk.size, fk.size, 100)
+    with pytest.raises(pa.PgpuError):
+        gj.run(bufs[0], bufs[1], dk.size, bufs[2], bufs[3], fk.size)
+    gj.close()
+    for b in bufs:
+        b.free()
+
+
+@pytest.mark.parametrize("nD,nF", [(0, 1000), (1000, 0)])
+def test_groupjoin_empty_sides(ctx, nD, nF):
+    dk = np.arange(max(nD, 1), dtype=np.int32)
+    fk = np.arange(max(nF, 1), dtype=np.int32)
+    bufs = [_dev(ctx, a) for a in (dk, dk, fk, fk.astype(np.float64))]
+    gj = pa.GroupJoin(ctx, max(nD, 1), max(nF, 1), 10)
+    assert gj.run(bufs[0], bufs[1], nD, bufs[2], bufs[3], nF) == (0, 0)
     gj.close()
     for b in bufs:
         b.free()
