@@ -499,12 +499,15 @@ def run_c3(args):
         "config": {"workload": f"join + group-by FP64 sum: fact {nF} rows (int32 orderkey, FP64 price), "
                                f"dimension {nD} rows (int32 orderkey, int32 custkey), group by custkey",
                    "dimension_rows": nD, "fact_rows": nF, "groups": ng, "path": path},
-        "roofline": roofline(per_kernel, kb, None),
+        "roofline": roofline(per_kernel, kb, args.traffic or os.path.join(ROOT, "profiles", "traffic_c3.json")),
         "job_roofline": {"compulsory_bytes": compulsory, "achieved_GBps": compulsory / (ms_step / 1e3) / 1e9,
                          "frac": compulsory / (ms_step / 1e3) / HBM_PEAK},
         "kernels": per_kernel,
         "result": {"groups": ng, "joined": nj, "total": total},
     }
+    rd, cp = ctx.membw(1 << 30, 10)
+    line["hbm_calibration"] = {"read_GBps": rd, "copy_GBps": cp,
+                               "note": "streaming uint4 read / copy kernels on this device, same run"}
     if not args.no_cpu_baseline:
         gj.close()
         for b in (dk, dc, fk, fp):
