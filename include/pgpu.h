@@ -383,6 +383,48 @@ int pgpu_agg_finish(pgpu_agg* a, uint64_t* n_groups, uint32_t* out_row_bytes);
 int pgpu_agg_pull(pgpu_agg* a, uint64_t first, uint64_t n, void* dst);
 
 /* ------------------------------------------------------------------ */
+/* Row-format range partitioning (gpupartition).  Stands in for          */
+/* PartitionOp (operators/partition.cpp:44-404) with its exactrange hash  */
+/* (ExactRangeValueHasher, hash.h:444-479):                              */
+/*   bucket = (key - min) / ((max - min + buckets) / buckets)            */
+/* in signed 64-bit arithmetic truncating toward zero, key = the signed  */
+/* 4- or 8-byte column at key_offset (an 8-byte dec column is taken by   */
+/* its bits, as numericalize does).  Output bucket b = every input row   */
+/* of bucket b in input order, source 0's rows first (the reference's    */
+/* idxstart order over worker threads), optionally sorted by a column    */
+/* (sortAllInPage; ties keep input order).                               */
+/* ------------------------------------------------------------------ */
+enum pgpu_col_type { PGPU_COL_INT32 = 1, PGPU_COL_INT64 = 2, PGPU_COL_F64 = 3 };
+#define PGPU_PART_MAX_BUCKETS 127 /* PartitionOp: buckets < MAX_THREADS (128) */
+#define PGPU_PART_MAX_SOURCES 256
+
+typedef struct pgpu_part_desc {
+    uint32_t row_bytes;
+    uint32_t key_offset, key_bytes; /* key_bytes 4 or 8                     */
+    uint32_t buckets;               /* 1..PGPU_PART_MAX_BUCKETS             */
+    int64_t range_min, range_max;
+    uint32_t sort;                  /* 0: keep input order in a bucket      */
+    uint32_t sort_offset, sort_type; /* enum pgpu_col_type                  */
+    uint32_t pad;
+} pgpu_part_desc;
+
+typedef struct pgpu_part pgpu_part;
+int pgpu_part_create(pgpu_ctx* ctx, const pgpu_part_desc* desc, pgpu_part** out);
+int pgpu_part_destroy(pgpu_part* p);
+/* Thread-safe: stages n rows of input stream `source` (copied before
+ * return).  Streams are concatenated in source order. */
+int pgpu_part_append(pgpu_part* p, uint32_t source, const void* rows, uint64_t n);
+/* Uploads the staged rows and partitions (and sorts) them on the device;
+ * counts (optional) receives the rows per bucket.  A key outside the
+ * buckets (the reference indexes past its histogram there) fails with
+ * PGPU_EOVERFLOW. */
+int pgpu_part_finish(pgpu_part* p, uint64_t* counts);
+/* Copies rows [first, first + n) of bucket b into dst. */
+int pgpu_part_pull(pgpu_part* p, uint32_t bucket, uint64_t first, uint64_t n, void* dst);
+/* Drops the staged input and the result; the handle accepts new rows. */
+int pgpu_part_reset(pgpu_part* p);
+
+/* ------------------------------------------------------------------ */
 /* Join + group-by FP64 sum over device-resident columns (C3 shape):    */
 /*   SUM(F.val) GROUP BY D.grp over F JOIN D ON F.key = D.key            */
 /* Stands in for the plan hashjoin(build D, probe F, projection          */

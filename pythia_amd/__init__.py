@@ -15,7 +15,7 @@ import numpy as np
 from . import _lib
 from ._lib import ALGO_AUTO, ALGO_GLOBAL, ALGO_RADIX, GEN, PgpuError, call, lib  # noqa: F401
 
-__all__ = ["Context", "DeviceBuffer", "JoinPlan", "GroupJoin", "BatchPlan", "batch_autotune", "Server",
+__all__ = ["Context", "DeviceBuffer", "JoinPlan", "GroupJoin", "Partition", "BatchPlan", "batch_autotune", "Server",
            "c5_trace", "device_count", "PgpuError", "GEN", "ALGO_AUTO", "ALGO_RADIX", "ALGO_GLOBAL",
            "SERVE_BATCHED", "SERVE_PLAIN"]
 
@@ -213,6 +213,52 @@ class GroupJoin:
 
     def times(self):
         return _times("pgpu_groupjoin_times", self.handle)
+
+
+COL_INT32, COL_INT64, COL_F64 = 1, 2, 3
+
+
+class Partition:
+    """Stable exactrange partitioning of packed rows (the gpupartition
+    operator's device side): rows appended per source, partitioned on
+    finish(), pulled per bucket."""
+
+    def __init__(self, ctx: Context, row_bytes: int, key_offset: int, key_bytes: int, range_min: int,
+                 range_max: int, buckets: int, sort_offset: int | None = None, sort_type: int = COL_INT64):
+        self.ctx, self.row_bytes, self.buckets = ctx, row_bytes, buckets
+        d = _lib.PartDesc(row_bytes=row_bytes, key_offset=key_offset, key_bytes=key_bytes, buckets=buckets,
+                          range_min=range_min, range_max=range_max, sort=0 if sort_offset is None else 1,
+                          sort_offset=sort_offset or 0, sort_type=sort_type)
+        h = C.c_void_p()
+        call("pgpu_part_create", ctx.handle, C.byref(d), C.byref(h))
+        self.handle = h
+        self.counts = None
+
+    def close(self):
+        if self.handle:
+            lib().pgpu_part_destroy(self.handle)
+            self.handle = None
+
+    def append(self, source: int, rows: np.ndarray):
+        rows = np.ascontiguousarray(rows)
+        assert rows.dtype.itemsize == self.row_bytes
+        call("pgpu_part_append", self.handle, source, rows.ctypes.data_as(C.c_void_p), len(rows))
+
+    def finish(self) -> np.ndarray:
+        counts = np.zeros(self.buckets, np.uint64)
+        call("pgpu_part_finish", self.handle, counts.ctypes.data_as(C.c_void_p))
+        self.counts = counts
+        return counts
+
+    def pull(self, bucket: int, dtype) -> np.ndarray:
+        n = int(self.counts[bucket])
+        out = np.zeros(n, dtype=dtype)
+        call("pgpu_part_pull", self.handle, bucket, 0, n, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def reset(self):
+        call("pgpu_part_reset", self.handle)
+        self.counts = None
 
 
 def _batch_array(joins):

@@ -75,6 +75,12 @@ class ServerStats(C.Structure):
                                           "completed", "batches", "queued", "partitioned")]
 
 
+class PartDesc(C.Structure):
+    _fields_ = [("row_bytes", C.c_uint32), ("key_offset", C.c_uint32), ("key_bytes", C.c_uint32),
+                ("buckets", C.c_uint32), ("range_min", C.c_int64), ("range_max", C.c_int64), ("sort", C.c_uint32),
+                ("sort_offset", C.c_uint32), ("sort_type", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -140,6 +146,12 @@ _PROTOS = {
     "pgpu_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "pgpu_c5_trace": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                 C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pgpu_part_create": (C.c_int, [C.c_void_p, C.POINTER(PartDesc), C.POINTER(C.c_void_p)]),
+    "pgpu_part_destroy": (C.c_int, [C.c_void_p]),
+    "pgpu_part_append": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "pgpu_part_finish": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pgpu_part_pull": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
+    "pgpu_part_reset": (C.c_int, [C.c_void_p]),
     "pgpu_groupjoin_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pgpu_groupjoin_destroy": (C.c_int, [C.c_void_p]),
     "pgpu_groupjoin_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,

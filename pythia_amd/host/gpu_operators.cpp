@@ -517,3 +517,152 @@ Schema GpuAggregateCount::foldinit(libconfig::Setting&) {
     s.add(CT_LONG);
     return s;
 }
+
+// ------------------------------------------------------------------ gpupartition
+void GpuPartitionOp::init(libconfig::Config& root, libconfig::Setting& cfg) {
+    Operator::init(root, cfg);
+    schema = nextOp->getOutSchema();
+    attribute_ = (unsigned)(int)cfg["attr"];
+    if (attribute_ >= schema.columns()) throw InvalidParameter("partition: attr out of range");
+    // exactrange as TupleHasher::create builds it (hash.cpp:182-187); the
+    // hashed field goes through numericalize (hash.h:215-234): 4 or 8 bytes
+    rmin_ = (int)cfg["range"][0];
+    rmax_ = (int)cfg["range"][1];
+    const int b = (int)cfg["buckets"];
+    if (b < 1 || b >= MAX_THREADS) throw InvalidParameter("partition: buckets must be in 1..127 (one thread each)");
+    if (rmax_ < rmin_) throw InvalidParameter("partition: range max below min");
+    buckets_ = (unsigned)b;
+    const unsigned w = schema.getColumnWidth(attribute_);
+    if (w != 4 && w != 8) throw IllegalConversionException();
+    if (cfg.exists("maxtuplesinM"))
+        (void)(int)cfg["maxtuplesinM"];
+    else
+        (void)(int)cfg["maxtuples"];
+    std::string sort = cfg["sort"];
+    sort_ = sort == "yes";
+    sortattr_ = attribute_;
+    if (sort_) {
+        int sa = (int)attribute_;
+        cfg.lookupValue("sortattr", sa);
+        if (sa < 0 || (unsigned)sa >= schema.columns()) throw InvalidParameter("partition: sortattr out of range");
+        sortattr_ = (unsigned)sa;
+        // sortAllInPage sorts int, long / date and dec keys (partition.cpp:228-249)
+        const ColumnType t = schema.getColumnType(sortattr_);
+        if (t != CT_INTEGER && t != CT_LONG && t != CT_DATE && t != CT_DECIMAL) throw NotYetImplemented();
+    }
+    barrier_.init((int)buckets_);
+    knobs.read(cfg);
+    counts_.assign(buckets_, 0);
+    state.clear();
+    state.resize(MAX_THREADS);
+}
+
+void GpuPartitionOp::threadInit(unsigned short tid) {
+    if (tid >= buckets_) throw InvalidParameter("partition: run on exactly `buckets` threads (thread id >= buckets)");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ctx) ctx = pythia::acquire_device(knobs.device, knobs.hbmbudget);
+    if (!part_) {
+        pgpu_part_desc d{};
+        d.row_bytes = schema.getTupleSize();
+        d.key_offset = schema.getColumnOffset(attribute_);
+        d.key_bytes = schema.getColumnWidth(attribute_);
+        d.buckets = buckets_;
+        d.range_min = rmin_;
+        d.range_max = rmax_;
+        d.sort = sort_ ? 1 : 0;
+        if (sort_) {
+            d.sort_offset = schema.getColumnOffset(sortattr_);
+            const ColumnType t = schema.getColumnType(sortattr_);
+            d.sort_type = t == CT_INTEGER ? PGPU_COL_INT32 : t == CT_DECIMAL ? PGPU_COL_F64 : PGPU_COL_INT64;
+        }
+        check(pgpu_part_create(ctx, &d, &part_), "pgpu_part_create");
+    }
+    state[tid].rows.clear();
+    state[tid].n = state[tid].pos = 0;
+}
+
+Operator::ResultCode GpuPartitionOp::scanStart(unsigned short tid, Page* idx, Schema& idxschema) {
+    ThreadState& st = state[tid];
+    ResultCode rc = nextOp->scanStart(tid, idx, idxschema);
+    if (rc != Ready) {
+        barrier_.break_();
+        return rc;
+    }
+    const unsigned ts = schema.getTupleSize();
+    std::vector<char> rows;
+    GetNextResultT r{Ready, nullptr};
+    try {
+        do {
+            r = nextOp->getNext(tid);
+            if (r.first == Error) break;
+            const uint64_t n = r.second->getNumTuples();
+            rows.insert(rows.end(), r.second->data(), r.second->data() + n * ts);
+            if (rows.size() >= (64u << 20)) {
+                check(pgpu_part_append(part_, tid, rows.data(), rows.size() / ts), "pgpu_part_append");
+                rows.clear();
+            }
+        } while (r.first == Ready);
+        if (r.first != Error && !rows.empty())
+            check(pgpu_part_append(part_, tid, rows.data(), rows.size() / ts), "pgpu_part_append");
+    } catch (...) {
+        barrier_.break_();
+        throw;
+    }
+    rc = nextOp->scanStop(tid);
+    if (r.first == Error || rc != Ready) {
+        barrier_.break_();
+        return Error;
+    }
+    if (!barrier_.Arrive()) return Error;  // every thread has staged its input
+    if (tid == 0) {
+        try {
+            check(pgpu_part_finish(part_, counts_.data()), "pgpu_part_finish");
+        } catch (...) {
+            barrier_.break_();
+            throw;
+        }
+    }
+    if (!barrier_.Arrive()) return Error;  // the partitions exist
+    st.n = counts_[tid];
+    st.pos = 0;
+    st.rows.resize(st.n * ts);
+    check(pgpu_part_pull(part_, tid, 0, st.n, st.rows.data()), "pgpu_part_pull");
+    return Ready;
+}
+
+Operator::GetNextResultT GpuPartitionOp::getNext(unsigned short tid) {
+    ThreadState& st = state[tid];
+    const unsigned ts = schema.getTupleSize();
+    const uint64_t maxout = buffsize / ts;
+    const uint64_t rem = st.n - st.pos;
+    const uint64_t k = rem <= maxout ? rem : maxout;
+    if (k == 0) {
+        st.view.reset();
+        return {Finished, &EmptyPage};
+    }
+    st.view.reset(new Page(st.rows.data() + st.pos * ts, k * ts, nullptr, ts));
+    st.pos += k;
+    return {rem <= maxout ? Finished : Ready, st.view.get()};
+}
+
+Operator::ResultCode GpuPartitionOp::scanStop(unsigned short tid) {
+    const bool ok = barrier_.Arrive();  // every thread is done with its partition
+    ThreadState& st = state[tid];
+    st.view.reset();
+    std::vector<char>().swap(st.rows);
+    st.n = st.pos = 0;
+    if (tid == 0 && part_) check(pgpu_part_reset(part_), "pgpu_part_reset");
+    return ok ? Ready : Error;
+}
+
+void GpuPartitionOp::threadClose(unsigned short tid) {
+    state[tid].view.reset();
+    std::vector<char>().swap(state[tid].rows);
+}
+
+void GpuPartitionOp::destroy() {
+    if (part_) pgpu_part_destroy(part_);
+    part_ = nullptr;
+    if (ctx) pythia::release_device(ctx);
+    ctx = nullptr;
+}

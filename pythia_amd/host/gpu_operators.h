@@ -1,5 +1,6 @@
-// `gpuhashjoin`, `gpuaggregate_sum` and `gpuaggregate_count`: drop-ins for
-// Pythia's `hashjoin`, `aggregate_sum` and `aggregate_count` plan types.
+// `gpuhashjoin`, `gpuaggregate_sum`, `gpuaggregate_count` and `gpupartition`:
+// drop-ins for Pythia's `hashjoin`, `aggregate_sum`, `aggregate_count` and
+// `partition` plan types.
 // They read exactly the reference operators' configuration keys and produce
 // the same output schema and Page protocol; the join / aggregation itself
 // runs on the GPU through the C ABI of pgpu.h (nothing else crosses to the
@@ -7,6 +8,7 @@
 // group: device, hbmbudget (bytes), probebatch (probe rows per device call).
 #pragma once
 
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -120,4 +122,43 @@ class GpuAggregateCount : public GpuAggregate {
   protected:
     Schema foldinit(libconfig::Setting& node) override;  // one CT_LONG count
     uint32_t kind() const override { return PGPU_AGG_COUNT; }
+};
+
+class GpuPartitionOp : public SingleInputOp {
+  public:
+    void accept(Visitor* v) override { v->visit(static_cast<SingleInputOp*>(this)); }
+    // Keys as PartitionOp::init (operators/partition.cpp:44-101): attr,
+    // range = [min, max] and buckets of its exactrange hash (buckets = the
+    // threads running the operator, < 128), maxtuples | maxtuplesinM (required
+    // there to size the staging area; read and ignored here), sort = "yes" |
+    // "no", sortattr (defaults to attr).
+    void init(libconfig::Config& root, libconfig::Setting& node) override;
+    void threadInit(unsigned short threadid) override;
+    // Drains the child (scanStart .. scanStop), waits for every thread, the
+    // first thread partitions all rows on the device, and each thread pulls
+    // its own partition (operators/partition.cpp:271-359).
+    ResultCode scanStart(unsigned short threadid, Page* indexdatapage, Schema& indexdataschema) override;
+    // Pages of at most buffsize / tuplesize rows over the thread's partition;
+    // the last one (possibly empty) comes with Finished (partition.cpp:376-404).
+    GetNextResultT getNext(unsigned short threadid) override;
+    ResultCode scanStop(unsigned short threadid) override;
+    void threadClose(unsigned short threadid) override;
+    void destroy() override;
+
+  private:
+    struct ThreadState {
+        std::vector<char> rows;  // this thread's partition
+        uint64_t n = 0, pos = 0;
+        std::unique_ptr<Page> view;
+    };
+    unsigned attribute_ = 0, sortattr_ = 0, buckets_ = 0;
+    int64_t rmin_ = 0, rmax_ = 0;
+    bool sort_ = false;
+    Barrier barrier_;
+    pgpu_part* part_ = nullptr;
+    std::vector<uint64_t> counts_;
+    std::vector<ThreadState> state;
+    GpuKnobs knobs;
+    pgpu_ctx* ctx = nullptr;
+    std::mutex mu;
 };

@@ -456,3 +456,59 @@ void CallStateChecker::threadClose(unsigned short tid) {
     if (state_[tid] == Init || state_[tid] == Closed) bad("threadClose without threadInit", tid);
     state_[tid] = Closed;
 }
+
+// ------------------------------------------------------------------ threadidprepend
+void ThreadIdPrependOp::init(libconfig::Config& root, libconfig::Setting& node) {
+    Operator::init(root, node);
+    schema.add(CT_INTEGER);
+    Schema& in = nextOp->getOutSchema();
+    for (unsigned i = 0; i < in.columns(); ++i) schema.add(in.get(i));
+    st_.assign(MAX_THREADS, State());
+}
+
+void ThreadIdPrependOp::threadInit(unsigned short tid) {
+    if (buffsize < schema.getTupleSize()) throw InvalidParameter("buffsize smaller than one output tuple");
+    st_[tid] = State();
+    st_[tid].out = new Page(buffsize, schema.getTupleSize());
+}
+
+Operator::ResultCode ThreadIdPrependOp::scanStart(unsigned short tid, Page* p, Schema& s) {
+    State& st = st_[tid];
+    st.in = nullptr;
+    st.rc = Ready;
+    st.off = 0;
+    return nextOp->scanStart(tid, p, s);
+}
+
+Operator::GetNextResultT ThreadIdPrependOp::getNext(unsigned short tid) {
+    State& st = st_[tid];
+    Page* out = st.out;
+    out->clear();
+    const unsigned insz = nextOp->getOutSchema().getTupleSize();
+    const CtInt id = tid;
+    for (;;) {
+        if (st.in) {
+            while (const void* t = st.in->getTupleOffset(st.off)) {
+                void* dst = out->allocateTuple();
+                if (!dst) return {Ready, out};  // page full; resume at st.off
+                schema.writeData(dst, 0, &id);
+                memcpy(schema.calcOffset(dst, 1), t, insz);
+                ++st.off;
+            }
+        }
+        if (st.rc == Finished) {
+            st.in = nullptr;
+            return {Finished, out};
+        }
+        GetNextResultT r = nextOp->getNext(tid);
+        if (r.first == Error) return {Error, out};
+        st.in = r.second;
+        st.rc = r.first;
+        st.off = 0;
+    }
+}
+
+void ThreadIdPrependOp::threadClose(unsigned short tid) {
+    delete st_[tid].out;
+    st_[tid].out = nullptr;
+}

@@ -289,6 +289,27 @@ class MergeOp : public SingleInputOp {
     std::condition_variable cv_;
 };
 
+// threadidprepend (ThreadIdPrependOp, operators/threadidprepend.cpp:42-110):
+// each output tuple is (int id of the worker thread that produced it, the
+// input tuple's columns).
+class ThreadIdPrependOp : public SingleInputOp {
+  public:
+    void init(libconfig::Config& root, libconfig::Setting& node) override;
+    void threadInit(unsigned short tid) override;
+    ResultCode scanStart(unsigned short tid, Page* p, Schema& s) override;
+    GetNextResultT getNext(unsigned short tid) override;
+    void threadClose(unsigned short tid) override;
+
+  private:
+    struct State {
+        Page* out = nullptr;
+        Page* in = nullptr;  // child page being mapped
+        ResultCode rc = Ready;
+        uint64_t off = 0;
+    };
+    std::vector<State> st_;
+};
+
 // CallStateChecker contract (operators/checker_callstate.cpp:53-159): fails
 // the query when a parent violates the lifecycle protocol of its child.
 class CallStateChecker : public SingleInputOp {

@@ -39,7 +39,7 @@ void constructsubtree(libconfig::Config& cfg, libconfig::Setting& node, Operator
         constructsubtree(cfg, node["build"], &op->buildOp);
         sanitycheck(cfg, node, "probe");
         constructsubtree(cfg, node["probe"], &op->probeOp);
-    } else if (type == "hashjoin" || type == "aggregate_sum" || type == "aggregate_count") {
+    } else if (type == "hashjoin" || type == "aggregate_sum" || type == "aggregate_count" || type == "partition") {
         throw MissingParameterException("`" + type + "' is the reference engine's CPU operator; this engine runs `gpu" +
                                         type + "'.");
     } else {
@@ -52,6 +52,10 @@ void constructsubtree(libconfig::Config& cfg, libconfig::Setting& node, Operator
             op = new MergeOp();
         else if (type == "checker_callstate")
             op = new CallStateChecker();
+        else if (type == "gpupartition")
+            op = new GpuPartitionOp();
+        else if (type == "threadidprepend")
+            op = new ThreadIdPrependOp();
         else
             throw MissingParameterException("`" + type + "' is neither a built-in nor a user-defined type.");
         *root = op;

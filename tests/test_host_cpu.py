@@ -40,6 +40,27 @@ def test_scan_merge_protocol_returns_every_row_once(tmp_path, threads):
     assert np.all(out["c1"] == out["c0"] + 0.25)
 
 
+@pytest.mark.parametrize("threads,buffsize", [(1, 20), (3, 20), (5, 4096)])
+def test_threadidprepend(tmp_path, threads, buffsize):
+    """threadidprepend (operators/threadidprepend.cpp:42-110): output =
+    (int worker id, input columns); each worker's rows carry its own id."""
+    f = write_text(tmp_path, "t.tbl", [(i, i ^ 0xABCDEF) for i in range(1, 2001)])
+    tl = ", ".join(str(i) for i in range(threads))
+    plan = textwrap.dedent(f"""
+        path = "{tmp_path}/"; buffsize = {buffsize};
+        treeroot = {{ name = "merge"; input = {{ name = "tid"; input = {{ name = "scan"; }}; }}; }};
+        merge = {{ type = "merge"; threads = {threads}; }};
+        tid = {{ type = "threadidprepend"; }};
+        scan = {{ type = "parallelscan"; filetype = "text"; files = ( "{f}" ); mapping = ( [ {tl} ] );
+                  schema = ( "long", "long" ); }};
+    """)
+    out, meta = P.run(plan)
+    assert meta["schema"] == "int,long,long"
+    assert sorted(out["c1"].tolist()) == list(range(1, 2001))
+    assert np.all((out["c2"] ^ 0xABCDEF) == out["c1"])
+    assert np.all((out["c0"] >= 0) & (out["c0"] < threads))
+
+
 def test_config_syntax_features(tmp_path):
     f = write_text(tmp_path, "t.tbl", [(i, i) for i in range(5)])
     plan = f"""
