@@ -127,9 +127,10 @@ typedef struct pgpu_join_cfg {
     uint32_t probe_payload; /* 1: probe rows carry a payload summed per match        */
     uint32_t algo;          /* enum pgpu_join_algo                                   */
     uint32_t join_variant;  /* LDS join shape (slots x threads, probe rows in flight  */
-                            /* per thread): 0/1 = 4096 x 512 x 8, 2 = 4096 x 256 x   */
-                            /* 32, 3 = 2048 x 256 x 16, 4 = 4096 x 512 x 16; the     */
-                            /* result does not depend on it                         */
+                            /* per thread): 0/6 = 4096 x 512 x 6, 1 = 4096 x 512 x 8,*/
+                            /* 2 = 4096 x 256 x 32, 3 = 2048 x 256 x 16, 4 = 4096 x  */
+                            /* 512 x 16, 5 = 4096 x 512 x 8 at one workgroup per CU; */
+                            /* the result does not depend on it                     */
 } pgpu_join_cfg;
 
 typedef struct pgpu_join_result {

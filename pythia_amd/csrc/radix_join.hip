@@ -308,8 +308,10 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
 // and the units' output runs for (p1, p2) are contiguous too, starting at
 // base(p1,p2) + H[b0][p1,p2].  G > 1 keeps units near a tile when blocks
 // are small.
+// Two workgroups per CU where the registers allow it (pass-2 units are short:
+// the second workgroup hides one unit's start-up and tail).
 template <typename KT, bool HV, int IPT>
-__global__ __launch_bounds__(kScatterThreads) void k_scatter2(const Elem<KT, HV>* __restrict__ in, uint32_t nblk,
+__global__ __launch_bounds__(kScatterThreads, IPT <= 16 ? 4 : 1) void k_scatter2(const Elem<KT, HV>* __restrict__ in, uint32_t nblk,
                                                               uint32_t G, int B, int B2,
                                                               const uint64_t* __restrict__ off1,
                                                               const uint32_t* __restrict__ Hs,
@@ -924,10 +926,12 @@ int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const 
                        (const Elem<KT, true>*)p->R2, p->baseR, p->totR, (const Elem<KT, HVS>*)p->S2, p->baseS,    \
                        p->totS, p->ioff, p->items, nb, B, p->part)
         switch (p->cfg.join_variant) {
+            case 1: PGPU_JOIN(4096, 512, 2048, 8, 4); break;
             case 2: PGPU_JOIN(4096, 256, 2048, 32, 2); break;
             case 3: PGPU_JOIN(2048, 256, 1280, 16, 4); break;
             case 4: PGPU_JOIN(4096, 512, 2048, 16, 4); break;
-            default: PGPU_JOIN(4096, 512, 2048, 8, 4); break;
+            case 5: PGPU_JOIN(4096, 512, 2048, 8, 2); break;
+            default: PGPU_JOIN(4096, 512, 2048, 6, 4); break;  // no VGPR spills at 4 waves per SIMD
         }
 #undef PGPU_JOIN
         PGPU_LAUNCH_CHECK();
@@ -979,7 +983,7 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
     if (cfg->radix_bits != 0 && (cfg->radix_bits < 2 || cfg->radix_bits > kMaxBits))
         return fail(PGPU_EINVAL, "radix_bits must be 0 or in [2, %d] (got %u)", kMaxBits, cfg->radix_bits);
     if (cfg->algo > PGPU_ALGO_GLOBAL) return fail(PGPU_EINVAL, "unknown join algorithm %u", cfg->algo);
-    if (cfg->join_variant > 4) return fail(PGPU_EINVAL, "unknown LDS join variant %u", cfg->join_variant);
+    if (cfg->join_variant > 6) return fail(PGPU_EINVAL, "unknown LDS join variant %u", cfg->join_variant);
     if (max_build >= (1ull << 32) || max_probe >= (1ull << 40))
         return fail(PGPU_EINVAL, "relation too large (build < 2^32, probe < 2^40 rows)");
     PGPU_TRY(ctx_activate(ctx));

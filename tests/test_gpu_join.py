@@ -84,6 +84,22 @@ def test_c2_radix_matches_closed_form(ctx, bits, zipf):
         b.free()
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("bits", [4, 9])
+def test_join_variants_match_closed_form(ctx, variant, bits):
+    """Every LDS join shape gives the same count and sum (4 bits: 4K build
+    rows per partition, several LDS pieces per partition)."""
+    nR, nS = 1 << 16, 1 << 20
+    rk, rp, sk = _c2_device(ctx, nR, nS, 0.0)
+    plan = pa.JoinPlan(ctx, 8, nR, nS, radix_bits=bits, algo=pa.ALGO_RADIX, join_variant=variant)
+    r = plan.count_sum(rk, rp, nR, sk, None, nS)
+    ref = O.c2_closed(nR, nS, 0.0, threads=THREADS)
+    assert (r.matches, r.sum_build) == (ref.count, ref.sum_build)
+    plan.close()
+    for b in (rk, rp, sk):
+        b.free()
+
+
 def test_c2_global_matches_closed_form(ctx):
     nR, nS = 1 << 18, 1 << 21
     rk, rp, sk = _c2_device(ctx, nR, nS, 0.5)
