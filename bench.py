@@ -383,7 +383,8 @@ def run_c1(args):
     T = cpu_cores()
     groups = ", ".join(str(i) for i in range(T))
 
-    def text(agg):
+    def text(agg, resident=False):
+        gpu = "gpu = { resident = true; probebatch = 1048576L; };" if resident else ""
         return textwrap.dedent(f"""
             path = "./"; buffsize = 1048576;
             treeroot = {{ name = "merge"; input = {{ name = "agg"; input = {{ name = "join";
@@ -392,7 +393,7 @@ def run_c1(args):
             agg = {{ type = "gpu{agg}"; fields = (); {'sumfield = 0;' if agg == 'aggregate_sum' else ''} }};
             join = {{ type = "gpuhashjoin"; hash = {{ fn = "modulo"; buckets = 65536; }};
                       threadgroups = ( [ {groups} ] ); tuplesperbucket = 16; allocpolicy = "local";
-                      buildjattr = 0; probejattr = 0; projection = ( "B$1" ); }};
+                      buildjattr = 0; probejattr = 0; projection = ( "B$1" ); {gpu} }};
             r = {{ type = "memscan"; table = "c1_r"; schema = ( "int", "int" ); }};
             s = {{ type = "memscan"; table = "c1_s"; schema = ( "int", "int" ); }};
         """)
@@ -403,18 +404,31 @@ def run_c1(args):
         out, meta = P.run(text("aggregate_sum"))
         secs.append(meta["seconds"])
     cnt, cmeta = P.run(text("aggregate_count"))
-    op_s = float(np.median(secs))
+    page_s = float(np.median(secs))
+    page_sum32 = int(np.int64(out["c0"].astype(np.int64).sum()).astype(np.int32)) if len(out) else 0
+    page_count = int(cnt["c0"].sum())
+    # the same plan with gpu { resident = true; }: relations kept in HBM across
+    # queries (copied on the first run, outside the timed runs), the aggregate
+    # folding the join's output in HBM
+    P.run(text("aggregate_sum", resident=True))
+    rsecs = []
+    for _ in range(max(1, args.steps)):
+        out, meta = P.run(text("aggregate_sum", resident=True))
+        rsecs.append(meta["seconds"])
+    cnt, _ = P.run(text("aggregate_count", resident=True))
+    op_s = float(np.median(rsecs))
     sum32 = int(np.int64(out["c0"].astype(np.int64).sum()).astype(np.int32)) if len(out) else 0
     count = int(cnt["c0"].sum())
     P.clear_tables()
-    print(f"# c1 operator API {1e3 * op_s:.1f} ms per query, device columns {dev_ms:.3f} ms", file=sys.stderr,
-          flush=True)
+    P.clear_resident()
+    print(f"# c1 operator API: resident {1e3 * op_s:.2f} ms per query, host pages {1e3 * page_s:.1f} ms; "
+          f"device columns {dev_ms:.3f} ms", file=sys.stderr, flush=True)
     line = {
         "metric": METRIC,
         "value": (nR + nS) / op_s,
         "unit": "tuples/s",
         "n_gpus": 1,
-        "steps": len(secs),
+        "steps": len(rsecs),
         "warmup": 1,
         "ms_per_step": 1e3 * op_s,
         "higher_is_better": True,
@@ -423,8 +437,11 @@ def run_c1(args):
         "dtype": "int32",
         "data": "synthetic (counter-based SplitMix64 generators)",
         "config": {"workload": f"queryhashjoin plan through the operator API: R 2^20 (int32 key, int32 payload) "
-                               f"x S 2^24, gpuhashjoin -> gpuaggregate_sum / gpuaggregate_count -> merge({T}), "
-                               f"host-resident pages (PCIe-bound)", "threads": T},
+                               f"x S 2^24, gpuhashjoin (gpu.resident: relations in HBM) -> gpuaggregate_sum / "
+                               f"gpuaggregate_count (folded in HBM) -> merge({T})", "threads": T},
+        "operator_api_host_pages": {"ms_per_query": 1e3 * page_s, "tuples_per_s": (nR + nS) / page_s,
+                                    "note": "same plan without gpu.resident: every page crosses the host link",
+                                    "same_result": bool(page_count == count and page_sum32 == sum32)},
         "device_columns": {"ms_per_join": dev_ms, "tuples_per_s": (nR + nS) / (dev_ms / 1e3),
                            "matches": dres.matches, "sum_build": dres.sum_build},
         "result": {"count": count, "sum_int32": sum32,

@@ -350,6 +350,16 @@ int pgpu_hj_build_finish(pgpu_hj* h);
 int pgpu_hj_probe(pgpu_hj* h, uint32_t slot, const void* rows, uint64_t n, uint64_t* n_out);
 /* Copies up to cap_rows further output rows of `slot` into dst. */
 int pgpu_hj_pull(pgpu_hj* h, uint32_t slot, void* dst, uint64_t cap_rows, uint64_t* n_rows);
+/* Device-resident inputs and outputs (tables kept in HBM across queries, an
+ * aggregate fused over the join): no reference counterpart — the reference
+ * moves every tuple through host pages.  build_device builds the table from
+ * n rows already in device memory (they must stay valid until the handle is
+ * destroyed; exclusive with build_append); probe_device joins n device rows;
+ * output hands out the device address of the slot's not yet pulled output
+ * rows (valid until the slot's next probe) and marks them pulled. */
+int pgpu_hj_build_device(pgpu_hj* h, const void* d_rows, uint64_t n);
+int pgpu_hj_probe_device(pgpu_hj* h, uint32_t slot, const void* d_rows, uint64_t n, uint64_t* n_out);
+int pgpu_hj_output(pgpu_hj* h, uint32_t slot, const void** d_rows, uint64_t* n_rows);
 int pgpu_hj_stats_get(pgpu_hj* h, pgpu_hj_stats* out);
 
 /* ------------------------------------------------------------------ */
@@ -378,6 +388,9 @@ int pgpu_agg_create(pgpu_ctx* ctx, const pgpu_agg_desc* desc, pgpu_agg** out);
 int pgpu_agg_destroy(pgpu_agg* a);
 /* Thread-safe; rows are copied to HBM and folded before return. */
 int pgpu_agg_append(pgpu_agg* a, const void* rows, uint64_t n);
+/* As pgpu_agg_append for rows already in device memory (for example a join's
+ * pgpu_hj_output); no reference counterpart. */
+int pgpu_agg_append_device(pgpu_agg* a, const void* d_rows, uint64_t n);
 /* Compacts the groups; output rows = key columns then the accumulator. */
 int pgpu_agg_finish(pgpu_agg* a, uint64_t* n_groups, uint32_t* out_row_bytes);
 /* Copies output rows [first, first + n) into dst. */

@@ -29,6 +29,10 @@ const char* pythia_last_error(void);
 /* Makes rows available to plans as `type = "memscan"; table = "<name>";`. */
 int pythia_register_table(const char* name, const void* rows, uint64_t nrows, uint32_t row_bytes);
 int pythia_clear_tables(void);
+/* Frees the device copies that plans with gpu { resident = true; } keep in
+ * HBM across queries (no reference counterpart: the reference re-reads its
+ * inputs into pages on every query). */
+int pythia_clear_resident(void);
 int pythia_run_query(const char* plan_text, pythia_result* out);
 void pythia_free_result(pythia_result* r);
 

@@ -360,6 +360,7 @@ struct pgpu_hj {
     uint64_t nbuild = 0;
     bool built = false;
     char* d_build = nullptr;
+    bool build_external = false;  // d_build is the caller's device buffer
     JSlot* d_table = nullptr;
     int logcap = 0;
     struct Slot {
@@ -389,6 +390,7 @@ int slot_reserve(pgpu_ctx* ctx, char** p, uint64_t* cap, uint64_t need) {
     *cap = c;
     return PGPU_OK;
 }
+int hj_build_table(pgpu_hj* h);
 }  // namespace
 
 extern "C" {
@@ -436,7 +438,7 @@ int pgpu_hj_destroy(pgpu_hj* h) {
         if (s.d_cursor) (void)ctx_free(h->ctx, s.d_cursor);
         if (s.h_cursor) (void)hipHostFree(s.h_cursor);
     }
-    if (h->d_build) (void)ctx_free(h->ctx, h->d_build);
+    if (h->d_build && !h->build_external) (void)ctx_free(h->ctx, h->d_build);
     if (h->d_table) (void)ctx_free(h->ctx, h->d_table);
     delete h;
     return PGPU_OK;
@@ -462,6 +464,33 @@ int pgpu_hj_build_finish(pgpu_hj* h) {
     const uint64_t bytes = h->staged.size();
     PGPU_TRY(ctx_alloc(h->ctx, std::max<uint64_t>(bytes, 8), (void**)&h->d_build));
     if (bytes) PGPU_HIP(hipMemcpyAsync(h->d_build, h->staged.data(), bytes, hipMemcpyHostToDevice, st));
+    PGPU_TRY(hj_build_table(h));
+    h->stats.h2d_bytes += bytes;
+    std::vector<char>().swap(h->staged);
+    h->built = true;
+    return PGPU_OK;
+}
+
+int pgpu_hj_build_device(pgpu_hj* h, const void* d_rows, uint64_t n) {
+    if (!h || (n && !d_rows)) return fail(PGPU_EINVAL, "pgpu_hj_build_device: null argument");
+    std::lock_guard<std::mutex> g(h->mu);
+    if (h->built || h->nbuild) return fail(PGPU_ESTATE, "pgpu_hj_build_device: the table already has build rows");
+    if (n >= kEmptyIdx) return fail(PGPU_EINVAL, "build side exceeds 2^32-1 rows");
+    PGPU_TRY(ctx_activate(h->ctx));
+    h->d_build = (char*)d_rows;
+    h->build_external = true;
+    h->nbuild = n;
+    PGPU_TRY(hj_build_table(h));
+    h->built = true;
+    return PGPU_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Claims and fills the open-addressing table over h->d_build's h->nbuild rows.
+int hj_build_table(pgpu_hj* h) {
+    hipStream_t st = h->ctx->stream;
     h->logcap = (int)ilog2_ceil(std::max<uint64_t>(2 * h->nbuild, 16));
     const uint64_t cap = 1ull << h->logcap;
     PGPU_TRY(ctx_alloc(h->ctx, cap * sizeof(JSlot), (void**)&h->d_table));
@@ -476,32 +505,13 @@ int pgpu_hj_build_finish(pgpu_hj* h) {
     PGPU_HIP(hipStreamSynchronize(st));
     h->stats.build_rows = h->nbuild;
     h->stats.table_slots = cap;
-    h->stats.h2d_bytes += bytes;
-    std::vector<char>().swap(h->staged);
-    h->built = true;
     return PGPU_OK;
 }
 
-int pgpu_hj_probe(pgpu_hj* h, uint32_t slot, const void* rows, uint64_t n, uint64_t* n_out) {
-    if (!h || !n_out || (n && !rows)) return fail(PGPU_EINVAL, "pgpu_hj_probe: null argument");
-    if (slot >= h->slots.size()) return fail(PGPU_EINVAL, "pgpu_hj_probe: slot %u out of range", slot);
-    if (!h->built) return fail(PGPU_ESTATE, "pgpu_hj_probe before build_finish");
-    PGPU_TRY(ctx_activate(h->ctx));
-    auto& s = h->slots[slot];
-    if (!s.st) PGPU_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
-    if (!s.d_cursor) {
-        PGPU_TRY(ctx_alloc(h->ctx, 8, (void**)&s.d_cursor));
-        PGPU_HIP(hipHostMalloc((void**)&s.h_cursor, 8, hipHostMallocDefault));
-    }
-    s.nout = s.pulled = 0;
-    *n_out = 0;
-    if (n == 0 || h->nbuild == 0) {
-        h->stats.probe_rows += n;
-        return PGPU_OK;
-    }
-    const uint32_t prb = h->jd.d.probe_row_bytes, orb = std::max<uint32_t>(h->jd.out_row_bytes, 1);
-    PGPU_TRY(slot_reserve(h->ctx, &s.d_in, &s.in_cap, n * prb));
-    PGPU_HIP(hipMemcpyAsync(s.d_in, rows, n * prb, hipMemcpyHostToDevice, s.st));
+// Probes n rows that are already in device memory (d_in) on the slot's
+// stream; the output rows stay in the slot's device buffer.
+int hj_probe_rows(pgpu_hj* h, pgpu_hj::Slot& s, const char* d_in, uint64_t n, uint64_t* n_out) {
+    const uint32_t orb = std::max<uint32_t>(h->jd.out_row_bytes, 1);
     uint64_t cap_rows = s.out_cap / orb;
     if (cap_rows < n) {
         PGPU_TRY(slot_reserve(h->ctx, &s.d_out, &s.out_cap, n * orb));
@@ -511,7 +521,7 @@ int pgpu_hj_probe(pgpu_hj* h, uint32_t slot, const void* rows, uint64_t n, uint6
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, s.st, s.d_cursor, 1, 0ull);
         PGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_jprobe, dim3(grid_for(n)), dim3(256), 0, s.st, h->jd, h->d_build, h->d_table,
-                           h->logcap, s.d_in, n, s.d_out, cap_rows, s.d_cursor);
+                           h->logcap, d_in, n, s.d_out, cap_rows, s.d_cursor);
         PGPU_LAUNCH_CHECK();
         PGPU_HIP(hipMemcpyAsync(s.h_cursor, s.d_cursor, 8, hipMemcpyDeviceToHost, s.st));
         PGPU_HIP(hipStreamSynchronize(s.st));
@@ -528,9 +538,65 @@ int pgpu_hj_probe(pgpu_hj* h, uint32_t slot, const void* rows, uint64_t n, uint6
     }
     h->stats.probe_rows += n;
     h->stats.output_rows += s.nout;
-    h->stats.h2d_bytes += n * prb;
     *n_out = s.nout;
     return PGPU_OK;
+}
+
+// Checks a probe call and prepares the slot; *skip when there is nothing to join.
+int hj_probe_begin(pgpu_hj* h, uint32_t slot, uint64_t n, uint64_t* n_out, bool* skip) {
+    if (slot >= h->slots.size()) return fail(PGPU_EINVAL, "pgpu_hj_probe: slot %u out of range", slot);
+    if (!h->built) return fail(PGPU_ESTATE, "pgpu_hj_probe before build_finish");
+    PGPU_TRY(ctx_activate(h->ctx));
+    auto& s = h->slots[slot];
+    if (!s.st) PGPU_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+    if (!s.d_cursor) {
+        PGPU_TRY(ctx_alloc(h->ctx, 8, (void**)&s.d_cursor));
+        PGPU_HIP(hipHostMalloc((void**)&s.h_cursor, 8, hipHostMallocDefault));
+    }
+    s.nout = s.pulled = 0;
+    *n_out = 0;
+    *skip = n == 0 || h->nbuild == 0;
+    if (*skip) h->stats.probe_rows += n;
+    return PGPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pgpu_hj_probe_device(pgpu_hj* h, uint32_t slot, const void* d_rows, uint64_t n, uint64_t* n_out) {
+    if (!h || !n_out || (n && !d_rows)) return fail(PGPU_EINVAL, "pgpu_hj_probe_device: null argument");
+    bool skip = false;
+    PGPU_TRY(hj_probe_begin(h, slot, n, n_out, &skip));
+    if (skip) return PGPU_OK;
+    return hj_probe_rows(h, h->slots[slot], (const char*)d_rows, n, n_out);
+}
+
+int pgpu_hj_output(pgpu_hj* h, uint32_t slot, const void** d_rows, uint64_t* n_rows) {
+    if (!h || !d_rows || !n_rows) return fail(PGPU_EINVAL, "pgpu_hj_output: null argument");
+    if (slot >= h->slots.size()) return fail(PGPU_EINVAL, "pgpu_hj_output: slot out of range");
+    auto& s = h->slots[slot];
+    *d_rows = s.d_out ? s.d_out + s.pulled * (uint64_t)h->jd.out_row_bytes : nullptr;
+    *n_rows = s.nout - s.pulled;
+    s.pulled = s.nout;
+    return PGPU_OK;
+}
+
+int pgpu_hj_probe(pgpu_hj* h, uint32_t slot, const void* rows, uint64_t n, uint64_t* n_out) {
+    if (!h || !n_out || (n && !rows)) return fail(PGPU_EINVAL, "pgpu_hj_probe: null argument");
+    bool skip = false;
+    PGPU_TRY(hj_probe_begin(h, slot, n, n_out, &skip));
+    if (skip) return PGPU_OK;
+    auto& s = h->slots[slot];
+    if (!s.st) PGPU_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+    if (!s.d_cursor) {
+        PGPU_TRY(ctx_alloc(h->ctx, 8, (void**)&s.d_cursor));
+        PGPU_HIP(hipHostMalloc((void**)&s.h_cursor, 8, hipHostMallocDefault));
+    }
+    const uint32_t prb = h->jd.d.probe_row_bytes;
+    PGPU_TRY(slot_reserve(h->ctx, &s.d_in, &s.in_cap, n * prb));
+    PGPU_HIP(hipMemcpyAsync(s.d_in, rows, n * prb, hipMemcpyHostToDevice, s.st));
+    h->stats.h2d_bytes += n * prb;
+    return hj_probe_rows(h, s, s.d_in, n, n_out);
 }
 
 int pgpu_hj_pull(pgpu_hj* h, uint32_t slot, void* dst, uint64_t cap_rows, uint64_t* n_rows) {
@@ -662,17 +728,21 @@ int pgpu_agg_destroy(pgpu_agg* g) {
     return PGPU_OK;
 }
 
-int pgpu_agg_append(pgpu_agg* g, const void* rows, uint64_t n) {
-    if (!g || (n && !rows)) return fail(PGPU_EINVAL, "pgpu_agg_append: null argument");
-    std::lock_guard<std::mutex> lk(g->mu);
-    if (g->finished) return fail(PGPU_ESTATE, "pgpu_agg_append after finish");
-    if (!n) return PGPU_OK;
-    PGPU_TRY(ctx_activate(g->ctx));
+}  // extern "C"
+
+namespace {
+// Folds n device-resident rows into the table (caller holds g->mu; the
+// context is current).  `upload`: rows are on the host and go through d_in.
+int agg_fold(pgpu_agg* g, const void* rows, uint64_t n, bool upload) {
     hipStream_t st = g->ctx->stream;
-    // Keep the table at most half full even if every row is a new group.
-    PGPU_HIP(hipMemcpyAsync(g->h_ngroups, g->d_ngroups, 8, hipMemcpyDeviceToHost, st));
-    PGPU_HIP(hipStreamSynchronize(st));
-    uint64_t need = g->a.nkeys ? *g->h_ngroups + n : 1;  // no GROUP BY: one group
+    // Keep the table at most half full even if every row is a new group
+    // (no GROUP BY: one group).
+    uint64_t need = 1;
+    if (g->a.nkeys) {
+        PGPU_HIP(hipMemcpyAsync(g->h_ngroups, g->d_ngroups, 8, hipMemcpyDeviceToHost, st));
+        PGPU_HIP(hipStreamSynchronize(st));
+        need = *g->h_ngroups + n;
+    }
     if (need > g->cap / 2) {
         uint64_t nc = g->cap;
         while (need > nc / 2) nc *= 2;
@@ -680,17 +750,42 @@ int pgpu_agg_append(pgpu_agg* g, const void* rows, uint64_t n) {
         // require, bounded by rows appended
         PGPU_TRY(agg_grow(g, nc));
     }
-    PGPU_TRY(slot_reserve(g->ctx, &g->d_in, &g->in_cap, n * g->a.row_bytes));
-    PGPU_HIP(hipMemcpyAsync(g->d_in, rows, n * g->a.row_bytes, hipMemcpyHostToDevice, st));
+    const char* src = (const char*)rows;
+    if (upload) {
+        PGPU_TRY(slot_reserve(g->ctx, &g->d_in, &g->in_cap, n * g->a.row_bytes));
+        PGPU_HIP(hipMemcpyAsync(g->d_in, rows, n * g->a.row_bytes, hipMemcpyHostToDevice, st));
+        src = g->d_in;
+    }
     if (g->a.nkeys == 0)
         hipLaunchKernelGGL(k_agg_single, dim3((unsigned)std::min<uint64_t>(grid_for(n), 1024)), dim3(256), 0, st, g->a,
-                           g->d_in, n, g->d_table, g->cap - 1, g->d_ngroups);
+                           src, n, g->d_table, g->cap - 1, g->d_ngroups);
     else
-        hipLaunchKernelGGL(k_agg_append, dim3(grid_for(n)), dim3(256), 0, st, g->a, g->d_in, n, g->d_table,
+        hipLaunchKernelGGL(k_agg_append, dim3(grid_for(n)), dim3(256), 0, st, g->a, src, n, g->d_table,
                            g->cap - 1, g->d_ngroups);
     PGPU_LAUNCH_CHECK();
     PGPU_HIP(hipStreamSynchronize(st));
     return PGPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pgpu_agg_append(pgpu_agg* g, const void* rows, uint64_t n) {
+    if (!g || (n && !rows)) return fail(PGPU_EINVAL, "pgpu_agg_append: null argument");
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (g->finished) return fail(PGPU_ESTATE, "pgpu_agg_append after finish");
+    if (!n) return PGPU_OK;
+    PGPU_TRY(ctx_activate(g->ctx));
+    return agg_fold(g, rows, n, true);
+}
+
+int pgpu_agg_append_device(pgpu_agg* g, const void* d_rows, uint64_t n) {
+    if (!g || (n && !d_rows)) return fail(PGPU_EINVAL, "pgpu_agg_append_device: null argument");
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (g->finished) return fail(PGPU_ESTATE, "pgpu_agg_append_device after finish");
+    if (!n) return PGPU_OK;
+    PGPU_TRY(ctx_activate(g->ctx));
+    return agg_fold(g, d_rows, n, false);
 }
 
 int pgpu_agg_finish(pgpu_agg* g, uint64_t* n_groups, uint32_t* out_row_bytes) {

@@ -19,6 +19,7 @@ void GpuKnobs::read(libconfig::Setting& node) {
     if (g.lookupValue("hbmbudget", b)) hbmbudget = (uint64_t)b;
     long long pb = 0;
     if (g.lookupValue("probebatch", pb) && pb > 0) probebatch = (uint64_t)pb;
+    g.lookupValue("resident", resident);
 }
 
 // The checks TupleHasher::create applies to a hash node
@@ -124,6 +125,17 @@ void GpuHashJoinOp::init(libconfig::Config& root, libconfig::Setting& node) {
     knobs.read(node);
     tables.assign(groupleader.size(), nullptr);
     state.assign(MAX_THREADS, ThreadState());
+    if (knobs.resident) {
+        const char* rows;
+        uint64_t n;
+        std::string key;
+        auto* b = dynamic_cast<ParallelScanOp*>(buildOp);
+        auto* p = dynamic_cast<ParallelScanOp*>(probeOp);
+        if (!b || !p || !b->sharedRows(&rows, &n, &key) || !p->sharedRows(&rows, &n, &key))
+            throw InvalidParameter("gpu.resident: both join inputs must be scans without a thread mapping");
+        if (groupleader.size() != 1) throw InvalidParameter("gpu.resident: the join must have one thread group");
+        resident_ = true;
+    }
 }
 
 void GpuHashJoinOp::fail_group(unsigned short g) { barriers.at(g)->break_(); }
@@ -140,8 +152,11 @@ void GpuHashJoinOp::threadInit(unsigned short tid) {
             }
             pgpu_hj_desc d{};
             Schema& ps = probeOp->getOutSchema();
-            d.build_row_bytes = sbuild.getTupleSize();
-            d.build_key_offset = 0;
+            Schema& bs = buildOp->getOutSchema();
+            // staged build rows are sbuild (key + projected build columns);
+            // resident build rows are the scan's own tuples
+            d.build_row_bytes = resident_ ? bs.getTupleSize() : sbuild.getTupleSize();
+            d.build_key_offset = resident_ ? bs.getColumnOffset(joinattr1) : 0;
             d.build_key_bytes = sbuild.getColumnWidth(0);
             d.probe_row_bytes = ps.getTupleSize();
             d.probe_key_offset = ps.getColumnOffset(joinattr2);
@@ -151,7 +166,10 @@ void GpuHashJoinOp::threadInit(unsigned short tid) {
             unsigned buildcol = 1;
             for (uint32_t j = 0; j < d.nproj; ++j) {
                 if (projection[j].first == BuildSide) {
-                    d.proj[j] = {0, sbuild.getColumnOffset(buildcol), sbuild.getColumnWidth(buildcol)};
+                    if (resident_)
+                        d.proj[j] = {0, bs.getColumnOffset(projection[j].second), bs.getColumnWidth(projection[j].second)};
+                    else
+                        d.proj[j] = {0, sbuild.getColumnOffset(buildcol), sbuild.getColumnWidth(buildcol)};
                     ++buildcol;
                 } else {
                     d.proj[j] = {1, ps.getColumnOffset(projection[j].second), ps.getColumnWidth(projection[j].second)};
@@ -171,6 +189,7 @@ void GpuHashJoinOp::threadInit(unsigned short tid) {
 }
 
 Operator::ResultCode GpuHashJoinOp::scanStart(unsigned short tid, Page* idx, Schema& idxschema) {
+    if (resident_) return scanStartResident(tid, idx, idxschema);
     const unsigned short g = threadgroups[tid];
     Schema& bs = buildOp->getOutSchema();
     if (buildOp->scanStart(tid, idx, idxschema) == Error) {
@@ -268,6 +287,24 @@ Operator::GetNextResultT GpuHashJoinOp::getNext(unsigned short tid) {
     const unsigned ts = schema.getTupleSize();
     const uint64_t cap = out->capacity() / ts;
     pgpu_hj* t = tables[threadgroups[tid]];
+    while (resident_) {
+        if (st.pending) {
+            const uint64_t used = out->getNumTuples();
+            uint64_t got = 0;
+            check(pgpu_hj_pull(t, tid, out->data() + used * ts, std::min(cap - used, st.pending), &got),
+                  "pgpu_hj_pull");
+            out->setUsed((used + got) * ts);
+            st.pending -= got;
+            if (out->getNumTuples() == cap) return {Ready, out};
+            continue;
+        }
+        uint64_t nout = 0;
+        if (!probeNextResident(tid, &nout)) {
+            st.done = true;
+            return {Finished, out};
+        }
+        st.pending = nout;
+    }
     while (true) {
         if (st.pending) {
             const uint64_t used = out->getNumTuples();
@@ -294,6 +331,74 @@ Operator::GetNextResultT GpuHashJoinOp::getNext(unsigned short tid) {
 }
 
 Operator::ResultCode GpuHashJoinOp::scanStop(unsigned short tid) { return probeOp->scanStop(tid); }
+
+// Resident mode: the leader builds the table straight from the device copy
+// of the build relation; the probe relation's device copy is then consumed
+// in probebatch chunks from a shared cursor.  The child scans only see
+// scanStart / scanStop (their pages are not read).
+Operator::ResultCode GpuHashJoinOp::scanStartResident(unsigned short tid, Page* idx, Schema& idxschema) {
+    const unsigned short g = threadgroups[tid];
+    if (buildOp->scanStart(tid, idx, idxschema) == Error || buildOp->scanStop(tid) == Error) {
+        fail_group(g);
+        return Error;
+    }
+    if (groupleader[g] == tid) {
+        try {
+            auto* b = dynamic_cast<ParallelScanOp*>(buildOp);
+            auto* p = dynamic_cast<ParallelScanOp*>(probeOp);
+            const char* rows;
+            uint64_t n;
+            std::string key;
+            b->sharedRows(&rows, &n, &key);
+            const unsigned bts = buildOp->getOutSchema().getTupleSize();
+            const void* d_build = pythia::resident_rows(knobs.device, knobs.hbmbudget, key, rows, n * bts);
+            check(pgpu_hj_build_device(tables[g], d_build, n), "pgpu_hj_build_device");
+            p->sharedRows(&rows, &n, &key);
+            const unsigned pts = probeOp->getOutSchema().getTupleSize();
+            d_probe_ = (const char*)pythia::resident_rows(knobs.device, knobs.hbmbudget, key, rows, n * pts);
+            nprobe_rows_ = n;
+            probe_cursor_ = 0;
+        } catch (...) {
+            fail_group(g);
+            throw;
+        }
+    }
+    if (!barriers[g]->Arrive()) return Error;
+    ResultCode rc = probeOp->scanStart(tid, idx, idxschema);
+    if (rc == Error) return Error;
+    ThreadState& st = state[tid];
+    Page* keep = st.out;
+    st = ThreadState();
+    st.out = keep;
+    if (nprobe_rows_ == 0) {  // an empty probe side is Finished (operators/join.cpp:444-459)
+        st.done = true;
+        return Finished;
+    }
+    return rc;
+}
+
+// Joins the next probebatch chunk of the resident probe relation on thread
+// tid's slot; false when the relation is exhausted.
+bool GpuHashJoinOp::probeNextResident(unsigned short tid, uint64_t* nout) {
+    const uint64_t c = probe_cursor_.fetch_add(knobs.probebatch);
+    if (c >= nprobe_rows_) return false;
+    const uint64_t n = std::min<uint64_t>(knobs.probebatch, nprobe_rows_ - c);
+    const unsigned pts = probeOp->getOutSchema().getTupleSize();
+    check(pgpu_hj_probe_device(tables[threadgroups[tid]], tid, d_probe_ + c * pts, n, nout), "pgpu_hj_probe_device");
+    return true;
+}
+
+bool GpuHashJoinOp::nextDeviceBatch(unsigned short tid, const void** d_rows, uint64_t* n) {
+    ThreadState& st = state[tid];
+    if (st.done) return false;
+    uint64_t nout = 0;
+    if (!probeNextResident(tid, &nout)) {
+        st.done = true;
+        return false;
+    }
+    check(pgpu_hj_output(tables[threadgroups[tid]], tid, d_rows, n), "pgpu_hj_output");
+    return true;
+}
 
 void GpuHashJoinOp::threadClose(unsigned short tid) {
     delete state[tid].out;
@@ -416,8 +521,19 @@ Operator::ResultCode GpuAggregate::scanStart(unsigned short tid, Page* idx, Sche
     const unsigned ts = nextOp->getOutSchema().getTupleSize();
     std::vector<char> rows;
     GetNextResultT r{Ready, nullptr};
+    // Over a resident gpuhashjoin on the same device the join's output rows
+    // are folded where they are, in HBM (no pages cross the host link).
+    auto* join = dynamic_cast<GpuHashJoinOp*>(nextOp);
+    const bool fused = join && join->residentMode() && join->device() == knobs.device;
     try {
-        do {
+        if (fused) {
+            const void* d = nullptr;
+            uint64_t n = 0;
+            while (join->nextDeviceBatch(tid, &d, &n))
+                if (n) check(pgpu_agg_append_device(t, d, n), "pgpu_agg_append_device");
+            r.first = Finished;
+        }
+        while (!fused) {
             r = nextOp->getNext(tid);
             if (r.first == Error) break;
             const uint64_t n = r.second->getNumTuples();
@@ -426,7 +542,8 @@ Operator::ResultCode GpuAggregate::scanStart(unsigned short tid, Page* idx, Sche
                 check(pgpu_agg_append(t, rows.data(), rows.size() / ts), "pgpu_agg_append");
                 rows.clear();
             }
-        } while (r.first == Ready);
+            if (r.first != Ready) break;
+        }
         if (r.first != Error && !rows.empty())
             check(pgpu_agg_append(t, rows.data(), rows.size() / ts), "pgpu_agg_append");
     } catch (...) {

@@ -5,9 +5,13 @@
 // the same output schema and Page protocol; the join / aggregation itself
 // runs on the GPU through the C ABI of pgpu.h (nothing else crosses to the
 // device).  Optional extra keys, ignored by the CPU types, live in a `gpu`
-// group: device, hbmbudget (bytes), probebatch (probe rows per device call).
+// group: device, hbmbudget (bytes), probebatch (probe rows per device call),
+// resident (gpuhashjoin over unmapped scans: the scanned relations are kept in
+// HBM across queries and joined there; a gpuaggregate directly above such a
+// join folds its output in HBM).
 #pragma once
 
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -19,6 +23,7 @@ struct GpuKnobs {
     int device = 0;
     uint64_t hbmbudget = 0;
     uint64_t probebatch = 1 << 20;
+    bool resident = false;
     void read(libconfig::Setting& node);
 };
 
@@ -42,6 +47,17 @@ class GpuHashJoinOp : public DualInputOp {
     const Schema& buildSchema() const { return sbuild; }
     pgpu_hj_stats stats(unsigned short group);
 
+    // Resident mode (gpu { resident = true; }): both inputs are device copies
+    // of the child scans' relations, one thread group, probe rows taken from
+    // a shared cursor in probebatch chunks.
+    bool residentMode() const { return resident_; }
+    int device() const { return knobs.device; }
+    // Resident mode: joins thread tid's next probe chunk and hands out the
+    // output rows in HBM (valid until the thread's next call); false when the
+    // probe relation is exhausted.  Used between scanStart and scanStop in
+    // place of getNext.
+    bool nextDeviceBatch(unsigned short tid, const void** d_rows, uint64_t* n);
+
   private:
     struct ThreadState {
         Page* out = nullptr;
@@ -64,6 +80,13 @@ class GpuHashJoinOp : public DualInputOp {
     GpuKnobs knobs;
     pgpu_ctx* ctx = nullptr;
     std::mutex mu;
+    // resident mode
+    bool resident_ = false;
+    const char* d_probe_ = nullptr;
+    uint64_t nprobe_rows_ = 0;
+    std::atomic<uint64_t> probe_cursor_{0};
+    ResultCode scanStartResident(unsigned short tid, Page* idx, Schema& idxschema);
+    bool probeNextResident(unsigned short tid, uint64_t* nout);
 };
 
 class GpuAggregate : public SingleInputOp {

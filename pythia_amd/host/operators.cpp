@@ -2,6 +2,8 @@
 // of the host engine the GPU operators plug into.
 #include "operators.h"
 
+#include <sys/stat.h>
+
 #include <cstdio>
 #include <fstream>
 #include <map>
@@ -157,7 +159,14 @@ void ParallelScanOp::init(libconfig::Config& root, libconfig::Setting& node) {
         }
     }
     std::vector<std::vector<char>> per_file;
+    source_key_ = "scan:" + type + ":" + seps + ":";
+    for (unsigned i = 0; i < schema.columns(); ++i)
+        source_key_ += std::to_string((int)schema.getColumnType(i)) + "/" + std::to_string(schema.getColumnWidth(i)) + ",";
     for (auto& f : files) {
+        struct stat sb {};
+        if (stat((path + f).c_str(), &sb) == 0)
+            source_key_ += "|" + path + f + ":" + std::to_string((long long)sb.st_size) + ":" +
+                           std::to_string((long long)sb.st_mtim.tv_sec) + "." + std::to_string((long long)sb.st_mtim.tv_nsec);
         std::ifstream in(path + f, std::ios::binary);
         if (!in) throw MissingParameterException("cannot open " + path + f);
         std::stringstream ss;
@@ -187,6 +196,14 @@ void ParallelScanOp::init(libconfig::Config& root, libconfig::Setting& node) {
         groups_.back()->rows = std::move(per_file[g]);
         groups_.back()->n = groups_.back()->rows.size() / schema.getTupleSize();
     }
+}
+
+bool ParallelScanOp::sharedRows(const char** rows, uint64_t* nrows, std::string* key) const {
+    if (groups_.size() != 1) return false;
+    *rows = groups_[0]->rows.data();
+    *nrows = groups_[0]->n;
+    *key = source_key_;
+    return true;
 }
 
 void ParallelScanOp::setRows(std::vector<char>&& rows) {
@@ -237,9 +254,11 @@ void MemoryScanOp::init(libconfig::Config& root, libconfig::Setting& node) {
     schema = Schema::create(node["schema"]);
     std::string name = node["table"];
     std::vector<char> rows;
-    if (!pythia::lookup_table(name, schema.getTupleSize(), rows))
+    uint64_t generation = 0;
+    if (!pythia::lookup_table(name, schema.getTupleSize(), rows, &generation))
         throw MissingParameterException("memscan: no registered table `" + name + "' with this row width");
     setRows(std::move(rows));
+    source_key_ = "memscan:" + name + ":" + std::to_string(generation) + ":" + std::to_string(schema.getTupleSize());
 }
 
 // ------------------------------------------------------------ merge

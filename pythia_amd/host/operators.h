@@ -225,8 +225,14 @@ class ParallelScanOp : public ZeroInputOp {
     void threadClose(unsigned short tid) override;
     void destroy() override;
 
+    // An unmapped scan's whole relation and a key naming its contents (file
+    // paths, sizes and modification times, or a registered table and its
+    // generation), for device-resident copies; false for a mapped scan.
+    bool sharedRows(const char** rows, uint64_t* nrows, std::string* key) const;
+
   protected:
     void setRows(std::vector<char>&& rows);
+    std::string source_key_;
     struct Group {
         std::vector<char> rows;
         uint64_t n = 0;

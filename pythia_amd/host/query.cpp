@@ -112,6 +112,11 @@ int pythia_clear_tables(void) {
     return 0;
 }
 
+int pythia_clear_resident(void) {
+    pythia::clear_resident();
+    return 0;
+}
+
 // The executequery driver loop (drivers/executequery.cpp:32-101): create,
 // threadInit, scanStart, getNext until Finished, scanStop, threadClose,
 // destroy; every output row is collected.

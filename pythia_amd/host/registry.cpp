@@ -1,5 +1,6 @@
 #include "registry.h"
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -9,7 +10,19 @@ namespace pythia {
 
 namespace {
 std::mutex g_mu;
-std::map<std::string, std::pair<unsigned, std::vector<char>>> g_tables;
+struct Table {
+    unsigned row_bytes = 0;
+    std::vector<char> rows;
+    uint64_t generation = 0;
+};
+std::map<std::string, Table> g_tables;
+uint64_t g_generation = 0;
+struct Resident {
+    pgpu_ctx* ctx = nullptr;
+    void* d = nullptr;
+};
+std::mutex g_res_mu;  // held while a source is copied; never taken under g_mu
+std::map<std::pair<int, std::string>, Resident> g_resident;
 struct Dev {
     pgpu_ctx* ctx = nullptr;
     int refs = 0;
@@ -20,8 +33,9 @@ std::map<int, Dev> g_devs;
 void register_table(const std::string& name, const void* rows, uint64_t nrows, unsigned row_bytes) {
     std::lock_guard<std::mutex> g(g_mu);
     auto& t = g_tables[name];
-    t.first = row_bytes;
-    t.second.assign((const char*)rows, (const char*)rows + nrows * row_bytes);
+    t.row_bytes = row_bytes;
+    t.rows.assign((const char*)rows, (const char*)rows + nrows * row_bytes);
+    t.generation = ++g_generation;
 }
 
 void clear_tables() {
@@ -29,11 +43,12 @@ void clear_tables() {
     g_tables.clear();
 }
 
-bool lookup_table(const std::string& name, unsigned row_bytes, std::vector<char>& out) {
+bool lookup_table(const std::string& name, unsigned row_bytes, std::vector<char>& out, uint64_t* generation) {
     std::lock_guard<std::mutex> g(g_mu);
     auto it = g_tables.find(name);
-    if (it == g_tables.end() || it->second.first != row_bytes) return false;
-    out = it->second.second;
+    if (it == g_tables.end() || it->second.row_bytes != row_bytes) return false;
+    out = it->second.rows;
+    if (generation) *generation = it->second.generation;
     return true;
 }
 
@@ -59,6 +74,33 @@ void release_device(pgpu_ctx* ctx) {
             return;
         }
     }
+}
+
+const void* resident_rows(int device, uint64_t hbm_budget, const std::string& key, const char* rows, uint64_t bytes) {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    auto it = g_resident.find({device, key});
+    if (it != g_resident.end()) return it->second.d;
+    Resident r;
+    r.ctx = acquire_device(device, hbm_budget);
+    int rc = pgpu_alloc(r.ctx, std::max<uint64_t>(bytes, 256), &r.d);
+    if (rc == PGPU_OK && bytes) rc = pgpu_h2d(r.ctx, r.d, rows, bytes);
+    if (rc != PGPU_OK) {
+        const std::string msg = std::string("resident copy of a scan source: ") + pgpu_last_error();
+        if (r.d) pgpu_free(r.ctx, r.d);
+        release_device(r.ctx);
+        throw QueryExecutionError(msg);
+    }
+    g_resident[{device, key}] = r;
+    return r.d;
+}
+
+void clear_resident() {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    for (auto& e : g_resident) {
+        pgpu_free(e.second.ctx, e.second.d);
+        release_device(e.second.ctx);
+    }
+    g_resident.clear();
 }
 
 }  // namespace pythia

@@ -66,6 +66,11 @@ def clear_tables():
     lib().pythia_clear_tables()
 
 
+def clear_resident():
+    """Frees the HBM copies kept by plans with gpu { resident = true; }."""
+    lib().pythia_clear_resident()
+
+
 def run(plan: str):
     """Runs a plan; returns (rows as a structured array, Result metadata)."""
     r = Result()

@@ -208,3 +208,86 @@ def test_join_then_groupby_sum_fp64(tmp_path):
     for c, s in zip(out["c0"], out["c1"]):
         exact = cents[c] / 100.0
         assert abs(s - exact) <= 1e-12 * abs(exact)
+
+
+@pytest.mark.parametrize("threads,batch", [(1, 1 << 20), (4, 997)])
+def test_resident_join_matches_oracle(tmp_path, threads, batch):
+    """gpu { resident = true; }: the scanned relations are joined from their
+    HBM copies; the output is the same byte-exact multiset as the reference
+    join, on the first run and on a second run that reuses the copies."""
+    rng = np.random.default_rng(40 + threads)
+    nb, np_ = 3000, 20000
+    bs = O.schema(("int",), ("char", 6), ("dec",))
+    ps = O.schema(("long",), ("long",), ("int",))
+    bk = rng.integers(-500, 500, nb).astype(np.int32)
+    bc = np.array([b"k%04d" % (x % 10000) for x in rng.integers(0, 10000, nb)], dtype="S6")
+    brow = O.rows(bs, bk, bc, rng.random(nb))
+    prow = O.rows(ps, rng.integers(-600, 600, np_), rng.integers(-2**62, 2**62, np_),
+                  rng.integers(-1000, 1000, np_).astype(np.int32))
+    proj = ["P$2", "B$1", "P$0", "B$2", "B$0", "P$1"]
+    d = O.join_desc(bs, ps, 0, 0, proj, buckets=256, tuplesperbucket=4, threads=2)
+    exp, _ = O.hashjoin(d, brow, prow, cap=len(prow) * 20)
+    fb, fp = write_bin(tmp_path, "b.bin", brow), write_bin(tmp_path, "p.bin", prow)
+    plan = join_plan(tmp_path, f'"{fb}"', f'"{fp}"', threads, '"int", "char(5)", "dec"', '"long", "long", "int"',
+                     ", ".join(f'"{x}"' for x in proj), buffsize=4096, filetype="binary", checker=False,
+                     extra=f"gpu = {{ resident = true; probebatch = {batch}L; }};")
+    try:
+        for _ in range(2):
+            out, meta = P.run(plan)
+            rb = meta["row_bytes"]
+            assert len(out) == len(exp) > 0
+            got = sorted(out.tobytes()[i * rb:(i + 1) * rb] for i in range(len(out)))
+            want = sorted(exp.tobytes()[i * rb:(i + 1) * rb] for i in range(len(exp)))
+            assert got == want
+    finally:
+        P.clear_resident()
+
+
+def _resident_agg_plan(agg, threads, extra_agg=""):
+    groups = ", ".join(str(i) for i in range(threads))
+    sumfield = "sumfield = 0;" if agg == "aggregate_sum" else ""
+    return textwrap.dedent(f"""
+        path = "./"; buffsize = 65536;
+        treeroot = {{ name = "merge"; input = {{ name = "agg"; input = {{ name = "join";
+                      build = {{ name = "r"; }}; probe = {{ name = "s"; }}; }}; }}; }};
+        merge = {{ type = "merge"; threads = {threads}; }};
+        agg = {{ type = "gpu{agg}"; fields = (); {sumfield} {extra_agg} }};
+        join = {{ type = "gpuhashjoin"; hash = {{ fn = "modulo"; buckets = 4096; }};
+                  threadgroups = ( [ {groups} ] ); tuplesperbucket = 4; allocpolicy = "local";
+                  buildjattr = 0; probejattr = 0; projection = ( "B$1" );
+                  gpu = {{ resident = true; probebatch = 30000L; }}; }};
+        r = {{ type = "memscan"; table = "res_r"; schema = ( "int", "int" ); }};
+        s = {{ type = "memscan"; table = "res_s"; schema = ( "int", "int" ); }};
+    """)
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_resident_join_with_fused_aggregate(threads):
+    """The queryhashjoin plan with a resident join: gpuaggregate_sum /
+    gpuaggregate_count fold the join output in HBM.  The int32 sum wraps as
+    the reference's CtInt accumulator (each thread's partial, then the
+    consumer's sum, all mod 2^32); re-registering a table replaces its HBM
+    copy."""
+    rng = np.random.default_rng(threads)
+    rdt = np.dtype({"names": ["k", "v"], "formats": [np.int32, np.int32], "offsets": [0, 4], "itemsize": 8})
+    try:
+        for rep in range(2):
+            nR, nS = 5000 + rep, 200_000
+            R = np.empty(nR, rdt)
+            R["k"] = rng.permutation(nR).astype(np.int32) * 3
+            R["v"] = rng.integers(-2**31, 2**31 - 1, nR).astype(np.int32)
+            S = np.empty(nS, rdt)
+            S["k"] = (rng.integers(0, nR + 500, nS) * 3).astype(np.int32)
+            S["v"] = 0
+            P.register_table("res_r", R)
+            P.register_table("res_s", S)
+            bv = dict(zip(R["k"].tolist(), R["v"].tolist()))
+            matched = [bv[k] for k in S["k"].tolist() if k in bv]
+            out, _ = P.run(_resident_agg_plan("aggregate_sum", threads))
+            cnt, _ = P.run(_resident_agg_plan("aggregate_count", threads))
+            assert int(cnt["c0"].sum()) == len(matched)
+            got = int(out["c0"].astype(np.int64).sum()) & 0xFFFFFFFF
+            assert got == sum(matched) & 0xFFFFFFFF
+    finally:
+        P.clear_tables()
+        P.clear_resident()

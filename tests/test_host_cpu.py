@@ -145,3 +145,24 @@ def test_device_failure_is_an_error_not_a_hang(tmp_path):
     fp = write_text(tmp_path, "p.tbl", [(i, i + 0.1) for i in range(1, 50)])
     with pytest.raises(P.QueryError):
         P.run(join_plan(tmp_path, fb, fp, 4, extra="gpu = { hbmbudget = 1L; };"))
+
+
+@pytest.mark.parametrize("case", ["mapped", "groups", "not_scan"])
+def test_resident_join_needs_unmapped_scans_and_one_group(tmp_path, case):
+    f = write_text(tmp_path, "t.tbl", [(i, i) for i in range(10)])
+    groups = "( [ 0 ], [ 1 ] )" if case == "groups" else "( [ 0, 1 ] )"
+    mapping = "mapping = ( [ 0, 1 ] );" if case == "mapped" else ""
+    probe = 'name = "chk"; input = { name = "p"; };' if case == "not_scan" else 'name = "p";'
+    plan = textwrap.dedent(f"""
+        path = "{tmp_path}/"; buffsize = 64;
+        treeroot = {{ name = "merge"; input = {{ name = "join"; build = {{ name = "b"; }}; probe = {{ {probe} }}; }}; }};
+        merge = {{ type = "merge"; threads = 2; }};
+        chk = {{ type = "checker_callstate"; }};
+        join = {{ type = "gpuhashjoin"; hash = {{ fn = "modulo"; buckets = 16; }}; threadgroups = {groups};
+                  tuplesperbucket = 2; allocpolicy = "local"; buildjattr = 0; probejattr = 0;
+                  projection = ( "B$1" ); gpu = {{ resident = true; }}; }};
+        b = {{ type = "parallelscan"; filetype = "text"; files = ( "{f}" ); schema = ( "long", "long" ); {mapping} }};
+        p = {{ type = "parallelscan"; filetype = "text"; files = ( "{f}" ); schema = ( "long", "long" ); }};
+    """)
+    with pytest.raises(P.QueryError, match="gpu.resident"):
+        P.run(plan)
