@@ -190,44 +190,51 @@ __global__ __launch_bounds__(512) void k_colscan(uint32_t* __restrict__ H, uint3
     }
 }
 
-// base[i] = sum of in[0..i) for i <= n, single workgroup.
+// base[i] = sum of in[0..i) for i <= n (n <= 16384), single workgroup: each
+// thread holds its 16 consecutive counts in registers (one load batch).
 __global__ __launch_bounds__(1024) void k_binscan(const uint32_t* __restrict__ in, uint32_t n,
                                                   uint64_t* __restrict__ base) {
-    const uint32_t per = (n + 1023) / 1024;
-    const uint32_t t0 = threadIdx.x * per;
+    constexpr int PER = 16;
+    const uint32_t t0 = threadIdx.x * PER;
+    uint32_t v[PER];
     uint64_t s = 0;
-    for (uint32_t i = 0; i < per; ++i)
-        if (t0 + i < n) s += in[t0 + i];
-    uint64_t total = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        v[i] = t0 + i < n ? in[t0 + i] : 0u;
+        s += v[i];
+    }
     __shared__ uint64_t tot_sh;
     uint64_t ex = block_excl_scan1024(s, &tot_sh);
-    (void)total;
-    for (uint32_t i = 0; i < per; ++i) {
-        if (t0 + i < n) {
-            base[t0 + i] = ex;
-            ex += in[t0 + i];
-        }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        if (t0 + i < n) base[t0 + i] = ex;
+        ex += v[i];
     }
     __syncthreads();
     if (threadIdx.x == 0) base[n] = tot_sh;
 }
 
 // off1[b][p1] = base[p1 << B2] + sum_p2 Hs[b][p1, p2] for b < nblk;
-// off1[nblk][p1] = base[(p1 + 1) << B2] (end of partition p1).
+// off1[nblk][p1] = base[(p1 + 1) << B2] (end of partition p1).  One
+// wavefront per (b, p1): its 2^B2 (<= 128) counts are read coalesced and
+// reduced with shuffles.
 __global__ __launch_bounds__(256) void k_off1(const uint32_t* __restrict__ Hs, const uint64_t* __restrict__ base,
                                               uint32_t nblk, int B, int B2, uint64_t* __restrict__ off1) {
     const uint32_t nb1 = 1u << (B - B2), nb2 = 1u << B2, nb = 1u << B;
-    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= (uint64_t)(nblk + 1) * nb1) return;
     const uint32_t b = (uint32_t)(t / nb1), p1 = (uint32_t)(t % nb1);
     if (b == nblk) {
-        off1[t] = base[(uint64_t)(p1 + 1) << B2];
+        if (lane == 0) off1[t] = base[(uint64_t)(p1 + 1) << B2];
         return;
     }
     const uint32_t* row = Hs + (uint64_t)b * nb + ((uint64_t)p1 << B2);
-    uint64_t s = 0;
-    for (uint32_t j = 0; j < nb2; ++j) s += row[j];
-    off1[t] = base[(uint64_t)p1 << B2] + s;
+    uint64_t s = 0;  // column prefixes: their sum over 128 bins can pass 2^32
+    for (uint32_t j = lane; j < nb2; j += 64) s += row[j];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) off1[t] = base[(uint64_t)p1 << B2] + s;
 }
 
 template <typename KT, bool HV, int IPT>
@@ -827,10 +834,10 @@ int plan_alloc(pgpu_join_plan* p, uint64_t bytes, void** out) {
 // Elements per LDS staging tile for an element of `esz` bytes.
 inline uint64_t tile_elems(uint64_t esz) { return kTileBytes / esz; }
 
-// Rows per histogram/scatter block: at most 512 blocks, tile-aligned.
+// Rows per histogram/scatter block: at most 256 blocks (one per CU), tile-aligned.
 inline uint64_t block_rows(uint64_t n, uint64_t esz) {
     const uint64_t T = tile_elems(esz);
-    uint64_t blk = ceil_div(std::max<uint64_t>(n, 1), 512);
+    uint64_t blk = ceil_div(std::max<uint64_t>(n, 1), 256);
     blk = ceil_div(blk, T) * T;
     return std::max<uint64_t>(blk, T);
 }
@@ -873,7 +880,7 @@ int run_partition(pgpu_join_plan* p, const KT* keys, const KT* vals, uint64_t n,
         hipLaunchKernelGGL(k_binscan, dim3(1), dim3(1024), 0, st, tot, nb, base);
         PGPU_LAUNCH_CHECK();
         const uint64_t nt = (uint64_t)(nblk + 1) * nb1;
-        hipLaunchKernelGGL(k_off1, dim3((unsigned)ceil_div(nt, 256)), dim3(256), 0, st, H, base, nblk, B, B2, off1);
+        hipLaunchKernelGGL(k_off1, dim3((unsigned)ceil_div(nt, 4)), dim3(256), 0, st, H, base, nblk, B, B2, off1);
         PGPU_LAUNCH_CHECK();
     }
     {
