@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "batch_internal.h"
 #include "join_common.h"
@@ -297,13 +298,15 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
     static_assert(PIECE < 65536, "16-bit LDS match counters");
     constexpr KT EMPTY = KeyEmpty<KT>::v;
     using SR = Row<KT, HVS>;
-    constexpr int U = sizeof(SR) >= 16 ? 8 : 16;  // probe rows prefetched per thread
+    constexpr int U = sizeof(SR) >= 16 ? 3 : (W ? 4 : 6);  // probe rows per thread per batch
+    constexpr int RB = (PIECE + T - 1) / T;       // build rows per thread per piece
     __shared__ LSlot<KT> tab[CAP];
     __shared__ uint32_t tc[W ? CAP / 2 : 1];
     __shared__ unsigned long long sp_cnt, sp_sum;
     __shared__ unsigned long long red[3][T / 64];
     __shared__ uint32_t s_item;
     const int tid = threadIdx.x;
+    const unsigned short* tc16 = (const unsigned short*)tc;  // little-endian halves of tc[]
     while (true) {
         if (tid == 0) s_item = atomicAdd(queue, 1u);
         __syncthreads();
@@ -316,18 +319,45 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
         const uint32_t nR = bndR[jb.rbnd + d + 1] - bndR[jb.rbnd + d];
         const uint64_t sb = jb.sdst + bndS[jb.sbnd + d];
         const uint64_t se = jb.sdst + bndS[jb.sbnd + d + 1];
-        const int hs = 64 - (int)jb.bits - L;
+        // LDS slot = bits [64 - bits - L, 64 - bits) of hmix(k): they lie in
+        // the high word of the product (bits + L <= 32), computed without its
+        // low half.
+        const int sh = 32 - (int)jb.bits - L;
+        auto slot_of = [&](KT key) -> uint32_t {
+            const uint64_t kk = (uint64_t)(int64_t)key;
+            const uint32_t lo = (uint32_t)kk, hi = (uint32_t)(kk >> 32);
+            const uint32_t h = __umulhi(lo, (uint32_t)kFib) + lo * (uint32_t)(kFib >> 32) + hi * (uint32_t)kFib;
+            return (h >> sh) & (CAP - 1);
+        };
         unsigned long long acc_c = 0, acc_b = 0, acc_p = 0;
-        SR x[U];
-        auto fetch = [&](uint64_t q) {
+        SR x[U] = {};
+        // rows of the batch at b0 this thread holds: b0 + tid + u*T < se
+        auto rows_in = [&](uint64_t b0) -> uint32_t {
+            const uint64_t r0 = b0 + tid;
+            return r0 < se ? (uint32_t)min<uint64_t>(U, (se - r0 + T - 1) / T) : 0u;
+        };
+        auto fetch = [&](uint64_t b0) {
+            const uint32_t cnt = rows_in(b0);
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint64_t idx = q + (uint64_t)u * T;
-                if (idx < se) x[u] = nt_load(&S[idx]);
+            for (int u = 0; u < U; ++u)
+                if ((uint32_t)u < cnt) x[u] = nt_load(&S[b0 + tid + (uint64_t)u * T]);
+        };
+        Row<KT, true> br[RB];
+        auto fetch_build = [&](uint32_t p0) {
+            const uint32_t pn = min(PIECE, nR - p0);
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                const uint32_t i = tid + b * T;
+                if (i < pn) br[b] = nt_load(&R[rb + p0 + i]);
             }
         };
         const bool work = nR && se > sb;
-        if (work) fetch(sb + tid);
+        // Neither the build rows nor the first probe batch depend on the
+        // table: both are in flight while it is cleared.
+        if (work) {
+            fetch_build(0);
+            fetch(sb);
+        }
         for (uint32_t p0 = 0; work && p0 < nR; p0 += PIECE) {
             for (int i = tid; i < CAP; i += T) {
                 tab[i].key = EMPTY;
@@ -341,14 +371,16 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
             if (tid == 0) sp_cnt = sp_sum = 0;
             __syncthreads();
             const uint32_t pn = min(PIECE, nR - p0);
-            for (uint32_t i = tid; i < pn; i += T) {
-                const Row<KT, true> r = nt_load(&R[rb + p0 + i]);
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                if (tid + b * T >= pn) continue;
+                const Row<KT, true> r = br[b];
                 if (r.k == EMPTY) {
                     atomicAdd(&sp_cnt, 1ull);
                     atomicAdd(&sp_sum, (unsigned long long)(int64_t)r.v);
                     continue;
                 }
-                uint32_t slot = (uint32_t)(hmix(r.k) >> hs) & (CAP - 1);
+                uint32_t slot = slot_of(r.k);
                 while (true) {
                     KT prev;
                     if constexpr (W)
@@ -367,39 +399,71 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
                     slot = (slot + 1) & (CAP - 1);
                 }
             }
+            if (p0 + PIECE < nR) fetch_build(p0 + PIECE);  // next piece, if any
             __syncthreads();
-            if (p0) fetch(sb + tid);
-            auto probe = [&](const SR& e) {
-                unsigned long long c = 0, sbv = 0;
-                if (e.k == EMPTY) {
-                    c = sp_cnt;
-                    sbv = sp_sum;
-                } else {
-                    uint32_t slot = (uint32_t)(hmix(e.k) >> hs) & (CAP - 1);
-                    while (true) {
-                        const LSlot<KT> t = tab[slot];
-                        if (t.key == e.k) {
-                            if constexpr (W)
-                                c = (tc[slot >> 1] >> (16 * (slot & 1))) & 0xFFFFu;
-                            else
-                                c = t.cnt;
-                            sbv = t.sum;
-                            break;
-                        }
-                        if (t.key == EMPTY) break;
-                        slot = (slot + 1) & (CAP - 1);
-                    }
+            if (p0) fetch(sb);  // later pieces re-read the item's probe rows
+            const unsigned long long spc = sp_cnt, sps = sp_sum;
+            // One batch: all U slot reads in flight at once, collision chains
+            // walked only by the lanes that need them, selects instead of
+            // branches; FULL batches (every thread holds U rows) skip the
+            // row-count masks.
+            auto probe_batch = [&](auto full_tag, uint32_t cnt) {
+                constexpr bool FULL = decltype(full_tag)::value;
+                uint32_t slot[U];
+                LSlot<KT> t[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    slot[u] = slot_of(x[u].k);
+                    t[u] = tab[slot[u]];
                 }
-                acc_c += c;
-                acc_b += sbv;
-                if constexpr (HVS) acc_p += c * (unsigned long long)(int64_t)e.v;
-            };
-            for (uint64_t q = sb + tid; q < se;) {
+                bool any = false;
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if (q + (uint64_t)u * T < se) probe(x[u]);
-                q += (uint64_t)U * T;
-                if (q < se) fetch(q);
+                    any |= (FULL || (uint32_t)u < cnt) && t[u].key != x[u].k && t[u].key != EMPTY;
+                if (any) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        if (!FULL && (uint32_t)u >= cnt) continue;
+                        while (t[u].key != x[u].k && t[u].key != EMPTY) {
+                            slot[u] = (slot[u] + 1) & (CAP - 1);
+                            t[u] = tab[slot[u]];
+                        }
+                    }
+                }
+                uint32_t bc = 0;  // fits 32 bits: U rows x PIECE matches
+                unsigned long long bs = 0, bp = 0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool hit = t[u].key == x[u].k;
+                    uint32_t raw;
+                    if constexpr (W)
+                        raw = tc16[slot[u]];
+                    else
+                        raw = t[u].cnt;
+                    uint32_t c = hit ? raw : 0u;
+                    unsigned long long sbv = hit ? t[u].sum : 0ull;
+                    if (x[u].k == EMPTY) {  // the empty marker as a real key: side accumulator
+                        c = (uint32_t)spc;
+                        sbv = sps;
+                    }
+                    if (!FULL && (uint32_t)u >= cnt) {
+                        c = 0;
+                        sbv = 0;
+                    }
+                    bc += c;
+                    bs += sbv;
+                    if constexpr (HVS) bp += (unsigned long long)c * (unsigned long long)(int64_t)x[u].v;
+                }
+                acc_c += bc;
+                acc_b += bs;
+                acc_p += bp;
+            };
+            for (uint64_t b0 = sb; b0 < se; b0 += (uint64_t)U * T) {
+                if (b0 + (uint64_t)U * T <= se)
+                    probe_batch(std::true_type{}, (uint32_t)U);
+                else
+                    probe_batch(std::false_type{}, rows_in(b0));
+                if (b0 + (uint64_t)U * T < se) fetch(b0 + (uint64_t)U * T);
             }
             __syncthreads();
         }
