@@ -251,7 +251,8 @@ typedef struct pgpu_server_cfg {
     uint32_t mode;        /* enum pgpu_serve_mode                                       */
     uint32_t slots;       /* batches in flight at once (streams); 0 = 3                 */
     uint32_t max_batch;   /* queries per launch; 0 = 256                                */
-    uint32_t max_items;   /* 64K-row random-probe work items per launch; 0 = 32768      */
+    uint32_t max_items;   /* random-probe work items per launch, counted as 16K-row    */
+                          /* items at admission (smaller ones when they fit); 0 = 32768 */
     uint32_t part_min_rows; /* probe batches of at least this many rows (and a quarter */
                             /* of their table) take the partitioned path; 0 = 2^20,    */
                             /* 1 = every batch, UINT32_MAX = none                      */

@@ -50,7 +50,9 @@ using pgpu::batch::Seg;
 
 namespace {
 
-constexpr uint64_t kItemRows = 16384;  // probe rows per random-probe work item
+constexpr uint64_t kItemRows = 16384;  // largest random-probe work item (rows)
+constexpr uint64_t kMinItemRows = 1024;  // smallest one: half a pass of the probe loop
+constexpr uint64_t kBatchItems = 512;    // work items a batch aims for (two per CU)
 constexpr int kProbeThreads = 256;
 constexpr int kProbeU = 8;             // independent probes in flight per thread
 constexpr int kVariant = 2;            // LDS join shape of the partitioned path
@@ -303,6 +305,16 @@ uint32_t chunk_rows_for(uint64_t m) {
     return (uint32_t)std::min<uint64_t>(kChunkRows, std::max<uint64_t>(4096, ceil_div(c, 4096) * 4096));
 }
 
+// Rows per random-probe work item of a batch whose random-probe queries hold
+// `rows` rows in all: about kBatchItems items, so a lone small query spreads
+// over many CUs instead of walking its probes on one (a 2^14-row probe of a
+// 2^22-row table takes ~180 us on one workgroup, its random slot reads
+// latency-bound), while saturated batches keep 16K-row items.
+uint64_t item_rows_for(uint64_t rows) {
+    const uint64_t r = ceil_div(ceil_div(std::max<uint64_t>(rows, 1), kBatchItems), kMinItemRows) * kMinItemRows;
+    return std::min(kItemRows, r);
+}
+
 bool partitioned(const pgpu_server* s, const Query* q) {
     const uint64_t pmin = s->cfg.part_min_rows ? s->cfg.part_min_rows : (1u << 20);
     if (pmin == UINT32_MAX || q->m == 0) return false;
@@ -332,7 +344,7 @@ void reap(pgpu_server* s, Slot& sl) {
 // no lock held; returns true if it launched something.
 bool launch_batch(pgpu_server* s, Slot& sl) {
     std::vector<Query*> qs;
-    uint64_t items = 0, rows = 0, hist = 0, bnd = 0, chunks = 0, pitems = 0;
+    uint64_t items = 0, rows = 0, hist = 0, bnd = 0, chunks = 0, pitems = 0, rrows = 0;
     const double t = now_s();
     {
         std::lock_guard<std::mutex> g(s->mu);
@@ -355,6 +367,7 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
                 const uint64_t qi = ceil_div(q->m, kItemRows);
                 if (items + qi > s->cfg.max_items) break;
                 items += qi;
+                rrows += q->m;
             }
             s->pending.pop_front();
             s->reserved += q->reserve;
@@ -367,13 +380,18 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
     const uint32_t bid = (uint32_t)s->batches++;
     uint32_t ni = 0, nseg = 0, nchunk = 0, nitem = 0;
     uint64_t prow = 0, phist = 0, pbnd = 0;
+    // Admission counted kItemRows items; smaller ones are used when they fit.
+    uint64_t irows = item_rows_for(rrows), nsmall = 0;
+    for (Query* q : qs)
+        if (!partitioned(s, q)) nsmall += ceil_div(q->m, irows);
+    if (nsmall > s->cfg.max_items) irows = kItemRows;
     for (uint32_t i = 0; i < qs.size(); ++i) {
         Query* q = qs[i];
         q->batch = bid;
         q->batch_size = (uint32_t)qs.size();
         if (!partitioned(s, q)) {
-            for (uint64_t r = 0; r < q->m; r += kItemRows)
-                sl.h.items[ni++] = QItem{q->probe, r, std::min(q->m, r + kItemRows), i, q->table};
+            for (uint64_t r = 0; r < q->m; r += irows)
+                sl.h.items[ni++] = QItem{q->probe, r, std::min(q->m, r + irows), i, q->table};
             continue;
         }
         const Table& tb = s->tables[q->table];
