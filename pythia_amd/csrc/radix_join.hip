@@ -530,6 +530,12 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
         for (int u = 0; u < U; ++u)
             if ((uint32_t)u < cnt) x[u] = nt_load(&S[rb + tid + u * kJoinThreads]);
     };
+    // Unpredicated fetch of the batch at `rb` into z: rows past the item's
+    // end read its last row (sN > 0) and are masked by the probe.
+    auto fetch_clamped = [&](Elem<KT, HVS>(&z)[U], uint32_t rb) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) z[u] = nt_load(&S[min(rb + tid + u * kJoinThreads, sN - 1)]);
+    };
     constexpr int RB = (kJoinPiece + kJoinThreads - 1) / kJoinThreads;  // build rows per thread per piece
     Elem<KT, true> br[RB];
     auto fetch_build = [&](uint32_t p0) {
@@ -549,6 +555,7 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
     const unsigned short* tc16 = (const unsigned short*)tc;  // little-endian halves of tc[]
 
     for (uint32_t p0 = 0; p0 < nR; p0 += kJoinPiece) {
+        if (p0) fetch_build(p0);  // later pieces (skewed build partitions) fetch their rows here
         for (int i = tid; i < CAP; i += kJoinThreads) {
             tab[i].key = EMPTY;
             tab[i].sum = 0;
@@ -579,7 +586,6 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
                 slot = (slot + 1) & (CAP - 1);
             }
         }
-        if (p0 + kJoinPiece < nR) fetch_build(p0 + kJoinPiece);  // next piece, if any
         __syncthreads();
         if (p0) fetch(0);  // later pieces re-read the item's probe rows
         const unsigned long long spc = sp_cnt, sps = sp_sum;
@@ -587,7 +593,7 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
         // walked only by the lanes that need them, selects instead of
         // branches.  FULL batches (every thread holds U rows) skip the
         // row-count masks.
-        auto probe_batch = [&](auto full_tag, uint32_t cnt) {
+        auto probe_batch = [&](auto full_tag, const Elem<KT, HVS>(&x)[U], uint32_t cnt) {
             constexpr bool FULL = decltype(full_tag)::value;
             uint32_t slot[U];
             LSlot t[U];
@@ -636,12 +642,20 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
             acc_b += bs;
             acc_p += bp;
         };
-        for (uint32_t rb = 0; rb < sN; rb += BATCH) {
+        auto probe = [&](const Elem<KT, HVS>(&z)[U], uint32_t rb) {
             if (rb + BATCH <= sN)
-                probe_batch(std::true_type{}, (uint32_t)U);
+                probe_batch(std::true_type{}, z, (uint32_t)U);
             else
-                probe_batch(std::false_type{}, rows_in(rb));
-            if (rb + BATCH < sN) fetch(rb + BATCH);
+                probe_batch(std::false_type{}, z, rows_in(rb));
+        };
+        // Two batches in flight: the next batch's rows load while this one
+        // probes the LDS table.
+        Elem<KT, HVS> y[U];
+        for (uint32_t rb = 0; rb < sN; rb += 2 * BATCH) {
+            fetch_clamped(y, rb + BATCH);
+            probe(x, rb);
+            fetch_clamped(x, rb + 2 * BATCH);
+            if (rb + BATCH < sN) probe(y, rb + BATCH);
         }
         __syncthreads();
     }
