@@ -65,8 +65,9 @@ uint32_t bits_for(uint64_t n, uint32_t target);
 int launch_hist(hipStream_t st, int key_bytes, const Seg* segs, const Chunk* chunks, uint32_t chunk_base,
                 uint32_t nchunks, uint32_t* H);
 int launch_scan(hipStream_t st, const Seg* segs, uint32_t seg_base, uint32_t nsegs, uint32_t* H, uint32_t* bnd);
-int launch_scatter(hipStream_t st, int key_bytes, bool hv, const Seg* segs, const Chunk* chunks, uint32_t chunk_base,
-                   uint32_t nchunks, const uint32_t* H, void* out);
+// max_bits: the largest partition bits of the launched chunks' segments.
+int launch_scatter(hipStream_t st, int key_bytes, bool hv, uint32_t max_bits, const Seg* segs, const Chunk* chunks,
+                   uint32_t chunk_base, uint32_t nchunks, const uint32_t* H, void* out);
 // Persistent join over n_items work items; `grid` from join_grid().
 int launch_join(hipStream_t st, int key_bytes, bool hvs, int variant, int grid, const Job* jobs,
                 const uint32_t* item_job, uint32_t n_items, const void* R, const void* S, const uint32_t* bndR,

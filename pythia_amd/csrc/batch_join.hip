@@ -154,18 +154,23 @@ __device__ __forceinline__ void load_group(const KT* key, const KT* pay, bool ve
     }
 }
 
-template <typename KT, bool HV>
+// TB bytes of LDS staging per tile, ND partitions at most (the launch's
+// largest 2^bits): fan-outs up to 1024 take 128 KB tiles, twice the rows per
+// partition run of the 64 KB tile that 4096 partitions leave room for, so
+// fewer partial cache lines reach HBM.
+template <typename KT, bool HV, int TB, int ND>
 __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs, const Chunk* __restrict__ chunks,
                                                    uint32_t chunk_base, const uint32_t* __restrict__ H,
                                                    Row<KT, HV>* __restrict__ out) {
     using E = Row<KT, HV>;
     constexpr int T = 1024;
     constexpr int G = 16 / sizeof(KT);            // rows per 16-byte key load
-    constexpr int TR = kScatterTileBytes / sizeof(E);  // rows per tile
+    constexpr int TR = TB / sizeof(E);            // rows per tile
     constexpr int NG = TR / (T * G);              // row groups per thread per tile
     static_assert(NG >= 1 && TR % (T * G) == 0, "tile shape");
+    static_assert(ND >= T && ND <= kND, "partition table");
     __shared__ E stage[TR];
-    __shared__ uint32_t cnt[kND], start[kND], cur[kND];
+    __shared__ uint32_t cnt[ND], start[ND], cur[ND];
     __shared__ uint32_t ws[T / 64 + 1];
     const uint32_t ci = chunk_base + blockIdx.x;
     const Chunk c = chunks[ci];
@@ -200,7 +205,7 @@ __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs,
     }
     __syncthreads();
     const int sh = 64 - (int)s.bits;
-    constexpr int PER = kND / T;
+    constexpr int PER = ND / T;
     for (uint64_t t0 = r0; t0 < r1; t0 += TR) {
         const uint64_t te = min<uint64_t>(r1, t0 + TR);
         const uint32_t tn = (uint32_t)(te - t0);
@@ -298,7 +303,7 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
     static_assert(PIECE < 65536, "16-bit LDS match counters");
     constexpr KT EMPTY = KeyEmpty<KT>::v;
     using SR = Row<KT, HVS>;
-    constexpr int U = sizeof(SR) >= 16 ? 3 : (W ? 4 : 6);  // probe rows per thread per batch
+    constexpr int U = sizeof(SR) >= 16 ? 3 : 4;  // probe rows per thread per batch (two batches in flight)
     constexpr int RB = (PIECE + T - 1) / T;       // build rows per thread per piece
     __shared__ LSlot<KT> tab[CAP];
     __shared__ uint32_t tc[W ? CAP / 2 : 1];
@@ -342,6 +347,12 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
             for (int u = 0; u < U; ++u)
                 if ((uint32_t)u < cnt) x[u] = nt_load(&S[b0 + tid + (uint64_t)u * T]);
         };
+        // Unpredicated fetch of the batch at b0 into z: rows past the item's
+        // end read its last row (se > sb) and are masked by the probe.
+        auto fetch_clamped = [&](SR(&z)[U], uint64_t b0) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) z[u] = nt_load(&S[min(b0 + tid + (uint64_t)u * T, se - 1)]);
+        };
         Row<KT, true> br[RB];
         auto fetch_build = [&](uint32_t p0) {
             const uint32_t pn = min(PIECE, nR - p0);
@@ -359,6 +370,7 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
             fetch(sb);
         }
         for (uint32_t p0 = 0; work && p0 < nR; p0 += PIECE) {
+            if (p0) fetch_build(p0);  // later pieces (skewed build partitions) fetch their rows here
             for (int i = tid; i < CAP; i += T) {
                 tab[i].key = EMPTY;
                 tab[i].sum = 0;
@@ -399,7 +411,6 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
                     slot = (slot + 1) & (CAP - 1);
                 }
             }
-            if (p0 + PIECE < nR) fetch_build(p0 + PIECE);  // next piece, if any
             __syncthreads();
             if (p0) fetch(sb);  // later pieces re-read the item's probe rows
             const unsigned long long spc = sp_cnt, sps = sp_sum;
@@ -407,7 +418,7 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
             // walked only by the lanes that need them, selects instead of
             // branches; FULL batches (every thread holds U rows) skip the
             // row-count masks.
-            auto probe_batch = [&](auto full_tag, uint32_t cnt) {
+            auto probe_batch = [&](auto full_tag, const SR(&x)[U], uint32_t cnt) {
                 constexpr bool FULL = decltype(full_tag)::value;
                 uint32_t slot[U];
                 LSlot<KT> t[U];
@@ -458,12 +469,20 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
                 acc_b += bs;
                 acc_p += bp;
             };
-            for (uint64_t b0 = sb; b0 < se; b0 += (uint64_t)U * T) {
+            auto probe = [&](const SR(&z)[U], uint64_t b0) {
                 if (b0 + (uint64_t)U * T <= se)
-                    probe_batch(std::true_type{}, (uint32_t)U);
+                    probe_batch(std::true_type{}, z, (uint32_t)U);
                 else
-                    probe_batch(std::false_type{}, rows_in(b0));
-                if (b0 + (uint64_t)U * T < se) fetch(b0 + (uint64_t)U * T);
+                    probe_batch(std::false_type{}, z, rows_in(b0));
+            };
+            // Two batches in flight: the next batch's rows load while this
+            // one probes the LDS table.
+            SR y[U];
+            for (uint64_t b0 = sb; b0 < se; b0 += 2ull * U * T) {
+                fetch_clamped(y, b0 + (uint64_t)U * T);
+                probe(x, b0);
+                fetch_clamped(x, b0 + 2ull * U * T);
+                if (b0 + (uint64_t)U * T < se) probe(y, b0 + (uint64_t)U * T);
             }
             __syncthreads();
         }
@@ -569,23 +588,35 @@ int launch_scan(hipStream_t st, const Seg* segs, uint32_t seg_base, uint32_t nse
     return PGPU_OK;
 }
 
-int launch_scatter(hipStream_t st, int kb, bool hv, const Seg* segs, const Chunk* chunks, uint32_t chunk_base,
-                   uint32_t n, const uint32_t* H, void* out) {
+template <typename KT, bool HV>
+void launch_scatter_t(hipStream_t st, uint32_t max_bits, const Seg* segs, const Chunk* chunks, uint32_t chunk_base,
+                      uint32_t n, const uint32_t* H, void* out) {
+    // 4-byte rows already hold 16K rows in the 64 KB tile
+    if constexpr (sizeof(Row<KT, HV>) >= 8) {
+        if (max_bits <= 10) {
+            hipLaunchKernelGGL((k_bscatter<KT, HV, 2 * kScatterTileBytes, 1024>), dim3(n), dim3(1024), 0, st, segs,
+                               chunks, chunk_base, H, (Row<KT, HV>*)out);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_bscatter<KT, HV, kScatterTileBytes, kND>), dim3(n), dim3(1024), 0, st, segs, chunks,
+                       chunk_base, H, (Row<KT, HV>*)out);
+}
+
+int launch_scatter(hipStream_t st, int kb, bool hv, uint32_t max_bits, const Seg* segs, const Chunk* chunks,
+                   uint32_t chunk_base, uint32_t n, const uint32_t* H, void* out) {
     if (!n) return PGPU_OK;
+    if (max_bits > (uint32_t)kMaxBits) return fail(PGPU_EINVAL, "scatter: %u partition bits (at most %d)", max_bits, kMaxBits);
     if (kb == 8) {
         if (hv)
-            hipLaunchKernelGGL((k_bscatter<int64_t, true>), dim3(n), dim3(1024), 0, st, segs, chunks, chunk_base, H,
-                               (Row<int64_t, true>*)out);
+            launch_scatter_t<int64_t, true>(st, max_bits, segs, chunks, chunk_base, n, H, out);
         else
-            hipLaunchKernelGGL((k_bscatter<int64_t, false>), dim3(n), dim3(1024), 0, st, segs, chunks, chunk_base, H,
-                               (Row<int64_t, false>*)out);
+            launch_scatter_t<int64_t, false>(st, max_bits, segs, chunks, chunk_base, n, H, out);
     } else {
         if (hv)
-            hipLaunchKernelGGL((k_bscatter<int32_t, true>), dim3(n), dim3(1024), 0, st, segs, chunks, chunk_base, H,
-                               (Row<int32_t, true>*)out);
+            launch_scatter_t<int32_t, true>(st, max_bits, segs, chunks, chunk_base, n, H, out);
         else
-            hipLaunchKernelGGL((k_bscatter<int32_t, false>), dim3(n), dim3(1024), 0, st, segs, chunks, chunk_base, H,
-                               (Row<int32_t, false>*)out);
+            launch_scatter_t<int32_t, false>(st, max_bits, segs, chunks, chunk_base, n, H, out);
     }
     PGPU_LAUNCH_CHECK();
     return PGPU_OK;

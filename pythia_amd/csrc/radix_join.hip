@@ -338,12 +338,14 @@ __global__ __launch_bounds__(kScatterThreads, IPT <= 16 ? 4 : 1) void k_scatter2
     const uint64_t e = off1[(uint64_t)b1 * nb1 + p1];
     if (s >= e) return;
     const uint32_t mask2 = nb2 - 1;
-    for (uint32_t d = tid; d < nb2; d += kScatterThreads) {
-        const uint64_t pp = ((uint64_t)p1 << B2) + d;
-        run[d] = base[pp] + Hs[(uint64_t)b0 * nb + pp];
-        hist[d] = 0;
+    static_assert(kScatterThreads >= 128, "one thread per pass-2 digit");
+    // The run starts and the unit's first tile are read together: the tile
+    // does not wait for the offsets.
+    uint64_t rbase = 0;
+    if ((uint32_t)tid < nb2) {
+        const uint64_t pp = ((uint64_t)p1 << B2) + tid;
+        rbase = base[pp] + Hs[(uint64_t)b0 * nb + pp];
     }
-    __syncthreads();
     E el[IPT];
     auto load = [&](uint64_t t0, uint32_t cnt) {
 #pragma unroll
@@ -354,6 +356,11 @@ __global__ __launch_bounds__(kScatterThreads, IPT <= 16 ? 4 : 1) void k_scatter2
     };
     uint32_t cnt = (uint32_t)min<uint64_t>(T, e - s);
     load(s, cnt);
+    if ((uint32_t)tid < nb2) {
+        run[tid] = rbase;
+        hist[tid] = 0;
+    }
+    __syncthreads();
     for (uint64_t t0 = s; t0 < e;) {
         uint32_t dg[IPT], rk[IPT];
 #pragma unroll
@@ -907,9 +914,9 @@ int run_partition(pgpu_join_plan* p, const KT* keys, const KT* vals, uint64_t n,
     {
         snprintf(name, sizeof name, "scatter2_%s", tag);
         Timed tm(p->timer, name);
-        // group pass-1 blocks so a pass-2 unit averages about two tiles
+        // group pass-1 blocks so a pass-2 unit averages about one tile
         const uint64_t T = tile_elems(sizeof(E));
-        const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nblk, 2 * T * nb1 / blk));
+        const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nblk, T * nb1 / blk));
         const uint32_t ngrp = (nblk + G - 1) / G;
         hipLaunchKernelGGL((k_scatter2<KT, HV, IPT>), dim3(ngrp * nb1), dim3(kScatterThreads), 0, st, P1, nblk, G, B,
                            B2, off1, H, base, P2);

@@ -378,7 +378,7 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
     if (qs.empty()) return false;
     ctx_activate(s->ctx);
     const uint32_t bid = (uint32_t)s->batches++;
-    uint32_t ni = 0, nseg = 0, nchunk = 0, nitem = 0;
+    uint32_t ni = 0, nseg = 0, nchunk = 0, nitem = 0, maxbits = 0;
     uint64_t prow = 0, phist = 0, pbnd = 0;
     // Admission counted kItemRows items; smaller ones are used when they fit.
     uint64_t irows = item_rows_for(rrows), nsmall = 0;
@@ -406,6 +406,7 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
         sg.chunk_rows = chunk_rows_for(q->m);
         sg.nchunks = (uint32_t)ceil_div(q->m, sg.chunk_rows);
         sg.bits = tb.bits;
+        maxbits = std::max(maxbits, tb.bits);
         for (uint32_t c = 0; c < sg.nchunks; ++c)
             sl.h.chunks[nchunk++] = Chunk{nseg, 0, (uint64_t)c * sg.chunk_rows};
         Job jb{};
@@ -430,7 +431,8 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
     if (rc == PGPU_OK && nseg) {
         rc = batch::launch_hist(sl.st, 8, sl.d.segs, sl.d.chunks, 0, nchunk, sl.H);
         if (rc == PGPU_OK) rc = batch::launch_scan(sl.st, sl.d.segs, 0, nseg, sl.H, sl.bnd);
-        if (rc == PGPU_OK) rc = batch::launch_scatter(sl.st, 8, false, sl.d.segs, sl.d.chunks, 0, nchunk, sl.H, sl.SP);
+        if (rc == PGPU_OK)
+            rc = batch::launch_scatter(sl.st, 8, false, maxbits, sl.d.segs, sl.d.chunks, 0, nchunk, sl.H, sl.SP);
         if (rc == PGPU_OK)
             rc = batch::launch_join(sl.st, 8, false, kVariant, std::min<int>(s->join_grid, (int)nitem), sl.d.jobs,
                                     sl.d.item_job, nitem, s->R, sl.SP, s->RB, sl.bnd, queue, sl.d_res);
@@ -579,7 +581,7 @@ int partition_table(pgpu_server* s, Table& t) {
     if (rc == PGPU_OK) rc = batch::launch_hist(st, 8, (Seg*)d_seg, (Chunk*)d_chunks, 0, nc, (uint32_t*)d_H);
     if (rc == PGPU_OK) rc = batch::launch_scan(st, (Seg*)d_seg, 0, 1, (uint32_t*)d_H, s->RB);
     if (rc == PGPU_OK)
-        rc = batch::launch_scatter(st, 8, true, (Seg*)d_seg, (Chunk*)d_chunks, 0, nc, (uint32_t*)d_H, s->R);
+        rc = batch::launch_scatter(st, 8, true, t.bits, (Seg*)d_seg, (Chunk*)d_chunks, 0, nc, (uint32_t*)d_H, s->R);
     if (rc == PGPU_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(PGPU_EDEVICE, "table partition failed");
     if (d_seg) ctx_free(s->ctx, d_seg);
     if (d_chunks) ctx_free(s->ctx, d_chunks);
