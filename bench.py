@@ -326,7 +326,7 @@ def run_c4(args):
                    "joins": len(joins), "build_rows": nb, "probe_rows": ns,
                    "variant": info.cfg.variant, "part_rows": info.cfg.part_rows, "items": info.items,
                    "persistent_workgroups": info.grid},
-        "roofline": roofline(per_kernel, kb, args.traffic),
+        "roofline": roofline(per_kernel, kb, args.traffic or os.path.join(ROOT, "profiles", "traffic_c4.json")),
         "job_roofline": {"compulsory_bytes": compulsory,
                          "achieved_GBps": compulsory / (ms_step / 1e3) / 1e9,
                          "frac": compulsory / (ms_step / 1e3) / HBM_PEAK},
@@ -335,6 +335,9 @@ def run_c4(args):
         "autotune": tune,
         "kernels": per_kernel,
     }
+    rd, cp = ctx.membw(1 << 30, 10)
+    line["hbm_calibration"] = {"read_GBps": rd, "copy_GBps": cp,
+                               "note": "streaming uint4 read / copy kernels on this device, same run"}
     if not args.no_cpu_baseline:
         plan.close()
         for b in bufs:
