@@ -44,6 +44,17 @@ def cpu_cores():
     return int(os.environ.get("PYTHIA_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def kernel_bytes_c2(nR: int, nS: int, esR: int = 16, esS: int = 8, kb: int = 8) -> dict:
     """Algorithmic HBM bytes of one launch of each kernel of the radix join."""
     return {
@@ -373,6 +384,11 @@ def run_c1(args):
     for _ in range(args.steps):
         dres = plan.count_sum(cols["C1_RKEY"], cols["C1_RPAY"], nR, cols["C1_SFK"], cols["C1_SPAY"], nS)
     dev_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    plan.timing(True)
+    for _ in range(args.steps):
+        plan.count_sum(cols["C1_RKEY"], cols["C1_RPAY"], nR, cols["C1_SFK"], cols["C1_SPAY"], nS)
+    dev_kernels = per_kernel_times(plan.times())
+    plan.timing(False)
     plan.close()
     rdt = np.dtype({"names": ["k", "v"], "formats": [np.int32, np.int32], "offsets": [0, 4], "itemsize": 8})
     R = np.empty(nR, rdt)
@@ -446,7 +462,14 @@ def run_c1(args):
                                     "note": "same plan without gpu.resident: every page crosses the host link",
                                     "same_result": bool(page_count == count and page_sum32 == sum32)},
         "device_columns": {"ms_per_join": dev_ms, "tuples_per_s": (nR + nS) / (dev_ms / 1e3),
-                           "matches": dres.matches, "sum_build": dres.sum_build},
+                           "matches": dres.matches, "sum_build": dres.sum_build, "kernels": dev_kernels},
+        # the device-column join's kernels (the operator path's time is host-side page traffic)
+        "roofline": dict(roofline(dev_kernels, kernel_bytes_c2(nR, nS, esR=8, esS=8, kb=4), args.traffic),
+                         of="device_columns join"),
+        "job_roofline": {"compulsory_bytes": nR * 8 + nS * 8 + 16,
+                         "achieved_GBps": (nR * 8 + nS * 8 + 16) / (dev_ms / 1e3) / 1e9,
+                         "frac": (nR * 8 + nS * 8 + 16) / (dev_ms / 1e3) / HBM_PEAK,
+                         "of": "device_columns join; compulsory bytes: both relations' columns once"},
         "result": {"count": count, "sum_int32": sum32,
                    "matches_device_path": bool(count == dres.matches and
                                                sum32 == int(np.int64(dres.sum_build % 2**32).astype(np.int32)))},
@@ -681,6 +704,8 @@ def main():
               "(the path shards by radix partition; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     line = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
+    if isinstance(line.get("cpu_baseline"), dict):
+        line["cpu_baseline"]["cpu_model"] = cpu_model()
     print(json.dumps(line), flush=True)
 
 
