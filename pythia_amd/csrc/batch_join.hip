@@ -170,7 +170,9 @@ __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs,
     static_assert(NG >= 1 && TR % (T * G) == 0, "tile shape");
     static_assert(ND >= T && ND <= kND, "partition table");
     __shared__ E stage[TR];
-    __shared__ uint32_t cnt[ND], start[ND], cur[ND];
+    // cur: next output row of each partition (segment-relative); delta: the
+    // output row of stage row j of partition d is delta[d] + j (mod 2^32)
+    __shared__ uint32_t cnt[ND], start[ND], cur[ND], delta[ND];
     __shared__ uint32_t ws[T / 64 + 1];
     const uint32_t ci = chunk_base + blockIdx.x;
     const Chunk c = chunks[ci];
@@ -229,10 +231,17 @@ __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs,
         }
         uint32_t total;
         uint32_t ex = block_excl_scan<T>(sum, ws, &total);
+        // each thread owns bins [tid * PER, tid * PER + PER): their counters
+        // are consumed here and reset for the next tile
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const uint32_t idx = threadIdx.x * PER + u;
-            if (idx < nd) start[idx] = ex;
+            if (idx < nd) {
+                start[idx] = ex;
+                delta[idx] = cur[idx] - ex;
+                cur[idx] += loc[u];
+                cnt[idx] = 0;
+            }
             ex += loc[u];
         }
         __syncthreads();
@@ -250,17 +259,29 @@ __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs,
             }
         }
         __syncthreads();
-        for (uint32_t pos = threadIdx.x; pos < tn; pos += T) {
-            const E e = stage[pos];
-            const uint32_t d = (uint32_t)(hmix(e.k) >> sh);
-            dst[cur[d] + (pos - start[d])] = e;  // runs: keep L2 write-combining
+        // runs of each partition at its exact offsets (plain stores: the L2
+        // merges the partial lines at run ends).  The next tile rewrites
+        // stage and delta only after its first barrier, which every thread
+        // reaches after these stores.
+        if (tn == (uint32_t)TR) {
+            constexpr int SB = 4;  // rows whose LDS reads are issued together
+#pragma unroll
+            for (int b0 = 0; b0 < TR / T; b0 += SB) {
+                E e[SB];
+                uint32_t dl[SB];
+#pragma unroll
+                for (int u = 0; u < SB; ++u) e[u] = stage[threadIdx.x + (b0 + u) * T];
+#pragma unroll
+                for (int u = 0; u < SB; ++u) dl[u] = delta[(uint32_t)(hmix(e[u].k) >> sh)];
+#pragma unroll
+                for (int u = 0; u < SB; ++u) dst[dl[u] + (uint32_t)(threadIdx.x + (b0 + u) * T)] = e[u];
+            }
+        } else {
+            for (uint32_t pos = threadIdx.x; pos < tn; pos += T) {
+                const E e = stage[pos];
+                dst[delta[(uint32_t)(hmix(e.k) >> sh)] + pos] = e;
+            }
         }
-        __syncthreads();
-        for (uint32_t d = threadIdx.x; d < nd; d += T) {
-            cur[d] += cnt[d];
-            cnt[d] = 0;
-        }
-        __syncthreads();
     }
 }
 
