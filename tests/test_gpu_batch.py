@@ -178,6 +178,22 @@ def test_batch_rejects_bad_arguments(ctx):
     b.free()
 
 
+@pytest.mark.parametrize("variant,part_rows", [(1, 1024), (2, 2048)])
+def test_batch_wide_fanout_vs_closed_form(ctx, variant, part_rows):
+    """A join large enough for more than 2^10 partitions (the 12-bit cap at
+    1024 rows per partition, 11 bits at 2048) sends the whole launch through
+    the 64 KB-tile scatter; small joins ride along in the same launch."""
+    sizes = np.array([(1 << 22) + 12345, 3000, 1 << 21, 700], dtype=np.uint64)
+    joins, bufs = _c4_batch(ctx, 100, sizes)
+    plan = pa.BatchPlan(ctx, joins, key_bytes=4, probe_payload=True, variant=variant, part_rows=part_rows)
+    res = plan.run()
+    ref = O.c4_closed_many(100, sizes, 8 * sizes, threads=THREADS)
+    _check(res, ref)
+    assert max(r.radix_bits for r in res) >= 11
+    plan.close()
+    _free(bufs)
+
+
 def test_batch_full_c4_shape_vs_closed_form(ctx):
     """The full batch: 1024 joins, build sizes round(2^U(10, 22)), probe 8x,
     every join's count and sums against its closed form."""
