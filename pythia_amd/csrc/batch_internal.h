@@ -42,7 +42,9 @@ struct Job {
     uint64_t rbnd, sbnd;  // first bound word of each side
     uint32_t item0;
     uint32_t bits;
-    uint64_t pad;
+    uint32_t split;       // work items per partition (0 or 1: one); item item0 + d * split + k
+                          // probes the k-th of `split` equal slices of partition d's probe rows
+    uint32_t pad;
 };
 
 // Host-side builder of a side's segment and chunk tables.
@@ -64,7 +66,9 @@ uint32_t bits_for(uint64_t n, uint32_t target);
 
 int launch_hist(hipStream_t st, int key_bytes, const Seg* segs, const Chunk* chunks, uint32_t chunk_base,
                 uint32_t nchunks, uint32_t* H);
-int launch_scan(hipStream_t st, const Seg* segs, uint32_t seg_base, uint32_t nsegs, uint32_t* H, uint32_t* bnd);
+// max_bits: the largest partition bits of the launched segments.
+int launch_scan(hipStream_t st, uint32_t max_bits, const Seg* segs, uint32_t seg_base, uint32_t nsegs, uint32_t* H,
+                uint32_t* bnd);
 // max_bits: the largest partition bits of the launched chunks' segments.
 int launch_scatter(hipStream_t st, int key_bytes, bool hv, uint32_t max_bits, const Seg* segs, const Chunk* chunks,
                    uint32_t chunk_base, uint32_t nchunks, const uint32_t* H, void* out);

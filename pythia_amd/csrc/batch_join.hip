@@ -81,26 +81,49 @@ __global__ __launch_bounds__(512) void k_bhist(const Seg* __restrict__ segs, con
     for (uint32_t i = threadIdx.x; i < nd; i += 512) out[i] = h[i];
 }
 
-__global__ __launch_bounds__(1024) void k_bscan(const Seg* __restrict__ segs, uint32_t seg_base,
-                                                uint32_t* __restrict__ H, uint32_t* __restrict__ bnd) {
-    __shared__ uint32_t tot[kND];
+// Offsets of every segment in three launches, spread over the device (one
+// large segment, e.g. a single served query of 2^12 partitions x 256 chunks,
+// is 4 MB of histogram):
+//   k_bcol   block (g, s): column totals of bins [64 g, 64 g + 64) of
+//            segment s over its chunks (4 waves, a quarter of the chunks
+//            each), into bnd[d]
+//   k_bbins  block s: exclusive scan of the totals in place; bnd[nd] = n
+//   k_bapply block (g, s): h[c][d] = bnd[d] + sum of h[c'][d] for c' < c
+constexpr int kColBins = 64;
+
+__device__ __forceinline__ uint32_t col_sum(const uint32_t* h, uint32_t nd, uint32_t d, uint32_t c0, uint32_t c1) {
+    uint32_t t = 0;
+#pragma unroll 8
+    for (uint32_t c = c0; c < c1; ++c) t += h[(uint64_t)c * nd + d];
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_bcol(const Seg* __restrict__ segs, uint32_t seg_base,
+                                              const uint32_t* __restrict__ H, uint32_t* __restrict__ bnd) {
+    __shared__ uint32_t part[4][kColBins];
+    const Seg s = segs[seg_base + blockIdx.y];
+    const uint32_t nd = 1u << s.bits, nc = s.nchunks;
+    if (blockIdx.x * kColBins >= nd) return;  // whole block: fewer bins than groups
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t d = blockIdx.x * kColBins + lane;
+    const uint32_t c0 = (uint32_t)((uint64_t)nc * w / 4), c1 = (uint32_t)((uint64_t)nc * (w + 1) / 4);
+    part[w][lane] = d < nd ? col_sum(H + s.hist, nd, d, c0, c1) : 0u;
+    __syncthreads();
+    if (w == 0 && d < nd) bnd[s.bnd + d] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+}
+
+__global__ __launch_bounds__(1024) void k_bbins(const Seg* __restrict__ segs, uint32_t seg_base,
+                                                uint32_t* __restrict__ bnd) {
     __shared__ uint32_t ws[1024 / 64 + 1];
     const Seg s = segs[seg_base + blockIdx.x];
-    const uint32_t nd = 1u << s.bits, nc = s.nchunks;
-    uint32_t* h = H + s.hist;
-    for (uint32_t d = threadIdx.x; d < nd; d += 1024) {
-        uint32_t t = 0;
-#pragma unroll 8
-        for (uint32_t c = 0; c < nc; ++c) t += h[(uint64_t)c * nd + d];
-        tot[d] = t;
-    }
-    __syncthreads();
+    const uint32_t nd = 1u << s.bits;
+    uint32_t* b = bnd + s.bnd;
     constexpr int PER = kND / 1024;
     uint32_t loc[PER], sum = 0;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const uint32_t idx = threadIdx.x * PER + u;
-        loc[u] = idx < nd ? tot[idx] : 0u;
+        loc[u] = idx < nd ? b[idx] : 0u;
         sum += loc[u];
     }
     uint32_t total;
@@ -108,22 +131,32 @@ __global__ __launch_bounds__(1024) void k_bscan(const Seg* __restrict__ segs, ui
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const uint32_t idx = threadIdx.x * PER + u;
-        if (idx < nd) tot[idx] = ex;
+        if (idx < nd) b[idx] = ex;
         ex += loc[u];
     }
-    __syncthreads();
-    uint32_t* b = bnd + s.bnd;
-    for (uint32_t d = threadIdx.x; d < nd; d += 1024) {
-        uint32_t run = tot[d];
-        b[d] = run;
-#pragma unroll 8
-        for (uint32_t c = 0; c < nc; ++c) {
-            const uint32_t v = h[(uint64_t)c * nd + d];
-            h[(uint64_t)c * nd + d] = run;
-            run += v;
-        }
-    }
     if (threadIdx.x == 0) b[nd] = (uint32_t)s.n;
+}
+
+__global__ __launch_bounds__(256) void k_bapply(const Seg* __restrict__ segs, uint32_t seg_base,
+                                                uint32_t* __restrict__ H, const uint32_t* __restrict__ bnd) {
+    __shared__ uint32_t part[4][kColBins];
+    const Seg s = segs[seg_base + blockIdx.y];
+    const uint32_t nd = 1u << s.bits, nc = s.nchunks;
+    if (blockIdx.x * kColBins >= nd) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t d = blockIdx.x * kColBins + lane;
+    const uint32_t c0 = (uint32_t)((uint64_t)nc * w / 4), c1 = (uint32_t)((uint64_t)nc * (w + 1) / 4);
+    uint32_t* h = H + s.hist;
+    part[w][lane] = d < nd ? col_sum(h, nd, d, c0, c1) : 0u;
+    __syncthreads();
+    if (d >= nd) return;
+    uint32_t run = bnd[s.bnd + d];
+    for (int q = 0; q < w; ++q) run += part[q][lane];
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t v = h[(uint64_t)c * nd + d];
+        h[(uint64_t)c * nd + d] = run;
+        run += v;
+    }
 }
 
 // Loads rows [g*G, g*G+G) of a chunk-relative group into k/v (bounds per row).
@@ -340,11 +373,15 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
         if (item >= n_items) break;
         const uint32_t j = item_job[item];
         const Job jb = jobs[j];
-        const uint32_t d = item - jb.item0;
+        const uint32_t K = jb.split > 1 ? jb.split : 1u;
+        const uint32_t d = (item - jb.item0) / K, ks = (item - jb.item0) % K;
         const uint64_t rb = jb.rdst + bndR[jb.rbnd + d];
         const uint32_t nR = bndR[jb.rbnd + d + 1] - bndR[jb.rbnd + d];
-        const uint64_t sb = jb.sdst + bndS[jb.sbnd + d];
-        const uint64_t se = jb.sdst + bndS[jb.sbnd + d + 1];
+        // slice ks of K of the partition's probe rows (every slice builds the
+        // partition's table: large probes of small tables spread over the CUs)
+        const uint64_t s0 = bndS[jb.sbnd + d], sl = bndS[jb.sbnd + d + 1] - s0;
+        const uint64_t sb = jb.sdst + s0 + sl * ks / K;
+        const uint64_t se = jb.sdst + s0 + sl * (ks + 1) / K;
         // LDS slot = bits [64 - bits - L, 64 - bits) of hmix(k): they lie in
         // the high word of the product (bits + L <= 32), computed without its
         // low half.
@@ -602,9 +639,16 @@ int launch_hist(hipStream_t st, int kb, const Seg* segs, const Chunk* chunks, ui
     return PGPU_OK;
 }
 
-int launch_scan(hipStream_t st, const Seg* segs, uint32_t seg_base, uint32_t nsegs, uint32_t* H, uint32_t* bnd) {
+int launch_scan(hipStream_t st, uint32_t max_bits, const Seg* segs, uint32_t seg_base, uint32_t nsegs, uint32_t* H,
+                uint32_t* bnd) {
     if (!nsegs) return PGPU_OK;
-    hipLaunchKernelGGL(k_bscan, dim3(nsegs), dim3(1024), 0, st, segs, seg_base, H, bnd);
+    if (max_bits > (uint32_t)kMaxBits) return fail(PGPU_EINVAL, "scan: %u partition bits (at most %d)", max_bits, kMaxBits);
+    const uint32_t groups = (uint32_t)ceil_div(1u << max_bits, kColBins);
+    hipLaunchKernelGGL(k_bcol, dim3(groups, nsegs), dim3(256), 0, st, segs, seg_base, H, bnd);
+    PGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bbins, dim3(nsegs), dim3(1024), 0, st, segs, seg_base, bnd);
+    PGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bapply, dim3(groups, nsegs), dim3(256), 0, st, segs, seg_base, H, bnd);
     PGPU_LAUNCH_CHECK();
     return PGPU_OK;
 }

@@ -64,8 +64,8 @@ int enqueue(pgpu_batch_plan* p, bool timed) {
     };
     PGPU_TRY(timed_run("batch_zero", [&] { return launch_zero(st, p->res, words); }));
     PGPU_TRY(timed_run("batch_hist", [&] { return launch_hist(st, kb, p->segs, p->chunks, 0, nbc + nsc, p->H); }));
-    PGPU_TRY(timed_run("batch_scan", [&] { return launch_scan(st, p->segs, 0, 2 * p->n, p->H, p->bnd); }));
     const uint32_t mb = p->bits.empty() ? 0u : *std::max_element(p->bits.begin(), p->bits.end());
+    PGPU_TRY(timed_run("batch_scan", [&] { return launch_scan(st, mb, p->segs, 0, 2 * p->n, p->H, p->bnd); }));
     PGPU_TRY(timed_run("batch_scatter_build",
                        [&] { return launch_scatter(st, kb, true, mb, p->segs, p->chunks, 0, nbc, p->H, p->RP); }));
     PGPU_TRY(timed_run("batch_scatter_probe",

@@ -58,6 +58,9 @@ constexpr int kProbeU = 8;             // independent probes in flight per threa
 constexpr int kVariant = 2;            // LDS join shape of the partitioned path
 constexpr uint32_t kPartTarget = 3584; // build rows per resident partition (one LDS piece
                                        // holds 4096, so a partition rarely needs two)
+constexpr uint64_t kSliceRows = 32768; // probe rows per join work item, about (partitions
+                                       // with more are split into slices)
+constexpr uint32_t kMaxSplit = 64;
 
 struct __align__(16) SSlot {
     long long key;
@@ -315,6 +318,13 @@ uint64_t item_rows_for(uint64_t rows) {
     return std::min(kItemRows, r);
 }
 
+// Work items per partition of a partitioned query: probes much larger than
+// their table (few partitions) are cut into slices of about kSliceRows rows.
+uint32_t split_for(uint64_t m, uint32_t bits) {
+    const uint64_t per = ceil_div(std::max<uint64_t>(m, 1), 1ull << bits);
+    return (uint32_t)std::min<uint64_t>(kMaxSplit, std::max<uint64_t>(1, ceil_div(per, kSliceRows)));
+}
+
 bool partitioned(const pgpu_server* s, const Query* q) {
     const uint64_t pmin = s->cfg.part_min_rows ? s->cfg.part_min_rows : (1u << 20);
     if (pmin == UINT32_MAX || q->m == 0) return false;
@@ -354,15 +364,16 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
             if (partitioned(s, q)) {
                 const uint32_t bits = s->tables[q->table].bits;
                 const uint64_t nc = ceil_div(q->m, chunk_rows_for(q->m));
+                const uint64_t ni = (1ull << bits) * split_for(q->m, bits);
                 if (rows + q->m > s->caps.rows || hist + (nc << bits) > s->caps.hist ||
                     bnd + (1ull << bits) + 1 > s->caps.bnd || chunks + nc > s->caps.chunks ||
-                    pitems + (1ull << bits) > s->caps.pitems)
+                    pitems + ni > s->caps.pitems)
                     break;
                 rows += q->m;
                 hist += nc << bits;
                 bnd += (1ull << bits) + 1;
                 chunks += nc;
-                pitems += 1ull << bits;
+                pitems += ni;
             } else {
                 const uint64_t qi = ceil_div(q->m, kItemRows);
                 if (items + qi > s->cfg.max_items) break;
@@ -416,7 +427,8 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
         jb.sbnd = pbnd;
         jb.bits = tb.bits;
         jb.item0 = nitem;
-        for (uint32_t d = 0; d < (1u << tb.bits); ++d) sl.h.item_job[nitem++] = i;
+        jb.split = split_for(q->m, tb.bits);
+        for (uint64_t d = 0; d < ((uint64_t)jb.split << tb.bits); ++d) sl.h.item_job[nitem++] = i;
         sl.h.jobs[i] = jb;
         sl.h.segs[nseg++] = sg;
         prow += q->m;
@@ -430,7 +442,7 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
     if (rc == PGPU_OK && hipGetLastError() != hipSuccess) rc = fail(PGPU_EDEVICE, "probe launch failed");
     if (rc == PGPU_OK && nseg) {
         rc = batch::launch_hist(sl.st, 8, sl.d.segs, sl.d.chunks, 0, nchunk, sl.H);
-        if (rc == PGPU_OK) rc = batch::launch_scan(sl.st, sl.d.segs, 0, nseg, sl.H, sl.bnd);
+        if (rc == PGPU_OK) rc = batch::launch_scan(sl.st, maxbits, sl.d.segs, 0, nseg, sl.H, sl.bnd);
         if (rc == PGPU_OK)
             rc = batch::launch_scatter(sl.st, 8, false, maxbits, sl.d.segs, sl.d.chunks, 0, nchunk, sl.H, sl.SP);
         if (rc == PGPU_OK)
@@ -579,7 +591,7 @@ int partition_table(pgpu_server* s, Table& t) {
         rc = fail(PGPU_EDEVICE, "table partition upload failed");
     const uint32_t nc = (uint32_t)lay.chunks.size();
     if (rc == PGPU_OK) rc = batch::launch_hist(st, 8, (Seg*)d_seg, (Chunk*)d_chunks, 0, nc, (uint32_t*)d_H);
-    if (rc == PGPU_OK) rc = batch::launch_scan(st, (Seg*)d_seg, 0, 1, (uint32_t*)d_H, s->RB);
+    if (rc == PGPU_OK) rc = batch::launch_scan(st, t.bits, (Seg*)d_seg, 0, 1, (uint32_t*)d_H, s->RB);
     if (rc == PGPU_OK)
         rc = batch::launch_scatter(st, 8, true, t.bits, (Seg*)d_seg, (Chunk*)d_chunks, 0, nc, (uint32_t*)d_H, s->R);
     if (rc == PGPU_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(PGPU_EDEVICE, "table partition failed");
