@@ -34,16 +34,42 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z;
 }
 
-__device__ __forceinline__ int64_t load_key(const char* p, uint32_t bytes) {
-    if (bytes == 4) {
-        int32_t v;
-        __builtin_memcpy(&v, p, 4);
-        return v;
+// Packed rows (no padding: schema.cpp:82-117) align a column only when every
+// width before it is a multiple of 4 (no char(n) column).  Kernels take an
+// `a4` flag, true when the layout and the base pointers are 4-byte aligned,
+// and then use word accesses; otherwise bytes.
+__device__ __forceinline__ uint32_t ld_u32(const char* p, bool a4) {
+    if (a4) return *(const uint32_t*)p;
+    uint32_t v;
+    __builtin_memcpy(&v, p, 4);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t ld_u64(const char* p, bool a4) {
+    if (a4) {
+        const uint32_t* q = (const uint32_t*)p;  // 8-byte columns may sit at 4 mod 8
+        return (uint64_t)q[0] | ((uint64_t)q[1] << 32);
     }
-    int64_t v;
+    uint64_t v;
     __builtin_memcpy(&v, p, 8);
     return v;
 }
+
+// bytes % 4 == 0 when a4
+__device__ __forceinline__ void copy_cols(char* dst, const char* src, uint32_t bytes, bool a4) {
+    if (a4) {
+        for (uint32_t w = 0; w < bytes / 4; ++w) ((uint32_t*)dst)[w] = ((const uint32_t*)src)[w];
+        return;
+    }
+    for (uint32_t b = 0; b < bytes; ++b) dst[b] = src[b];
+}
+
+__device__ __forceinline__ int64_t load_key(const char* p, uint32_t bytes, bool a4) {
+    if (bytes == 4) return (int32_t)ld_u32(p, a4);
+    return (int64_t)ld_u64(p, a4);
+}
+
+__host__ __device__ inline bool aligned4(uint64_t v) { return (v & 3) == 0; }
 
 struct JSlot {
     int64_t key;
@@ -55,6 +81,7 @@ struct JoinDev {
     pgpu_hj_desc d;
     uint32_t out_row_bytes;
     uint32_t dst_off[PGPU_MAX_PROJ];
+    uint32_t layout_a4;  // every row width, offset and projected width a multiple of 4
 };
 
 __global__ __launch_bounds__(256) void k_jclear(JSlot* t, uint64_t cap) {
@@ -70,8 +97,9 @@ __global__ __launch_bounds__(256) void k_jbuild(const char* __restrict__ rows, u
                                                 int logcap) {
     const uint64_t mask = (1ull << logcap) - 1;
     const uint64_t stride = (uint64_t)gridDim.x * 256;
+    const bool a4 = aligned4((uintptr_t)rows | row_bytes | key_off);
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-        const int64_t k = load_key(rows + i * row_bytes + key_off, key_bytes);
+        const int64_t k = load_key(rows + i * row_bytes + key_off, key_bytes, a4);
         uint64_t s = mix64((uint64_t)k) & mask;
         while (atomicCAS(&t[s].idx, kEmptyIdx, (uint32_t)i) != kEmptyIdx) s = (s + 1) & mask;
         t[s].key = k;
@@ -103,13 +131,14 @@ __global__ __launch_bounds__(256) void k_jprobe(JoinDev jd, const char* __restri
                                                 uint64_t cap_rows, unsigned long long* __restrict__ cursor) {
     const pgpu_hj_desc& d = jd.d;
     const uint64_t mask = (1ull << logcap) - 1;
+    const bool a4 = jd.layout_a4 && aligned4((uintptr_t)build | (uintptr_t)probe | (uintptr_t)out);
     __shared__ unsigned long long s_base;
     // grid-stride over whole blocks so every thread reaches the scans
     for (uint64_t b0 = (uint64_t)blockIdx.x * 256; b0 < n; b0 += (uint64_t)gridDim.x * 256) {
         const uint64_t i = b0 + threadIdx.x;
         const bool live = i < n;
         const char* prow = probe + (live ? i : 0) * d.probe_row_bytes;
-        const int64_t k = live ? load_key(prow + d.probe_key_offset, d.probe_key_bytes) : 0;
+        const int64_t k = live ? load_key(prow + d.probe_key_offset, d.probe_key_bytes, a4) : 0;
         uint32_t cnt = 0;
         const uint64_t h0 = mix64((uint64_t)k) & mask;
         if (live) {
@@ -128,8 +157,7 @@ __global__ __launch_bounds__(256) void k_jprobe(JoinDev jd, const char* __restri
                     const char* brow = build + (uint64_t)t[s].idx * d.build_row_bytes;
                     for (uint32_t c = 0; c < d.nproj; ++c) {
                         const char* src = (d.proj[c].side ? prow : brow) + d.proj[c].src_offset;
-                        char* dst = o + jd.dst_off[c];
-                        for (uint32_t b = 0; b < d.proj[c].bytes; ++b) dst[b] = src[b];
+                        copy_cols(o + jd.dst_off[c], src, d.proj[c].bytes, a4);
                     }
                 }
                 ++pos;
@@ -148,12 +176,20 @@ struct AggDev {
     uint32_t key_words;  // ceil(total key bytes / 8)
     uint32_t key_total;
     uint32_t slot_words; // 1 (state) + key_words + 1 (acc)
+    uint32_t layout_a4;  // row width, value and key offsets and key widths multiples of 4
 };
 
-__device__ __forceinline__ void gather_key(const AggDev& a, const char* row, uint64_t* kw) {
-    char buf[kMaxKeyWords * 8];
+__device__ __forceinline__ void gather_key(const AggDev& a, const char* row, uint64_t* kw, bool a4) {
 #pragma unroll
     for (int w = 0; w < kMaxKeyWords; ++w) kw[w] = 0;
+    if (a4) {  // whole 4-byte words, packed in key order
+        uint32_t o = 0;
+        for (uint32_t c = 0; c < a.nkeys; ++c)
+            for (uint32_t b = 0; b < a.key_bytes[c]; b += 4, o += 4)
+                kw[o / 8] |= (uint64_t)ld_u32(row + a.key_offset[c] + b, true) << (8 * (o % 8));
+        return;
+    }
+    char buf[kMaxKeyWords * 8];
     uint32_t o = 0;
     for (uint32_t c = 0; c < a.nkeys; ++c) {
         for (uint32_t b = 0; b < a.key_bytes[c]; ++b) buf[o + b] = row[a.key_offset[c] + b];
@@ -163,26 +199,17 @@ __device__ __forceinline__ void gather_key(const AggDev& a, const char* row, uin
     for (uint32_t w = 0; w < a.key_words; ++w) __builtin_memcpy(&kw[w], buf + 8 * w, 8);
 }
 
-__device__ __forceinline__ void fold(const AggDev& a, unsigned long long* acc, const char* row) {
+__device__ __forceinline__ void fold(const AggDev& a, unsigned long long* acc, const char* row, bool a4) {
     switch (a.kind) {
-        case PGPU_AGG_SUM_I32: {
-            int32_t v;
-            __builtin_memcpy(&v, row + a.val_offset, 4);
-            atomicAdd((unsigned int*)acc, (unsigned int)v);  // low word of the little-endian slot
+        case PGPU_AGG_SUM_I32:
+            atomicAdd((unsigned int*)acc, ld_u32(row + a.val_offset, a4));  // low word of the little-endian slot
             break;
-        }
-        case PGPU_AGG_SUM_I64: {
-            int64_t v;
-            __builtin_memcpy(&v, row + a.val_offset, 8);
-            atomicAdd(acc, (unsigned long long)v);
+        case PGPU_AGG_SUM_I64:
+            atomicAdd(acc, (unsigned long long)ld_u64(row + a.val_offset, a4));
             break;
-        }
-        case PGPU_AGG_SUM_F64: {
-            double v;
-            __builtin_memcpy(&v, row + a.val_offset, 8);
-            atomicAdd((double*)acc, v);
+        case PGPU_AGG_SUM_F64:
+            atomicAdd((double*)acc, __longlong_as_double((long long)ld_u64(row + a.val_offset, a4)));
             break;
-        }
         default:
             atomicAdd(acc, 1ull);
     }
@@ -193,7 +220,7 @@ __device__ __forceinline__ void fold(const AggDev& a, unsigned long long* acc, c
 // [state][key words...][acc]; state 0 empty, 1 being written, 2 ready.
 __device__ void agg_insert(const AggDev& a, unsigned long long* t, uint64_t mask, const uint64_t* kw,
                            const char* row, const unsigned long long* partial,
-                           unsigned long long* __restrict__ ngroups) {
+                           unsigned long long* __restrict__ ngroups, bool a4 = false) {
     uint64_t h = 0x9E3779B97F4A7C15ull;
     for (uint32_t w = 0; w < a.key_words; ++w) h = mix64(h ^ kw[w]);
     uint64_t s = h & mask;
@@ -225,7 +252,7 @@ __device__ void agg_insert(const AggDev& a, unsigned long long* t, uint64_t mask
                 else
                     atomicAdd(acc, *partial);
             } else {
-                fold(a, acc, row);
+                fold(a, acc, row, a4);
             }
             return;
         }
@@ -237,11 +264,12 @@ __global__ __launch_bounds__(256) void k_agg_append(AggDev a, const char* __rest
                                                     unsigned long long* __restrict__ t, uint64_t mask,
                                                     unsigned long long* __restrict__ ngroups) {
     const uint64_t stride = (uint64_t)gridDim.x * 256;
+    const bool a4 = a.layout_a4 && aligned4((uintptr_t)rows);
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
         const char* row = rows + i * a.row_bytes;
         uint64_t kw[kMaxKeyWords];
-        gather_key(a, row, kw);
-        agg_insert(a, t, mask, kw, row, nullptr, ngroups);
+        gather_key(a, row, kw, a4);
+        agg_insert(a, t, mask, kw, row, nullptr, ngroups, a4);
     }
 }
 
@@ -255,29 +283,21 @@ __global__ __launch_bounds__(256) void k_agg_single(AggDev a, const char* __rest
     __shared__ unsigned long long red[256 / 64];
     __shared__ double redf[256 / 64];
     const uint64_t stride = (uint64_t)gridDim.x * 256;
+    const bool a4 = a.layout_a4 && aligned4((uintptr_t)rows);
     unsigned long long acc = 0;
     double accf = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
         const char* row = rows + i * a.row_bytes;
         switch (a.kind) {
-            case PGPU_AGG_SUM_I32: {
-                int32_t v;
-                __builtin_memcpy(&v, row + a.val_offset, 4);
-                acc += (unsigned long long)(uint32_t)v;  // low 32 bits are the wrapped int32 sum
+            case PGPU_AGG_SUM_I32:
+                acc += ld_u32(row + a.val_offset, a4);  // low 32 bits are the wrapped int32 sum
                 break;
-            }
-            case PGPU_AGG_SUM_I64: {
-                int64_t v;
-                __builtin_memcpy(&v, row + a.val_offset, 8);
-                acc += (unsigned long long)v;
+            case PGPU_AGG_SUM_I64:
+                acc += ld_u64(row + a.val_offset, a4);
                 break;
-            }
-            case PGPU_AGG_SUM_F64: {
-                double v;
-                __builtin_memcpy(&v, row + a.val_offset, 8);
-                accf += v;
+            case PGPU_AGG_SUM_F64:
+                accf += __longlong_as_double((long long)ld_u64(row + a.val_offset, a4));
                 break;
-            }
             default:
                 acc += 1;
         }
@@ -420,6 +440,9 @@ int pgpu_hj_create(pgpu_ctx* ctx, const pgpu_hj_desc* d, pgpu_hj** out) {
         o += p.bytes;
     }
     h->jd.out_row_bytes = o;
+    uint32_t lay = d->build_row_bytes | d->build_key_offset | d->probe_row_bytes | d->probe_key_offset | o;
+    for (uint32_t c = 0; c < d->nproj; ++c) lay |= d->proj[c].src_offset | d->proj[c].bytes | h->jd.dst_off[c];
+    h->jd.layout_a4 = aligned4(lay) ? 1u : 0u;
     h->slots.resize(d->slots);
     *out = h;
     return PGPU_OK;
@@ -696,6 +719,9 @@ int pgpu_agg_create(pgpu_ctx* ctx, const pgpu_agg_desc* d, pgpu_agg** out) {
     g->a.key_total = kt;
     g->a.key_words = (kt + 7) / 8;
     g->a.slot_words = 2 + g->a.key_words;
+    uint32_t lay = d->row_bytes | (d->kind != PGPU_AGG_COUNT ? d->val_offset : 0u);
+    for (uint32_t c = 0; c < d->nkeys; ++c) lay |= d->key_offset[c] | d->key_bytes[c];
+    g->a.layout_a4 = aligned4(lay) ? 1u : 0u;
     g->acc_bytes = vb;
     g->out_row_bytes = kt + vb;
     g->cap = 1ull << ilog2_ceil(std::max<uint64_t>(2 * (d->groups_hint ? d->groups_hint : 32768), 64));
@@ -757,7 +783,9 @@ int agg_fold(pgpu_agg* g, const void* rows, uint64_t n, bool upload) {
         src = g->d_in;
     }
     if (g->a.nkeys == 0)
-        hipLaunchKernelGGL(k_agg_single, dim3((unsigned)std::min<uint64_t>(grid_for(n), 1024)), dim3(256), 0, st, g->a,
+        // every block ends in one atomic on the single slot: few blocks, or
+        // the same-address atomics serialise (1024 blocks: ~50 us)
+        hipLaunchKernelGGL(k_agg_single, dim3((unsigned)std::min<uint64_t>(grid_for(n), 64)), dim3(256), 0, st, g->a,
                            src, n, g->d_table, g->cap - 1, g->d_ngroups);
     else
         hipLaunchKernelGGL(k_agg_append, dim3(grid_for(n)), dim3(256), 0, st, g->a, src, n, g->d_table,
