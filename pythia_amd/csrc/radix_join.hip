@@ -51,7 +51,9 @@ constexpr int kJoinThreads = 512;
 constexpr int kJoinCap = 4096;            // LDS table slots
 constexpr int kJoinPiece = 2048;          // build rows per LDS piece (load <= 0.5)
 constexpr uint32_t kItemRows = 32768;     // probe rows per join item
-constexpr int kTileBytes = 65536;         // LDS staging tile of the scatters
+constexpr int kTileBytes = 65536;         // LDS staging tile of the pass-2 scatter (two workgroups per CU)
+constexpr int kTileBytes1 = 131072;       // pass 1: one workgroup per CU; twice the rows per partition run
+                                          // per tile halves the partial lines written per byte
 constexpr int kPartSlots = 2048;          // partial-result slots of the join kernels
 static_assert(kJoinPiece < 65536, "LDS match counters are 16 bits wide");
 static_assert(2 * kJoinPiece <= kJoinCap, "LDS table load factor above 0.5");
@@ -854,10 +856,14 @@ int plan_alloc(pgpu_join_plan* p, uint64_t bytes, void** out) {
 
 // Elements per LDS staging tile for an element of `esz` bytes.
 inline uint64_t tile_elems(uint64_t esz) { return kTileBytes / esz; }
+// pass-1 rows per thread: the 128 KB tile, at most 32 rows (4-byte rows
+// would otherwise spill registers; 16K of them already fill 64 KB)
+constexpr int pass1_ipt(int esz) { return kTileBytes1 / esz / kScatterThreads < 32 ? kTileBytes1 / esz / kScatterThreads : 32; }
+inline uint64_t tile1_elems(uint64_t esz) { return (uint64_t)pass1_ipt((int)esz) * kScatterThreads; }
 
 // Rows per histogram/scatter block: at most 256 blocks (one per CU), tile-aligned.
 inline uint64_t block_rows(uint64_t n, uint64_t esz) {
-    const uint64_t T = tile_elems(esz);
+    const uint64_t T = tile1_elems(esz);
     uint64_t blk = ceil_div(std::max<uint64_t>(n, 1), 256);
     blk = ceil_div(blk, T) * T;
     return std::max<uint64_t>(blk, T);
@@ -885,7 +891,8 @@ int run_partition(pgpu_join_plan* p, const KT* keys, const KT* vals, uint64_t n,
     const uint32_t nb = 1u << B, nb1 = 1u << (B - B2);
     const uint64_t blk = block_rows(n, sizeof(E));
     const uint32_t nblk = (uint32_t)ceil_div(std::max<uint64_t>(n, 1), blk);
-    constexpr int IPT = kTileBytes / (int)sizeof(E) / kScatterThreads;
+    constexpr int IPT = kTileBytes / (int)sizeof(E) / kScatterThreads;    // pass 2
+    constexpr int IPT1 = pass1_ipt((int)sizeof(E));                       // pass 1
     char name[48];
     {
         snprintf(name, sizeof name, "hist_%s", tag);
@@ -907,7 +914,7 @@ int run_partition(pgpu_join_plan* p, const KT* keys, const KT* vals, uint64_t n,
     {
         snprintf(name, sizeof name, "scatter1_%s", tag);
         Timed tm(p->timer, name);
-        hipLaunchKernelGGL((k_scatter1<KT, HV, IPT>), dim3(nblk), dim3(kScatterThreads), 0, st, keys, vals, n, blk,
+        hipLaunchKernelGGL((k_scatter1<KT, HV, IPT1>), dim3(nblk), dim3(kScatterThreads), 0, st, keys, vals, n, blk,
                            B, B2, off1, P1);
         PGPU_LAUNCH_CHECK();
     }
