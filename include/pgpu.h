@@ -390,7 +390,9 @@ int pgpu_agg_destroy(pgpu_agg* a);
 /* Thread-safe; rows are copied to HBM and folded before return. */
 int pgpu_agg_append(pgpu_agg* a, const void* rows, uint64_t n);
 /* As pgpu_agg_append for rows already in device memory (for example a join's
- * pgpu_hj_output); no reference counterpart. */
+ * pgpu_hj_output); no reference counterpart.  Each aggregate folds on a
+ * stream of its own, ordered after work already queued on the context's
+ * stream (such as pgpu_gen writing the rows). */
 int pgpu_agg_append_device(pgpu_agg* a, const void* d_rows, uint64_t n);
 /* Compacts the groups; output rows = key columns then the accumulator. */
 int pgpu_agg_finish(pgpu_agg* a, uint64_t* n_groups, uint32_t* out_row_bytes);

@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -85,10 +86,18 @@ struct pgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
-    uint64_t used = 0;
+    uint64_t used = 0;  // live allocations (cached blocks excluded)
     uint64_t peak = 0;
     std::mutex mu;
     std::unordered_map<void*, uint64_t> allocs;
+    // Reuse pools for the resources the operators of one query create and
+    // drop (a query's operators live for that query only): streams, pinned
+    // host words and device blocks, handed out again by exact size.
+    std::vector<hipStream_t> stream_pool;
+    std::vector<unsigned long long*> host_words;
+    std::vector<void*> host_chunks;
+    std::multimap<uint64_t, void*> cache;  // bytes -> block
+    uint64_t cached = 0;                   // bytes held in `cache` (count against the budget)
 };
 
 namespace pgpu {
@@ -97,4 +106,17 @@ int ctx_alloc(pgpu_ctx* ctx, uint64_t bytes, void** p);
 int ctx_free(pgpu_ctx* ctx, void* p);
 // Makes ctx->device current on the calling thread.
 int ctx_activate(pgpu_ctx* ctx);
+// As ctx_alloc, but first takes a cached block of the same rounded size.
+int ctx_alloc_reuse(pgpu_ctx* ctx, uint64_t bytes, void** p);
+// Keeps a block from ctx_alloc / ctx_alloc_reuse for reuse instead of
+// freeing it (hipFree would wait for the whole device).  Every stream that
+// used the block must have finished with it.
+int ctx_free_reuse(pgpu_ctx* ctx, void* p);
+// A pooled non-blocking stream at the default priority; put back only
+// after it has been synchronized.
+int ctx_stream_get(pgpu_ctx* ctx, hipStream_t* st);
+void ctx_stream_put(pgpu_ctx* ctx, hipStream_t st);
+// One pinned host word (8 bytes) from the context's pool.
+int ctx_host_word(pgpu_ctx* ctx, unsigned long long** w);
+void ctx_host_word_put(pgpu_ctx* ctx, unsigned long long* w);
 }  // namespace pgpu

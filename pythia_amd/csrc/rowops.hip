@@ -125,45 +125,85 @@ __device__ __forceinline__ uint32_t block_scan256(uint32_t v, uint32_t* total) {
     return pre + x - v;
 }
 
-__global__ __launch_bounds__(256) void k_jprobe(JoinDev jd, const char* __restrict__ build,
+// Sum of one value per thread over a 256-thread block, returned to every thread.
+__device__ __forceinline__ unsigned long long block_sum256(unsigned long long v) {
+    __shared__ unsigned long long ws[4];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const unsigned long long t = ws[0] + ws[1] + ws[2] + ws[3];
+    __syncthreads();
+    return t;
+}
+
+// The probe runs in two launches over the same split of the rows into
+// contiguous per-block chunks: k_jcount counts each block's matches,
+// k_jwrite starts each block at the sum of the counts before it and writes
+// its rows in probe order.  No block takes a global atomic (one cursor
+// shared by thousands of blocks serialises them), and the output order is
+// fixed by the input order.
+constexpr unsigned kProbeBlocks = 2048;
+
+__device__ __forceinline__ uint32_t probe_matches(const JSlot* __restrict__ t, uint64_t mask, int64_t k) {
+    uint32_t c = 0;
+    for (uint64_t s = mix64((uint64_t)k) & mask; t[s].idx != kEmptyIdx; s = (s + 1) & mask) c += t[s].key == k;
+    return c;
+}
+
+__global__ __launch_bounds__(256) void k_jcount(JoinDev jd, const JSlot* __restrict__ t, int logcap,
+                                                const char* __restrict__ probe, uint64_t n, uint64_t chunk,
+                                                unsigned long long* __restrict__ counts) {
+    const pgpu_hj_desc& d = jd.d;
+    const uint64_t mask = (1ull << logcap) - 1;
+    const bool a4 = jd.layout_a4 && aligned4((uintptr_t)probe);
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    unsigned long long c = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += 256)
+        c += probe_matches(t, mask, load_key(probe + i * d.probe_row_bytes + d.probe_key_offset, d.probe_key_bytes, a4));
+    c = block_sum256(c);
+    if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// counts[gridDim.x] receives the total match count (written by the last
+// block); rows at or beyond cap_rows are counted but not written.
+__global__ __launch_bounds__(256) void k_jwrite(JoinDev jd, const char* __restrict__ build,
                                                 const JSlot* __restrict__ t, int logcap,
-                                                const char* __restrict__ probe, uint64_t n, char* __restrict__ out,
-                                                uint64_t cap_rows, unsigned long long* __restrict__ cursor) {
+                                                const char* __restrict__ probe, uint64_t n, uint64_t chunk,
+                                                unsigned long long* __restrict__ counts, char* __restrict__ out,
+                                                uint64_t cap_rows) {
     const pgpu_hj_desc& d = jd.d;
     const uint64_t mask = (1ull << logcap) - 1;
     const bool a4 = jd.layout_a4 && aligned4((uintptr_t)build | (uintptr_t)probe | (uintptr_t)out);
-    __shared__ unsigned long long s_base;
-    // grid-stride over whole blocks so every thread reaches the scans
-    for (uint64_t b0 = (uint64_t)blockIdx.x * 256; b0 < n; b0 += (uint64_t)gridDim.x * 256) {
+    unsigned long long pre = 0;
+    for (unsigned j = threadIdx.x; j < blockIdx.x; j += 256) pre += counts[j];
+    uint64_t base = block_sum256(pre);
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) counts[gridDim.x] = base + counts[blockIdx.x];
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    for (uint64_t b0 = lo; b0 < hi; b0 += 256) {  // whole tiles, so every thread reaches the scans
         const uint64_t i = b0 + threadIdx.x;
-        const bool live = i < n;
-        const char* prow = probe + (live ? i : 0) * d.probe_row_bytes;
+        const bool live = i < hi;
+        const char* prow = probe + (live ? i : lo) * d.probe_row_bytes;
         const int64_t k = live ? load_key(prow + d.probe_key_offset, d.probe_key_bytes, a4) : 0;
-        uint32_t cnt = 0;
         const uint64_t h0 = mix64((uint64_t)k) & mask;
-        if (live) {
-            for (uint64_t s = h0; t[s].idx != kEmptyIdx; s = (s + 1) & mask) cnt += t[s].key == k;
-        }
+        const uint32_t cnt = live ? probe_matches(t, mask, k) : 0u;
         uint32_t total;
         const uint32_t off = block_scan256(cnt, &total);
-        if (threadIdx.x == 0) s_base = total ? atomicAdd(cursor, (unsigned long long)total) : 0ull;
-        __syncthreads();
-        uint64_t pos = s_base + off;
-        if (live && cnt) {
-            for (uint64_t s = h0; t[s].idx != kEmptyIdx; s = (s + 1) & mask) {
-                if (t[s].key != k) continue;
-                if (pos < cap_rows) {
-                    char* o = out + pos * jd.out_row_bytes;
-                    const char* brow = build + (uint64_t)t[s].idx * d.build_row_bytes;
-                    for (uint32_t c = 0; c < d.nproj; ++c) {
-                        const char* src = (d.proj[c].side ? prow : brow) + d.proj[c].src_offset;
-                        copy_cols(o + jd.dst_off[c], src, d.proj[c].bytes, a4);
-                    }
+        uint64_t pos = base + off;
+        base += total;
+        if (!cnt) continue;
+        for (uint64_t s = h0; t[s].idx != kEmptyIdx; s = (s + 1) & mask) {
+            if (t[s].key != k) continue;
+            if (pos < cap_rows) {
+                char* o = out + pos * jd.out_row_bytes;
+                const char* brow = build + (uint64_t)t[s].idx * d.build_row_bytes;
+                for (uint32_t c = 0; c < d.nproj; ++c) {
+                    const char* src = (d.proj[c].side ? prow : brow) + d.proj[c].src_offset;
+                    copy_cols(o + jd.dst_off[c], src, d.proj[c].bytes, a4);
                 }
-                ++pos;
             }
+            ++pos;
         }
-        __syncthreads();
     }
 }
 
@@ -273,54 +313,97 @@ __global__ __launch_bounds__(256) void k_agg_append(AggDev a, const char* __rest
     }
 }
 
-// No GROUP BY: one group.  Each workgroup folds its rows in registers and
-// LDS (in the accumulator's type, so an int32 sum still wraps mod 2^32) and
-// folds the partial into the single slot with one atomic, instead of one
-// same-address atomic per row.
-__global__ __launch_bounds__(256) void k_agg_single(AggDev a, const char* __restrict__ rows, uint64_t n,
-                                                    unsigned long long* __restrict__ t, uint64_t mask,
-                                                    unsigned long long* __restrict__ ngroups) {
+// No GROUP BY: one group, folded in two launches with no same-address
+// atomics.  k_agg_partial: each workgroup sums its rows (in the accumulator's
+// type, so an int32 sum still wraps mod 2^32; several independent loads per
+// thread in flight) into partial[blockIdx.x].  k_agg_single: one workgroup
+// sums the partials in a fixed order and folds the total into the single
+// slot.
+constexpr unsigned kAggPartials = 1024;
+constexpr int kAggUnroll = 4;
+
+__device__ __forceinline__ void acc_row(const AggDev& a, const char* row, bool a4, unsigned long long& acc,
+                                        double& accf) {
+    switch (a.kind) {
+        case PGPU_AGG_SUM_I32:
+            acc += ld_u32(row + a.val_offset, a4);  // low 32 bits are the wrapped int32 sum
+            break;
+        case PGPU_AGG_SUM_I64:
+            acc += ld_u64(row + a.val_offset, a4);
+            break;
+        case PGPU_AGG_SUM_F64:
+            accf += __longlong_as_double((long long)ld_u64(row + a.val_offset, a4));
+            break;
+        default:
+            acc += 1;
+    }
+}
+
+// Sums of (acc, accf) over a 256-thread block, valid in thread 0.
+__device__ __forceinline__ void block_sum2(unsigned long long& acc, double& accf) {
     __shared__ unsigned long long red[256 / 64];
     __shared__ double redf[256 / 64];
-    const uint64_t stride = (uint64_t)gridDim.x * 256;
-    const bool a4 = a.layout_a4 && aligned4((uintptr_t)rows);
-    unsigned long long acc = 0;
-    double accf = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-        const char* row = rows + i * a.row_bytes;
-        switch (a.kind) {
-            case PGPU_AGG_SUM_I32:
-                acc += ld_u32(row + a.val_offset, a4);  // low 32 bits are the wrapped int32 sum
-                break;
-            case PGPU_AGG_SUM_I64:
-                acc += ld_u64(row + a.val_offset, a4);
-                break;
-            case PGPU_AGG_SUM_F64:
-                accf += __longlong_as_double((long long)ld_u64(row + a.val_offset, a4));
-                break;
-            default:
-                acc += 1;
-        }
-    }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         acc += __shfl_xor(acc, d, 64);
         accf += __shfl_xor(accf, d, 64);
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) {
-        red[w] = acc;
-        redf[w] = accf;
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = acc;
+        redf[threadIdx.x >> 6] = accf;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long p = 0;
-        double pf = 0;
-        for (int k = 0; k < 256 / 64; ++k) {
-            p += red[k];
-            pf += redf[k];
+        acc = red[0] + red[1] + red[2] + red[3];
+        accf = ((redf[0] + redf[1]) + redf[2]) + redf[3];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_agg_partial(AggDev a, const char* __restrict__ rows, uint64_t n,
+                                                     unsigned long long* __restrict__ partial) {
+    const bool a4 = a.layout_a4 && aligned4((uintptr_t)rows);
+    unsigned long long acc = 0;
+    double accf = 0;
+    const uint64_t step = (uint64_t)gridDim.x * 256 * kAggUnroll;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * 256 * kAggUnroll + threadIdx.x; i0 < n; i0 += step) {
+        if (i0 + 256 * (kAggUnroll - 1) < n) {
+            // all loads issued before the first add
+            uint64_t v[kAggUnroll];
+#pragma unroll
+            for (int u = 0; u < kAggUnroll; ++u) {
+                const char* p = rows + (i0 + 256 * u) * a.row_bytes + a.val_offset;
+                v[u] = a.kind == PGPU_AGG_COUNT ? 1u : a.kind == PGPU_AGG_SUM_I32 ? ld_u32(p, a4) : ld_u64(p, a4);
+            }
+#pragma unroll
+            for (int u = 0; u < kAggUnroll; ++u) {
+                if (a.kind == PGPU_AGG_SUM_F64)
+                    accf += __longlong_as_double((long long)v[u]);
+                else
+                    acc += v[u];
+            }
+        } else {
+            for (uint64_t i = i0; i < n; i += 256) acc_row(a, rows + i * a.row_bytes, a4, acc, accf);
         }
-        if (a.kind == PGPU_AGG_SUM_F64) p = (unsigned long long)__double_as_longlong(pf);
+    }
+    block_sum2(acc, accf);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = a.kind == PGPU_AGG_SUM_F64 ? (unsigned long long)__double_as_longlong(accf) : acc;
+}
+
+__global__ __launch_bounds__(256) void k_agg_single(AggDev a, const unsigned long long* __restrict__ partial,
+                                                    unsigned np, unsigned long long* __restrict__ t, uint64_t mask,
+                                                    unsigned long long* __restrict__ ngroups) {
+    unsigned long long acc = 0;
+    double accf = 0;
+    for (unsigned j = threadIdx.x; j < np; j += 256) {
+        if (a.kind == PGPU_AGG_SUM_F64)
+            accf += __longlong_as_double((long long)partial[j]);
+        else
+            acc += partial[j];
+    }
+    block_sum2(acc, accf);
+    if (threadIdx.x == 0) {
+        unsigned long long p = a.kind == PGPU_AGG_SUM_F64 ? (unsigned long long)__double_as_longlong(accf) : acc;
         uint64_t kw[kMaxKeyWords] = {};
         agg_insert(a, t, mask, kw, nullptr, &p, ngroups);
     }
@@ -389,7 +472,7 @@ struct pgpu_hj {
         uint64_t in_cap = 0;
         char* d_out = nullptr;
         uint64_t out_cap = 0;  // rows
-        unsigned long long* d_cursor = nullptr;
+        unsigned long long* d_cursor = nullptr;  // per-block match counts, then the total
         unsigned long long* h_cursor = nullptr;
         uint64_t nout = 0, pulled = 0;
     };
@@ -402,11 +485,11 @@ using namespace pgpu;
 namespace {
 int slot_reserve(pgpu_ctx* ctx, char** p, uint64_t* cap, uint64_t need) {
     if (*cap >= need && *p) return PGPU_OK;
-    if (*p) PGPU_TRY(ctx_free(ctx, *p));
+    if (*p) PGPU_TRY(ctx_free_reuse(ctx, *p));
     *p = nullptr;
     *cap = 0;
     uint64_t c = std::max<uint64_t>(need, 4096);
-    PGPU_TRY(ctx_alloc(ctx, c, (void**)p));
+    PGPU_TRY(ctx_alloc_reuse(ctx, c, (void**)p));
     *cap = c;
     return PGPU_OK;
 }
@@ -451,18 +534,22 @@ int pgpu_hj_create(pgpu_ctx* ctx, const pgpu_hj_desc* d, pgpu_hj** out) {
 int pgpu_hj_destroy(pgpu_hj* h) {
     if (!h) return PGPU_OK;
     (void)ctx_activate(h->ctx);
+    // every stream is synchronized before its buffers go back to the pools
+    bool synced = true;
+    for (auto& s : h->slots)
+        if (s.st) synced = hipStreamSynchronize(s.st) == hipSuccess && synced;
+    synced = hipStreamSynchronize(h->ctx->stream) == hipSuccess && synced;
+    auto drop = [&](void* p) { (void)(synced ? ctx_free_reuse(h->ctx, p) : ctx_free(h->ctx, p)); };
     for (auto& s : h->slots) {
-        if (s.st) {
-            (void)hipStreamSynchronize(s.st);
-            (void)hipStreamDestroy(s.st);
-        }
-        if (s.d_in) (void)ctx_free(h->ctx, s.d_in);
-        if (s.d_out) (void)ctx_free(h->ctx, s.d_out);
-        if (s.d_cursor) (void)ctx_free(h->ctx, s.d_cursor);
-        if (s.h_cursor) (void)hipHostFree(s.h_cursor);
+        if (s.st && synced) ctx_stream_put(h->ctx, s.st);
+        else if (s.st) (void)hipStreamDestroy(s.st);
+        if (s.d_in) drop(s.d_in);
+        if (s.d_out) drop(s.d_out);
+        if (s.d_cursor) drop(s.d_cursor);
+        if (s.h_cursor) ctx_host_word_put(h->ctx, s.h_cursor);
     }
-    if (h->d_build && !h->build_external) (void)ctx_free(h->ctx, h->d_build);
-    if (h->d_table) (void)ctx_free(h->ctx, h->d_table);
+    if (h->d_build && !h->build_external) drop(h->d_build);
+    if (h->d_table) drop(h->d_table);
     delete h;
     return PGPU_OK;
 }
@@ -485,7 +572,7 @@ int pgpu_hj_build_finish(pgpu_hj* h) {
     if (h->nbuild >= kEmptyIdx) return fail(PGPU_EINVAL, "build side exceeds 2^32-1 rows");
     hipStream_t st = h->ctx->stream;
     const uint64_t bytes = h->staged.size();
-    PGPU_TRY(ctx_alloc(h->ctx, std::max<uint64_t>(bytes, 8), (void**)&h->d_build));
+    PGPU_TRY(ctx_alloc_reuse(h->ctx, std::max<uint64_t>(bytes, 8), (void**)&h->d_build));
     if (bytes) PGPU_HIP(hipMemcpyAsync(h->d_build, h->staged.data(), bytes, hipMemcpyHostToDevice, st));
     PGPU_TRY(hj_build_table(h));
     h->stats.h2d_bytes += bytes;
@@ -516,7 +603,7 @@ int hj_build_table(pgpu_hj* h) {
     hipStream_t st = h->ctx->stream;
     h->logcap = (int)ilog2_ceil(std::max<uint64_t>(2 * h->nbuild, 16));
     const uint64_t cap = 1ull << h->logcap;
-    PGPU_TRY(ctx_alloc(h->ctx, cap * sizeof(JSlot), (void**)&h->d_table));
+    PGPU_TRY(ctx_alloc_reuse(h->ctx, cap * sizeof(JSlot), (void**)&h->d_table));
     hipLaunchKernelGGL(k_jclear, dim3(grid_for(cap)), dim3(256), 0, st, h->d_table, cap);
     PGPU_LAUNCH_CHECK();
     if (h->nbuild) {
@@ -540,20 +627,25 @@ int hj_probe_rows(pgpu_hj* h, pgpu_hj::Slot& s, const char* d_in, uint64_t n, ui
         PGPU_TRY(slot_reserve(h->ctx, &s.d_out, &s.out_cap, n * orb));
         cap_rows = s.out_cap / orb;
     }
+    // contiguous chunks of whole 256-row tiles, at most kProbeBlocks of them
+    const uint64_t chunk = ceil_div(ceil_div(n, kProbeBlocks), 256) * 256;
+    const unsigned nb = (unsigned)ceil_div(n, chunk);
+    hipLaunchKernelGGL(k_jcount, dim3(nb), dim3(256), 0, s.st, h->jd, h->d_table, h->logcap, d_in, n, chunk,
+                       s.d_cursor);
+    PGPU_LAUNCH_CHECK();
     for (int attempt = 0; attempt < 2; ++attempt) {
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, s.st, s.d_cursor, 1, 0ull);
+        hipLaunchKernelGGL(k_jwrite, dim3(nb), dim3(256), 0, s.st, h->jd, h->d_build, h->d_table, h->logcap, d_in,
+                           n, chunk, s.d_cursor, s.d_out, cap_rows);
         PGPU_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_jprobe, dim3(grid_for(n)), dim3(256), 0, s.st, h->jd, h->d_build, h->d_table,
-                           h->logcap, d_in, n, s.d_out, cap_rows, s.d_cursor);
-        PGPU_LAUNCH_CHECK();
-        PGPU_HIP(hipMemcpyAsync(s.h_cursor, s.d_cursor, 8, hipMemcpyDeviceToHost, s.st));
+        PGPU_HIP(hipMemcpyAsync(s.h_cursor, s.d_cursor + nb, 8, hipMemcpyDeviceToHost, s.st));
         PGPU_HIP(hipStreamSynchronize(s.st));
         const uint64_t total = *s.h_cursor;
         if (total <= cap_rows) {
             s.nout = total;
             break;
         }
-        // Output larger than the buffer: grow to the exact size and re-run.
+        // Output larger than the buffer: grow to the exact size and write again
+        // (the per-block counts stay valid).
         PGPU_TRY(slot_reserve(h->ctx, &s.d_out, &s.out_cap, total * orb));
         cap_rows = s.out_cap / orb;
         h->stats.probe_reruns++;
@@ -571,11 +663,9 @@ int hj_probe_begin(pgpu_hj* h, uint32_t slot, uint64_t n, uint64_t* n_out, bool*
     if (!h->built) return fail(PGPU_ESTATE, "pgpu_hj_probe before build_finish");
     PGPU_TRY(ctx_activate(h->ctx));
     auto& s = h->slots[slot];
-    if (!s.st) PGPU_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
-    if (!s.d_cursor) {
-        PGPU_TRY(ctx_alloc(h->ctx, 8, (void**)&s.d_cursor));
-        PGPU_HIP(hipHostMalloc((void**)&s.h_cursor, 8, hipHostMallocDefault));
-    }
+    if (!s.st) PGPU_TRY(ctx_stream_get(h->ctx, &s.st));
+    if (!s.d_cursor) PGPU_TRY(ctx_alloc_reuse(h->ctx, 8 * (kProbeBlocks + 1), (void**)&s.d_cursor));
+    if (!s.h_cursor) PGPU_TRY(ctx_host_word(h->ctx, &s.h_cursor));
     s.nout = s.pulled = 0;
     *n_out = 0;
     *skip = n == 0 || h->nbuild == 0;
@@ -610,11 +700,6 @@ int pgpu_hj_probe(pgpu_hj* h, uint32_t slot, const void* rows, uint64_t n, uint6
     PGPU_TRY(hj_probe_begin(h, slot, n, n_out, &skip));
     if (skip) return PGPU_OK;
     auto& s = h->slots[slot];
-    if (!s.st) PGPU_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
-    if (!s.d_cursor) {
-        PGPU_TRY(ctx_alloc(h->ctx, 8, (void**)&s.d_cursor));
-        PGPU_HIP(hipHostMalloc((void**)&s.h_cursor, 8, hipHostMallocDefault));
-    }
     const uint32_t prb = h->jd.d.probe_row_bytes;
     PGPU_TRY(slot_reserve(h->ctx, &s.d_in, &s.in_cap, n * prb));
     PGPU_HIP(hipMemcpyAsync(s.d_in, rows, n * prb, hipMemcpyHostToDevice, s.st));
@@ -658,16 +743,23 @@ struct pgpu_agg {
     char* d_in = nullptr;
     uint64_t in_cap = 0;
     char* d_out = nullptr;
+    uint64_t out_cap = 0;
+    unsigned long long* d_partial = nullptr;  // per-workgroup sums of the single-group fold
     uint64_t ngroups_out = 0;
     uint32_t out_row_bytes = 0, acc_bytes = 0;
     bool finished = false;
+    // Each aggregate runs on a stream of its own, so that the per-thread
+    // aggregates of one query fold concurrently instead of queueing on the
+    // context's stream.
+    hipStream_t st = nullptr;
+    hipEvent_t ctx_done = nullptr;  // orders device-row folds after the context's stream
 };
 
 namespace {
 int agg_alloc_table(pgpu_agg* g, uint64_t cap, unsigned long long** t) {
     const uint64_t words = cap * g->a.slot_words;
-    PGPU_TRY(ctx_alloc(g->ctx, words * 8, (void**)t));
-    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(words)), dim3(256), 0, g->ctx->stream, *t, words, 0ull);
+    PGPU_TRY(ctx_alloc_reuse(g->ctx, words * 8, (void**)t));
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(words)), dim3(256), 0, g->st, *t, words, 0ull);
     PGPU_LAUNCH_CHECK();
     return PGPU_OK;
 }
@@ -675,14 +767,14 @@ int agg_alloc_table(pgpu_agg* g, uint64_t cap, unsigned long long** t) {
 int agg_grow(pgpu_agg* g, uint64_t newcap) {
     unsigned long long* nt = nullptr;
     PGPU_TRY(agg_alloc_table(g, newcap, &nt));
-    hipStream_t st = g->ctx->stream;
+    hipStream_t st = g->st;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, g->d_ngroups, 1, 0ull);
     PGPU_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_agg_rehash, dim3(grid_for(g->cap)), dim3(256), 0, st, g->a, g->d_table, g->cap, nt,
                        newcap - 1, g->d_ngroups);
     PGPU_LAUNCH_CHECK();
     PGPU_HIP(hipStreamSynchronize(st));
-    PGPU_TRY(ctx_free(g->ctx, g->d_table));
+    PGPU_TRY(ctx_free_reuse(g->ctx, g->d_table));
     g->d_table = nt;
     g->cap = newcap;
     return PGPU_OK;
@@ -725,12 +817,20 @@ int pgpu_agg_create(pgpu_ctx* ctx, const pgpu_agg_desc* d, pgpu_agg** out) {
     g->acc_bytes = vb;
     g->out_row_bytes = kt + vb;
     g->cap = 1ull << ilog2_ceil(std::max<uint64_t>(2 * (d->groups_hint ? d->groups_hint : 32768), 64));
-    int rc = agg_alloc_table(g, g->cap, &g->d_table);
-    if (rc == PGPU_OK) rc = ctx_alloc(ctx, 8, (void**)&g->d_ngroups);
-    if (rc == PGPU_OK && hipHostMalloc((void**)&g->h_ngroups, 8, hipHostMallocDefault) != hipSuccess)
-        rc = fail(PGPU_ENOMEM, "hipHostMalloc failed");
+    int rc = ctx_stream_get(ctx, &g->st);
+    if (rc == PGPU_OK && hipEventCreateWithFlags(&g->ctx_done, hipEventDisableTiming) != hipSuccess) {
+        g->ctx_done = nullptr;
+        rc = fail(PGPU_EDEVICE, "hipEventCreateWithFlags failed");
+    }
+    if (rc == PGPU_OK) rc = agg_alloc_table(g, g->cap, &g->d_table);
+    // the compacted output for as many groups as the table holds before it
+    // grows, so that finish allocates nothing unless the table grew
+    if (rc == PGPU_OK) rc = slot_reserve(ctx, &g->d_out, &g->out_cap, (g->cap / 2) * g->out_row_bytes);
+    if (rc == PGPU_OK) rc = ctx_alloc_reuse(ctx, 8, (void**)&g->d_ngroups);
+    if (rc == PGPU_OK && d->nkeys == 0) rc = ctx_alloc_reuse(ctx, 8 * kAggPartials, (void**)&g->d_partial);
+    if (rc == PGPU_OK) rc = ctx_host_word(ctx, &g->h_ngroups);
     if (rc == PGPU_OK) {
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, ctx->stream, g->d_ngroups, 1, 0ull);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, g->st, g->d_ngroups, 1, 0ull);
         if (hipGetLastError() != hipSuccess) rc = fail(PGPU_EDEVICE, "launch failed");
     }
     if (rc != PGPU_OK) {
@@ -744,12 +844,18 @@ int pgpu_agg_create(pgpu_ctx* ctx, const pgpu_agg_desc* d, pgpu_agg** out) {
 int pgpu_agg_destroy(pgpu_agg* g) {
     if (!g) return PGPU_OK;
     (void)ctx_activate(g->ctx);
-    (void)hipStreamSynchronize(g->ctx->stream);
-    if (g->d_table) (void)ctx_free(g->ctx, g->d_table);
-    if (g->d_ngroups) (void)ctx_free(g->ctx, g->d_ngroups);
-    if (g->h_ngroups) (void)hipHostFree(g->h_ngroups);
-    if (g->d_in) (void)ctx_free(g->ctx, g->d_in);
-    if (g->d_out) (void)ctx_free(g->ctx, g->d_out);
+    // the stream is synchronized before its buffers go back to the pools
+    const bool synced = !g->st || hipStreamSynchronize(g->st) == hipSuccess;
+    auto drop = [&](void* p) { (void)(synced ? ctx_free_reuse(g->ctx, p) : ctx_free(g->ctx, p)); };
+    if (g->d_table) drop(g->d_table);
+    if (g->d_ngroups) drop(g->d_ngroups);
+    if (g->h_ngroups) ctx_host_word_put(g->ctx, g->h_ngroups);
+    if (g->d_in) drop(g->d_in);
+    if (g->d_out) drop(g->d_out);
+    if (g->d_partial) drop(g->d_partial);
+    if (g->st && synced) ctx_stream_put(g->ctx, g->st);
+    else if (g->st) (void)hipStreamDestroy(g->st);
+    if (g->ctx_done) (void)hipEventDestroy(g->ctx_done);
     delete g;
     return PGPU_OK;
 }
@@ -760,7 +866,7 @@ namespace {
 // Folds n device-resident rows into the table (caller holds g->mu; the
 // context is current).  `upload`: rows are on the host and go through d_in.
 int agg_fold(pgpu_agg* g, const void* rows, uint64_t n, bool upload) {
-    hipStream_t st = g->ctx->stream;
+    hipStream_t st = g->st;
     // Keep the table at most half full even if every row is a new group
     // (no GROUP BY: one group).
     uint64_t need = 1;
@@ -781,13 +887,18 @@ int agg_fold(pgpu_agg* g, const void* rows, uint64_t n, bool upload) {
         PGPU_TRY(slot_reserve(g->ctx, &g->d_in, &g->in_cap, n * g->a.row_bytes));
         PGPU_HIP(hipMemcpyAsync(g->d_in, rows, n * g->a.row_bytes, hipMemcpyHostToDevice, st));
         src = g->d_in;
+    } else {
+        // rows a kernel on the context's stream may still be writing
+        PGPU_HIP(hipEventRecord(g->ctx_done, g->ctx->stream));
+        PGPU_HIP(hipStreamWaitEvent(st, g->ctx_done, 0));
     }
-    if (g->a.nkeys == 0)
-        // every block ends in one atomic on the single slot: few blocks, or
-        // the same-address atomics serialise (1024 blocks: ~50 us)
-        hipLaunchKernelGGL(k_agg_single, dim3((unsigned)std::min<uint64_t>(grid_for(n), 64)), dim3(256), 0, st, g->a,
-                           src, n, g->d_table, g->cap - 1, g->d_ngroups);
-    else
+    if (g->a.nkeys == 0) {
+        const unsigned np = (unsigned)std::min<uint64_t>(ceil_div(n, 256 * kAggUnroll), kAggPartials);
+        hipLaunchKernelGGL(k_agg_partial, dim3(np), dim3(256), 0, st, g->a, src, n, g->d_partial);
+        PGPU_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_agg_single, dim3(1), dim3(256), 0, st, g->a, g->d_partial, np, g->d_table, g->cap - 1,
+                           g->d_ngroups);
+    } else
         hipLaunchKernelGGL(k_agg_append, dim3(grid_for(n)), dim3(256), 0, st, g->a, src, n, g->d_table,
                            g->cap - 1, g->d_ngroups);
     PGPU_LAUNCH_CHECK();
@@ -820,12 +931,12 @@ int pgpu_agg_finish(pgpu_agg* g, uint64_t* n_groups, uint32_t* out_row_bytes) {
     if (!g || !n_groups) return fail(PGPU_EINVAL, "pgpu_agg_finish: null argument");
     std::lock_guard<std::mutex> lk(g->mu);
     PGPU_TRY(ctx_activate(g->ctx));
-    hipStream_t st = g->ctx->stream;
+    hipStream_t st = g->st;
     if (!g->finished) {
         PGPU_HIP(hipMemcpyAsync(g->h_ngroups, g->d_ngroups, 8, hipMemcpyDeviceToHost, st));
         PGPU_HIP(hipStreamSynchronize(st));
         const uint64_t ng = *g->h_ngroups;
-        PGPU_TRY(ctx_alloc(g->ctx, std::max<uint64_t>(ng * g->out_row_bytes, 8), (void**)&g->d_out));
+        PGPU_TRY(slot_reserve(g->ctx, &g->d_out, &g->out_cap, ng * g->out_row_bytes));
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, g->d_ngroups, 1, 0ull);
         PGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_agg_compact, dim3(grid_for(g->cap)), dim3(256), 0, st, g->a, g->d_table, g->cap,
@@ -848,7 +959,9 @@ int pgpu_agg_pull(pgpu_agg* g, uint64_t first, uint64_t n, void* dst) {
     if (first + n > g->ngroups_out) return fail(PGPU_EINVAL, "pgpu_agg_pull: range beyond the groups");
     if (!n) return PGPU_OK;
     PGPU_TRY(ctx_activate(g->ctx));
-    PGPU_HIP(hipMemcpy(dst, g->d_out + first * g->out_row_bytes, n * g->out_row_bytes, hipMemcpyDeviceToHost));
+    PGPU_HIP(hipMemcpyAsync(dst, g->d_out + first * g->out_row_bytes, n * g->out_row_bytes, hipMemcpyDeviceToHost,
+                            g->st));
+    PGPU_HIP(hipStreamSynchronize(g->st));
     return PGPU_OK;
 }
 

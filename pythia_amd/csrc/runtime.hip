@@ -32,13 +32,26 @@ int ctx_activate(pgpu_ctx* ctx) {
     return PGPU_OK;
 }
 
+namespace {
+uint64_t round_bytes(uint64_t bytes) { return bytes ? (bytes + 255) & ~uint64_t(255) : 256; }
+
+// Frees every cached block (ctx->mu held).
+void release_cache(pgpu_ctx* ctx) {
+    if (ctx->cache.empty()) return;
+    (void)hipSetDevice(ctx->device);
+    for (auto& kv : ctx->cache) (void)hipFree(kv.second);
+    ctx->cache.clear();
+    ctx->cached = 0;
+}
+}  // namespace
+
 int ctx_alloc(pgpu_ctx* ctx, uint64_t bytes, void** p) {
     if (!ctx || !p) return fail(PGPU_EINVAL, "ctx_alloc: null argument");
     *p = nullptr;
-    if (bytes == 0) bytes = 256;
-    bytes = (bytes + 255) & ~uint64_t(255);
+    bytes = round_bytes(bytes);
     {
         std::lock_guard<std::mutex> g(ctx->mu);
+        if (ctx->used + ctx->cached + bytes > ctx->budget) release_cache(ctx);
         if (ctx->used + bytes > ctx->budget)
             return fail(PGPU_EBUDGET,
                         "HBM budget exceeded: %llu B in use + %llu B requested > %llu B budget",
@@ -77,6 +90,84 @@ int ctx_free(pgpu_ctx* ctx, void* p) {
     PGPU_TRY(ctx_activate(ctx));
     PGPU_HIP(hipFree(p));
     return PGPU_OK;
+}
+
+int ctx_alloc_reuse(pgpu_ctx* ctx, uint64_t bytes, void** p) {
+    if (!ctx || !p) return fail(PGPU_EINVAL, "ctx_alloc_reuse: null argument");
+    bytes = round_bytes(bytes);
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto it = ctx->cache.find(bytes);
+        if (it != ctx->cache.end() && ctx->used + bytes <= ctx->budget) {
+            *p = it->second;
+            ctx->cache.erase(it);
+            ctx->cached -= bytes;
+            ctx->used += bytes;
+            if (ctx->used > ctx->peak) ctx->peak = ctx->used;
+            ctx->allocs[*p] = bytes;
+            return PGPU_OK;
+        }
+    }
+    return ctx_alloc(ctx, bytes, p);
+}
+
+int ctx_free_reuse(pgpu_ctx* ctx, void* p) {
+    if (!p) return PGPU_OK;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto it = ctx->allocs.find(p);
+        if (it == ctx->allocs.end()) return fail(PGPU_EINVAL, "ctx_free_reuse: %p was not allocated by this context", p);
+        const uint64_t bytes = it->second;
+        // keep at most a quarter of the budget cached
+        if (ctx->cached + bytes <= ctx->budget / 4) {
+            ctx->allocs.erase(it);
+            ctx->used -= bytes;
+            ctx->cache.emplace(bytes, p);
+            ctx->cached += bytes;
+            return PGPU_OK;
+        }
+    }
+    return ctx_free(ctx, p);
+}
+
+int ctx_stream_get(pgpu_ctx* ctx, hipStream_t* st) {
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (!ctx->stream_pool.empty()) {
+            *st = ctx->stream_pool.back();
+            ctx->stream_pool.pop_back();
+            return PGPU_OK;
+        }
+    }
+    PGPU_TRY(ctx_activate(ctx));
+    PGPU_HIP(hipStreamCreateWithFlags(st, hipStreamNonBlocking));
+    return PGPU_OK;
+}
+
+void ctx_stream_put(pgpu_ctx* ctx, hipStream_t st) {
+    if (!st) return;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->stream_pool.push_back(st);
+}
+
+int ctx_host_word(pgpu_ctx* ctx, unsigned long long** w) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->host_words.empty()) {
+        constexpr int kWords = 512;
+        void* chunk = nullptr;
+        PGPU_HIP(hipHostMalloc(&chunk, kWords * 8, hipHostMallocDefault));
+        ctx->host_chunks.push_back(chunk);
+        for (int i = kWords - 1; i >= 0; --i) ctx->host_words.push_back((unsigned long long*)chunk + i);
+    }
+    *w = ctx->host_words.back();
+    ctx->host_words.pop_back();
+    return PGPU_OK;
+}
+
+void ctx_host_word_put(pgpu_ctx* ctx, unsigned long long* w) {
+    if (!w) return;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->host_words.push_back(w);
 }
 
 // ---------------------------------------------------------------- timer
@@ -191,6 +282,9 @@ int pgpu_ctx_destroy(pgpu_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
     ctx->allocs.clear();
+    release_cache(ctx);
+    for (auto st : ctx->stream_pool) (void)hipStreamDestroy(st);
+    for (auto c : ctx->host_chunks) (void)hipHostFree(c);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PGPU_OK;

@@ -291,3 +291,55 @@ def test_resident_join_with_fused_aggregate(threads):
     finally:
         P.clear_tables()
         P.clear_resident()
+
+
+@pytest.mark.parametrize("vtype", ["long", "dec"])
+def test_resident_large_probe_and_single_group_sums(vtype):
+    """A resident probe batch of 1.2M rows (several 256-row tiles per
+    workgroup) whose output, with every build key twice, outgrows the first
+    output buffer (the write pass runs again into a larger one); the
+    single-group fold then runs its grid-stride loop over ~2.3M join rows.
+    int64 sums wrap mod 2^64 exactly; FP64 sums within 1e-12 relative of the
+    exact sum (positive values)."""
+    rng = np.random.default_rng(11 if vtype == "long" else 12)
+    vdt = np.int64 if vtype == "long" else np.float64
+    rdt = np.dtype({"names": ["k", "v"], "formats": [np.int32, vdt], "offsets": [0, 4], "itemsize": 12})
+    sdt = np.dtype({"names": ["k", "v"], "formats": [np.int32, np.int32], "offsets": [0, 4], "itemsize": 8})
+    nR, nS = 100_000, 1_300_000
+    R = np.empty(nR, rdt)
+    R["k"] = (rng.permutation(nR) // 2 * 3).astype(np.int32)  # every key twice
+    R["v"] = rng.integers(-2**62, 2**62, nR) if vtype == "long" else rng.random(nR) * 1e6
+    S = np.empty(nS, sdt)
+    S["k"] = (rng.integers(0, nR // 2 + 1000, nS) * 3).astype(np.int32)
+    S["v"] = 0
+    nkeys = nR // 2 + 1000
+    mult = np.bincount(R["k"] // 3, minlength=nkeys)
+    pk = S["k"] // 3
+    try:
+        P.register_table("res_r", R)
+        P.register_table("res_s", S)
+        plan = _resident_agg_plan("aggregate_sum", 2).replace("probebatch = 30000L", "probebatch = 1200000L")
+        plan = plan.replace('r = { type = "memscan"; table = "res_r"; schema = ( "int", "int" ); };',
+                            f'r = {{ type = "memscan"; table = "res_r"; schema = ( "int", "{vtype}" ); }};')
+        assert f'"int", "{vtype}"' in plan
+        cplan = _resident_agg_plan("aggregate_count", 2).replace("probebatch = 30000L", "probebatch = 1200000L")
+        cplan = cplan.replace('table = "res_r"; schema = ( "int", "int" )', f'table = "res_r"; schema = ( "int", "{vtype}" )')
+        cnt, _ = P.run(cplan)
+        assert int(cnt["c0"].sum()) == int(mult[pk].sum()) > nS
+        out, meta = P.run(plan)
+        if vtype == "long":
+            ksum = np.zeros(nkeys, np.uint64)
+            np.add.at(ksum, R["k"] // 3, R["v"].view(np.uint64))
+            with np.errstate(over="ignore"):
+                want = int(ksum[pk].sum(dtype=np.uint64))
+                got = int(out["c0"].astype(np.int64).view(np.uint64).sum(dtype=np.uint64))
+            assert got == want
+        else:
+            ksum = np.zeros(nkeys, np.float64)
+            np.add.at(ksum, R["k"] // 3, R["v"])
+            want = float(np.sum(ksum[pk].astype(np.longdouble)))
+            got = float(out["c0"].sum())
+            assert abs(got - want) <= 1e-12 * abs(want)
+    finally:
+        P.clear_tables()
+        P.clear_resident()
