@@ -71,7 +71,7 @@ __device__ __forceinline__ int64_t load_key(const char* p, uint32_t bytes, bool 
 
 __host__ __device__ inline bool aligned4(uint64_t v) { return (v & 3) == 0; }
 
-struct JSlot {
+struct alignas(16) JSlot {  // one 16-byte load per probe step
     int64_t key;
     uint32_t idx;
     uint32_t pad;
@@ -138,29 +138,57 @@ __device__ __forceinline__ unsigned long long block_sum256(unsigned long long v)
 }
 
 // The probe runs in two launches over the same split of the rows into
-// contiguous per-block chunks: k_jcount counts each block's matches,
-// k_jwrite starts each block at the sum of the counts before it and writes
-// its rows in probe order.  No block takes a global atomic (one cursor
-// shared by thousands of blocks serialises them), and the output order is
-// fixed by the input order.
+// contiguous per-block chunks.  k_jcount probes every row once and records
+// (match count << 32 | first match's build row) per row, plus each block's
+// total; k_jwrite starts each block at the sum of the totals before it and
+// writes its rows in probe order: a row with one match (the primary-key
+// case) is written from its record without touching the table again, only
+// rows with several matches walk it a second time.  No block takes a global
+// atomic (one cursor shared by thousands of blocks serialises them).  Each
+// thread holds kProbeU consecutive rows, so their loads are in flight
+// together and a wave reads one contiguous span.
 constexpr unsigned kProbeBlocks = 2048;
-
-__device__ __forceinline__ uint32_t probe_matches(const JSlot* __restrict__ t, uint64_t mask, int64_t k) {
-    uint32_t c = 0;
-    for (uint64_t s = mix64((uint64_t)k) & mask; t[s].idx != kEmptyIdx; s = (s + 1) & mask) c += t[s].key == k;
-    return c;
-}
+constexpr int kProbeU = 4;
+constexpr unsigned kProbeTile = 256 * kProbeU;
 
 __global__ __launch_bounds__(256) void k_jcount(JoinDev jd, const JSlot* __restrict__ t, int logcap,
                                                 const char* __restrict__ probe, uint64_t n, uint64_t chunk,
-                                                unsigned long long* __restrict__ counts) {
+                                                unsigned long long* __restrict__ counts,
+                                                unsigned long long* __restrict__ rec) {
     const pgpu_hj_desc& d = jd.d;
     const uint64_t mask = (1ull << logcap) - 1;
     const bool a4 = jd.layout_a4 && aligned4((uintptr_t)probe);
     const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
     unsigned long long c = 0;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += 256)
-        c += probe_matches(t, mask, load_key(probe + i * d.probe_row_bytes + d.probe_key_offset, d.probe_key_bytes, a4));
+    for (uint64_t b0 = lo; b0 < hi; b0 += kProbeTile) {
+        const uint64_t i0 = b0 + (uint64_t)threadIdx.x * kProbeU;
+        int64_t k[kProbeU];
+        uint64_t sl[kProbeU];
+        JSlot e[kProbeU];
+#pragma unroll
+        for (int u = 0; u < kProbeU; ++u)
+            k[u] = i0 + u < hi ? load_key(probe + (i0 + u) * d.probe_row_bytes + d.probe_key_offset, d.probe_key_bytes, a4)
+                               : 0;
+#pragma unroll
+        for (int u = 0; u < kProbeU; ++u) {
+            sl[u] = mix64((uint64_t)k[u]) & mask;
+            e[u] = t[sl[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kProbeU; ++u) {
+            if (i0 + u >= hi) break;
+            uint32_t cnt = 0, first = 0;
+            for (JSlot x = e[u]; x.idx != kEmptyIdx; x = t[sl[u]]) {
+                if (x.key == k[u]) {
+                    if (!cnt) first = x.idx;
+                    ++cnt;
+                }
+                sl[u] = (sl[u] + 1) & mask;
+            }
+            rec[i0 + u] = ((unsigned long long)cnt << 32) | first;
+            c += cnt;
+        }
+    }
     c = block_sum256(c);
     if (threadIdx.x == 0) counts[blockIdx.x] = c;
 }
@@ -170,7 +198,8 @@ __global__ __launch_bounds__(256) void k_jcount(JoinDev jd, const JSlot* __restr
 __global__ __launch_bounds__(256) void k_jwrite(JoinDev jd, const char* __restrict__ build,
                                                 const JSlot* __restrict__ t, int logcap,
                                                 const char* __restrict__ probe, uint64_t n, uint64_t chunk,
-                                                unsigned long long* __restrict__ counts, char* __restrict__ out,
+                                                unsigned long long* __restrict__ counts,
+                                                const unsigned long long* __restrict__ rec, char* __restrict__ out,
                                                 uint64_t cap_rows) {
     const pgpu_hj_desc& d = jd.d;
     const uint64_t mask = (1ull << logcap) - 1;
@@ -180,29 +209,40 @@ __global__ __launch_bounds__(256) void k_jwrite(JoinDev jd, const char* __restri
     uint64_t base = block_sum256(pre);
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) counts[gridDim.x] = base + counts[blockIdx.x];
     const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
-    for (uint64_t b0 = lo; b0 < hi; b0 += 256) {  // whole tiles, so every thread reaches the scans
-        const uint64_t i = b0 + threadIdx.x;
-        const bool live = i < hi;
-        const char* prow = probe + (live ? i : lo) * d.probe_row_bytes;
-        const int64_t k = live ? load_key(prow + d.probe_key_offset, d.probe_key_bytes, a4) : 0;
-        const uint64_t h0 = mix64((uint64_t)k) & mask;
-        const uint32_t cnt = live ? probe_matches(t, mask, k) : 0u;
+    auto emit = [&](uint64_t pos, const char* prow, uint32_t bidx) {
+        if (pos >= cap_rows) return;
+        char* o = out + pos * jd.out_row_bytes;
+        const char* brow = build + (uint64_t)bidx * d.build_row_bytes;
+        for (uint32_t c = 0; c < d.nproj; ++c) {
+            const char* src = (d.proj[c].side ? prow : brow) + d.proj[c].src_offset;
+            copy_cols(o + jd.dst_off[c], src, d.proj[c].bytes, a4);
+        }
+    };
+    for (uint64_t b0 = lo; b0 < hi; b0 += kProbeTile) {  // whole tiles, so every thread reaches the scans
+        const uint64_t i0 = b0 + (uint64_t)threadIdx.x * kProbeU;
+        unsigned long long r[kProbeU];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int u = 0; u < kProbeU; ++u) {
+            r[u] = i0 + u < hi ? rec[i0 + u] : 0ull;
+            mine += (uint32_t)(r[u] >> 32);
+        }
         uint32_t total;
-        const uint32_t off = block_scan256(cnt, &total);
-        uint64_t pos = base + off;
+        uint64_t pos = base + block_scan256(mine, &total);
         base += total;
-        if (!cnt) continue;
-        for (uint64_t s = h0; t[s].idx != kEmptyIdx; s = (s + 1) & mask) {
-            if (t[s].key != k) continue;
-            if (pos < cap_rows) {
-                char* o = out + pos * jd.out_row_bytes;
-                const char* brow = build + (uint64_t)t[s].idx * d.build_row_bytes;
-                for (uint32_t c = 0; c < d.nproj; ++c) {
-                    const char* src = (d.proj[c].side ? prow : brow) + d.proj[c].src_offset;
-                    copy_cols(o + jd.dst_off[c], src, d.proj[c].bytes, a4);
-                }
+        if (!mine) continue;
+#pragma unroll
+        for (int u = 0; u < kProbeU; ++u) {
+            const uint32_t cnt = (uint32_t)(r[u] >> 32);
+            if (!cnt) continue;
+            const char* prow = probe + (i0 + u) * d.probe_row_bytes;
+            if (cnt == 1) {
+                emit(pos++, prow, (uint32_t)r[u]);
+                continue;
             }
-            ++pos;
+            const int64_t k = load_key(prow + d.probe_key_offset, d.probe_key_bytes, a4);
+            for (uint64_t s = mix64((uint64_t)k) & mask; t[s].idx != kEmptyIdx; s = (s + 1) & mask)
+                if (t[s].key == k) emit(pos++, prow, t[s].idx);
         }
     }
 }
@@ -472,6 +512,8 @@ struct pgpu_hj {
         uint64_t in_cap = 0;
         char* d_out = nullptr;
         uint64_t out_cap = 0;  // rows
+        char* d_rec = nullptr;  // per probe row: match count << 32 | first match
+        uint64_t rec_cap = 0;
         unsigned long long* d_cursor = nullptr;  // per-block match counts, then the total
         unsigned long long* h_cursor = nullptr;
         uint64_t nout = 0, pulled = 0;
@@ -546,6 +588,7 @@ int pgpu_hj_destroy(pgpu_hj* h) {
         if (s.d_in) drop(s.d_in);
         if (s.d_out) drop(s.d_out);
         if (s.d_cursor) drop(s.d_cursor);
+        if (s.d_rec) drop(s.d_rec);
         if (s.h_cursor) ctx_host_word_put(h->ctx, s.h_cursor);
     }
     if (h->d_build && !h->build_external) drop(h->d_build);
@@ -627,15 +670,17 @@ int hj_probe_rows(pgpu_hj* h, pgpu_hj::Slot& s, const char* d_in, uint64_t n, ui
         PGPU_TRY(slot_reserve(h->ctx, &s.d_out, &s.out_cap, n * orb));
         cap_rows = s.out_cap / orb;
     }
-    // contiguous chunks of whole 256-row tiles, at most kProbeBlocks of them
-    const uint64_t chunk = ceil_div(ceil_div(n, kProbeBlocks), 256) * 256;
+    // contiguous chunks of whole tiles, at most kProbeBlocks of them
+    const uint64_t chunk = ceil_div(ceil_div(n, kProbeBlocks), kProbeTile) * kProbeTile;
     const unsigned nb = (unsigned)ceil_div(n, chunk);
+    PGPU_TRY(slot_reserve(h->ctx, &s.d_rec, &s.rec_cap, n * 8));
+    const auto* rec = (const unsigned long long*)s.d_rec;
     hipLaunchKernelGGL(k_jcount, dim3(nb), dim3(256), 0, s.st, h->jd, h->d_table, h->logcap, d_in, n, chunk,
-                       s.d_cursor);
+                       s.d_cursor, (unsigned long long*)s.d_rec);
     PGPU_LAUNCH_CHECK();
     for (int attempt = 0; attempt < 2; ++attempt) {
         hipLaunchKernelGGL(k_jwrite, dim3(nb), dim3(256), 0, s.st, h->jd, h->d_build, h->d_table, h->logcap, d_in,
-                           n, chunk, s.d_cursor, s.d_out, cap_rows);
+                           n, chunk, s.d_cursor, rec, s.d_out, cap_rows);
         PGPU_LAUNCH_CHECK();
         PGPU_HIP(hipMemcpyAsync(s.h_cursor, s.d_cursor + nb, 8, hipMemcpyDeviceToHost, s.st));
         PGPU_HIP(hipStreamSynchronize(s.st));
