@@ -174,40 +174,63 @@ __global__ __launch_bounds__(256) void k_jcount(JoinDev jd, const JSlot* __restr
             sl[u] = mix64((uint64_t)k[u]) & mask;
             e[u] = t[sl[u]];
         }
+        // the kProbeU collision chains are walked in lock step: one round of
+        // dependent loads per step instead of one per row and step
+        uint32_t cnt[kProbeU], first[kProbeU];
+        bool act[kProbeU];
+#pragma unroll
+        for (int u = 0; u < kProbeU; ++u) {
+            cnt[u] = first[u] = 0;
+            act[u] = i0 + u < hi;
+        }
+        while (true) {
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < kProbeU; ++u) {
+                if (!act[u]) continue;
+                if (e[u].idx == kEmptyIdx) {
+                    act[u] = false;
+                    continue;
+                }
+                if (e[u].key == k[u]) {
+                    if (!cnt[u]) first[u] = e[u].idx;
+                    ++cnt[u];
+                }
+                sl[u] = (sl[u] + 1) & mask;
+                any = true;
+            }
+            if (!any) break;
+#pragma unroll
+            for (int u = 0; u < kProbeU; ++u)
+                if (act[u]) e[u] = t[sl[u]];
+        }
 #pragma unroll
         for (int u = 0; u < kProbeU; ++u) {
             if (i0 + u >= hi) break;
-            uint32_t cnt = 0, first = 0;
-            for (JSlot x = e[u]; x.idx != kEmptyIdx; x = t[sl[u]]) {
-                if (x.key == k[u]) {
-                    if (!cnt) first = x.idx;
-                    ++cnt;
-                }
-                sl[u] = (sl[u] + 1) & mask;
-            }
-            rec[i0 + u] = ((unsigned long long)cnt << 32) | first;
-            c += cnt;
+            rec[i0 + u] = ((unsigned long long)cnt[u] << 32) | first[u];
+            c += cnt[u];
         }
     }
     c = block_sum256(c);
     if (threadIdx.x == 0) counts[blockIdx.x] = c;
 }
 
-// counts[gridDim.x] receives the total match count (written by the last
-// block); rows at or beyond cap_rows are counted but not written.
+// The last block writes the total match count to pinned host memory
+// (*host_total, read after the stream synchronizes: no copy on the probe's
+// path); rows at or beyond cap_rows are counted but not written.
 __global__ __launch_bounds__(256) void k_jwrite(JoinDev jd, const char* __restrict__ build,
                                                 const JSlot* __restrict__ t, int logcap,
                                                 const char* __restrict__ probe, uint64_t n, uint64_t chunk,
-                                                unsigned long long* __restrict__ counts,
+                                                const unsigned long long* __restrict__ counts,
                                                 const unsigned long long* __restrict__ rec, char* __restrict__ out,
-                                                uint64_t cap_rows) {
+                                                uint64_t cap_rows, unsigned long long* host_total) {
     const pgpu_hj_desc& d = jd.d;
     const uint64_t mask = (1ull << logcap) - 1;
     const bool a4 = jd.layout_a4 && aligned4((uintptr_t)build | (uintptr_t)probe | (uintptr_t)out);
     unsigned long long pre = 0;
     for (unsigned j = threadIdx.x; j < blockIdx.x; j += 256) pre += counts[j];
     uint64_t base = block_sum256(pre);
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) counts[gridDim.x] = base + counts[blockIdx.x];
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *host_total = base + counts[blockIdx.x];
     const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
     auto emit = [&](uint64_t pos, const char* prow, uint32_t bidx) {
         if (pos >= cap_rows) return;
@@ -514,7 +537,7 @@ struct pgpu_hj {
         uint64_t out_cap = 0;  // rows
         char* d_rec = nullptr;  // per probe row: match count << 32 | first match
         uint64_t rec_cap = 0;
-        unsigned long long* d_cursor = nullptr;  // per-block match counts, then the total
+        unsigned long long* d_cursor = nullptr;  // per-block match counts
         unsigned long long* h_cursor = nullptr;
         uint64_t nout = 0, pulled = 0;
     };
@@ -644,7 +667,8 @@ namespace {
 // Claims and fills the open-addressing table over h->d_build's h->nbuild rows.
 int hj_build_table(pgpu_hj* h) {
     hipStream_t st = h->ctx->stream;
-    h->logcap = (int)ilog2_ceil(std::max<uint64_t>(2 * h->nbuild, 16));
+    // load factor at most 1/4 (1/2 beyond 2^26 build rows): short collision chains
+    h->logcap = (int)ilog2_ceil(std::max<uint64_t>((h->nbuild <= (1ull << 26) ? 4 : 2) * h->nbuild, 16));
     const uint64_t cap = 1ull << h->logcap;
     PGPU_TRY(ctx_alloc_reuse(h->ctx, cap * sizeof(JSlot), (void**)&h->d_table));
     hipLaunchKernelGGL(k_jclear, dim3(grid_for(cap)), dim3(256), 0, st, h->d_table, cap);
@@ -680,9 +704,8 @@ int hj_probe_rows(pgpu_hj* h, pgpu_hj::Slot& s, const char* d_in, uint64_t n, ui
     PGPU_LAUNCH_CHECK();
     for (int attempt = 0; attempt < 2; ++attempt) {
         hipLaunchKernelGGL(k_jwrite, dim3(nb), dim3(256), 0, s.st, h->jd, h->d_build, h->d_table, h->logcap, d_in,
-                           n, chunk, s.d_cursor, rec, s.d_out, cap_rows);
+                           n, chunk, s.d_cursor, rec, s.d_out, cap_rows, s.h_cursor);
         PGPU_LAUNCH_CHECK();
-        PGPU_HIP(hipMemcpyAsync(s.h_cursor, s.d_cursor + nb, 8, hipMemcpyDeviceToHost, s.st));
         PGPU_HIP(hipStreamSynchronize(s.st));
         const uint64_t total = *s.h_cursor;
         if (total <= cap_rows) {
@@ -709,7 +732,7 @@ int hj_probe_begin(pgpu_hj* h, uint32_t slot, uint64_t n, uint64_t* n_out, bool*
     PGPU_TRY(ctx_activate(h->ctx));
     auto& s = h->slots[slot];
     if (!s.st) PGPU_TRY(ctx_stream_get(h->ctx, &s.st));
-    if (!s.d_cursor) PGPU_TRY(ctx_alloc_reuse(h->ctx, 8 * (kProbeBlocks + 1), (void**)&s.d_cursor));
+    if (!s.d_cursor) PGPU_TRY(ctx_alloc_reuse(h->ctx, 8 * kProbeBlocks, (void**)&s.d_cursor));
     if (!s.h_cursor) PGPU_TRY(ctx_host_word(h->ctx, &s.h_cursor));
     s.nout = s.pulled = 0;
     *n_out = 0;
