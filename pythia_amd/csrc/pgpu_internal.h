@@ -98,6 +98,8 @@ struct pgpu_ctx {
     std::vector<void*> host_chunks;
     std::multimap<uint64_t, void*> cache;  // bytes -> block
     uint64_t cached = 0;                   // bytes held in `cache` (count against the budget)
+    std::multimap<uint64_t, void*> host_cache;  // pinned host staging blocks: bytes -> block
+    std::unordered_map<void*, uint64_t> host_blocks;  // every pinned staging block and its size
 };
 
 namespace pgpu {
@@ -116,6 +118,10 @@ int ctx_free_reuse(pgpu_ctx* ctx, void* p);
 // after it has been synchronized.
 int ctx_stream_get(pgpu_ctx* ctx, hipStream_t* st);
 void ctx_stream_put(pgpu_ctx* ctx, hipStream_t st);
+// Pinned host staging of at least `bytes` (sizes rounded up to 1 MiB and
+// reused by size); host_free_reuse keeps it for the next caller.
+int ctx_host_alloc_reuse(pgpu_ctx* ctx, uint64_t bytes, void** p, uint64_t* cap);
+void ctx_host_free_reuse(pgpu_ctx* ctx, void* p);
 // One pinned host word (8 bytes) from the context's pool.
 int ctx_host_word(pgpu_ctx* ctx, unsigned long long** w);
 void ctx_host_word_put(pgpu_ctx* ctx, unsigned long long* w);

@@ -518,6 +518,13 @@ inline unsigned grid_for(uint64_t n) { return (unsigned)std::max<uint64_t>(1, st
 }  // namespace
 
 // ====================================================================== join
+namespace {
+struct Stage {  // a pinned host staging block from the context's pool
+    char* p = nullptr;
+    uint64_t cap = 0;
+};
+}  // namespace
+
 struct pgpu_hj {
     pgpu_ctx* ctx = nullptr;
     JoinDev jd{};
@@ -535,6 +542,7 @@ struct pgpu_hj {
         uint64_t in_cap = 0;
         char* d_out = nullptr;
         uint64_t out_cap = 0;  // rows
+        Stage hin, hout;        // pinned staging of host probe rows / pulled output rows
         char* d_rec = nullptr;  // per probe row: match count << 32 | first match
         uint64_t rec_cap = 0;
         unsigned long long* d_cursor = nullptr;  // per-block match counts
@@ -559,6 +567,41 @@ int slot_reserve(pgpu_ctx* ctx, char** p, uint64_t* cap, uint64_t need) {
     return PGPU_OK;
 }
 int hj_build_table(pgpu_hj* h);
+
+// Pinned staging of host rows: pageable copies go through the runtime's own
+// bounce buffer one at a time, so many workers' copies queue behind each
+// other.  Rows are copied into a pinned block in 4 MiB pieces, each piece's
+// DMA queued on `st` as soon as it is copied (the CPU copy of the next piece
+// overlaps it).  The stream must have finished with the previous contents.
+int stage_reserve(pgpu_ctx* ctx, Stage& sg, uint64_t bytes) {
+    if (sg.cap >= bytes && sg.p) return PGPU_OK;
+    ctx_host_free_reuse(ctx, sg.p);
+    sg.p = nullptr;
+    sg.cap = 0;
+    return ctx_host_alloc_reuse(ctx, bytes, (void**)&sg.p, &sg.cap);
+}
+
+int stage_h2d(pgpu_ctx* ctx, Stage& sg, hipStream_t st, char* d_dst, const char* src, uint64_t bytes) {
+    if (!bytes) return PGPU_OK;
+    PGPU_TRY(stage_reserve(ctx, sg, bytes));
+    constexpr uint64_t kPiece = 4ull << 20;
+    for (uint64_t o = 0; o < bytes; o += kPiece) {
+        const uint64_t b = std::min(kPiece, bytes - o);
+        memcpy(sg.p + o, src + o, b);
+        PGPU_HIP(hipMemcpyAsync(d_dst + o, sg.p + o, b, hipMemcpyHostToDevice, st));
+    }
+    return PGPU_OK;
+}
+
+// Device rows to host memory through the pinned staging (one DMA, then a CPU copy).
+int stage_d2h(pgpu_ctx* ctx, Stage& sg, hipStream_t st, char* dst, const char* d_src, uint64_t bytes) {
+    if (!bytes) return PGPU_OK;
+    PGPU_TRY(stage_reserve(ctx, sg, bytes));
+    PGPU_HIP(hipMemcpyAsync(sg.p, d_src, bytes, hipMemcpyDeviceToHost, st));
+    PGPU_HIP(hipStreamSynchronize(st));
+    memcpy(dst, sg.p, bytes);
+    return PGPU_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -612,6 +655,10 @@ int pgpu_hj_destroy(pgpu_hj* h) {
         if (s.d_out) drop(s.d_out);
         if (s.d_cursor) drop(s.d_cursor);
         if (s.d_rec) drop(s.d_rec);
+        if (synced) {
+            ctx_host_free_reuse(h->ctx, s.hin.p);
+            ctx_host_free_reuse(h->ctx, s.hout.p);
+        }
         if (s.h_cursor) ctx_host_word_put(h->ctx, s.h_cursor);
     }
     if (h->d_build && !h->build_external) drop(h->d_build);
@@ -770,7 +817,7 @@ int pgpu_hj_probe(pgpu_hj* h, uint32_t slot, const void* rows, uint64_t n, uint6
     auto& s = h->slots[slot];
     const uint32_t prb = h->jd.d.probe_row_bytes;
     PGPU_TRY(slot_reserve(h->ctx, &s.d_in, &s.in_cap, n * prb));
-    PGPU_HIP(hipMemcpyAsync(s.d_in, rows, n * prb, hipMemcpyHostToDevice, s.st));
+    PGPU_TRY(stage_h2d(h->ctx, s.hin, s.st, s.d_in, (const char*)rows, n * prb));
     h->stats.h2d_bytes += n * prb;
     return hj_probe_rows(h, s, s.d_in, n, n_out);
 }
@@ -784,8 +831,7 @@ int pgpu_hj_pull(pgpu_hj* h, uint32_t slot, void* dst, uint64_t cap_rows, uint64
     if (!k) return PGPU_OK;
     PGPU_TRY(ctx_activate(h->ctx));
     const uint64_t orb = h->jd.out_row_bytes;
-    PGPU_HIP(hipMemcpyAsync(dst, s.d_out + s.pulled * orb, k * orb, hipMemcpyDeviceToHost, s.st));
-    PGPU_HIP(hipStreamSynchronize(s.st));
+    PGPU_TRY(stage_d2h(h->ctx, s.hout, s.st, (char*)dst, s.d_out + s.pulled * orb, k * orb));
     s.pulled += k;
     h->stats.d2h_bytes += k * orb;
     return PGPU_OK;
@@ -813,6 +859,7 @@ struct pgpu_agg {
     char* d_out = nullptr;
     uint64_t out_cap = 0;
     unsigned long long* d_partial = nullptr;  // per-workgroup sums of the single-group fold
+    Stage hin;                                // pinned staging of appended host rows
     uint64_t ngroups_out = 0;
     uint32_t out_row_bytes = 0, acc_bytes = 0;
     bool finished = false;
@@ -921,6 +968,7 @@ int pgpu_agg_destroy(pgpu_agg* g) {
     if (g->d_in) drop(g->d_in);
     if (g->d_out) drop(g->d_out);
     if (g->d_partial) drop(g->d_partial);
+    if (synced) ctx_host_free_reuse(g->ctx, g->hin.p);
     if (g->st && synced) ctx_stream_put(g->ctx, g->st);
     else if (g->st) (void)hipStreamDestroy(g->st);
     if (g->ctx_done) (void)hipEventDestroy(g->ctx_done);
@@ -953,7 +1001,7 @@ int agg_fold(pgpu_agg* g, const void* rows, uint64_t n, bool upload) {
     const char* src = (const char*)rows;
     if (upload) {
         PGPU_TRY(slot_reserve(g->ctx, &g->d_in, &g->in_cap, n * g->a.row_bytes));
-        PGPU_HIP(hipMemcpyAsync(g->d_in, rows, n * g->a.row_bytes, hipMemcpyHostToDevice, st));
+        PGPU_TRY(stage_h2d(g->ctx, g->hin, st, g->d_in, (const char*)rows, n * g->a.row_bytes));
         src = g->d_in;
     } else {
         // rows a kernel on the context's stream may still be writing

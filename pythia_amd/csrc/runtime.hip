@@ -6,6 +6,7 @@
 // resource: every allocation is charged against the context's HBM budget,
 // and an allocation that would exceed it fails with PGPU_EBUDGET instead of
 // blocking, so admission control can act on it.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 
@@ -150,6 +151,33 @@ void ctx_stream_put(pgpu_ctx* ctx, hipStream_t st) {
     ctx->stream_pool.push_back(st);
 }
 
+int ctx_host_alloc_reuse(pgpu_ctx* ctx, uint64_t bytes, void** p, uint64_t* cap) {
+    const uint64_t b = std::max<uint64_t>(1, (bytes + (1ull << 20) - 1) >> 20) << 20;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto it = ctx->host_cache.find(b);
+        if (it != ctx->host_cache.end()) {
+            *p = it->second;
+            ctx->host_cache.erase(it);
+            *cap = b;
+            return PGPU_OK;
+        }
+    }
+    PGPU_TRY(ctx_activate(ctx));
+    PGPU_HIP(hipHostMalloc(p, b, hipHostMallocDefault));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->host_blocks[*p] = b;
+    *cap = b;
+    return PGPU_OK;
+}
+
+void ctx_host_free_reuse(pgpu_ctx* ctx, void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->host_blocks.find(p);
+    if (it != ctx->host_blocks.end()) ctx->host_cache.emplace(it->second, p);
+}
+
 int ctx_host_word(pgpu_ctx* ctx, unsigned long long** w) {
     std::lock_guard<std::mutex> g(ctx->mu);
     if (ctx->host_words.empty()) {
@@ -285,6 +313,7 @@ int pgpu_ctx_destroy(pgpu_ctx* ctx) {
     release_cache(ctx);
     for (auto st : ctx->stream_pool) (void)hipStreamDestroy(st);
     for (auto c : ctx->host_chunks) (void)hipHostFree(c);
+    for (auto& kv : ctx->host_blocks) (void)hipHostFree(kv.first);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PGPU_OK;

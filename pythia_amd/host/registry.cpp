@@ -23,9 +23,13 @@ struct Resident {
 };
 std::mutex g_res_mu;  // held while a source is copied; never taken under g_mu
 std::map<std::pair<int, std::string>, Resident> g_resident;
+// A device context outlives the queries that use it (an idle one is kept
+// with refs == 0): its pooled streams, pinned words and device blocks serve
+// the next query, so a repeated query allocates nothing.
 struct Dev {
     pgpu_ctx* ctx = nullptr;
     int refs = 0;
+    uint64_t budget = 0;  // as requested at creation
 };
 std::map<int, Dev> g_devs;
 }  // namespace
@@ -59,7 +63,18 @@ void check(int rc, const char* what) {
 pgpu_ctx* acquire_device(int device, uint64_t hbm_budget) {
     std::lock_guard<std::mutex> g(g_mu);
     Dev& d = g_devs[device];
-    if (!d.ctx) check(pgpu_ctx_create(device, hbm_budget, &d.ctx), "pgpu_ctx_create");
+    if (d.ctx && d.refs == 0 && d.budget != hbm_budget) {  // idle, other budget: start afresh
+        pgpu_ctx_destroy(d.ctx);
+        d.ctx = nullptr;
+    }
+    if (!d.ctx) {
+        pgpu_ctx* c = nullptr;
+        const int rc = pgpu_ctx_create(device, hbm_budget, &c);
+        if (rc != PGPU_OK) g_devs.erase(device);
+        check(rc, "pgpu_ctx_create");
+        d.ctx = c;
+        d.budget = hbm_budget;
+    }
     ++d.refs;
     return d.ctx;
 }
@@ -67,13 +82,11 @@ pgpu_ctx* acquire_device(int device, uint64_t hbm_budget) {
 void release_device(pgpu_ctx* ctx) {
     if (!ctx) return;
     std::lock_guard<std::mutex> g(g_mu);
-    for (auto it = g_devs.begin(); it != g_devs.end(); ++it) {
-        if (it->second.ctx == ctx && --it->second.refs == 0) {
-            pgpu_ctx_destroy(ctx);
-            g_devs.erase(it);
+    for (auto& e : g_devs)
+        if (e.second.ctx == ctx && e.second.refs > 0) {
+            --e.second.refs;  // at zero the context stays, idle, for the next query
             return;
         }
-    }
 }
 
 const void* resident_rows(int device, uint64_t hbm_budget, const std::string& key, const char* rows, uint64_t bytes) {

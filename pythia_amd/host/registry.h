@@ -24,7 +24,9 @@ const void* resident_rows(int device, uint64_t hbm_budget, const std::string& ke
 void clear_resident();
 
 // Shared device context; the first acquirer's budget (0 = all free HBM)
-// applies.  Throws QueryExecutionError with the library's message on failure.
+// applies.  It is kept when its last user releases it (the next query reuses
+// its pools) unless the next acquirer asks for another budget.  Throws
+// QueryExecutionError with the library's message on failure.
 pgpu_ctx* acquire_device(int device, uint64_t hbm_budget);
 void release_device(pgpu_ctx* ctx);
 
