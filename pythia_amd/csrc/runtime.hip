@@ -63,6 +63,14 @@ int ctx_alloc(pgpu_ctx* ctx, uint64_t bytes, void** p) {
     }
     PGPU_TRY(ctx_activate(ctx));
     hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {  // memory taken outside the library: drop the kept blocks, try again
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            release_cache(ctx);
+        }
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         std::lock_guard<std::mutex> g(ctx->mu);
         ctx->used -= bytes;
