@@ -3,19 +3,23 @@
 //
 // Join (replaces HashJoinOp build + probe + constructOutputTuple,
 // operators/join.cpp:475-667): the build rows (sbuild layout) are uploaded
-// once; an open-addressing multimap in HBM holds (key, row index) per build
-// row.  A probe batch is uploaded, each thread counts its matches, the
-// workgroup scans the counts, reserves one contiguous output range with a
-// single atomic, and writes the projected rows there (output compaction
-// without a second pass).  When the output buffer is too small the batch is
-// re-run into a larger one.
+// once; an open-addressing multimap in HBM (at most a quarter full) holds
+// (key, row index) per build row.  A probe batch is joined in two launches
+// over per-workgroup chunks: a count pass records each row's match count and
+// first match, a write pass places every workgroup's rows after the counts
+// of the workgroups before it and writes the projected rows in probe order
+// (no global atomics).  When the output buffer is too small the write pass
+// runs again into a larger one.  Host rows and pulled output go through
+// pooled pinned staging.
 //
 // Aggregation (replaces GenericAggregate::remember and the AggregateSum /
 // AggregateCount folds, operators/genericaggregate.cpp:240-282): an
 // open-addressing table in HBM keyed by the packed group-key bytes; a slot is
 // claimed with a state CAS (empty -> writing -> ready) and folded with
 // hardware atomics in the accumulator's type.  The table grows (rehash) when
-// half full.  finish() compacts the ready slots into output rows.
+// half full.  finish() compacts the ready slots into output rows.  Without a
+// GROUP BY the rows are summed per workgroup and then in one workgroup (no
+// same-address atomics).  Each aggregate runs on a stream of its own.
 #include <algorithm>
 #include <cstring>
 
