@@ -585,25 +585,34 @@ int stage_reserve(pgpu_ctx* ctx, Stage& sg, uint64_t bytes) {
     return ctx_host_alloc_reuse(ctx, bytes, (void**)&sg.p, &sg.cap);
 }
 
+constexpr uint64_t kStageMax = 32ull << 20;  // pinned staging per user: larger copies go in rounds
+
 int stage_h2d(pgpu_ctx* ctx, Stage& sg, hipStream_t st, char* d_dst, const char* src, uint64_t bytes) {
     if (!bytes) return PGPU_OK;
-    PGPU_TRY(stage_reserve(ctx, sg, bytes));
+    PGPU_TRY(stage_reserve(ctx, sg, std::min(bytes, kStageMax)));
     constexpr uint64_t kPiece = 4ull << 20;
-    for (uint64_t o = 0; o < bytes; o += kPiece) {
-        const uint64_t b = std::min(kPiece, bytes - o);
-        memcpy(sg.p + o, src + o, b);
-        PGPU_HIP(hipMemcpyAsync(d_dst + o, sg.p + o, b, hipMemcpyHostToDevice, st));
+    for (uint64_t r = 0; r < bytes; r += sg.cap) {
+        if (r) PGPU_HIP(hipStreamSynchronize(st));  // the previous round's DMAs have read the staging
+        const uint64_t rb = std::min(sg.cap, bytes - r);
+        for (uint64_t o = 0; o < rb; o += kPiece) {
+            const uint64_t b = std::min(kPiece, rb - o);
+            memcpy(sg.p + o, src + r + o, b);
+            PGPU_HIP(hipMemcpyAsync(d_dst + r + o, sg.p + o, b, hipMemcpyHostToDevice, st));
+        }
     }
     return PGPU_OK;
 }
 
-// Device rows to host memory through the pinned staging (one DMA, then a CPU copy).
+// Device rows to host memory through the pinned staging (a DMA, then a CPU copy, per round).
 int stage_d2h(pgpu_ctx* ctx, Stage& sg, hipStream_t st, char* dst, const char* d_src, uint64_t bytes) {
     if (!bytes) return PGPU_OK;
-    PGPU_TRY(stage_reserve(ctx, sg, bytes));
-    PGPU_HIP(hipMemcpyAsync(sg.p, d_src, bytes, hipMemcpyDeviceToHost, st));
-    PGPU_HIP(hipStreamSynchronize(st));
-    memcpy(dst, sg.p, bytes);
+    PGPU_TRY(stage_reserve(ctx, sg, std::min(bytes, kStageMax)));
+    for (uint64_t r = 0; r < bytes; r += sg.cap) {
+        const uint64_t rb = std::min(sg.cap, bytes - r);
+        PGPU_HIP(hipMemcpyAsync(sg.p, d_src + r, rb, hipMemcpyDeviceToHost, st));
+        PGPU_HIP(hipStreamSynchronize(st));
+        memcpy(dst + r, sg.p, rb);
+    }
     return PGPU_OK;
 }
 }  // namespace

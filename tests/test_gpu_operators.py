@@ -343,3 +343,42 @@ def test_resident_large_probe_and_single_group_sums(vtype):
     finally:
         P.clear_tables()
         P.clear_resident()
+
+
+def test_host_pages_large_batches_stage_in_rounds():
+    """Host-page join with a 5M-row probe batch (40 MB, more than one pinned
+    staging round up) and 64 MB output pages (pulls of more than one round
+    down): the output multiset is byte-exact against numpy."""
+    rng = np.random.default_rng(21)
+    dt = np.dtype({"names": ["k", "v"], "formats": [np.int32, np.int32], "offsets": [0, 4], "itemsize": 8})
+    nR, nS = 100_000, 5_000_000
+    R = np.empty(nR, dt)
+    R["k"] = rng.permutation(nR).astype(np.int32)
+    R["v"] = rng.integers(-2**31, 2**31 - 1, nR)
+    S = np.empty(nS, dt)
+    S["k"] = rng.integers(-5000, nR, nS).astype(np.int32)  # a few probe keys match nothing
+    S["v"] = rng.integers(-2**31, 2**31 - 1, nS)
+    bv = np.full(nR, 0, np.int64)
+    bv[R["k"]] = R["v"]
+    hit = S["k"] >= 0
+    want = np.sort(bv[S["k"][hit]].astype(np.uint32).astype(np.uint64)
+                   | (S["v"][hit].astype(np.uint32).astype(np.uint64) << 32))
+    plan = textwrap.dedent("""
+        path = "./"; buffsize = 67108864;
+        treeroot = { name = "merge"; input = { name = "join"; build = { name = "r"; }; probe = { name = "s"; }; }; };
+        merge = { type = "merge"; threads = 1; };
+        join = { type = "gpuhashjoin"; hash = { fn = "modulo"; buckets = 4096; }; threadgroups = ( [ 0 ] );
+                 tuplesperbucket = 4; allocpolicy = "local"; buildjattr = 0; probejattr = 0;
+                 projection = ( "B$1", "P$1" ); gpu = { probebatch = 8388608L; }; };
+        r = { type = "memscan"; table = "big_r"; schema = ( "int", "int" ); };
+        s = { type = "memscan"; table = "big_s"; schema = ( "int", "int" ); };
+    """)
+    try:
+        P.register_table("big_r", R)
+        P.register_table("big_s", S)
+        out, meta = P.run(plan)
+        assert meta["row_bytes"] == 8 and len(out) == len(want)
+        got = np.sort(np.frombuffer(out.tobytes(), np.uint64))
+        assert np.array_equal(got, want)
+    finally:
+        P.clear_tables()
