@@ -181,9 +181,18 @@ int ctx_host_alloc_reuse(pgpu_ctx* ctx, uint64_t bytes, void** p, uint64_t* cap)
 
 void ctx_host_free_reuse(pgpu_ctx* ctx, void* p) {
     if (!p) return;
+    constexpr uint64_t kHostCacheMax = 1ull << 30;  // pinned staging kept for reuse, at most
     std::lock_guard<std::mutex> g(ctx->mu);
     auto it = ctx->host_blocks.find(p);
-    if (it != ctx->host_blocks.end()) ctx->host_cache.emplace(it->second, p);
+    if (it == ctx->host_blocks.end()) return;
+    uint64_t kept = 0;
+    for (auto& kv : ctx->host_cache) kept += kv.first;
+    if (kept + it->second <= kHostCacheMax) {
+        ctx->host_cache.emplace(it->second, p);
+        return;
+    }
+    (void)hipHostFree(p);
+    ctx->host_blocks.erase(it);
 }
 
 int ctx_host_word(pgpu_ctx* ctx, unsigned long long** w) {
