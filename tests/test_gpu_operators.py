@@ -382,3 +382,62 @@ def test_host_pages_large_batches_stage_in_rounds():
         assert np.array_equal(got, want)
     finally:
         P.clear_tables()
+
+
+@pytest.mark.parametrize("global_", [False, True])
+def test_resident_join_fused_groupby_sum(global_):
+    """A resident join whose output is grouped where it lies in HBM:
+    sum(B$2) group by B$1, int64 sums wrapping mod 2^64, against numpy.
+    Thread-local aggregates emit per-thread partial groups (summed here);
+    the global one emits every group once."""
+    rng = np.random.default_rng(33 if global_ else 34)
+    rdt = np.dtype({"names": ["k", "g", "v"], "formats": [np.int32, np.int32, np.int64], "offsets": [0, 4, 8],
+                    "itemsize": 16})
+    sdt = np.dtype({"names": ["k", "p"], "formats": [np.int32, np.int32], "offsets": [0, 4], "itemsize": 8})
+    nR, nS, ng = 50_000, 400_000, 3000
+    R = np.empty(nR, rdt)
+    R["k"] = rng.permutation(nR).astype(np.int32)
+    R["g"] = rng.integers(0, ng, nR)
+    R["v"] = rng.integers(-2**62, 2**62, nR)
+    S = np.empty(nS, sdt)
+    S["k"] = rng.integers(0, nR + 2000, nS).astype(np.int32)
+    S["p"] = 0
+    hit = S["k"] < nR
+    pos = np.empty(nR, np.int64)
+    pos[R["k"]] = np.arange(nR)
+    m = pos[S["k"][hit]]
+    want = {}
+    sums = np.zeros(ng, np.uint64)
+    np.add.at(sums, R["g"][m], R["v"][m].view(np.uint64))
+    present = np.bincount(R["g"][m], minlength=ng) > 0
+    want = {int(g): int(np.int64(sums[g].view(np.int64))) for g in np.nonzero(present)[0]}
+    agg = 'type = "gpuaggregate_sum"; field = 0; sumfield = 1; hash = { fn = "modulo"; buckets = 4096; field = 0; };'
+    if global_:
+        agg += " global = true; threads = 3;"
+    plan = textwrap.dedent(f"""
+        path = "./"; buffsize = 65536;
+        treeroot = {{ name = "merge"; input = {{ name = "agg"; input = {{ name = "join";
+                      build = {{ name = "r"; }}; probe = {{ name = "s"; }}; }}; }}; }};
+        merge = {{ type = "merge"; threads = 3; }};
+        agg = {{ {agg} }};
+        join = {{ type = "gpuhashjoin"; hash = {{ fn = "modulo"; buckets = 4096; }};
+                  threadgroups = ( [ 0, 1, 2 ] ); tuplesperbucket = 4; allocpolicy = "local";
+                  buildjattr = 0; probejattr = 0; projection = ( "B$1", "B$2" );
+                  gpu = {{ resident = true; probebatch = 50000L; }}; }};
+        r = {{ type = "memscan"; table = "gb_r"; schema = ( "int", "int", "long" ); }};
+        s = {{ type = "memscan"; table = "gb_s"; schema = ( "int", "int" ); }};
+    """)
+    try:
+        P.register_table("gb_r", R)
+        P.register_table("gb_s", S)
+        out, meta = P.run(plan)
+        assert meta["schema"] == "int,long"
+        got = {}
+        for g, v in zip(out["c0"].tolist(), out["c1"].tolist()):
+            if global_:
+                assert g not in got  # every group once
+            got[g] = (got.get(g, 0) + v + 2**63) % 2**64 - 2**63
+        assert got == want
+    finally:
+        P.clear_tables()
+        P.clear_resident()
