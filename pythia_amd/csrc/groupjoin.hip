@@ -263,7 +263,7 @@ constexpr uint32_t kIR2 = 98304;                               // rows per aggre
 constexpr int kT = 1024;                                       // threads per workgroup (scatter, probe, agg)
 constexpr int kMinmaxBlocks = 256;
 constexpr int kRangeTile = 4096;                               // rows per LDS staging tile: key-range scatter
-constexpr int kGroupTile = 8192;                               // rows per LDS staging tile: group-range scatter
+constexpr int kGroupTile = 12288;                              // rows per LDS staging tile: group-range scatter
 
 struct Item {
     uint32_t part, r0, r1, pad;
