@@ -618,7 +618,7 @@ __global__ __launch_bounds__(kT) void k_phist(const Item* __restrict__ items, co
     const Item it = items[blockIdx.x];
     for (uint32_t i = threadIdx.x; i < P2pad; i += kT) h[i] = 0;
     build_slice(tab, dlk, dgid, bD[it.part], bD[it.part + 1], dup);
-    constexpr int U = 8;  // rows per thread per step, all loads issued before the lookups
+    constexpr int U = 16;  // rows per thread per step, all loads issued before the lookups
     for (uint32_t t0 = it.r0; t0 < it.r1; t0 += U * kT) {
         uint32_t lk[U];
 #pragma unroll
@@ -734,7 +734,7 @@ __global__ __launch_bounds__(kT) void k_gagg(const Item* __restrict__ items, con
         lc[i] = 0;
     }
     __syncthreads();
-    constexpr int U = 4;  // rows per thread per step, all loads issued before the LDS adds
+    constexpr int U = 8;  // rows per thread per step, all loads issued before the LDS adds
     for (uint32_t t0 = it.r0; t0 < it.r1; t0 += U * kT) {
         uint32_t g[U];
         double v[U];
