@@ -61,6 +61,11 @@ int pgpu_hbm_usage(pgpu_ctx* ctx, uint64_t* used, uint64_t* budget, uint64_t* pe
 int pgpu_h2d(pgpu_ctx* ctx, void* d_dst, const void* src, uint64_t bytes);
 int pgpu_d2h(pgpu_ctx* ctx, void* dst, const void* d_src, uint64_t bytes);
 int pgpu_sync(pgpu_ctx* ctx);
+/* Pinned (page-locked) host memory, outside the HBM budget: device copies
+ * into it are direct DMA.  Stands in for the page buffers the operators
+ * allocate with numaallocate_local (util/numaallocate.cpp:60-181). */
+int pgpu_host_alloc(pgpu_ctx* ctx, uint64_t bytes, void** p);
+int pgpu_host_free(pgpu_ctx* ctx, void* p);
 /* Roofline calibration (no reference counterpart): sustained HBM GB/s of a
  * streaming read and of a read + write copy over `bytes`, `iters` times. */
 int pgpu_membw(pgpu_ctx* ctx, uint64_t bytes, int iters, double* read_gbps, double* copy_gbps);
@@ -129,8 +134,10 @@ typedef struct pgpu_join_cfg {
     uint32_t join_variant;  /* LDS join shape (slots x threads, probe rows in flight  */
                             /* per thread): 0/6 = 4096 x 512 x 6, 1 = 4096 x 512 x 8,*/
                             /* 2 = 4096 x 256 x 32, 3 = 2048 x 256 x 16, 4 = 4096 x  */
-                            /* 512 x 16, 5 = 4096 x 512 x 8 at one workgroup per CU; */
-                            /* the result does not depend on it                     */
+                            /* 512 x 16, 5 = 4096 x 512 x 8 at one workgroup per CU, */
+                            /* 7 = 4096 x 512 x 6 with 2560-row pieces, 8 = 8192 x  */
+                            /* 1024 x 6 with 5120-row pieces; the result does not    */
+                            /* depend on it                                          */
 } pgpu_join_cfg;
 
 typedef struct pgpu_join_result {
@@ -363,6 +370,15 @@ int pgpu_hj_probe_device(pgpu_hj* h, uint32_t slot, const void* d_rows, uint64_t
 int pgpu_hj_output(pgpu_hj* h, uint32_t slot, const void** d_rows, uint64_t* n_rows);
 int pgpu_hj_stats_get(pgpu_hj* h, pgpu_hj_stats* out);
 
+/* One 4- or 8-byte field of n packed rows in device memory (row_bytes apart,
+ * at `offset`) copied into a dense device column d_out (aligned to the field
+ * width).  Stands in for reading a column through Schema::calcOffset /
+ * asInt / asLong / asDecimal (schema.inl:141-202) on every tuple: the
+ * operators transpose resident rows into the columns the partitioned
+ * kernels stream. */
+int pgpu_rows_column(pgpu_ctx* ctx, const void* d_rows, uint64_t n, uint32_t row_bytes, uint32_t offset,
+                     uint32_t bytes, void* d_out);
+
 /* ------------------------------------------------------------------ */
 /* Row-format hash aggregation (gpuaggregate_sum / gpuaggregate_count). */
 /* Replaces GenericAggregate::remember + AggregateSum/AggregateCount    */
@@ -470,6 +486,16 @@ int pgpu_groupjoin_fetch(pgpu_groupjoin* p, int32_t* keys, double* sums, uint64_
 #define PGPU_GJ_PARTITIONED 1
 #define PGPU_GJ_GENERAL 2
 int pgpu_groupjoin_path(pgpu_groupjoin* p, int* path);
+/* Result rows [first, first + n) of the last run as the aggregate over the
+ * join emits them (GenericAggregate's output: group column then the
+ * accumulator, genericaggregate.cpp:96-104): the int32 group key followed by
+ * the FP64 sum (PGPU_GJ_ROW_SUM, AggregateSum over a dec column,
+ * aggregatesum.cpp:46-49) or the int64 count (PGPU_GJ_ROW_COUNT,
+ * aggregatecount.cpp:37-42), packed into 12-byte rows, copied into host
+ * memory dst (by direct DMA when dst comes from pgpu_host_alloc). */
+#define PGPU_GJ_ROW_SUM 1
+#define PGPU_GJ_ROW_COUNT 2
+int pgpu_groupjoin_rows(pgpu_groupjoin* p, int acc, uint64_t first, uint64_t n, void* dst);
 int pgpu_groupjoin_timing(pgpu_groupjoin* p, int enable);
 int pgpu_groupjoin_times(pgpu_groupjoin* p, pgpu_kernel_time* out, int cap, int* n);
 

@@ -23,6 +23,13 @@ typedef struct pythia_result {
     char schema[256];    /* e.g. "long,dec"                           */
     double seconds;      /* scanStart .. scanStop wall time           */
     uint64_t getnext_calls;
+    /* How the GPU operators ran (no reference counterpart): the OR of the
+     * fused forms the plan's aggregates took over resident joins (1: join
+     * rows folded in HBM, 2: count / sum fused into the radix join, 4: group
+     * sum fused into the partitioned group join), and the join output bytes
+     * copied to host pages. */
+    uint32_t gpu_fused;
+    uint64_t join_d2h_bytes;
 } pythia_result;
 
 const char* pythia_last_error(void);

@@ -824,6 +824,22 @@ inline uint32_t pad64(uint64_t p) { return (uint32_t)((p + 63) & ~63ull); }
 
 }  // namespace
 
+// Output rows of an aggregate over the group join: the int32 group key then
+// an 8-byte accumulator (the FP64 sum, or the int64 count), packed into 12
+// bytes with no padding (schema.cpp:82-117), for result rows [first, first + n).
+__global__ __launch_bounds__(256) void k_gj_rows(const int32_t* __restrict__ okey, const double* __restrict__ osum,
+                                                 const uint64_t* __restrict__ ocnt, int count, uint64_t first,
+                                                 uint64_t n, uint32_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const uint64_t a = count ? ocnt[first + i] : (uint64_t)__double_as_longlong(osum[first + i]);
+        uint32_t* r = out + 3 * i;
+        r[0] = (uint32_t)okey[first + i];
+        r[1] = (uint32_t)a;
+        r[2] = (uint32_t)(a >> 32);
+    }
+}
+
 struct pgpu_groupjoin {
     pgpu_ctx* ctx = nullptr;
     uint64_t maxD = 0, maxF = 0, maxG = 0;
@@ -839,6 +855,7 @@ struct pgpu_groupjoin {
     int32_t* okey = nullptr;
     double* osum = nullptr;
     uint64_t* ocnt = nullptr;
+    uint32_t* orow = nullptr;  // packed (key, accumulator) rows for pgpu_groupjoin_rows
     // partitioned path (absent when the plan's fact rows reach 2^31)
     bool part_ok = false;
     int32_t* mm = nullptr;
@@ -1059,6 +1076,7 @@ int pgpu_groupjoin_create(pgpu_ctx* ctx, uint64_t max_dim, uint64_t max_fact, ui
     if (rc == PGPU_OK) rc = gj_alloc(p, (p->maxG + 1) * sizeof(int32_t), (void**)&p->okey);
     if (rc == PGPU_OK) rc = gj_alloc(p, (p->maxG + 1) * sizeof(double), (void**)&p->osum);
     if (rc == PGPU_OK) rc = gj_alloc(p, (p->maxG + 1) * sizeof(uint64_t), (void**)&p->ocnt);
+    if (rc == PGPU_OK) rc = gj_alloc(p, (p->maxG + 1) * 12, (void**)&p->orow);
     if (rc == PGPU_OK && hipHostMalloc((void**)&p->h_counters, kCtWords * sizeof(unsigned long long), 0) != hipSuccess)
         rc = fail(PGPU_ENOMEM, "hipHostMalloc failed");
     if (rc == PGPU_OK && max_fact < (1ull << 31)) rc = alloc_partitioned(p);
@@ -1135,6 +1153,24 @@ int pgpu_groupjoin_fetch(pgpu_groupjoin* p, int32_t* keys, double* sums, uint64_
     if (keys && n) PGPU_HIP(hipMemcpy(keys, p->okey, n * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (sums && n) PGPU_HIP(hipMemcpy(sums, p->osum, n * sizeof(double), hipMemcpyDeviceToHost));
     if (counts && n) PGPU_HIP(hipMemcpy(counts, p->ocnt, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return PGPU_OK;
+}
+
+int pgpu_groupjoin_rows(pgpu_groupjoin* p, int acc, uint64_t first, uint64_t n, void* dst) {
+    if (!p || (n && !dst)) return fail(PGPU_EINVAL, "pgpu_groupjoin_rows: null argument");
+    if (acc != PGPU_GJ_ROW_SUM && acc != PGPU_GJ_ROW_COUNT)
+        return fail(PGPU_EINVAL, "pgpu_groupjoin_rows: unknown accumulator %d", acc);
+    if (first > p->ngroups || n > p->ngroups - first)
+        return fail(PGPU_EINVAL, "pgpu_groupjoin_rows: rows [%llu, +%llu) past %llu groups", (unsigned long long)first,
+                    (unsigned long long)n, (unsigned long long)p->ngroups);
+    if (!n) return PGPU_OK;
+    PGPU_TRY(ctx_activate(p->ctx));
+    hipStream_t st = p->ctx->stream;
+    hipLaunchKernelGGL(k_gj_rows, dim3(grid_for(n)), dim3(256), 0, st, p->okey, p->osum, p->ocnt,
+                       acc == PGPU_GJ_ROW_COUNT ? 1 : 0, first, n, p->orow);
+    PGPU_LAUNCH_CHECK();
+    PGPU_HIP(hipMemcpyAsync(dst, p->orow, n * 12, hipMemcpyDeviceToHost, st));
+    PGPU_HIP(hipStreamSynchronize(st));
     return PGPU_OK;
 }
 

@@ -966,6 +966,8 @@ int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const 
             case 3: PGPU_JOIN(2048, 256, 1280, 16, 4); break;
             case 4: PGPU_JOIN(4096, 512, 2048, 16, 4); break;
             case 5: PGPU_JOIN(4096, 512, 2048, 8, 2); break;
+            case 7: PGPU_JOIN(4096, 512, 2560, 6, 4); break;   // ~2048 build rows per partition (13 bits at 2^24)
+            case 8: PGPU_JOIN(8192, 1024, 5120, 6, 4); break;  // ~4096 build rows per partition (12 bits at 2^24)
             default: PGPU_JOIN(4096, 512, 2048, 6, 4); break;  // no VGPR spills at 4 waves per SIMD
         }
 #undef PGPU_JOIN
@@ -1018,9 +1020,12 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
     if (cfg->radix_bits != 0 && (cfg->radix_bits < 2 || cfg->radix_bits > kMaxBits))
         return fail(PGPU_EINVAL, "radix_bits must be 0 or in [2, %d] (got %u)", kMaxBits, cfg->radix_bits);
     if (cfg->algo > PGPU_ALGO_GLOBAL) return fail(PGPU_EINVAL, "unknown join algorithm %u", cfg->algo);
-    if (cfg->join_variant > 6) return fail(PGPU_EINVAL, "unknown LDS join variant %u", cfg->join_variant);
-    if (max_build >= (1ull << 32) || max_probe >= (1ull << 40))
-        return fail(PGPU_EINVAL, "relation too large (build < 2^32, probe < 2^40 rows)");
+    if (cfg->join_variant > 8) return fail(PGPU_EINVAL, "unknown LDS join variant %u", cfg->join_variant);
+    // Histogram columns, per-bin totals and the per-(block, bin) prefixes of
+    // k_colscan are 32-bit: one final partition may hold every row of a side,
+    // so each side must stay under 2^32 rows.
+    if (max_build >= (1ull << 32) || max_probe >= (1ull << 32))
+        return fail(PGPU_EINVAL, "relation too large (build and probe must each hold fewer than 2^32 rows)");
     PGPU_TRY(ctx_activate(ctx));
     auto* p = new pgpu_join_plan();
     p->ctx = ctx;

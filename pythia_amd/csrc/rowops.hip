@@ -20,6 +20,7 @@
 // half full.  finish() compacts the ready slots into output rows.  Without a
 // GROUP BY the rows are summed per workgroup and then in one workgroup (no
 // same-address atomics).  Each aggregate runs on a stream of its own.
+#include <atomic>
 #include <algorithm>
 #include <cstring>
 
@@ -554,7 +555,11 @@ struct pgpu_hj {
         uint64_t nout = 0, pulled = 0;
     };
     std::vector<Slot> slots;
-    pgpu_hj_stats stats{};
+    // Updated by every worker probing its own slot, hence atomic.
+    struct {
+        std::atomic<uint64_t> build_rows{0}, probe_rows{0}, output_rows{0}, table_slots{0};
+        std::atomic<uint64_t> h2d_bytes{0}, d2h_bytes{0}, probe_reruns{0};
+    } stats;
 };
 
 using namespace pgpu;
@@ -852,7 +857,13 @@ int pgpu_hj_pull(pgpu_hj* h, uint32_t slot, void* dst, uint64_t cap_rows, uint64
 
 int pgpu_hj_stats_get(pgpu_hj* h, pgpu_hj_stats* out) {
     if (!h || !out) return fail(PGPU_EINVAL, "pgpu_hj_stats_get: null argument");
-    *out = h->stats;
+    out->build_rows = h->stats.build_rows.load();
+    out->probe_rows = h->stats.probe_rows.load();
+    out->output_rows = h->stats.output_rows.load();
+    out->table_slots = h->stats.table_slots.load();
+    out->h2d_bytes = h->stats.h2d_bytes.load();
+    out->d2h_bytes = h->stats.d2h_bytes.load();
+    out->probe_reruns = h->stats.probe_reruns.load();
     return PGPU_OK;
 }
 
@@ -994,33 +1005,18 @@ int pgpu_agg_destroy(pgpu_agg* g) {
 namespace {
 // Folds n device-resident rows into the table (caller holds g->mu; the
 // context is current).  `upload`: rows are on the host and go through d_in.
-int agg_fold(pgpu_agg* g, const void* rows, uint64_t n, bool upload) {
+//
+// With a GROUP BY the rows are folded in pieces of at most cap/2 rows, and
+// before each piece the table doubles while the groups already seen fill
+// more than a quarter of it: even if every row of a piece were a new group
+// the table stays at most three quarters full (an insert always finds a
+// free slot), while its size follows the distinct groups, not the rows
+// appended.  Pieces are at least kAggMinPiece rows (the table is allowed
+// that much headroom) so that few groups over many rows take few launches.
+constexpr uint64_t kAggMinPiece = 1ull << 18;
+
+int agg_fold_piece(pgpu_agg* g, const char* src, uint64_t n) {
     hipStream_t st = g->st;
-    // Keep the table at most half full even if every row is a new group
-    // (no GROUP BY: one group).
-    uint64_t need = 1;
-    if (g->a.nkeys) {
-        PGPU_HIP(hipMemcpyAsync(g->h_ngroups, g->d_ngroups, 8, hipMemcpyDeviceToHost, st));
-        PGPU_HIP(hipStreamSynchronize(st));
-        need = *g->h_ngroups + n;
-    }
-    if (need > g->cap / 2) {
-        uint64_t nc = g->cap;
-        while (need > nc / 2) nc *= 2;
-        // grow lazily: never beyond what the distinct keys seen so far can
-        // require, bounded by rows appended
-        PGPU_TRY(agg_grow(g, nc));
-    }
-    const char* src = (const char*)rows;
-    if (upload) {
-        PGPU_TRY(slot_reserve(g->ctx, &g->d_in, &g->in_cap, n * g->a.row_bytes));
-        PGPU_TRY(stage_h2d(g->ctx, g->hin, st, g->d_in, (const char*)rows, n * g->a.row_bytes));
-        src = g->d_in;
-    } else {
-        // rows a kernel on the context's stream may still be writing
-        PGPU_HIP(hipEventRecord(g->ctx_done, g->ctx->stream));
-        PGPU_HIP(hipStreamWaitEvent(st, g->ctx_done, 0));
-    }
     if (g->a.nkeys == 0) {
         const unsigned np = (unsigned)std::min<uint64_t>(ceil_div(n, 256 * kAggUnroll), kAggPartials);
         hipLaunchKernelGGL(k_agg_partial, dim3(np), dim3(256), 0, st, g->a, src, n, g->d_partial);
@@ -1031,6 +1027,38 @@ int agg_fold(pgpu_agg* g, const void* rows, uint64_t n, bool upload) {
         hipLaunchKernelGGL(k_agg_append, dim3(grid_for(n)), dim3(256), 0, st, g->a, src, n, g->d_table,
                            g->cap - 1, g->d_ngroups);
     PGPU_LAUNCH_CHECK();
+    return PGPU_OK;
+}
+
+int agg_fold(pgpu_agg* g, const void* rows, uint64_t n, bool upload) {
+    hipStream_t st = g->st;
+    const char* src = (const char*)rows;
+    if (upload) {
+        PGPU_TRY(slot_reserve(g->ctx, &g->d_in, &g->in_cap, n * g->a.row_bytes));
+        PGPU_TRY(stage_h2d(g->ctx, g->hin, st, g->d_in, (const char*)rows, n * g->a.row_bytes));
+        src = g->d_in;
+    } else {
+        // rows a kernel on the context's stream may still be writing
+        PGPU_HIP(hipEventRecord(g->ctx_done, g->ctx->stream));
+        PGPU_HIP(hipStreamWaitEvent(st, g->ctx_done, 0));
+    }
+    if (g->a.nkeys == 0) {  // one group: the slot exists after the first fold
+        PGPU_TRY(agg_fold_piece(g, src, n));
+        PGPU_HIP(hipStreamSynchronize(st));
+        return PGPU_OK;
+    }
+    for (uint64_t done = 0; done < n;) {
+        PGPU_HIP(hipMemcpyAsync(g->h_ngroups, g->d_ngroups, 8, hipMemcpyDeviceToHost, st));
+        PGPU_HIP(hipStreamSynchronize(st));
+        const uint64_t ng = *g->h_ngroups;
+        const uint64_t want = std::min(n - done, kAggMinPiece);
+        uint64_t nc = g->cap;
+        while (ng > nc / 4 || nc / 2 < want) nc *= 2;
+        if (nc != g->cap) PGPU_TRY(agg_grow(g, nc));
+        const uint64_t piece = std::min(n - done, g->cap / 2);
+        PGPU_TRY(agg_fold_piece(g, src + done * g->a.row_bytes, piece));
+        done += piece;
+    }
     PGPU_HIP(hipStreamSynchronize(st));
     return PGPU_OK;
 }
@@ -1091,6 +1119,42 @@ int pgpu_agg_pull(pgpu_agg* g, uint64_t first, uint64_t n, void* dst) {
     PGPU_HIP(hipMemcpyAsync(dst, g->d_out + first * g->out_row_bytes, n * g->out_row_bytes, hipMemcpyDeviceToHost,
                             g->st));
     PGPU_HIP(hipStreamSynchronize(g->st));
+    return PGPU_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== columns
+namespace {
+// One 4- or 8-byte field of every packed row into a dense column.
+__global__ __launch_bounds__(256) void k_column(const char* __restrict__ rows, uint64_t n, uint32_t row_bytes,
+                                                uint32_t off, uint32_t bytes, char* __restrict__ out) {
+    const bool a4 = aligned4((uintptr_t)rows | row_bytes | off);
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const char* p = rows + i * row_bytes + off;
+        if (bytes == 4)
+            ((uint32_t*)out)[i] = ld_u32(p, a4);
+        else
+            ((uint64_t*)out)[i] = ld_u64(p, a4);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int pgpu_rows_column(pgpu_ctx* ctx, const void* d_rows, uint64_t n, uint32_t row_bytes, uint32_t offset,
+                     uint32_t bytes, void* d_out) {
+    if (!ctx || (n && (!d_rows || !d_out))) return fail(PGPU_EINVAL, "pgpu_rows_column: null argument");
+    if (bytes != 4 && bytes != 8) return fail(PGPU_EUNSUPPORTED, "pgpu_rows_column: fields of 4 or 8 bytes");
+    if ((uint64_t)offset + bytes > row_bytes) return fail(PGPU_EINVAL, "pgpu_rows_column: field outside the row");
+    if ((uintptr_t)d_out % bytes) return fail(PGPU_EINVAL, "pgpu_rows_column: output not aligned to the field");
+    if (!n) return PGPU_OK;
+    PGPU_TRY(ctx_activate(ctx));
+    hipLaunchKernelGGL(k_column, dim3(grid_for(n)), dim3(256), 0, ctx->stream, (const char*)d_rows, n, row_bytes,
+                       offset, bytes, (char*)d_out);
+    PGPU_LAUNCH_CHECK();
+    PGPU_HIP(hipStreamSynchronize(ctx->stream));
     return PGPU_OK;
 }
 

@@ -376,6 +376,22 @@ int pgpu_d2h(pgpu_ctx* ctx, void* dst, const void* d_src, uint64_t bytes) {
     return PGPU_OK;
 }
 
+int pgpu_host_alloc(pgpu_ctx* ctx, uint64_t bytes, void** p) {
+    if (!ctx || !p) return fail(PGPU_EINVAL, "pgpu_host_alloc: null argument");
+    *p = nullptr;
+    PGPU_TRY(ctx_activate(ctx));
+    PGPU_HIP(hipHostMalloc(p, std::max<uint64_t>(bytes, 1), hipHostMallocDefault));
+    return PGPU_OK;
+}
+
+int pgpu_host_free(pgpu_ctx* ctx, void* p) {
+    if (!ctx) return fail(PGPU_EINVAL, "pgpu_host_free: null ctx");
+    if (!p) return PGPU_OK;
+    PGPU_TRY(ctx_activate(ctx));
+    PGPU_HIP(hipHostFree(p));
+    return PGPU_OK;
+}
+
 int pgpu_sync(pgpu_ctx* ctx) {
     if (!ctx) return fail(PGPU_EINVAL, "pgpu_sync: null ctx");
     PGPU_TRY(ctx_activate(ctx));

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <set>
 #include <sstream>
 
@@ -293,6 +294,7 @@ Operator::GetNextResultT GpuHashJoinOp::getNext(unsigned short tid) {
             uint64_t got = 0;
             check(pgpu_hj_pull(t, tid, out->data() + used * ts, std::min(cap - used, st.pending), &got),
                   "pgpu_hj_pull");
+            pulled_bytes_ += got * ts;
             out->setUsed((used + got) * ts);
             st.pending -= got;
             if (out->getNumTuples() == cap) return {Ready, out};
@@ -311,6 +313,7 @@ Operator::GetNextResultT GpuHashJoinOp::getNext(unsigned short tid) {
             uint64_t got = 0;
             check(pgpu_hj_pull(t, tid, out->data() + used * ts, std::min(cap - used, st.pending), &got),
                   "pgpu_hj_pull");
+            pulled_bytes_ += got * ts;
             out->setUsed((used + got) * ts);
             st.pending -= got;
             if (out->getNumTuples() == cap) return {Ready, out};
@@ -351,13 +354,22 @@ Operator::ResultCode GpuHashJoinOp::scanStartResident(unsigned short tid, Page* 
             std::string key;
             b->sharedRows(&rows, &n, &key);
             const unsigned bts = buildOp->getOutSchema().getTupleSize();
-            const void* d_build = pythia::resident_rows(knobs.device, knobs.hbmbudget, key, rows, n * bts);
-            check(pgpu_hj_build_device(tables[g], d_build, n), "pgpu_hj_build_device");
+            d_build_ = (const char*)pythia::resident_rows(knobs.device, knobs.hbmbudget, key, rows, n * bts);
+            nbuild_rows_ = n;
+            build_key_ = key;
+            // a fused count/sum or group consumer runs the partitioned kernels
+            // over columns instead: no row-format table
+            if (fused_ != FuseCountSum && fused_ != FuseGroup)
+                check(pgpu_hj_build_device(tables[g], d_build_, n), "pgpu_hj_build_device");
             p->sharedRows(&rows, &n, &key);
             const unsigned pts = probeOp->getOutSchema().getTupleSize();
             d_probe_ = (const char*)pythia::resident_rows(knobs.device, knobs.hbmbudget, key, rows, n * pts);
             nprobe_rows_ = n;
+            probe_key_ = key;
             probe_cursor_ = 0;
+            std::lock_guard<std::mutex> lk(fuse_mu_);
+            fuse_done_ = false;
+            fuse_error_.clear();
         } catch (...) {
             fail_group(g);
             throw;
@@ -400,6 +412,142 @@ bool GpuHashJoinOp::nextDeviceBatch(unsigned short tid, const void** d_rows, uin
     return true;
 }
 
+static bool shared_size(Operator* op, uint64_t* n) {
+    auto* scan = dynamic_cast<ParallelScanOp*>(op);
+    const char* rows;
+    std::string key;
+    return scan && scan->sharedRows(&rows, n, &key);
+}
+
+bool GpuHashJoinOp::canFuseCountSum(int outcol) const {
+    if (!resident_) return false;
+    Schema& bs = buildOp->getOutSchema();
+    Schema& ps = probeOp->getOutSchema();
+    const unsigned kb = bs.getColumnWidth(joinattr1);
+    if ((kb != 4 && kb != 8) || ps.getColumnWidth(joinattr2) != kb) return false;
+    uint64_t nb = 0, np = 0;  // the radix plan takes fewer than 2^32 rows a side
+    if (!shared_size(buildOp, &nb) || !shared_size(probeOp, &np) || nb >= (1ull << 32) || np >= (1ull << 32))
+        return false;
+    if (outcol < 0) return true;
+    if ((size_t)outcol >= projection.size()) return false;
+    const JoinPrjT& c = projection[outcol];
+    Schema& s = c.first == BuildSide ? bs : ps;
+    const ColumnType t = s.getColumnType(c.second);
+    return s.getColumnWidth(c.second) == kb && (t == CT_INTEGER || t == CT_LONG);
+}
+
+bool GpuHashJoinOp::canFuseGroup(int groupcol, int sumcol) const {
+    if (!resident_ || groupcol < 0 || sumcol < 0 || (size_t)groupcol >= projection.size() ||
+        (size_t)sumcol >= projection.size())
+        return false;
+    Schema& bs = buildOp->getOutSchema();
+    Schema& ps = probeOp->getOutSchema();
+    if (bs.getColumnType(joinattr1) != CT_INTEGER || ps.getColumnType(joinattr2) != CT_INTEGER) return false;
+    const JoinPrjT& g = projection[groupcol];
+    const JoinPrjT& v = projection[sumcol];
+    if (g.first != BuildSide || bs.getColumnType(g.second) != CT_INTEGER) return false;
+    if (v.first != ProbeSide || ps.getColumnType(v.second) != CT_DECIMAL) return false;
+    uint64_t nb = 0, np = 0;  // the group join's dimension side stays under 2^31 rows
+    return shared_size(buildOp, &nb) && shared_size(probeOp, &np) && nb < (1ull << 31);
+}
+
+const void* GpuHashJoinOp::residentColumn(bool build, unsigned col) {
+    Schema& s = build ? buildOp->getOutSchema() : probeOp->getOutSchema();
+    return pythia::resident_column(knobs.device, knobs.hbmbudget, build ? build_key_ : probe_key_,
+                                   build ? d_build_ : d_probe_, build ? nbuild_rows_ : nprobe_rows_,
+                                   s.getTupleSize(), s.getColumnOffset(col), s.getColumnWidth(col));
+}
+
+// The queryhashjoin plan's join -> aggregate_count / aggregate_sum with no
+// GROUP BY (operators/join.cpp:503-562 feeding aggregatesum.cpp:81-107 /
+// aggregatecount.cpp:44-52) as one 2-pass radix join with a fused count +
+// sum: the summed column's values travel through the partitions as the
+// payload of their side; an output row's value is counted once per match.
+GpuHashJoinOp::CountSum GpuHashJoinOp::countSum(int outcol, bool* first) {
+    std::lock_guard<std::mutex> lk(fuse_mu_);
+    *first = !fuse_done_ && fuse_error_.empty();
+    if (!fuse_error_.empty()) throw QueryExecutionError(fuse_error_);
+    if (fuse_done_) return fuse_cs_;
+    fuse_cs_ = CountSum();
+    try {
+        if (nbuild_rows_ && nprobe_rows_) {
+            const void* rk = residentColumn(true, joinattr1);
+            const void* sk = residentColumn(false, joinattr2);
+            const void* rp = rk;  // a count still streams a build payload: the key itself
+            const void* sp = nullptr;
+            bool probe_side = false;
+            if (outcol >= 0) {
+                const JoinPrjT& c = projection[outcol];
+                if (c.first == BuildSide)
+                    rp = residentColumn(true, c.second);
+                else {
+                    sp = residentColumn(false, c.second);
+                    probe_side = true;
+                }
+            }
+            pgpu_join_cfg cfg{};
+            cfg.key_bytes = buildOp->getOutSchema().getColumnWidth(joinattr1);
+            cfg.probe_payload = probe_side ? 1 : 0;
+            cfg.algo = PGPU_ALGO_AUTO;
+            pgpu_join_result r{};
+            pythia::with_join_plan(knobs.device, knobs.hbmbudget, cfg, nbuild_rows_, nprobe_rows_,
+                                   [&](pgpu_join_plan* plan) {
+                                       check(pgpu_join_count_sum(plan, rk, rp, nbuild_rows_, sk, sp, nprobe_rows_, &r),
+                                             "pgpu_join_count_sum");
+                                   });
+            fuse_cs_.matches = r.matches;
+            fuse_cs_.sum = probe_side ? r.sum_probe : r.sum_build;
+        }
+    } catch (const std::exception& e) {
+        fuse_error_ = e.what();
+        throw;
+    }
+    fuse_done_ = true;
+    return fuse_cs_;
+}
+
+// hashjoin(build D, probe F) -> aggregate_sum(field = a D column, sumfield =
+// an F dec column) (operators/join.cpp:503-562 feeding
+// genericaggregate.cpp:240-282) as the partitioned group join; its result
+// rows are the aggregate's output rows.
+const char* GpuHashJoinOp::groupRows(int groupcol, int sumcol, uint64_t* n) {
+    std::lock_guard<std::mutex> lk(fuse_mu_);
+    if (!fuse_error_.empty()) throw QueryExecutionError(fuse_error_);
+    if (fuse_done_) {
+        *n = fuse_nrows_;
+        return fuse_rows_;
+    }
+    try {
+        fuse_nrows_ = 0;
+        if (nbuild_rows_ && nprobe_rows_) {
+            const auto* dk = (const int32_t*)residentColumn(true, joinattr1);
+            const auto* dg = (const int32_t*)residentColumn(true, projection[groupcol].second);
+            const auto* fk = (const int32_t*)residentColumn(false, joinattr2);
+            const auto* fv = (const double*)residentColumn(false, projection[sumcol].second);
+            pythia::with_groupjoin(knobs.device, knobs.hbmbudget, nbuild_rows_, nprobe_rows_, nbuild_rows_,
+                                   [&](pgpu_groupjoin* gj) {
+                                       uint64_t ng = 0;
+                                       check(pgpu_groupjoin_run(gj, dk, dg, nbuild_rows_, fk, fv, nprobe_rows_, &ng,
+                                                                nullptr),
+                                             "pgpu_groupjoin_run");
+                                       if (ng && !fuse_rows_)
+                                           fuse_rows_ = (char*)pythia::acquire_pinned(knobs.device, knobs.hbmbudget,
+                                                                                     ng * 12);
+                                       if (ng)
+                                           check(pgpu_groupjoin_rows(gj, PGPU_GJ_ROW_SUM, 0, ng, fuse_rows_),
+                                                 "pgpu_groupjoin_rows");
+                                       fuse_nrows_ = ng;
+                                   });
+        }
+    } catch (const std::exception& e) {
+        fuse_error_ = e.what();
+        throw;
+    }
+    fuse_done_ = true;
+    *n = fuse_nrows_;
+    return fuse_rows_;
+}
+
 void GpuHashJoinOp::threadClose(unsigned short tid) {
     delete state[tid].out;
     state[tid].out = nullptr;
@@ -418,6 +566,8 @@ void GpuHashJoinOp::destroy() {
     }
     for (auto* b : barriers) delete b;
     barriers.clear();
+    pythia::release_pinned(fuse_rows_);  // consumers have emitted every row by now
+    fuse_rows_ = nullptr;
     if (ctx) pythia::release_device(ctx);
     ctx = nullptr;
 }
@@ -469,6 +619,18 @@ void GpuAggregate::init(libconfig::Config& root, libconfig::Setting& cfg) {
     }
     knobs.read(cfg);
     state.assign(MAX_THREADS, ThreadState());
+    // Directly over a resident gpuhashjoin on the same device the join and
+    // this aggregate run as one device computation (see FuseKind).
+    auto* join = dynamic_cast<GpuHashJoinOp*>(nextOp);
+    if (join && join->residentMode() && join->device() == knobs.device) {
+        join_ = join;
+        fuse_ = GpuHashJoinOp::FuseRows;
+        if (aggfields.empty() && join->canFuseCountSum(sumField()))
+            fuse_ = GpuHashJoinOp::FuseCountSum;
+        else if (aggfields.size() == 1 && kind() == PGPU_AGG_SUM_F64 && join->canFuseGroup(aggfields[0], sumField()))
+            fuse_ = GpuHashJoinOp::FuseGroup;
+        join->setFused(fuse_);
+    }
 }
 
 pgpu_agg* GpuAggregate::newTable() {
@@ -496,6 +658,7 @@ void GpuAggregate::threadInit(unsigned short tid) {
     if (buffsize < schema.getTupleSize()) throw InvalidParameter("buffsize smaller than one output tuple");
     state[tid] = ThreadState();
     state[tid].out = new Page(buffsize, schema.getTupleSize());
+    if (fuse_ == GpuHashJoinOp::FuseCountSum || fuse_ == GpuHashJoinOp::FuseGroup) return;  // no group table
     if (!global_) {
         state[tid].agg = newTable();
         return;
@@ -510,7 +673,48 @@ void GpuAggregate::threadInit(unsigned short tid) {
     if (!barrier_.Arrive()) throw QueryExecutionError("gpuaggregate: a thread failed during threadInit");
 }
 
+// The fused forms: the join below computes the whole query once (its first
+// caller runs it) and each thread emits its share of the result: the one
+// row of a count / sum without GROUP BY comes from the thread that ran the
+// join (nothing when no tuple joined: no group exists), and the groups of
+// the group join are split evenly over the join's threads, each emitted
+// once (Global mode's rule, genericaggregate.cpp:324-412; in ThreadLocal
+// mode each group is then one thread's partial).
+Operator::ResultCode GpuAggregate::scanStartFused(unsigned short tid, Page* idx, Schema& idxschema) {
+    ThreadState& st = state[tid];
+    st.row.clear();
+    st.view.reset();
+    st.rows = nullptr;
+    st.first = st.end = st.pos = 0;
+    ResultCode rc = nextOp->scanStart(tid, idx, idxschema);
+    if (rc != Ready) return rc;  // an empty probe side joins nothing
+    if (fuse_ == GpuHashJoinOp::FuseCountSum) {
+        bool first = false;
+        const GpuHashJoinOp::CountSum cs = join_->countSum(sumField(), &first);
+        if (first && cs.matches) {
+            st.row.resize(schema.getTupleSize());
+            if (kind() == PGPU_AGG_SUM_I32) {
+                const int32_t v = (int32_t)(uint32_t)cs.sum;  // the CtInt accumulator wraps mod 2^32
+                memcpy(st.row.data(), &v, 4);
+            } else {
+                const int64_t v = kind() == PGPU_AGG_COUNT ? (int64_t)cs.matches : (int64_t)cs.sum;
+                memcpy(st.row.data(), &v, 8);
+            }
+        }
+    } else {
+        uint64_t n = 0;
+        st.rows = join_->groupRows(aggfields[0], sumField(), &n);
+        const uint64_t T = join_->groupThreads(), j = join_->groupPosition(tid);
+        st.first = j * n / T;
+        st.end = (j + 1) * n / T;
+        st.pos = st.first;
+    }
+    return nextOp->scanStop(tid);
+}
+
 Operator::ResultCode GpuAggregate::scanStart(unsigned short tid, Page* idx, Schema& idxschema) {
+    if (fuse_ == GpuHashJoinOp::FuseCountSum || fuse_ == GpuHashJoinOp::FuseGroup)
+        return scanStartFused(tid, idx, idxschema);
     ThreadState& st = state[tid];
     ResultCode rc = nextOp->scanStart(tid, idx, idxschema);
     if (rc != Ready) {
@@ -523,13 +727,12 @@ Operator::ResultCode GpuAggregate::scanStart(unsigned short tid, Page* idx, Sche
     GetNextResultT r{Ready, nullptr};
     // Over a resident gpuhashjoin on the same device the join's output rows
     // are folded where they are, in HBM (no pages cross the host link).
-    auto* join = dynamic_cast<GpuHashJoinOp*>(nextOp);
-    const bool fused = join && join->residentMode() && join->device() == knobs.device;
+    const bool fused = fuse_ == GpuHashJoinOp::FuseRows;
     try {
         if (fused) {
             const void* d = nullptr;
             uint64_t n = 0;
-            while (join->nextDeviceBatch(tid, &d, &n))
+            while (join_->nextDeviceBatch(tid, &d, &n))
                 if (n) check(pgpu_agg_append_device(t, d, n), "pgpu_agg_append_device");
             r.first = Finished;
         }
@@ -576,6 +779,23 @@ Operator::GetNextResultT GpuAggregate::getNext(unsigned short tid) {
     ThreadState& st = state[tid];
     Page* out = st.out;
     out->clear();
+    if (fuse_ == GpuHashJoinOp::FuseCountSum) {
+        if (!st.row.empty()) {
+            memcpy(out->data(), st.row.data(), st.row.size());
+            out->setUsed(st.row.size());
+            st.row.clear();
+        }
+        return {Finished, out};
+    }
+    if (fuse_ == GpuHashJoinOp::FuseGroup) {
+        // overlay pages over the join's pinned result rows (valid until destroy)
+        if (st.pos >= st.end) return {Finished, out};
+        const unsigned ts = schema.getTupleSize();
+        const uint64_t k = std::min<uint64_t>(out->capacity() / ts, st.end - st.pos);
+        st.view.reset(new Page((void*)(st.rows + st.pos * ts), k * ts, nullptr, ts));
+        st.pos += k;
+        return {st.pos >= st.end ? Finished : Ready, st.view.get()};
+    }
     if (st.pos >= st.end) return {Finished, out};
     const unsigned ts = schema.getTupleSize();
     const uint64_t k = std::min<uint64_t>(out->capacity() / ts, st.end - st.pos);
@@ -588,6 +808,8 @@ Operator::GetNextResultT GpuAggregate::getNext(unsigned short tid) {
 void GpuAggregate::threadClose(unsigned short tid) {
     delete state[tid].out;
     state[tid].out = nullptr;
+    state[tid].view.reset();
+    if (fuse_ == GpuHashJoinOp::FuseCountSum || fuse_ == GpuHashJoinOp::FuseGroup) return;
     if (!global_) {
         pgpu_agg_destroy(state[tid].agg);
         state[tid].agg = nullptr;
@@ -625,6 +847,7 @@ Schema GpuAggregateSum::foldinit(libconfig::Setting& node) {
         default: throw InvalidParameter("aggregate_sum sums int, long or dec columns");
     }
     valoff_ = in.getColumnOffset(sumfield);
+    sumfield_ = (int)sumfield;
     s.add(in.getColumnType(sumfield));
     return s;
 }

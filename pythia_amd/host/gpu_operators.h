@@ -14,6 +14,7 @@
 #include <atomic>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "operators.h"
@@ -58,6 +59,39 @@ class GpuHashJoinOp : public DualInputOp {
     // place of getNext.
     bool nextDeviceBatch(unsigned short tid, const void** d_rows, uint64_t* n);
 
+    // Fused consumers.  An aggregate directly above a resident join picks
+    // one of these during its init (after the join's); the join then builds
+    // no row-format table, and the aggregate calls the matching method below
+    // between the join's scanStart and scanStop instead of getNext.
+    //   FuseRows:     join output rows handed over in HBM (nextDeviceBatch)
+    //   FuseCountSum: no GROUP BY, count or sum of one projected integer
+    //                 column of the key's width: the 2-pass radix join with
+    //                 its fused count + sum over dense device columns
+    //   FuseGroup:    sum of a projected dec probe column grouped by one
+    //                 projected int build column, int keys: the partitioned
+    //                 group join
+    enum FuseKind { FuseNone = 0, FuseRows = 1, FuseCountSum = 2, FuseGroup = 4 };
+    bool canFuseCountSum(int outcol) const;              // outcol < 0: count only
+    bool canFuseGroup(int groupcol, int sumcol) const;
+    void setFused(FuseKind k) { fused_ = k; }
+    FuseKind fused() const { return fused_; }
+    struct CountSum {
+        uint64_t matches = 0;
+        uint64_t sum = 0;  // modulo 2^64 (the low 32 bits for an int column)
+    };
+    // Runs the join once per query (the caller that runs it gets *first =
+    // true); every caller receives the totals.
+    CountSum countSum(int outcol, bool* first);
+    // Runs the group join once per query; every caller receives the result
+    // rows (int group key, FP64 sum: 12 bytes each) in pinned host memory
+    // owned by the join until destroy().
+    const char* groupRows(int groupcol, int sumcol, uint64_t* n);
+    // Threads of the (single, in resident mode) thread group and tid's place in it.
+    unsigned short groupThreads() const { return groupsize.empty() ? 1 : groupsize[0]; }
+    unsigned short groupPosition(unsigned short tid) const { return threadposingrp[tid]; }
+    // Join output bytes copied to the host by pgpu_hj_pull during this query.
+    uint64_t pulledBytes() const { return pulled_bytes_; }
+
   private:
     struct ThreadState {
         Page* out = nullptr;
@@ -87,6 +121,19 @@ class GpuHashJoinOp : public DualInputOp {
     std::atomic<uint64_t> probe_cursor_{0};
     ResultCode scanStartResident(unsigned short tid, Page* idx, Schema& idxschema);
     bool probeNextResident(unsigned short tid, uint64_t* nout);
+    const void* residentColumn(bool build, unsigned col);
+    const char* d_build_ = nullptr;
+    uint64_t nbuild_rows_ = 0;
+    std::string build_key_, probe_key_;
+    // fused consumers
+    FuseKind fused_ = FuseNone;
+    std::mutex fuse_mu_;
+    bool fuse_done_ = false;  // reset by the leader at scanStart
+    std::string fuse_error_;  // a failed fused run: every caller throws it
+    CountSum fuse_cs_;
+    char* fuse_rows_ = nullptr;  // pinned, from the registry's pool
+    uint64_t fuse_nrows_ = 0;
+    std::atomic<uint64_t> pulled_bytes_{0};
 };
 
 class GpuAggregate : public SingleInputOp {
@@ -107,8 +154,13 @@ class GpuAggregate : public SingleInputOp {
     virtual Schema foldinit(libconfig::Setting& node) = 0;
     virtual uint32_t kind() const = 0;
     virtual uint32_t valOffset() const { return 0; }
+    virtual int sumField() const { return -1; }  // the summed input column (-1: a count)
 
     std::vector<unsigned short> aggfields;
+
+  public:
+    // The fused form chosen at init over a resident gpuhashjoin (FuseNone otherwise).
+    GpuHashJoinOp::FuseKind fused() const { return fuse_; }
 
   private:
     struct ThreadState {
@@ -116,7 +168,13 @@ class GpuAggregate : public SingleInputOp {
         pgpu_agg* agg = nullptr;  // ThreadLocal table
         uint64_t first = 0, end = 0, pos = 0;
         bool done = false;
+        std::vector<char> row;      // FuseCountSum: the one output row, if this thread emits it
+        const char* rows = nullptr; // FuseGroup: the join's result rows
+        std::shared_ptr<Page> view; // FuseGroup: overlay page over `rows`
     };
+    GpuHashJoinOp::FuseKind fuse_ = GpuHashJoinOp::FuseNone;
+    GpuHashJoinOp* join_ = nullptr;  // the join below, when fused
+    ResultCode scanStartFused(unsigned short tid, Page* idx, Schema& idxschema);
     pgpu_agg* newTable();
     bool global_ = false;
     int threads_ = 0;
@@ -136,9 +194,11 @@ class GpuAggregateSum : public GpuAggregate {
     Schema foldinit(libconfig::Setting& node) override;
     uint32_t kind() const override { return kind_; }
     uint32_t valOffset() const override { return valoff_; }
+    int sumField() const override { return sumfield_; }
 
   private:
     uint32_t kind_ = 0, valoff_ = 0;
+    int sumfield_ = -1;
 };
 
 class GpuAggregateCount : public GpuAggregate {

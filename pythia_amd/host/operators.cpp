@@ -419,6 +419,13 @@ Operator::ResultCode MergeOp::scanStop(unsigned short) {
     return rc;
 }
 
+std::string MergeOp::firstError() {
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto& wk : workers_)
+        if (!wk.error.empty()) return wk.error;
+    return std::string();
+}
+
 void MergeOp::threadClose(unsigned short) {
     for (int w = 0; w < nthreads_; ++w) command(w, DoThreadClose);
     for (int w = 0; w < nthreads_; ++w) {

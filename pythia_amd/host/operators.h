@@ -269,6 +269,8 @@ class MergeOp : public SingleInputOp {
     ResultCode scanStop(unsigned short tid) override;
     void threadClose(unsigned short tid) override;
     int threads() const { return nthreads_; }
+    // The message of the first worker that failed (empty if none did).
+    std::string firstError();
 
   private:
     enum Cmd { None, DoScanStart, DoGetNext, DoScanStop, DoThreadClose, Exit };

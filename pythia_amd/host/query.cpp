@@ -65,6 +65,29 @@ void constructsubtree(libconfig::Config& cfg, libconfig::Setting& node, Operator
     (*root)->init(cfg, cfg.lookup(name));
 }
 
+// Fused forms taken and join output bytes pulled to the host, over the tree.
+void gpu_report(Operator* op, uint32_t* fused, uint64_t* d2h) {
+    if (!op) return;
+    if (auto* a = dynamic_cast<GpuAggregate*>(op)) *fused |= (uint32_t)a->fused();
+    if (auto* j = dynamic_cast<GpuHashJoinOp*>(op)) *d2h += j->pulledBytes();
+    if (auto* d = dynamic_cast<DualInputOp*>(op)) {
+        gpu_report(d->buildOp, fused, d2h);
+        gpu_report(d->probeOp, fused, d2h);
+    } else if (auto* s1 = dynamic_cast<SingleInputOp*>(op)) {
+        gpu_report(s1->nextOp, fused, d2h);
+    }
+}
+
+// "<what>", followed by the failing merge worker's message when there is one.
+std::string failure(Operator* root, const char* what) {
+    std::string m = what;
+    if (auto* merge = dynamic_cast<MergeOp*>(root)) {
+        const std::string e = merge->firstError();
+        if (!e.empty()) m += ": " + e;
+    }
+    return m;
+}
+
 thread_local std::string g_err;
 
 int set_err(const std::string& m) {
@@ -134,19 +157,20 @@ int pythia_run_query(const char* plan_text, pythia_result* out) {
         out->row_bytes = q.getOutSchema().getTupleSize();
         q.threadInit();
         auto t0 = std::chrono::steady_clock::now();
-        if (q.scanStart() == Operator::Error) throw QueryExecutionError("scanStart returned Error");
+        if (q.scanStart() == Operator::Error) throw QueryExecutionError(failure(q.tree, "scanStart returned Error"));
         Operator::GetNextResultT r{Operator::Ready, nullptr};
         uint64_t calls = 0;
         while (r.first == Operator::Ready) {
             r = q.getNext();
             ++calls;
-            if (r.first == Operator::Error) throw QueryExecutionError("getNext returned Error");
+            if (r.first == Operator::Error) throw QueryExecutionError(failure(q.tree, "getNext returned Error"));
             rows.append(r.second->data(), r.second->getUsedSpace());
         }
-        if (q.scanStop() == Operator::Error) throw QueryExecutionError("scanStop returned Error");
+        if (q.scanStop() == Operator::Error) throw QueryExecutionError(failure(q.tree, "scanStop returned Error"));
         auto t1 = std::chrono::steady_clock::now();
         out->seconds = std::chrono::duration<double>(t1 - t0).count();
         out->getnext_calls = calls;
+        gpu_report(q.tree, &out->gpu_fused, &out->join_d2h_bytes);
         q.threadClose();
     } catch (const std::exception& e) {
         rc = set_err(e.what());

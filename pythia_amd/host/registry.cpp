@@ -2,7 +2,9 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <tuple>
 
 #include "operators.h"
 
@@ -63,9 +65,20 @@ void check(int rc, const char* what) {
 pgpu_ctx* acquire_device(int device, uint64_t hbm_budget) {
     std::lock_guard<std::mutex> g(g_mu);
     Dev& d = g_devs[device];
-    if (d.ctx && d.refs == 0 && d.budget != hbm_budget) {  // idle, other budget: start afresh
-        pgpu_ctx_destroy(d.ctx);
-        d.ctx = nullptr;
+    if (d.ctx && d.budget != hbm_budget) {
+        if (d.refs == 0) {  // idle, other budget: start afresh
+            pgpu_ctx_destroy(d.ctx);
+            d.ctx = nullptr;
+        } else {
+            // The live context (held by a running query or by device copies
+            // kept across queries) enforces the budget it was created with;
+            // silently running under it would ignore the one asked for.
+            throw QueryExecutionError("gpu.hbmbudget " + std::to_string(hbm_budget) + " on device " +
+                                      std::to_string(device) + ": the device is in use under an HBM budget of " +
+                                      std::to_string(d.budget) +
+                                      " bytes (0 = all free memory); finish those queries or release the "
+                                      "resident device copies (pythia_clear_resident) first");
+        }
     }
     if (!d.ctx) {
         pgpu_ctx* c = nullptr;
@@ -107,13 +120,176 @@ const void* resident_rows(int device, uint64_t hbm_budget, const std::string& ke
     return r.d;
 }
 
-void clear_resident() {
-    std::lock_guard<std::mutex> g(g_res_mu);
-    for (auto& e : g_resident) {
-        pgpu_free(e.second.ctx, e.second.d);
-        release_device(e.second.ctx);
+namespace {
+struct ColKey {
+    int device;
+    std::string key;
+    unsigned offset, bytes;
+    bool operator<(const ColKey& o) const {
+        return std::tie(device, key, offset, bytes) < std::tie(o.device, o.key, o.offset, o.bytes);
     }
-    g_resident.clear();
+};
+std::map<ColKey, Resident> g_columns;  // under g_res_mu
+
+template <typename P>
+struct PlanEntry {
+    std::mutex mu;  // held while a query uses the plan
+    pgpu_ctx* ctx = nullptr;
+    P* plan = nullptr;
+    uint64_t a = 0, b = 0, c = 0;  // capacities the plan was created with
+};
+std::mutex g_plan_mu;  // guards the maps (not the entries)
+std::map<std::tuple<int, uint32_t, uint32_t>, std::unique_ptr<PlanEntry<pgpu_join_plan>>> g_join_plans;
+std::map<int, std::unique_ptr<PlanEntry<pgpu_groupjoin>>> g_groupjoins;
+
+struct Pinned {
+    int device;
+    pgpu_ctx* ctx;
+    uint64_t bytes;
+    bool busy;
+};
+std::mutex g_pin_mu;
+std::map<void*, Pinned> g_pinned;
+}  // namespace
+
+const void* resident_column(int device, uint64_t hbm_budget, const std::string& key, const void* d_rows, uint64_t n,
+                            unsigned row_bytes, unsigned offset, unsigned bytes) {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    const ColKey ck{device, key, offset, bytes};
+    auto it = g_columns.find(ck);
+    if (it != g_columns.end()) return it->second.d;
+    Resident r;
+    r.ctx = acquire_device(device, hbm_budget);
+    int rc = pgpu_alloc(r.ctx, std::max<uint64_t>(n * bytes, 256), &r.d);
+    if (rc == PGPU_OK) rc = pgpu_rows_column(r.ctx, d_rows, n, row_bytes, offset, bytes, r.d);
+    if (rc != PGPU_OK) {
+        const std::string msg = std::string("resident column of a scan source: ") + pgpu_last_error();
+        if (r.d) pgpu_free(r.ctx, r.d);
+        release_device(r.ctx);
+        throw QueryExecutionError(msg);
+    }
+    g_columns[ck] = r;
+    return r.d;
+}
+
+void with_join_plan(int device, uint64_t hbm_budget, const pgpu_join_cfg& cfg, uint64_t max_build,
+                    uint64_t max_probe, const std::function<void(pgpu_join_plan*)>& fn) {
+    PlanEntry<pgpu_join_plan>* e;
+    {
+        std::lock_guard<std::mutex> g(g_plan_mu);
+        auto& slot = g_join_plans[std::make_tuple(device, cfg.key_bytes, cfg.probe_payload)];
+        if (!slot) slot.reset(new PlanEntry<pgpu_join_plan>());
+        e = slot.get();
+    }
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (!e->plan || e->a < max_build || e->b < max_probe) {
+        if (e->plan) pgpu_join_plan_destroy(e->plan);
+        e->plan = nullptr;
+        if (!e->ctx) e->ctx = acquire_device(device, hbm_budget);
+        // grow geometrically so a series of slightly larger inputs does not
+        // recreate the plan every time
+        const uint64_t nb = std::max(max_build, e->a + e->a / 2), np = std::max(max_probe, e->b + e->b / 2);
+        check(pgpu_join_plan_create(e->ctx, &cfg, nb, np, &e->plan), "pgpu_join_plan_create");
+        e->a = nb;
+        e->b = np;
+    }
+    fn(e->plan);
+}
+
+void with_groupjoin(int device, uint64_t hbm_budget, uint64_t max_dim, uint64_t max_fact, uint64_t max_groups,
+                    const std::function<void(pgpu_groupjoin*)>& fn) {
+    PlanEntry<pgpu_groupjoin>* e;
+    {
+        std::lock_guard<std::mutex> g(g_plan_mu);
+        auto& slot = g_groupjoins[device];
+        if (!slot) slot.reset(new PlanEntry<pgpu_groupjoin>());
+        e = slot.get();
+    }
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (!e->plan || e->a < max_dim || e->b < max_fact || e->c < max_groups) {
+        if (e->plan) pgpu_groupjoin_destroy(e->plan);
+        e->plan = nullptr;
+        if (!e->ctx) e->ctx = acquire_device(device, hbm_budget);
+        const uint64_t nd = std::max(max_dim, e->a), nf = std::max(max_fact, e->b), ng = std::max(max_groups, e->c);
+        check(pgpu_groupjoin_create(e->ctx, nd, nf, ng, &e->plan), "pgpu_groupjoin_create");
+        e->a = nd;
+        e->b = nf;
+        e->c = ng;
+    }
+    fn(e->plan);
+}
+
+void* acquire_pinned(int device, uint64_t hbm_budget, uint64_t bytes) {
+    {
+        std::lock_guard<std::mutex> g(g_pin_mu);
+        void* best = nullptr;
+        for (auto& kv : g_pinned)
+            if (!kv.second.busy && kv.second.device == device && kv.second.bytes >= bytes &&
+                (!best || kv.second.bytes < g_pinned[best].bytes))
+                best = kv.first;
+        if (best) {
+            g_pinned[best].busy = true;
+            return best;
+        }
+    }
+    pgpu_ctx* ctx = acquire_device(device, hbm_budget);
+    void* p = nullptr;
+    const int rc = pgpu_host_alloc(ctx, bytes, &p);
+    if (rc != PGPU_OK) {
+        release_device(ctx);
+        check(rc, "pgpu_host_alloc");
+    }
+    std::lock_guard<std::mutex> g(g_pin_mu);
+    g_pinned[p] = Pinned{device, ctx, bytes, true};
+    return p;
+}
+
+void release_pinned(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(g_pin_mu);
+    auto it = g_pinned.find(p);
+    if (it != g_pinned.end()) it->second.busy = false;
+}
+
+void clear_resident() {
+    {
+        std::lock_guard<std::mutex> g(g_res_mu);
+        for (auto& e : g_resident) {
+            pgpu_free(e.second.ctx, e.second.d);
+            release_device(e.second.ctx);
+        }
+        g_resident.clear();
+        for (auto& e : g_columns) {
+            pgpu_free(e.second.ctx, e.second.d);
+            release_device(e.second.ctx);
+        }
+        g_columns.clear();
+    }
+    {
+        std::lock_guard<std::mutex> g(g_plan_mu);
+        for (auto& kv : g_join_plans) {
+            std::lock_guard<std::mutex> lk(kv.second->mu);
+            if (kv.second->plan) pgpu_join_plan_destroy(kv.second->plan);
+            if (kv.second->ctx) release_device(kv.second->ctx);
+        }
+        g_join_plans.clear();
+        for (auto& kv : g_groupjoins) {
+            std::lock_guard<std::mutex> lk(kv.second->mu);
+            if (kv.second->plan) pgpu_groupjoin_destroy(kv.second->plan);
+            if (kv.second->ctx) release_device(kv.second->ctx);
+        }
+        g_groupjoins.clear();
+    }
+    std::lock_guard<std::mutex> g(g_pin_mu);
+    for (auto it = g_pinned.begin(); it != g_pinned.end();) {
+        if (it->second.busy) {  // still held by a running query: freed by a later call
+            ++it;
+            continue;
+        }
+        pgpu_host_free(it->second.ctx, it->first);
+        release_device(it->second.ctx);
+        it = g_pinned.erase(it);
+    }
 }
 
 }  // namespace pythia

@@ -16,7 +16,11 @@ NP = {"int": np.int32, "long": np.int64, "dec": np.float64, "date": np.int64}
 
 class Result(C.Structure):
     _fields_ = [("rows", C.c_void_p), ("nrows", C.c_uint64), ("row_bytes", C.c_uint32),
-                ("schema", C.c_char * 256), ("seconds", C.c_double), ("getnext_calls", C.c_uint64)]
+                ("schema", C.c_char * 256), ("seconds", C.c_double), ("getnext_calls", C.c_uint64),
+                ("gpu_fused", C.c_uint32), ("join_d2h_bytes", C.c_uint64)]
+
+# pythia_result.gpu_fused bits
+FUSED_ROWS, FUSED_COUNT_SUM, FUSED_GROUP = 1, 2, 4
 
 
 class QueryError(RuntimeError):
@@ -83,7 +87,8 @@ def run(plan: str):
         n = r.nrows
         buf = (C.c_char * (n * r.row_bytes)).from_address(r.rows) if n else b""
         out = np.frombuffer(bytes(buf), dtype=dt, count=n).copy() if n else np.zeros(0, dtype=dt)
-        meta = {"schema": schema, "seconds": r.seconds, "getnext_calls": r.getnext_calls, "row_bytes": r.row_bytes}
+        meta = {"schema": schema, "seconds": r.seconds, "getnext_calls": r.getnext_calls, "row_bytes": r.row_bytes,
+                "gpu_fused": r.gpu_fused, "join_d2h_bytes": r.join_d2h_bytes}
     finally:
         lib().pythia_free_result(C.byref(r))
     return out, meta

@@ -70,3 +70,28 @@ def test_host_generator_helpers_match_oracle():
     assert sz.min() >= 1 << 10 and sz.max() <= 1 << 22
     assert abs(np.log2(sz.astype(np.float64)).mean() - 16.0) < 0.2
     assert np.array_equal(sz[100:200], pa.sizes(4, 100, 10, 22, first=100))
+
+
+def test_radix_plan_size_limits_without_device():
+    """The radix plan keeps histogram columns and in-partition offsets in 32
+    bits, so each side must hold fewer than 2^32 rows: larger plans are
+    refused by the argument checks, which run before the context is used
+    (the context pointer below is never dereferenced)."""
+    from pythia_amd import _lib
+
+    l = C.CDLL(LIB)
+    l.pgpu_last_error.restype = C.c_char_p
+    l.pgpu_join_plan_create.argtypes = [C.c_void_p, C.POINTER(_lib.JoinCfg), C.c_uint64, C.c_uint64,
+                                        C.POINTER(C.c_void_p)]
+    fake_ctx = C.create_string_buffer(64)
+    out = C.c_void_p()
+    for kb in (4, 8):
+        cfg = _lib.JoinCfg(kb, 0, 0, _lib.ALGO_RADIX, 0)
+        for nb, npr in [(1 << 20, 1 << 32), (1 << 32, 1 << 20), (1 << 32, 1 << 40)]:
+            rc = l.pgpu_join_plan_create(C.addressof(fake_ctx), C.byref(cfg), nb, npr, C.byref(out))
+            assert rc == -1, (kb, nb, npr)
+            assert b"fewer than 2^32" in l.pgpu_last_error()
+            assert not out.value
+    bad = _lib.JoinCfg(8, 0, 0, _lib.ALGO_RADIX, 9)  # unknown LDS join shape
+    assert l.pgpu_join_plan_create(C.addressof(fake_ctx), C.byref(bad), 16, 16, C.byref(out)) == -1
+    assert b"variant" in l.pgpu_last_error()

@@ -84,7 +84,7 @@ def test_c2_radix_matches_closed_form(ctx, bits, zipf):
         b.free()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 7, 8])
 @pytest.mark.parametrize("bits", [4, 9])
 def test_join_variants_match_closed_form(ctx, variant, bits):
     """Every LDS join shape gives the same count and sum (4 bits: 4K build

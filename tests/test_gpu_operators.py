@@ -432,12 +432,44 @@ def test_resident_join_fused_groupby_sum(global_):
         P.register_table("gb_s", S)
         out, meta = P.run(plan)
         assert meta["schema"] == "int,long"
+        # the rows were grouped where the join wrote them: none reached the host
+        assert meta["gpu_fused"] == P.FUSED_ROWS and meta["join_d2h_bytes"] == 0
         got = {}
         for g, v in zip(out["c0"].tolist(), out["c1"].tolist()):
             if global_:
                 assert g not in got  # every group once
             got[g] = (got.get(g, 0) + v + 2**63) % 2**64 - 2**63
         assert got == want
+    finally:
+        P.clear_tables()
+        P.clear_resident()
+
+
+def test_budget_change_while_resident_copies_are_held():
+    """The device context held by resident copies keeps the HBM budget it was
+    created with: a plan asking for another budget fails with a clear error
+    instead of running under the old one; once the copies are released the
+    new budget applies (a 1-byte budget then fails its first allocation)."""
+    rng = np.random.default_rng(5)
+    rdt = np.dtype({"names": ["k", "v"], "formats": [np.int32, np.int32], "offsets": [0, 4], "itemsize": 8})
+    R = np.empty(1000, rdt)
+    R["k"], R["v"] = rng.permutation(1000), rng.integers(0, 100, 1000)
+    S = np.empty(5000, rdt)
+    S["k"], S["v"] = rng.integers(0, 1000, 5000), 0
+    tight = _resident_agg_plan("aggregate_sum", 1).replace("probebatch = 30000L;", "probebatch = 30000L; hbmbudget = 1L;")
+    assert "hbmbudget = 1L" in tight
+    try:
+        P.register_table("res_r", R)
+        P.register_table("res_s", S)
+        out, _ = P.run(_resident_agg_plan("aggregate_sum", 1))
+        by_key = np.empty(1000, np.int64)
+        by_key[R["k"]] = R["v"]
+        assert int(out["c0"].astype(np.int64).sum()) == int(by_key[S["k"]].sum())
+        with pytest.raises(P.QueryError, match="in use under an HBM budget"):
+            P.run(tight)
+        P.clear_resident()
+        with pytest.raises(P.QueryError, match="budget"):
+            P.run(tight)
     finally:
         P.clear_tables()
         P.clear_resident()
