@@ -50,6 +50,14 @@ static void validate_hash(const libconfig::Setting& node, const Schema& s, int f
     }
 }
 
+// Rows of a scan that shares one relation among its threads (resident mode).
+static bool shared_size(Operator* op, uint64_t* n) {
+    auto* scan = dynamic_cast<ParallelScanOp*>(op);
+    const char* rows;
+    std::string key;
+    return scan && scan->sharedRows(&rows, n, &key);
+}
+
 static bool is_int_key(ColumnType t) { return t == CT_INTEGER || t == CT_LONG || t == CT_DATE; }
 
 // createProjectionVector (operators/join.cpp:141-182): "B$i" / "P$j".
@@ -126,6 +134,7 @@ void GpuHashJoinOp::init(libconfig::Config& root, libconfig::Setting& node) {
     knobs.read(node);
     tables.assign(groupleader.size(), nullptr);
     state.assign(MAX_THREADS, ThreadState());
+    runs_.assign(MAX_THREADS, 0);
     if (knobs.resident) {
         const char* rows;
         uint64_t n;
@@ -341,6 +350,25 @@ Operator::ResultCode GpuHashJoinOp::scanStop(unsigned short tid) { return probeO
 // scanStart / scanStop (their pages are not read).
 Operator::ResultCode GpuHashJoinOp::scanStartResident(unsigned short tid, Page* idx, Schema& idxschema) {
     const unsigned short g = threadgroups[tid];
+    if (fused_ == FuseCountSum || fused_ == FuseGroup) {
+        // the consumer runs the whole join through claimRun(): no table to
+        // build, no barrier
+        ++runs_[tid];
+        if (buildOp->scanStart(tid, idx, idxschema) == Error || buildOp->scanStop(tid) == Error) return Error;
+        ResultCode rc = probeOp->scanStart(tid, idx, idxschema);
+        if (rc == Error) return Error;
+        uint64_t np = 0;
+        shared_size(probeOp, &np);
+        ThreadState& st = state[tid];
+        Page* keep = st.out;
+        st = ThreadState();
+        st.out = keep;
+        if (np == 0) {  // an empty probe side is Finished (operators/join.cpp:444-459)
+            st.done = true;
+            return Finished;
+        }
+        return rc;
+    }
     if (buildOp->scanStart(tid, idx, idxschema) == Error || buildOp->scanStop(tid) == Error) {
         fail_group(g);
         return Error;
@@ -357,19 +385,13 @@ Operator::ResultCode GpuHashJoinOp::scanStartResident(unsigned short tid, Page* 
             d_build_ = (const char*)pythia::resident_rows(knobs.device, knobs.hbmbudget, key, rows, n * bts);
             nbuild_rows_ = n;
             build_key_ = key;
-            // a fused count/sum or group consumer runs the partitioned kernels
-            // over columns instead: no row-format table
-            if (fused_ != FuseCountSum && fused_ != FuseGroup)
-                check(pgpu_hj_build_device(tables[g], d_build_, n), "pgpu_hj_build_device");
+            check(pgpu_hj_build_device(tables[g], d_build_, n), "pgpu_hj_build_device");
             p->sharedRows(&rows, &n, &key);
             const unsigned pts = probeOp->getOutSchema().getTupleSize();
             d_probe_ = (const char*)pythia::resident_rows(knobs.device, knobs.hbmbudget, key, rows, n * pts);
             nprobe_rows_ = n;
             probe_key_ = key;
             probe_cursor_ = 0;
-            std::lock_guard<std::mutex> lk(fuse_mu_);
-            fuse_done_ = false;
-            fuse_error_.clear();
         } catch (...) {
             fail_group(g);
             throw;
@@ -412,13 +434,6 @@ bool GpuHashJoinOp::nextDeviceBatch(unsigned short tid, const void** d_rows, uin
     return true;
 }
 
-static bool shared_size(Operator* op, uint64_t* n) {
-    auto* scan = dynamic_cast<ParallelScanOp*>(op);
-    const char* rows;
-    std::string key;
-    return scan && scan->sharedRows(&rows, n, &key);
-}
-
 bool GpuHashJoinOp::canFuseCountSum(int outcol) const {
     if (!resident_) return false;
     Schema& bs = buildOp->getOutSchema();
@@ -458,67 +473,87 @@ const void* GpuHashJoinOp::residentColumn(bool build, unsigned col) {
                                    s.getTupleSize(), s.getColumnOffset(col), s.getColumnWidth(col));
 }
 
+bool GpuHashJoinOp::claimRun(unsigned short tid) {
+    // a run nobody claimed (an empty probe side returns before claiming)
+    // leaves claimed_run_ behind: any older value is claimable
+    const uint64_t run = runs_[tid];
+    uint64_t cur = claimed_run_.load();
+    while (cur < run)
+        if (claimed_run_.compare_exchange_weak(cur, run)) return true;
+    return false;
+}
+
+void GpuHashJoinOp::residentInputs() {
+    const char* rows;
+    uint64_t n;
+    std::string key;
+    dynamic_cast<ParallelScanOp*>(buildOp)->sharedRows(&rows, &n, &key);
+    d_build_ = (const char*)pythia::resident_rows(knobs.device, knobs.hbmbudget, key, rows,
+                                                  n * buildOp->getOutSchema().getTupleSize());
+    nbuild_rows_ = n;
+    build_key_ = key;
+    dynamic_cast<ParallelScanOp*>(probeOp)->sharedRows(&rows, &n, &key);
+    d_probe_ = (const char*)pythia::resident_rows(knobs.device, knobs.hbmbudget, key, rows,
+                                                  n * probeOp->getOutSchema().getTupleSize());
+    nprobe_rows_ = n;
+    probe_key_ = key;
+}
+
 // The queryhashjoin plan's join -> aggregate_count / aggregate_sum with no
 // GROUP BY (operators/join.cpp:503-562 feeding aggregatesum.cpp:81-107 /
 // aggregatecount.cpp:44-52) as one 2-pass radix join with a fused count +
 // sum: the summed column's values travel through the partitions as the
 // payload of their side; an output row's value is counted once per match.
-GpuHashJoinOp::CountSum GpuHashJoinOp::countSum(int outcol, bool* first) {
+GpuHashJoinOp::CountSum GpuHashJoinOp::countSum(int outcol) {
     std::lock_guard<std::mutex> lk(fuse_mu_);
-    *first = !fuse_done_ && fuse_error_.empty();
-    if (!fuse_error_.empty()) throw QueryExecutionError(fuse_error_);
-    if (fuse_done_) return fuse_cs_;
-    fuse_cs_ = CountSum();
-    try {
-        if (nbuild_rows_ && nprobe_rows_) {
-            const void* rk = residentColumn(true, joinattr1);
-            const void* sk = residentColumn(false, joinattr2);
-            const void* rp = rk;  // a count still streams a build payload: the key itself
-            const void* sp = nullptr;
-            bool probe_side = false;
-            if (outcol >= 0) {
-                const JoinPrjT& c = projection[outcol];
-                if (c.first == BuildSide)
-                    rp = residentColumn(true, c.second);
-                else {
-                    sp = residentColumn(false, c.second);
-                    probe_side = true;
-                }
-            }
-            pgpu_join_cfg cfg{};
-            cfg.key_bytes = buildOp->getOutSchema().getColumnWidth(joinattr1);
-            cfg.probe_payload = probe_side ? 1 : 0;
-            cfg.algo = PGPU_ALGO_AUTO;
-            pgpu_join_result r{};
-            pythia::with_join_plan(knobs.device, knobs.hbmbudget, cfg, nbuild_rows_, nprobe_rows_,
-                                   [&](pgpu_join_plan* plan) {
-                                       check(pgpu_join_count_sum(plan, rk, rp, nbuild_rows_, sk, sp, nprobe_rows_, &r),
-                                             "pgpu_join_count_sum");
-                                   });
-            fuse_cs_.matches = r.matches;
-            fuse_cs_.sum = probe_side ? r.sum_probe : r.sum_build;
+    residentInputs();
+    CountSum cs;
+    if (!nbuild_rows_ || !nprobe_rows_) return cs;
+    const void* rk = residentColumn(true, joinattr1);
+    const void* sk = residentColumn(false, joinattr2);
+    const void* rp = rk;  // a count still streams a build payload: the key itself
+    const void* sp = nullptr;
+    bool probe_side = false;
+    if (outcol >= 0) {
+        const JoinPrjT& c = projection[outcol];
+        if (c.first == BuildSide)
+            rp = residentColumn(true, c.second);
+        else {
+            sp = residentColumn(false, c.second);
+            probe_side = true;
         }
-    } catch (const std::exception& e) {
-        fuse_error_ = e.what();
-        throw;
     }
-    fuse_done_ = true;
-    return fuse_cs_;
+    pgpu_join_cfg cfg{};
+    cfg.key_bytes = buildOp->getOutSchema().getColumnWidth(joinattr1);
+    cfg.probe_payload = probe_side ? 1 : 0;
+    cfg.algo = PGPU_ALGO_AUTO;
+    pgpu_join_result r{};
+    pythia::with_join_plan(knobs.device, knobs.hbmbudget, cfg, nbuild_rows_, nprobe_rows_, [&](pgpu_join_plan* plan) {
+        check(pgpu_join_count_sum(plan, rk, rp, nbuild_rows_, sk, sp, nprobe_rows_, &r), "pgpu_join_count_sum");
+    });
+    cs.matches = r.matches;
+    cs.sum = probe_side ? r.sum_probe : r.sum_build;
+    return cs;
 }
 
 // hashjoin(build D, probe F) -> aggregate_sum(field = a D column, sumfield =
 // an F dec column) (operators/join.cpp:503-562 feeding
 // genericaggregate.cpp:240-282) as the partitioned group join; its result
 // rows are the aggregate's output rows.
-const char* GpuHashJoinOp::groupRows(int groupcol, int sumcol, uint64_t* n) {
-    std::lock_guard<std::mutex> lk(fuse_mu_);
-    if (!fuse_error_.empty()) throw QueryExecutionError(fuse_error_);
-    if (fuse_done_) {
-        *n = fuse_nrows_;
+const char* GpuHashJoinOp::groupRows(unsigned short tid, int groupcol, int sumcol, uint64_t* n) {
+    const uint64_t run = runs_[tid];
+    if (!claimRun(tid)) {
+        std::unique_lock<std::mutex> lk(fuse_mu_);
+        fuse_cv_.wait(lk, [&] { return published_run_ >= run; });
+        if (!run_error_.empty()) throw QueryExecutionError(run_error_);
+        *n = published_rows_;
         return fuse_rows_;
     }
+    // the claimer alone touches the inputs and the pinned rows until it publishes
+    uint64_t ng = 0;
+    std::string err;
     try {
-        fuse_nrows_ = 0;
+        residentInputs();
         if (nbuild_rows_ && nprobe_rows_) {
             const auto* dk = (const int32_t*)residentColumn(true, joinattr1);
             const auto* dg = (const int32_t*)residentColumn(true, projection[groupcol].second);
@@ -526,25 +561,35 @@ const char* GpuHashJoinOp::groupRows(int groupcol, int sumcol, uint64_t* n) {
             const auto* fv = (const double*)residentColumn(false, projection[sumcol].second);
             pythia::with_groupjoin(knobs.device, knobs.hbmbudget, nbuild_rows_, nprobe_rows_, nbuild_rows_,
                                    [&](pgpu_groupjoin* gj) {
-                                       uint64_t ng = 0;
                                        check(pgpu_groupjoin_run(gj, dk, dg, nbuild_rows_, fk, fv, nprobe_rows_, &ng,
                                                                 nullptr),
                                              "pgpu_groupjoin_run");
-                                       if (ng && !fuse_rows_)
+                                       if (ng * 12 > fuse_rows_cap_) {
+                                           pythia::release_pinned(fuse_rows_);
+                                           fuse_rows_ = nullptr;
+                                           fuse_rows_cap_ = 0;
                                            fuse_rows_ = (char*)pythia::acquire_pinned(knobs.device, knobs.hbmbudget,
                                                                                      ng * 12);
+                                           fuse_rows_cap_ = ng * 12;
+                                       }
                                        if (ng)
                                            check(pgpu_groupjoin_rows(gj, PGPU_GJ_ROW_SUM, 0, ng, fuse_rows_),
                                                  "pgpu_groupjoin_rows");
-                                       fuse_nrows_ = ng;
                                    });
         }
     } catch (const std::exception& e) {
-        fuse_error_ = e.what();
-        throw;
+        err = e.what();
+        ng = 0;
     }
-    fuse_done_ = true;
-    *n = fuse_nrows_;
+    {
+        std::lock_guard<std::mutex> lk(fuse_mu_);
+        run_error_ = err;
+        published_rows_ = ng;
+        published_run_ = run;
+    }
+    fuse_cv_.notify_all();
+    if (!err.empty()) throw QueryExecutionError(err);
+    *n = ng;
     return fuse_rows_;
 }
 
@@ -568,6 +613,7 @@ void GpuHashJoinOp::destroy() {
     barriers.clear();
     pythia::release_pinned(fuse_rows_);  // consumers have emitted every row by now
     fuse_rows_ = nullptr;
+    fuse_rows_cap_ = 0;
     if (ctx) pythia::release_device(ctx);
     ctx = nullptr;
 }
@@ -673,13 +719,13 @@ void GpuAggregate::threadInit(unsigned short tid) {
     if (!barrier_.Arrive()) throw QueryExecutionError("gpuaggregate: a thread failed during threadInit");
 }
 
-// The fused forms: the join below computes the whole query once (its first
-// caller runs it) and each thread emits its share of the result: the one
-// row of a count / sum without GROUP BY comes from the thread that ran the
-// join (nothing when no tuple joined: no group exists), and the groups of
-// the group join are split evenly over the join's threads, each emitted
-// once (Global mode's rule, genericaggregate.cpp:324-412; in ThreadLocal
-// mode each group is then one thread's partial).
+// The fused forms: the one thread that claims this run of the plan runs the
+// whole join below.  A count / sum without GROUP BY is then one row, emitted
+// by that thread (nothing when no tuple joined: no group exists); the other
+// threads emit nothing.  The group join's groups are split evenly over the
+// join's threads, each emitted once (Global mode's rule, each group by one
+// thread, genericaggregate.cpp:324-412; in ThreadLocal mode each group is
+// then one thread's partial).
 Operator::ResultCode GpuAggregate::scanStartFused(unsigned short tid, Page* idx, Schema& idxschema) {
     ThreadState& st = state[tid];
     st.row.clear();
@@ -688,10 +734,9 @@ Operator::ResultCode GpuAggregate::scanStartFused(unsigned short tid, Page* idx,
     st.first = st.end = st.pos = 0;
     ResultCode rc = nextOp->scanStart(tid, idx, idxschema);
     if (rc != Ready) return rc;  // an empty probe side joins nothing
-    if (fuse_ == GpuHashJoinOp::FuseCountSum) {
-        bool first = false;
-        const GpuHashJoinOp::CountSum cs = join_->countSum(sumField(), &first);
-        if (first && cs.matches) {
+    if (fuse_ == GpuHashJoinOp::FuseCountSum && join_->claimRun(tid)) {
+        const GpuHashJoinOp::CountSum cs = join_->countSum(sumField());
+        if (cs.matches) {
             st.row.resize(schema.getTupleSize());
             if (kind() == PGPU_AGG_SUM_I32) {
                 const int32_t v = (int32_t)(uint32_t)cs.sum;  // the CtInt accumulator wraps mod 2^32
@@ -701,13 +746,13 @@ Operator::ResultCode GpuAggregate::scanStartFused(unsigned short tid, Page* idx,
                 memcpy(st.row.data(), &v, 8);
             }
         }
-    } else {
+    }
+    if (fuse_ == GpuHashJoinOp::FuseGroup) {
         uint64_t n = 0;
-        st.rows = join_->groupRows(aggfields[0], sumField(), &n);
+        st.rows = join_->groupRows(tid, aggfields[0], sumField(), &n);
         const uint64_t T = join_->groupThreads(), j = join_->groupPosition(tid);
-        st.first = j * n / T;
+        st.first = st.pos = j * n / T;
         st.end = (j + 1) * n / T;
-        st.pos = st.first;
     }
     return nextOp->scanStop(tid);
 }

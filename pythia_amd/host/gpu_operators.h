@@ -12,6 +12,7 @@
 #pragma once
 
 #include <atomic>
+#include <condition_variable>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -79,13 +80,17 @@ class GpuHashJoinOp : public DualInputOp {
         uint64_t matches = 0;
         uint64_t sum = 0;  // modulo 2^64 (the low 32 bits for an int column)
     };
-    // Runs the join once per query (the caller that runs it gets *first =
-    // true); every caller receives the totals.
-    CountSum countSum(int outcol, bool* first);
-    // Runs the group join once per query; every caller receives the result
-    // rows (int group key, FP64 sum: 12 bytes each) in pinned host memory
-    // owned by the join until destroy().
-    const char* groupRows(int groupcol, int sumcol, uint64_t* n);
+    // In the fused forms the threads do not meet at a barrier: after its
+    // scanStart each thread asks claimRun(), and the one thread that gets
+    // true for this run of the plan runs the join (countSum / groupRows) and
+    // emits the whole result; the others have nothing to emit.
+    bool claimRun(unsigned short tid);
+    CountSum countSum(int outcol);
+    // The group join's result rows of this run (int group key, FP64 sum: 12
+    // bytes each) in pinned host memory owned by the join until the next run
+    // or destroy(): the thread that claims the run computes them, the others
+    // wait for them (every thread then emits its share).
+    const char* groupRows(unsigned short tid, int groupcol, int sumcol, uint64_t* n);
     // Threads of the (single, in resident mode) thread group and tid's place in it.
     unsigned short groupThreads() const { return groupsize.empty() ? 1 : groupsize[0]; }
     unsigned short groupPosition(unsigned short tid) const { return threadposingrp[tid]; }
@@ -128,11 +133,15 @@ class GpuHashJoinOp : public DualInputOp {
     // fused consumers
     FuseKind fused_ = FuseNone;
     std::mutex fuse_mu_;
-    bool fuse_done_ = false;  // reset by the leader at scanStart
-    std::string fuse_error_;  // a failed fused run: every caller throws it
-    CountSum fuse_cs_;
-    char* fuse_rows_ = nullptr;  // pinned, from the registry's pool
-    uint64_t fuse_nrows_ = 0;
+    std::vector<uint64_t> runs_;             // per thread: scanStart calls so far
+    std::atomic<uint64_t> claimed_run_{0};   // the last run a thread claimed
+    void residentInputs();                   // device copies of both relations (fuse_mu_ held)
+    char* fuse_rows_ = nullptr;              // pinned, from the registry's pool
+    uint64_t fuse_rows_cap_ = 0;
+    std::condition_variable fuse_cv_;        // signals a published run (under fuse_mu_)
+    uint64_t published_run_ = 0;             // the last run whose group rows are out
+    uint64_t published_rows_ = 0;
+    std::string run_error_;                  // the claimer's failure, rethrown by the waiters
     std::atomic<uint64_t> pulled_bytes_{0};
 };
 

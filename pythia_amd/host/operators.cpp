@@ -285,7 +285,7 @@ void MergeOp::command(int w, Cmd c) {
     cv_.wait(g, [&] { return !workers_[w].busy; });
     workers_[w].cmd = c;
     workers_[w].busy = true;
-    cv_.notify_all();
+    wcv_[w]->notify_one();  // only worker w waits on its own variable
 }
 
 void MergeOp::waitIdle(int w) {
@@ -311,7 +311,7 @@ void MergeOp::workerMain(int w) {
         Cmd c;
         {
             std::unique_lock<std::mutex> g(mu_);
-            cv_.wait(g, [&] { return workers_[w].busy; });
+            wcv_[w]->wait(g, [&] { return workers_[w].busy; });
             c = workers_[w].cmd;
         }
         if (c == Exit) {
@@ -357,6 +357,8 @@ void MergeOp::workerMain(int w) {
 
 void MergeOp::threadInit(unsigned short) {
     workers_.assign(nthreads_, Worker());
+    wcv_.clear();
+    for (int w = 0; w < nthreads_; ++w) wcv_.emplace_back(new std::condition_variable());
     for (int w = 0; w < nthreads_; ++w) {
         workers_[w].busy = true;
         workers_[w].th = new std::thread(&MergeOp::workerMain, this, w);

@@ -294,7 +294,8 @@ class MergeOp : public SingleInputOp {
     bool finished_ = false;
     std::vector<Worker> workers_;
     std::mutex mu_;
-    std::condition_variable cv_;
+    std::condition_variable cv_;  // the consumer waits here for workers' results / idleness
+    std::vector<std::unique_ptr<std::condition_variable>> wcv_;  // worker w waits on wcv_[w] for a command
 };
 
 // threadidprepend (ThreadIdPrependOp, operators/threadidprepend.cpp:42-110):

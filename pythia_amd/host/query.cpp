@@ -148,7 +148,11 @@ int pythia_run_query(const char* plan_text, pythia_result* out) {
     memset(out, 0, sizeof *out);
     Query q;
     int rc = 0;
-    std::string rows;
+    // Output rows are collected in a buffer that keeps its capacity between
+    // queries on this thread: a repeated query writes into memory already
+    // mapped instead of growing (and page-faulting) a fresh one.
+    static thread_local std::string rows;
+    rows.clear();
     try {
         libconfig::Config cfg;
         cfg.readString(plan_text);
