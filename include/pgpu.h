@@ -166,6 +166,21 @@ int pgpu_join_count_sum(pgpu_join_plan* plan, const void* d_rkey, const void* d_
                         uint64_t n_build, const void* d_skey, const void* d_spay,
                         uint64_t n_probe, pgpu_join_result* out);
 
+/* The same join (count + sums, the global-table algorithm of
+ * PGPU_ALGO_GLOBAL) with no allocation at all: no reference counterpart; it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its hash
+ * table, partial results and result words live in a 32 MiB device array of
+ * the library, one per device, kept until the process ends; the inputs stay
+ * the caller's.  Builds of at most 2^20 rows (4-byte keys) or 2^19 rows
+ * (8-byte keys) are accepted, larger ones fail with PGPU_EINVAL.  Every call
+ * on a device shares that device's array: calls that overlap on one device
+ * (from several host threads or streams) race; each call clears what it
+ * needs and waits for its result on the context's stream. */
+int pgpu_join_count_sum_noalloc(pgpu_ctx* ctx, uint32_t key_bytes, uint32_t probe_payload, const void* d_rkey,
+                                const void* d_rpay, uint64_t n_build, const void* d_skey, const void* d_spay,
+                                uint64_t n_probe, pgpu_join_result* out);
+
 /* Per-kernel timing with HIP events on the stream the kernels run on.
  * enable != 0 starts recording (and clears totals); pgpu_join_plan_times
  * returns, per kernel, its name, launch count and summed milliseconds. */

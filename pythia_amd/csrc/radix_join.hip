@@ -33,6 +33,7 @@
 // GLOBAL (one open-addressing table in HBM, the "plain" per-query path):
 //   k_gclear, k_gbuild (CAS insert), k_gprobe.
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <cstring>
 
@@ -875,11 +876,13 @@ inline int choose_bits(uint64_t nR) {
     return std::max(2, std::min(kMaxBits, b));
 }
 
-struct Timed {
+struct Timed {  // t may be null (untimed launches)
     pgpu::KernelTimer* t;
     int idx;
-    Timed(pgpu::KernelTimer* t_, const char* name) : t(t_), idx(t_->start(name)) {}
-    ~Timed() { t->stop(idx); }
+    Timed(pgpu::KernelTimer* t_, const char* name) : t(t_), idx(t_ ? t_->start(name) : -1) {}
+    ~Timed() {
+        if (t) t->stop(idx);
+    }
 };
 
 template <typename KT, bool HV>
@@ -976,33 +979,43 @@ int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const 
     return PGPU_OK;
 }
 
+// The global-table join over explicit buffers: `table` of at least
+// 2^ceil(log2(2 nR)) slots, `special` (2 words), `part` (kPartSlots x 4
+// words, zeroed by the caller) receiving the per-workgroup partials.
 template <typename KT, bool HVS>
-int run_global(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const KT* sk, const KT* sv, uint64_t nS) {
-    hipStream_t st = p->ctx->stream;
+int launch_global(hipStream_t st, pgpu::KernelTimer* timer, void* table, unsigned long long* special,
+                  unsigned long long* part, const KT* rk, const KT* rv, uint64_t nR, const KT* sk, const KT* sv,
+                  uint64_t nS) {
     const int logcap = (int)pgpu::ilog2_ceil(std::max<uint64_t>(2 * nR, 2));
     const uint64_t cap = 1ull << logcap;
-    if (cap > p->gcap) return pgpu::fail(PGPU_EINVAL, "global table: %llu slots needed, plan holds %llu",
-                                         (unsigned long long)cap, (unsigned long long)p->gcap);
-    auto* t = (GSlot<KT>*)p->gtable;
+    auto* t = (GSlot<KT>*)table;
     {
-        Timed tm(p->timer, "gclear");
+        Timed tm(timer, "gclear");
         hipLaunchKernelGGL(k_gclear<KT>, dim3((unsigned)std::min<uint64_t>(ceil_div(cap, 256), 4096)), dim3(256), 0,
-                           st, t, cap, p->gspecial);
+                           st, t, cap, special);
         PGPU_LAUNCH_CHECK();
     }
     if (nR) {
-        Timed tm(p->timer, "gbuild");
+        Timed tm(timer, "gbuild");
         hipLaunchKernelGGL(k_gbuild<KT>, dim3((unsigned)std::min<uint64_t>(ceil_div(nR, 256), 8192)), dim3(256), 0,
-                           st, rk, rv, nR, t, logcap, p->gspecial);
+                           st, rk, rv, nR, t, logcap, special);
         PGPU_LAUNCH_CHECK();
     }
     if (nS) {
-        Timed tm(p->timer, "gprobe");
+        Timed tm(timer, "gprobe");
         hipLaunchKernelGGL((k_gprobe<KT, HVS>), dim3((unsigned)std::min<uint64_t>(ceil_div(nS, 256), 8192)),
-                           dim3(256), 0, st, sk, sv, nS, t, logcap, p->gspecial, p->part);
+                           dim3(256), 0, st, sk, sv, nS, t, logcap, special, part);
         PGPU_LAUNCH_CHECK();
     }
     return PGPU_OK;
+}
+
+template <typename KT, bool HVS>
+int run_global(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const KT* sk, const KT* sv, uint64_t nS) {
+    const uint64_t cap = 1ull << pgpu::ilog2_ceil(std::max<uint64_t>(2 * nR, 2));
+    if (cap > p->gcap) return pgpu::fail(PGPU_EINVAL, "global table: %llu slots needed, plan holds %llu",
+                                         (unsigned long long)cap, (unsigned long long)p->gcap);
+    return launch_global<KT, HVS>(p->ctx->stream, p->timer, p->gtable, p->gspecial, p->part, rk, rv, nR, sk, sv, nS);
 }
 
 }  // namespace
@@ -1206,6 +1219,111 @@ int pgpu_join_plan_times(pgpu_join_plan* p, pgpu_kernel_time* out, int cap, int*
         out[i].launches = t[i].launches;
         out[i].total_ms = t[i].ms;
     }
+    return PGPU_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== no-allocation join
+// pgpu_join_count_sum_noalloc: the global-table join whose table, partials
+// and result words live in `scratch`, a file-scope device array (each device
+// holds its own copy until the process ends).  Host code reaches a device's
+// copy only through the address hipGetSymbolAddress returns with that device
+// current; the array's name is not a device address on the host.
+namespace {
+constexpr uint64_t kScratchTableBytes = 32ull << 20;                 // global hash table
+constexpr uint64_t kScratchPartOffset = kScratchTableBytes;          // per-workgroup partials
+constexpr uint64_t kScratchPartBytes = 4ull * kPartSlots * sizeof(unsigned long long);
+constexpr uint64_t kScratchSpecialOffset = kScratchPartOffset + kScratchPartBytes;  // 2 words
+constexpr uint64_t kScratchResultOffset = kScratchSpecialOffset + 256;              // 4 words
+constexpr uint64_t kScratchBytes = kScratchResultOffset + 256;
+constexpr int kScratchDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+char* g_scratch_addr[kScratchDevices] = {};  // per device, from hipGetSymbolAddress
+
+// The calling thread's current device must be `device`.
+int scratch_address(int device, char** out) {
+    if (device < 0 || device >= kScratchDevices) return pgpu::fail(PGPU_EINVAL, "device %d out of range", device);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return pgpu::fail(PGPU_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                              hipGetErrorString(e));
+        g_scratch_addr[device] = (char*)p;
+    }
+    *out = g_scratch_addr[device];
+    return PGPU_OK;
+}
+
+// Build rows the scratch table holds: its 2^ceil(log2(2 n)) slots must fit.
+uint64_t noalloc_max_build(uint32_t key_bytes) {
+    const uint64_t slot = key_bytes == 8 ? sizeof(GSlot<int64_t>) : sizeof(GSlot<int32_t>);
+    uint64_t cap = 1;
+    while (cap * 2 * slot <= kScratchTableBytes) cap *= 2;
+    return cap / 2;
+}
+}  // namespace
+
+extern "C" {
+
+int pgpu_join_count_sum_noalloc(pgpu_ctx* ctx, uint32_t key_bytes, uint32_t probe_payload, const void* d_rkey,
+                                const void* d_rpay, uint64_t n_build, const void* d_skey, const void* d_spay,
+                                uint64_t n_probe, pgpu_join_result* out) {
+    if (!ctx || !out) return fail(PGPU_EINVAL, "pgpu_join_count_sum_noalloc: null argument");
+    if (key_bytes != 4 && key_bytes != 8)
+        return fail(PGPU_EUNSUPPORTED, "join keys must be 4 or 8 bytes (got %u)", key_bytes);
+    const uint64_t maxb = noalloc_max_build(key_bytes);
+    if (n_build > maxb)
+        return fail(PGPU_EINVAL, "pgpu_join_count_sum_noalloc: %llu build rows exceed its %llu-row table (%llu MiB scratch)",
+                    (unsigned long long)n_build, (unsigned long long)maxb,
+                    (unsigned long long)(kScratchTableBytes >> 20));
+    if ((n_build && (!d_rkey || !d_rpay)) || (n_probe && !d_skey) || (n_probe && probe_payload && !d_spay))
+        return fail(PGPU_EINVAL, "pgpu_join_count_sum_noalloc: null column");
+    PGPU_TRY(ctx_activate(ctx));
+    char* base = nullptr;
+    PGPU_TRY(scratch_address(ctx->device, &base));
+    auto* part = (unsigned long long*)(base + kScratchPartOffset);
+    auto* special = (unsigned long long*)(base + kScratchSpecialOffset);
+    auto* result = (unsigned long long*)(base + kScratchResultOffset);
+    hipStream_t st = ctx->stream;
+    // the partials start at zero (fill kernels, not memsets: capturable)
+    hipLaunchKernelGGL(k_fill64, dim3(8), dim3(256), 0, st, part, (uint64_t)4 * kPartSlots);
+    PGPU_LAUNCH_CHECK();
+    int rc = PGPU_OK;
+    if (n_build && n_probe) {
+        const bool hv = probe_payload != 0;
+        if (key_bytes == 8)
+            rc = hv ? launch_global<int64_t, true>(st, nullptr, base, special, part, (const int64_t*)d_rkey,
+                                                   (const int64_t*)d_rpay, n_build, (const int64_t*)d_skey,
+                                                   (const int64_t*)d_spay, n_probe)
+                    : launch_global<int64_t, false>(st, nullptr, base, special, part, (const int64_t*)d_rkey,
+                                                    (const int64_t*)d_rpay, n_build, (const int64_t*)d_skey, nullptr,
+                                                    n_probe);
+        else
+            rc = hv ? launch_global<int32_t, true>(st, nullptr, base, special, part, (const int32_t*)d_rkey,
+                                                   (const int32_t*)d_rpay, n_build, (const int32_t*)d_skey,
+                                                   (const int32_t*)d_spay, n_probe)
+                    : launch_global<int32_t, false>(st, nullptr, base, special, part, (const int32_t*)d_rkey,
+                                                    (const int32_t*)d_rpay, n_build, (const int32_t*)d_skey, nullptr,
+                                                    n_probe);
+    }
+    if (rc != PGPU_OK) return rc;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, part, kPartSlots, result);
+    PGPU_LAUNCH_CHECK();
+    unsigned long long h[4] = {0, 0, 0, 0};
+    PGPU_HIP(hipMemcpyAsync(h, result, sizeof h, hipMemcpyDeviceToHost, st));
+    PGPU_HIP(hipStreamSynchronize(st));
+    memset(out, 0, sizeof *out);
+    out->matches = h[0];
+    out->sum_build = h[1];
+    out->sum_probe = h[2];
+    out->radix_bits = 0;
+    out->lds_passes = (uint32_t)h[3];
     return PGPU_OK;
 }
 
