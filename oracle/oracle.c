@@ -114,7 +114,7 @@ int orc_hasher_init(orc_hasher* h, int fn, unsigned buckets, int64_t rmin, int64
         return 0;
     }
     if (buckets == 0) return -1; /* MissingParameterException, hash.cpp:61-64 */
-    if (fn != ORC_H_BYTES) {
+    if (fn != ORC_H_BYTES && fn != ORC_H_TPCHQ1MAGIC) {
         if (fieldmin != fieldmax) return -2; /* composite value hash */
         int t = s->type[fieldmin];
         if (t != ORC_INT && t != ORC_LONG && t != ORC_DATE) return -3; /* hash.cpp:158-168 */
@@ -154,6 +154,10 @@ int orc_hasher_init(orc_hasher* h, int fn, unsigned buckets, int64_t rmin, int64
     case ORC_H_BYTES: /* hash.h:150-193 */
         h->k = lg;
         h->buckets = 1u << lg;
+        break;
+    case ORC_H_TPCHQ1MAGIC: /* hash.h:129-140: four buckets whatever `buckets` says */
+        h->k = 2;
+        h->buckets = 4;
         break;
     default:
         return -4;
@@ -219,6 +223,11 @@ unsigned orc_hash_tuple(const orc_hasher* h, const void* tuple)
     const char* p = (const char*)tuple + h->offset;
     if (h->fn == ORC_H_ALWAYSZERO) return 0;
     if (h->fn == ORC_H_BYTES) return bytes_hash(h, p, h->size);
+    if (h->fn == ORC_H_TPCHQ1MAGIC) { /* hash.h:136-140: bits of the first 4 bytes of the span */
+        uint32_t v;
+        memcpy(&v, p, 4);
+        return (((v >> 4) | (v >> 16)) & 0x1u) | ((v >> 1) & 0x2u);
+    }
     int64_t v = 0;
     if (numericalize(p, h->size, &v) != 0) return 0;
     return orc_hash_value(h, v);

@@ -45,7 +45,8 @@ enum orc_hashfn {
     ORC_H_BYTES,
     ORC_H_ALWAYSZERO,
     ORC_H_WILLIS,
-    ORC_H_TPCHORDERKEY
+    ORC_H_TPCHORDERKEY,
+    ORC_H_TPCHQ1MAGIC
 };
 
 typedef struct orc_hasher {

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 INT, LONG, DEC, CHAR, DATE = 0, 1, 2, 3, 4
-H_MODULO, H_PARAMMODULO, H_KNUTH, H_RANGE, H_EXACTRANGE, H_BYTES, H_ALWAYSZERO, H_WILLIS, H_TPCH = range(9)
+H_MODULO, H_PARAMMODULO, H_KNUTH, H_RANGE, H_EXACTRANGE, H_BYTES, H_ALWAYSZERO, H_WILLIS, H_TPCH, H_TPCHQ1 = range(10)
 
 NP_OF = {INT: np.int32, LONG: np.int64, DEC: np.float64, DATE: np.int64}
 

@@ -16,6 +16,8 @@
 // usage: refjoin <c1|c2> <log2 R> <log2 S> <threads> <zipf s> <buckets> <tuplesperbucket> <sum|count|probesum>
 //        refjoin c4 <first join> <joins> <threads> <tuplesperbucket> <sum|probesum>
 //          (the batched-join family: one plan per join, buckets = 2^ceil(log2 n))
+//        refjoin hash <int|long|chars> <hash node> <value>...
+//          (hash values through the reference's TupleHasher::create)
 //        refjoin c3 <dimension rows> <fact rows> <threads>
 //          (join + group-by: hashjoin(D, F) -> aggregate_sum(field 0) per worker ->
 //           merge; the consumer folds the per-worker groups)
@@ -178,7 +180,8 @@ static void usage() {
     fprintf(stderr,
             "usage: refjoin <c1|c2> <log2R> <log2S> <threads> <zipf> <buckets> <tpb> <sum|count|probesum>\n"
             "       refjoin c4 <first> <joins> <threads> <tpb> <sum|probesum>\n"
-            "       refjoin c3 <dimension rows> <fact rows> <threads>\n");
+            "       refjoin c3 <dimension rows> <fact rows> <threads>\n"
+            "       refjoin hash <int|long|chars> <hash node> <value>...\n");
 }
 
 // The batched family: joins first..first+count-1, each its own plan; inputs
@@ -349,7 +352,56 @@ static int run_c3(int argc, char** argv) {
     return 0;
 }
 
+// refjoin hash <int|long|chars> <hash node> <value>...: builds the hasher
+// with the reference's own TupleHasher::create (hash.cpp:99-234) over a
+// one-column int or long schema, or two char(1) columns (`chars`, hashed as
+// fieldrange [0, 1]), and prints its bucket count and the hash of a tuple
+// holding each value (for `chars`, the value's four low bytes).
+static int run_hash(int argc, char** argv) {
+    if (argc < 4) {
+        usage();
+        return 2;
+    }
+    const std::string ty = argv[2];
+    Schema s;
+    if (ty == "int")
+        s.add(CT_INTEGER);
+    else if (ty == "long")
+        s.add(CT_LONG);
+    else {
+        s.add(CT_CHAR, 2);  // char(1): one byte and its terminator
+        s.add(CT_CHAR, 2);
+    }
+    Config cfg;  // libconfig 1.2 reads streams only
+    std::string text = std::string("h = ") + argv[3] + ";\n";
+    FILE* f = fmemopen(&text[0], text.size(), "r");
+    if (!f) return 1;
+    cfg.read(f);
+    fclose(f);
+    TupleHasher th = TupleHasher::create(s, cfg.lookup("h"));
+    printf("{\"buckets\": %u, \"hashes\": [", th.buckets());
+    for (int i = 4; i < argc; ++i) {
+        const long long v = strtoll(argv[i], NULL, 10);
+        char tuple[16];
+        memset(tuple, 0, sizeof tuple);
+        if (ty == "int") {
+            const int32_t x = (int32_t)v;
+            memcpy(tuple, &x, 4);
+        } else if (ty == "long") {
+            memcpy(tuple, &v, 8);
+        } else {
+            const uint32_t x = (uint32_t)v;
+            memcpy(tuple, &x, 4);
+        }
+        printf("%s%u", i > 4 ? ", " : "", th.hash(tuple));
+    }
+    printf("]}\n");
+    th.destroy();
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 2 && std::string(argv[1]) == "hash") return run_hash(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "c4") return run_c4(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "c3") return run_c3(argc, argv);
     if (argc < 9) {

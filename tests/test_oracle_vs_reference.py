@@ -88,3 +88,71 @@ def test_c3_reference_equals_oracle(threads):
     assert r["groups"] == int(np.count_nonzero(cents))
     exact = int(cents.sum()) / 100.0
     assert abs(r["total"] - exact) <= 1e-12 * exact
+
+
+# The oracle's hash functions against the reference's own TupleHasher::create
+# (hash.cpp:99-234; hash.h:129-479) on fixed vectors: values around zero, the
+# extremes of both key widths, the range bounds and random values.
+_I32 = [0, 1, -1, 2, 7, 8, 255, 256, 4095, 4096, 12345, -12345, -1000, -1001, -999, 99999, 100000, 100001,
+        2**31 - 1, -2**31, -2**31 + 1, 1 << 20, (1 << 20) - 1]
+_I64 = _I32 + [2**40 + 3, -(2**40), 2**63 - 1, -2**63, -2**63 + 1, 2**62, 0x123456789ABCDEF, -0x123456789ABCDEF]
+_RNG = np.random.default_rng(77)
+_I32 += [int(x) for x in _RNG.integers(-2**31, 2**31 - 1, 40)]
+_I64 += [int(x) for x in _RNG.integers(-2**63, 2**63 - 1, 40, dtype=np.int64)]
+_IN_RANGE = [int(x) for x in _RNG.integers(-1000, 100001, 40)]  # range hashes overflow (UB) far outside
+
+
+def _ref_hashes(ty, node, values):
+    out = subprocess.run([REF, "hash", ty, node] + [str(v) for v in values], stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True, timeout=60, cwd="/tmp")
+    assert out.returncode == 0, out.stderr
+    return json.loads(out.stdout)
+
+
+@pytest.mark.parametrize("ty", ["int", "long"])
+@pytest.mark.parametrize("node,fn,buckets,rmin,rmax,skip,values", [
+    ('{ fn = "modulo"; buckets = 1000; field = 0; }', O.H_MODULO, 1000, 0, 0, 0, None),
+    ('{ fn = "parammodulo"; buckets = 256; field = 0; offset = 7; skipbits = 3; }', O.H_PARAMMODULO, 256, 7, 0, 3,
+     None),
+    ('{ fn = "knuth"; buckets = 256; field = 0; offset = 5; skipbits = 2; }', O.H_KNUTH, 256, 5, 0, 2, None),
+    ('{ fn = "range"; buckets = 64; field = 0; range = [-1000, 100000]; }', O.H_RANGE, 64, -1000, 100000, 0,
+     "in_range"),
+    ('{ fn = "range"; buckets = 100; field = 0; range = [0, 1048575]; }', O.H_RANGE, 100, 0, 1048575, 0, "in_range"),
+    ('{ fn = "exactrange"; buckets = 10; field = 0; range = [-1000, 100000]; }', O.H_EXACTRANGE, 10, -1000, 100000,
+     0, "in_range"),
+    ('{ fn = "willis"; buckets = 4096; field = 0; }', O.H_WILLIS, 4096, 0, 0, 0, None),
+    ('{ fn = "willis"; buckets = 3; field = 0; }', O.H_WILLIS, 3, 0, 0, 0, None),
+    ('{ fn = "tpchorderkey"; buckets = 1024; field = 0; }', O.H_TPCH, 1024, 0, 0, 0, None),
+    ('{ fn = "bytes"; buckets = 4096; field = 0; }', O.H_BYTES, 4096, 0, 0, 0, None),
+    ('{ fn = "alwayszero"; }', O.H_ALWAYSZERO, 1, 0, 0, 0, None),
+])
+def test_hashers_equal_the_reference(ty, node, fn, buckets, rmin, rmax, skip, values):
+    if values == "in_range":
+        vals = [-1000, -999, 0, 1, 50000, 99999, 100000] + _IN_RANGE
+        vals = [v for v in vals if rmin <= v <= rmax]
+    else:
+        vals = _I32 if ty == "int" else _I64
+    s = O.schema((ty,))
+    h = O.hasher(fn, buckets, s, 0, rmin=rmin, rmax=rmax, skipbits=skip)
+    ref = _ref_hashes(ty, node, vals)
+    assert ref["buckets"] == h.buckets
+    r = O.rows(s, np.array(vals, np.int32 if ty == "int" else np.int64))
+    w = 4 if ty == "int" else 8
+    got = [O.lib().orc_hash_tuple(__import__("ctypes").byref(h), r.ctypes.data + i * w) for i in range(len(vals))]
+    assert got == ref["hashes"]
+
+
+def test_tpchq1magic_equals_the_reference():
+    """TpchQ1MagicByteHasher over the (returnflag, linestatus) char(1) pair:
+    every byte pattern of the flags TPC-H uses and random 4-byte spans."""
+    import ctypes
+
+    vals = [0, 16, 65536, 2, 4294967295, 0x0046_0041, 0x004F_0052, 0x0046_004E, 0x004F_0041]
+    vals += [int(x) for x in _RNG.integers(0, 2**32, 60)]
+    ref = _ref_hashes("chars", '{ fn = "tpchq1magic"; buckets = 4; fieldrange = [0, 1]; }', vals)
+    s = O.schema(("char", 1), ("char", 1))
+    h = O.hasher(O.H_TPCHQ1, 4, s, 0, fieldmax=1)
+    assert ref["buckets"] == h.buckets == 4
+    buf = np.array(vals, np.uint32)
+    got = [O.lib().orc_hash_tuple(ctypes.byref(h), buf.ctypes.data + 4 * i) for i in range(len(vals))]
+    assert got == ref["hashes"]
