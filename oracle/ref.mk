@@ -34,7 +34,7 @@ PY_OBJ := $(foreach s,$(PY_SRC),$(OUT)/obj/$(call flat,$(s)).o) $(OUT)/obj/$(cal
 LC_OBJ := $(foreach s,$(LC_SRC),$(OUT)/obj/libconfig__$(s).o) $(OUT)/obj/libconfig__libconfigcpp.o
 BZ_OBJ := $(foreach s,$(BZ_SRC),$(OUT)/obj/bzip2__$(s).o)
 
-all: $(OUT)/refjoin
+all: $(OUT)/refjoin $(OUT)/refgpujoin
 
 $(OUT)/obj:
 	mkdir -p $@
@@ -56,8 +56,13 @@ $(OUT)/obj/libconfig__libconfigcpp.o: $(REF)/externals/libconfig-1.2/libconfigcp
 $(OUT)/obj/bzip2__%.c.o: $(REF)/externals/bzip2-1.0.5/%.c | $(OUT)/obj
 	$(CC) $(CFLAGS) -D_FILE_OFFSET_BITS=64 -c $< -o $@
 
-$(OUT)/refjoin: ref_driver.cpp ../include/pgpu_gen.h $(PY_OBJ) $(LC_OBJ) $(BZ_OBJ)
+$(OUT)/refjoin: ref_driver.cpp ref_memscan.h ../include/pgpu_gen.h $(PY_OBJ) $(LC_OBJ) $(BZ_OBJ)
 	$(CXX) -std=gnu++98 -O3 -w -DSTATS_ALLOCATE -pthread $(INC) -I../include ref_driver.cpp $(PY_OBJ) $(LC_OBJ) $(BZ_OBJ) -lrt -o $@
+
+# The reference engine with the GPU operators' binding (integration/) linked
+# in, against libpgpu.so (built first by the package's build step).
+$(OUT)/refgpujoin: ref_gpu_driver.cpp ref_memscan.h ../integration/gpujoin.cpp ../integration/gpujoin.h ../include/pgpu.h ../pythia_amd/libpgpu.so $(PY_OBJ) $(LC_OBJ) $(BZ_OBJ)
+	$(CXX) -std=gnu++98 -O2 -w -DSTATS_ALLOCATE -pthread $(INC) -I../include ref_gpu_driver.cpp ../integration/gpujoin.cpp $(PY_OBJ) $(LC_OBJ) $(BZ_OBJ) -L../pythia_amd -lpgpu -Wl,-rpath,'$$ORIGIN/../../pythia_amd' -lrt -o $@
 
 clean:
 	rm -rf $(OUT)
