@@ -208,10 +208,11 @@ def test_hbm_budget_is_enforced():
     ctx.close()
 
 
-@pytest.mark.parametrize("zipf", [0.0, 1.0])
+@pytest.mark.parametrize("zipf", [0.0, 0.5, 1.0])
 def test_c2_full_shape(ctx, zipf):
-    """C2 at its full size: R 2^24 x (int64, int64), S 2^28 int64, 14 radix
-    bits; count and int64 sum bit-exact against the closed form."""
+    """C2 at its full size: R 2^24 x (int64, int64), S 2^28 int64 uniform or
+    Zipf s = 0.5 / 1.0, 14 radix bits; count and int64 sum bit-exact against
+    the closed form."""
     nR, nS = 1 << 24, 1 << 28
     rk, rp, sk = _c2_device(ctx, nR, nS, zipf)
     plan = pa.JoinPlan(ctx, 8, nR, nS, radix_bits=14, algo=pa.ALGO_RADIX)

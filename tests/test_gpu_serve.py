@@ -170,3 +170,61 @@ def test_duplicates_and_empty_marker_key(ctx):
     assert (cs.count, cs.isum % 2**64) == exact(pk)
     for b in (k, p, q):
         b.free()
+
+
+@pytest.fixture(scope="module")
+def c5_world(ctx):
+    """The C5 configuration's tables at full scale: 64 tables of
+    round(2^U(16, 23)) int64 rows (about 1.1e8 rows), and a query pool from
+    the C5 trace (probe batches of round(2^U(14, 25)) rows): its first 40
+    queries plus the first 8 whose probe batch holds 2^24 rows or more."""
+    ntab = 64
+    sizes = pa.sizes(5, ntab, 16, 23)
+    tabs = []
+    for t, n in enumerate(int(x) for x in sizes):
+        k, p = ctx.alloc(8 * n), ctx.alloc(8 * n)
+        ctx.gen("C2_RKEY", n, k, sub=t)
+        ctx.gen("C2_RPAY", n, p, sub=t)
+        tabs.append((k, p, n))
+    qt, qm, _ = pa.c5_trace(2000, 1.0, ntables=ntab, lo=14, hi=25)
+    big = [int(q) for q in np.nonzero(qm >= (1 << 24))[0][:8]]
+    sel = sorted(set(range(40)) | set(big))
+    probes = []
+    for q in sel:
+        t, m = int(qt[q]), int(qm[q])
+        b = ctx.alloc(8 * m)
+        ctx.gen("C2_SFK_UNIFORM", m, b, sub=t, n_build=tabs[t][2], offset=q << 32)
+        probes.append(b)
+    ctx.sync()
+    st, sm = qt[sel], qm[sel]
+    exp = [O.c5_closed(int(t), tabs[int(t)][2], q << 32, int(m), threads=THREADS) for q, t, m in zip(sel, st, sm)]
+    yield tabs, st, sm, probes, exp, len(big)
+    for b in probes:
+        b.free()
+    for k, p, _ in tabs:
+        k.free()
+        p.free()
+
+
+def test_c5_full_scale_poisson_2000_alone_and_in_the_mix(ctx, c5_world):
+    """At the C5 configuration's scale (64 resident tables up to 2^23 rows,
+    probe batches up to 2^25 rows, the ones that set the p99) every query's
+    count and int64 sum equals its closed form in a Poisson replay at
+    2000 queries/s and when it runs alone."""
+    tabs, qt, qm, probes, exp, nbig = c5_world
+    assert nbig == 8 and qm.max() >= 1 << 24
+    _, _, off = pa.c5_trace(len(qt), 2000.0, ntables=len(tabs), lo=14, hi=25)
+    s = _server(ctx, tabs)
+    res = s.replay(qt, probes, qm, off)
+    assert [(r.matches, r.sum_build) for r in res] == [(e.count, e.sum_build) for e in exp]
+    alone = []
+    for q in range(len(qt)):
+        s.submit(int(qt[q]), probes[q], int(qm[q]), tag=q)
+        s.drain()
+        (r,) = s.poll()
+        assert r.batch_size == 1
+        alone.append((r.matches, r.sum_build))
+    assert alone == [(e.count, e.sum_build) for e in exp]
+    st = s.stats()
+    assert st["partitioned"] >= 2 * nbig  # the large batches took the partitioned path
+    s.close()
