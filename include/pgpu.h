@@ -335,9 +335,13 @@ int pgpu_c5_trace(uint64_t seed, uint64_t count, uint32_t ntables, double lo, do
 /* Build rows are the reference's `sbuild` layout (join key then the     */
 /* projected build columns, packed); probe rows are the probe child's    */
 /* tuples.  Output rows are the projection, packed, in the output        */
-/* schema's order.  Keys are signed integers of 4 or 8 bytes compared by */
-/* value (an int key equals a long key of the same value).               */
+/* schema's order.  Keys compare as the reference's Comparator does:    */
+/* integers of 4 or 8 bytes by value, dec keys as doubles, char keys by   */
+/* strncmp (see pgpu_hj_desc).                                            */
 /* ------------------------------------------------------------------ */
+/* Column types where a kernel interprets a column (join keys, sort keys). */
+enum pgpu_col_type { PGPU_COL_INT32 = 1, PGPU_COL_INT64 = 2, PGPU_COL_F64 = 3, PGPU_COL_CHAR = 4 };
+
 typedef struct pgpu_proj_col {
     uint32_t side;       /* 0: build (sbuild) row, 1: probe row */
     uint32_t src_offset; /* byte offset in that row              */
@@ -352,6 +356,14 @@ typedef struct pgpu_hj_desc {
     uint32_t nproj;
     pgpu_proj_col proj[PGPU_MAX_PROJ];
     uint32_t slots;      /* independent probe streams (worker threads), >= 1 */
+    /* Key types (enum pgpu_col_type; 0 = a signed integer of key_bytes).
+     * Equality follows the reference's Comparator (comparator.cpp:43-413,
+     * rawcompfns.cpp): integer keys of either width by value; a dec (FP64)
+     * key against a dec or integer key as doubles (so +0 equals -0, a NaN
+     * key matches nothing); char keys against char keys as strncmp over the
+     * smaller key width (bytes up to the first NUL).  Other pairs are
+     * refused (PGPU_EUNSUPPORTED). */
+    uint32_t build_key_type, probe_key_type;
 } pgpu_hj_desc;
 
 typedef struct pgpu_hj_stats {
@@ -442,7 +454,6 @@ int pgpu_agg_pull(pgpu_agg* a, uint64_t first, uint64_t n, void* dst);
 /* idxstart order over worker threads), optionally sorted by a column    */
 /* (sortAllInPage; ties keep input order).                               */
 /* ------------------------------------------------------------------ */
-enum pgpu_col_type { PGPU_COL_INT32 = 1, PGPU_COL_INT64 = 2, PGPU_COL_F64 = 3 };
 #define PGPU_PART_MAX_BUCKETS 127 /* PartitionOp: buckets < MAX_THREADS (128) */
 #define PGPU_PART_MAX_SOURCES 256
 

@@ -22,7 +22,15 @@ void gpu_check(int rc, const char* what) {
     }
 }
 
-bool is_int_key(ColumnType t) { return t == CT_INTEGER || t == CT_LONG || t == CT_DATE; }
+// enum pgpu_col_type of a join key column (int, long and date: integers by value)
+unsigned key_type(ColumnType t) {
+    switch (t) {
+        case CT_DECIMAL: return PGPU_COL_F64;
+        case CT_CHAR: return PGPU_COL_CHAR;
+        case CT_INTEGER: return PGPU_COL_INT32;
+        default: return PGPU_COL_INT64;
+    }
+}
 
 // Byte offset of column `pos` in a tuple of schema s.
 unsigned column_offset(Schema& s, unsigned pos) {
@@ -117,8 +125,15 @@ void GpuHashJoinOp::init(libconfig::Config& root, libconfig::Setting& node) {
     (void)(int)node["tuplesperbucket"];
     std::string policy = node["allocpolicy"];
     (void)policy;
-    if (!is_int_key(bs.getColumnType(joinattr1)) || !is_int_key(ps.getColumnType(joinattr2)))
-        throw InvalidParameter();
+    // the key pairs Comparator::init accepts (comparator.cpp:43-413); the
+    // reference itself throws the same exception from createComparator
+    {
+        const ColumnType bt = bs.getColumnType(joinattr1), pt = ps.getColumnType(joinattr2);
+        const bool num = (bt == CT_INTEGER || bt == CT_LONG || bt == CT_DECIMAL) &&
+                         (pt == CT_INTEGER || pt == CT_LONG || pt == CT_DECIMAL);
+        if (!num && !(bt == CT_DATE && pt == CT_DATE) && !(bt == CT_CHAR && pt == CT_CHAR))
+            throw UnknownComparisonException();
+    }
     knobs.read(node);
 
     gpu_check(pgpu_ctx_create(knobs.device, knobs.hbmbudget, &ctx), "pgpu_ctx_create");
@@ -130,6 +145,8 @@ void GpuHashJoinOp::init(libconfig::Config& root, libconfig::Setting& node) {
     d.probe_row_bytes = ps.getTupleSize();
     d.probe_key_offset = column_offset(ps, joinattr2);
     d.probe_key_bytes = ps.getColumnWidth(joinattr2);
+    d.build_key_type = key_type(bs.getColumnType(joinattr1));
+    d.probe_key_type = key_type(ps.getColumnType(joinattr2));
     d.nproj = projection.size();
     if (d.nproj > PGPU_MAX_PROJ) throw InvalidParameter();
     unsigned buildcol = 1;

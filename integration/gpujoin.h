@@ -55,8 +55,9 @@ struct GpuPlanKnobs {
 
 /* Keys as HashJoinOp::init (operators/join.cpp:273-366): hash{fn, buckets,
  * ...}, tuplesperbucket, buildjattr, probejattr, projection, threadgroups
- * (both parsed by JoinOp::init), allocpolicy.  Join keys are int, long or
- * date columns compared by value. */
+ * (both parsed by JoinOp::init), allocpolicy.  Join keys compare as the
+ * reference's Comparator does: int / long / dec by value, date with date,
+ * char with char (strncmp). */
 class GpuHashJoinOp : public JoinOp {
   public:
     GpuHashJoinOp();

@@ -13,9 +13,9 @@
 // usage: refgpujoin c1 <log2 R> <log2 S> <threads> <cpu|gpu> <sum|count|probesum>
 //          the queryhashjoin plan (unit_tests/queryhashjoin.cpp) on the C1
 //          generators: join -> partial aggregate -> merge, consumer folds
-//        refgpujoin rows <R rows> <S rows> <threads> <cpu|gpu>
-//          join -> merge with a mixed projection, int build key against a long
-//          probe key, duplicate keys; digest of the output multiset
+//        refgpujoin rows <R rows> <S rows> <threads> <cpu|gpu> [int|dec|mixed|char|charw]
+//          join -> merge with a mixed projection, duplicate keys, misses, and
+//          the key types of run_rows; digest of the output multiset
 //        refgpujoin groups <D rows> <F rows> <threads> <cpu|gpu> <global 0|1>
 //          join -> aggregate_sum(field 0) -> merge with integer-valued dec
 //          prices (exact sums); digest of the folded groups
@@ -198,42 +198,96 @@ struct FoldRows {
     }
 };
 
-// build (int key, long payload, dec value), probe (long fk, int x): every
-// build key used twice, a tenth of the probe keys missing, negative keys.
+// build (key, long payload, dec value), probe (key, int x): every build key
+// used twice, a tenth of the probe keys missing, negative keys.  Key types
+// (`keys`): int  int build key against a long probe key (modulo hash)
+//           dec  dec against dec, some NaN keys (bytes hash: equal values
+//                have equal bytes, no -0 is generated)
+//           mixed int build key against a dec probe key (alwayszero hash:
+//                one bucket, so the comparator alone decides)
+//           char char(7) against char(7), NUL-padded (bytes hash)
+//           charw char(5) build against char(9) probe (alwayszero hash)
+void put_key(char* dst, ColumnType t, unsigned width, int64_t v, bool nan) {
+    if (t == CT_INTEGER) {
+        const int32_t x = (int32_t)v;
+        memcpy(dst, &x, 4);
+    } else if (t == CT_LONG) {
+        memcpy(dst, &v, 8);
+    } else if (t == CT_DECIMAL) {
+        double x = nan ? __builtin_nan("") : (double)v * 0.5;  // halves: exact, never -0
+        if (x == 0.0) x = 0.0;
+        memcpy(dst, &x, 8);
+    } else {
+        char buf[32];
+        memset(dst, 0, width);
+        snprintf(buf, sizeof buf, "k%lld", (long long)v);  // the same text within every width used
+        strncpy(dst, buf, width - 1);
+    }
+}
+
 int run_rows(int argc, char** argv) {
     if (argc < 6) return 2;
     const uint64_t nR = strtoull(argv[2], 0, 10), nS = strtoull(argv[3], 0, 10);
     const int T = atoi(argv[4]);
     const bool gpu = std::string(argv[5]) == "gpu";
+    const std::string keys = argc > 6 ? argv[6] : "int";
+    ColumnType bt = CT_INTEGER, pt = CT_LONG;
+    unsigned bw = 0, pw = 0;
+    std::string hashfn = "modulo";
+    if (keys == "dec") {
+        bt = pt = CT_DECIMAL;
+        hashfn = "bytes";
+    } else if (keys == "mixed") {
+        bt = CT_INTEGER;
+        pt = CT_DECIMAL;
+        hashfn = "alwayszero";
+    } else if (keys == "char") {
+        bt = pt = CT_CHAR;
+        bw = pw = 8;
+        hashfn = "bytes";
+    } else if (keys == "charw") {
+        bt = pt = CT_CHAR;
+        bw = 6;
+        pw = 10;
+        hashfn = "alwayszero";
+    } else if (keys != "int") {
+        return 2;
+    }
     Schema rs, ss;
-    rs.add(CT_INTEGER);
+    rs.add(bt, bw);
     rs.add(CT_LONG);
     rs.add(CT_DECIMAL);
-    ss.add(CT_LONG);
+    ss.add(pt, pw);
     ss.add(CT_INTEGER);
-    std::vector<char> R(nR * rs.getTupleSize()), S(nS * ss.getTupleSize());
-    const int64_t keys = (int64_t)(nR / 2 ? nR / 2 : 1);
+    const unsigned rts = rs.getTupleSize(), sts = ss.getTupleSize();
+    const unsigned bkw = rs.getColumnWidth(0), pkw = ss.getColumnWidth(0);
+    std::vector<char> R(nR * rts), S(nS * sts);
+    // charw: the build's char(5) holds at most 4 characters + NUL ("k" and up
+    // to 3 digits), so keys stay below 1000 there
+    const int64_t keys_n = keys == "charw" ? 900 : (int64_t)(nR / 2 ? nR / 2 : 1);
     for (uint64_t i = 0; i < nR; ++i) {
-        int32_t k = (int32_t)((int64_t)(i % keys) * 3 - keys);
-        int64_t p = (int64_t)mix(i);
-        double v = (double)(mix(i + 7) % 100000) / 8.0;
-        char* r = &R[i * rs.getTupleSize()];
-        memcpy(r, &k, 4);
-        memcpy(r + 4, &p, 8);
-        memcpy(r + 12, &v, 8);
+        char* r = &R[i * rts];
+        const int64_t k = (int64_t)(i % keys_n) * 3 - (keys == "char" || keys == "charw" ? 0 : keys_n);
+        put_key(r, bt, bkw, k, keys == "dec" && i % 97 == 5);
+        const int64_t p = (int64_t)mix(i);
+        const double v = (double)(mix(i + 7) % 100000) / 8.0;
+        memcpy(r + bkw, &p, 8);
+        memcpy(r + bkw + 8, &v, 8);
     }
     for (uint64_t j = 0; j < nS; ++j) {
         const uint64_t z = mix(j + 1000003);
-        int64_t k = (int64_t)(z % (uint64_t)(keys + keys / 10 + 1)) * 3 - keys;
-        int32_t x = (int32_t)(z >> 40);
-        char* s = &S[j * ss.getTupleSize()];
-        memcpy(s, &k, 8);
-        memcpy(s + 8, &x, 4);
+        const int64_t k = (int64_t)(z % (uint64_t)(keys_n + keys_n / 10 + 1)) * 3 -
+                          (keys == "char" || keys == "charw" ? 0 : keys_n);
+        char* s = &S[j * sts];
+        put_key(s, pt, pkw, k, keys == "dec" && j % 89 == 3);
+        const int32_t x = (int32_t)(z >> 40);
+        memcpy(s + pkw, &x, 4);
     }
     Config cfg;
-    base_config(cfg, nS * ss.getTupleSize(), T);
+    base_config(cfg, nS * sts, T);
     const char* proj[] = {"P$1", "B$2", "B$1", "P$0", "B$0"};
     Setting& jn = join_node(cfg, T, proj, 5);
+    jn["hash"]["fn"] = hashfn;
     Setting& mn = cfg.getRoot().add("merge", Setting::TypeGroup);
     mn.add("threads", Setting::TypeInt) = T;
     Setting& sc = cfg.getRoot().add("scan", Setting::TypeGroup);
@@ -255,10 +309,10 @@ int run_rows(int argc, char** argv) {
     double secs = 0;
     if (drive(q, f, &secs)) return 1;
     q.destroynofree();
-    printf("{\"mode\": \"rows\", \"ops\": \"%s\", \"rows\": %llu, \"row_bytes\": %u, \"sum\": %llu, \"xor\": %llu, "
-           "\"seconds\": %.6f}\n",
-           gpu ? "gpu" : "cpu", (unsigned long long)f.n, f.ts, (unsigned long long)f.sum, (unsigned long long)f.x,
-           secs);
+    printf("{\"mode\": \"rows\", \"keys\": \"%s\", \"ops\": \"%s\", \"rows\": %llu, \"row_bytes\": %u, "
+           "\"sum\": %llu, \"xor\": %llu, \"seconds\": %.6f}\n",
+           keys.c_str(), gpu ? "gpu" : "cpu", (unsigned long long)f.n, f.ts, (unsigned long long)f.sum,
+           (unsigned long long)f.x, secs);
     return 0;
 }
 
@@ -367,7 +421,7 @@ int main(int argc, char** argv) {
     if (rc == 2)
         fprintf(stderr,
                 "usage: refgpujoin c1 <log2 R> <log2 S> <threads> <cpu|gpu> <sum|count|probesum>\n"
-                "       refgpujoin rows <R rows> <S rows> <threads> <cpu|gpu>\n"
+                "       refgpujoin rows <R rows> <S rows> <threads> <cpu|gpu> [int|dec|mixed|char|charw]\n"
                 "       refgpujoin groups <D rows> <F rows> <threads> <cpu|gpu> <global 0|1>\n");
     return rc;
 }

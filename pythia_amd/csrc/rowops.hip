@@ -82,12 +82,57 @@ struct alignas(16) JSlot {  // one 16-byte load per probe step
     uint32_t pad;
 };
 
+// How join keys compare (Comparator::init, comparator.cpp:43-413):
+//   kKeyInt  integers of 4 or 8 bytes by value: the key is the sign-extended value
+//   kKeyDbl  a dec key on either side: the key is the canonical bit pattern
+//            of the value as a double (-0 -> +0); a NaN key matches nothing
+//   kKeyChr  char against char: strncmp over `nchar` bytes; the key is a hash
+//            of the bytes before the first NUL, equal hashes are confirmed
+//            against the build row's bytes
+enum : uint32_t { kKeyInt = 0, kKeyDbl = 1, kKeyChr = 2 };
+
 struct JoinDev {
     pgpu_hj_desc d;
     uint32_t out_row_bytes;
     uint32_t dst_off[PGPU_MAX_PROJ];
     uint32_t layout_a4;  // every row width, offset and projected width a multiple of 4
+    uint32_t key_mode;   // kKeyInt / kKeyDbl / kKeyChr
+    uint32_t nchar;      // kKeyChr: bytes compared (the smaller key width)
 };
+
+// The join key of the row field at p (type: enum pgpu_col_type, 0 = signed
+// integer); false when the key can match nothing (a NaN).
+__device__ __forceinline__ bool join_key(const char* p, uint32_t bytes, uint32_t type, uint32_t mode,
+                                         uint32_t nchar, bool a4, int64_t* key) {
+    if (mode == kKeyInt) {
+        *key = load_key(p, bytes, a4);
+        return true;
+    }
+    if (mode == kKeyDbl) {
+        const double v = type == PGPU_COL_F64 ? __longlong_as_double((long long)ld_u64(p, a4))
+                                              : (double)load_key(p, bytes, a4);
+        if (v != v) return false;
+        *key = v == 0.0 ? 0 : (int64_t)__double_as_longlong(v);
+        return true;
+    }
+    uint64_t h = 14695981039346656037ull;  // FNV-1a over the bytes before the first NUL
+    for (uint32_t i = 0; i < nchar; ++i) {
+        const unsigned char c = (unsigned char)p[i];
+        if (!c) break;
+        h = (h ^ c) * 1099511628211ull;
+    }
+    *key = (int64_t)h;
+    return true;
+}
+
+// strncmp(a, b, n) == 0
+__device__ __forceinline__ bool chars_equal(const char* a, const char* b, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        if (a[i] != b[i]) return false;
+        if (!a[i]) return true;
+    }
+    return true;
+}
 
 __global__ __launch_bounds__(256) void k_jclear(JSlot* t, uint64_t cap) {
     const uint64_t stride = (uint64_t)gridDim.x * 256;
@@ -97,14 +142,17 @@ __global__ __launch_bounds__(256) void k_jclear(JSlot* t, uint64_t cap) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_jbuild(const char* __restrict__ rows, uint64_t n, uint32_t row_bytes,
-                                                uint32_t key_off, uint32_t key_bytes, JSlot* __restrict__ t,
-                                                int logcap) {
+__global__ __launch_bounds__(256) void k_jbuild(JoinDev jd, const char* __restrict__ rows, uint64_t n,
+                                                JSlot* __restrict__ t, int logcap) {
+    const pgpu_hj_desc& d = jd.d;
     const uint64_t mask = (1ull << logcap) - 1;
     const uint64_t stride = (uint64_t)gridDim.x * 256;
-    const bool a4 = aligned4((uintptr_t)rows | row_bytes | key_off);
+    const bool a4 = aligned4((uintptr_t)rows | d.build_row_bytes | d.build_key_offset);
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-        const int64_t k = load_key(rows + i * row_bytes + key_off, key_bytes, a4);
+        int64_t k;
+        if (!join_key(rows + i * d.build_row_bytes + d.build_key_offset, d.build_key_bytes, d.build_key_type,
+                      jd.key_mode, jd.nchar, a4, &k))
+            continue;  // a NaN key joins nothing
         uint64_t s = mix64((uint64_t)k) & mask;
         while (atomicCAS(&t[s].idx, kEmptyIdx, (uint32_t)i) != kEmptyIdx) s = (s + 1) & mask;
         t[s].key = k;
@@ -156,7 +204,8 @@ constexpr unsigned kProbeBlocks = 2048;
 constexpr int kProbeU = 4;
 constexpr unsigned kProbeTile = 256 * kProbeU;
 
-__global__ __launch_bounds__(256) void k_jcount(JoinDev jd, const JSlot* __restrict__ t, int logcap,
+__global__ __launch_bounds__(256) void k_jcount(JoinDev jd, const char* __restrict__ build,
+                                                const JSlot* __restrict__ t, int logcap,
                                                 const char* __restrict__ probe, uint64_t n, uint64_t chunk,
                                                 unsigned long long* __restrict__ counts,
                                                 unsigned long long* __restrict__ rec) {
@@ -170,10 +219,14 @@ __global__ __launch_bounds__(256) void k_jcount(JoinDev jd, const JSlot* __restr
         int64_t k[kProbeU];
         uint64_t sl[kProbeU];
         JSlot e[kProbeU];
+        bool valid[kProbeU];
 #pragma unroll
-        for (int u = 0; u < kProbeU; ++u)
-            k[u] = i0 + u < hi ? load_key(probe + (i0 + u) * d.probe_row_bytes + d.probe_key_offset, d.probe_key_bytes, a4)
-                               : 0;
+        for (int u = 0; u < kProbeU; ++u) {
+            k[u] = 0;
+            valid[u] = i0 + u < hi &&
+                       join_key(probe + (i0 + u) * d.probe_row_bytes + d.probe_key_offset, d.probe_key_bytes,
+                                d.probe_key_type, jd.key_mode, jd.nchar, a4, &k[u]);
+        }
 #pragma unroll
         for (int u = 0; u < kProbeU; ++u) {
             sl[u] = mix64((uint64_t)k[u]) & mask;
@@ -186,7 +239,7 @@ __global__ __launch_bounds__(256) void k_jcount(JoinDev jd, const JSlot* __restr
 #pragma unroll
         for (int u = 0; u < kProbeU; ++u) {
             cnt[u] = first[u] = 0;
-            act[u] = i0 + u < hi;
+            act[u] = valid[u];
         }
         while (true) {
             bool any = false;
@@ -197,7 +250,10 @@ __global__ __launch_bounds__(256) void k_jcount(JoinDev jd, const JSlot* __restr
                     act[u] = false;
                     continue;
                 }
-                if (e[u].key == k[u]) {
+                if (e[u].key == k[u] &&
+                    (jd.key_mode != kKeyChr ||
+                     chars_equal(build + (uint64_t)e[u].idx * d.build_row_bytes + d.build_key_offset,
+                                 probe + (i0 + u) * d.probe_row_bytes + d.probe_key_offset, jd.nchar))) {
                     if (!cnt[u]) first[u] = e[u].idx;
                     ++cnt[u];
                 }
@@ -268,9 +324,14 @@ __global__ __launch_bounds__(256) void k_jwrite(JoinDev jd, const char* __restri
                 emit(pos++, prow, (uint32_t)r[u]);
                 continue;
             }
-            const int64_t k = load_key(prow + d.probe_key_offset, d.probe_key_bytes, a4);
+            int64_t k = 0;
+            join_key(prow + d.probe_key_offset, d.probe_key_bytes, d.probe_key_type, jd.key_mode, jd.nchar, a4, &k);
             for (uint64_t s = mix64((uint64_t)k) & mask; t[s].idx != kEmptyIdx; s = (s + 1) & mask)
-                if (t[s].key == k) emit(pos++, prow, t[s].idx);
+                if (t[s].key == k &&
+                    (jd.key_mode != kKeyChr ||
+                     chars_equal(build + (uint64_t)t[s].idx * d.build_row_bytes + d.build_key_offset,
+                                 prow + d.probe_key_offset, jd.nchar)))
+                    emit(pos++, prow, t[s].idx);
         }
     }
 }
@@ -627,8 +688,25 @@ extern "C" {
 int pgpu_hj_create(pgpu_ctx* ctx, const pgpu_hj_desc* d, pgpu_hj** out) {
     if (!ctx || !d || !out) return fail(PGPU_EINVAL, "pgpu_hj_create: null argument");
     *out = nullptr;
-    if ((d->build_key_bytes != 4 && d->build_key_bytes != 8) || (d->probe_key_bytes != 4 && d->probe_key_bytes != 8))
-        return fail(PGPU_EUNSUPPORTED, "join keys must be 4- or 8-byte integers");
+    // key pairs Comparator::init accepts (comparator.cpp:43-413)
+    const uint32_t bt = d->build_key_type, pt = d->probe_key_type;
+    auto is_int = [](uint32_t t, uint32_t b) {
+        return (t == 0 || t == PGPU_COL_INT32 || t == PGPU_COL_INT64) && (b == 4 || b == 8) &&
+               (t != PGPU_COL_INT32 || b == 4) && (t != PGPU_COL_INT64 || b == 8);
+    };
+    auto is_dbl = [](uint32_t t, uint32_t b) { return t == PGPU_COL_F64 && b == 8; };
+    auto is_chr = [](uint32_t t, uint32_t b) { return t == PGPU_COL_CHAR && b >= 1; };
+    uint32_t mode;
+    if (is_int(bt, d->build_key_bytes) && is_int(pt, d->probe_key_bytes))
+        mode = kKeyInt;
+    else if ((is_int(bt, d->build_key_bytes) || is_dbl(bt, d->build_key_bytes)) &&
+             (is_int(pt, d->probe_key_bytes) || is_dbl(pt, d->probe_key_bytes)))
+        mode = kKeyDbl;
+    else if (is_chr(bt, d->build_key_bytes) && is_chr(pt, d->probe_key_bytes))
+        mode = kKeyChr;
+    else
+        return fail(PGPU_EUNSUPPORTED,
+                    "join keys: integers (4 or 8 bytes), dec against dec or integer, or char against char");
     if (d->build_key_offset + d->build_key_bytes > d->build_row_bytes ||
         d->probe_key_offset + d->probe_key_bytes > d->probe_row_bytes)
         return fail(PGPU_EINVAL, "join key outside its row");
@@ -649,6 +727,8 @@ int pgpu_hj_create(pgpu_ctx* ctx, const pgpu_hj_desc* d, pgpu_hj** out) {
         o += p.bytes;
     }
     h->jd.out_row_bytes = o;
+    h->jd.key_mode = mode;
+    h->jd.nchar = std::min(d->build_key_bytes, d->probe_key_bytes);
     uint32_t lay = d->build_row_bytes | d->build_key_offset | d->probe_row_bytes | d->probe_key_offset | o;
     for (uint32_t c = 0; c < d->nproj; ++c) lay |= d->proj[c].src_offset | d->proj[c].bytes | h->jd.dst_off[c];
     h->jd.layout_a4 = aligned4(lay) ? 1u : 0u;
@@ -739,8 +819,7 @@ int hj_build_table(pgpu_hj* h) {
     hipLaunchKernelGGL(k_jclear, dim3(grid_for(cap)), dim3(256), 0, st, h->d_table, cap);
     PGPU_LAUNCH_CHECK();
     if (h->nbuild) {
-        hipLaunchKernelGGL(k_jbuild, dim3(grid_for(h->nbuild)), dim3(256), 0, st, h->d_build, h->nbuild,
-                           h->jd.d.build_row_bytes, h->jd.d.build_key_offset, h->jd.d.build_key_bytes,
+        hipLaunchKernelGGL(k_jbuild, dim3(grid_for(h->nbuild)), dim3(256), 0, st, h->jd, h->d_build, h->nbuild,
                            h->d_table, h->logcap);
         PGPU_LAUNCH_CHECK();
     }
@@ -764,7 +843,8 @@ int hj_probe_rows(pgpu_hj* h, pgpu_hj::Slot& s, const char* d_in, uint64_t n, ui
     const unsigned nb = (unsigned)ceil_div(n, chunk);
     PGPU_TRY(slot_reserve(h->ctx, &s.d_rec, &s.rec_cap, n * 8));
     const auto* rec = (const unsigned long long*)s.d_rec;
-    hipLaunchKernelGGL(k_jcount, dim3(nb), dim3(256), 0, s.st, h->jd, h->d_table, h->logcap, d_in, n, chunk,
+    hipLaunchKernelGGL(k_jcount, dim3(nb), dim3(256), 0, s.st, h->jd, h->d_build, h->d_table, h->logcap, d_in, n,
+                       chunk,
                        s.d_cursor, (unsigned long long*)s.d_rec);
     PGPU_LAUNCH_CHECK();
     for (int attempt = 0; attempt < 2; ++attempt) {

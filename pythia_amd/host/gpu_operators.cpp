@@ -60,6 +60,16 @@ static bool shared_size(Operator* op, uint64_t* n) {
 
 static bool is_int_key(ColumnType t) { return t == CT_INTEGER || t == CT_LONG || t == CT_DATE; }
 
+// enum pgpu_col_type of a join key column (int, long and date: integers by value)
+static uint32_t key_type(ColumnType t) {
+    switch (t) {
+        case CT_DECIMAL: return PGPU_COL_F64;
+        case CT_CHAR: return PGPU_COL_CHAR;
+        case CT_INTEGER: return PGPU_COL_INT32;
+        default: return PGPU_COL_INT64;
+    }
+}
+
 // createProjectionVector (operators/join.cpp:141-182): "B$i" / "P$j".
 static std::vector<GpuHashJoinOp::JoinPrjT> parse_projection(const libconfig::Setting& line) {
     std::vector<GpuHashJoinOp::JoinPrjT> ret;
@@ -129,8 +139,15 @@ void GpuHashJoinOp::init(libconfig::Config& root, libconfig::Setting& node) {
     std::string policy = node["allocpolicy"];
     if (policy == "striped")
         fprintf(stderr, " ** NUMA POLICY WARNING: gpuhashjoin keeps its table in HBM; allocpolicy is ignored.\n");
-    if (!is_int_key(bs.getColumnType(joinattr1)) || !is_int_key(ps.getColumnType(joinattr2)))
-        throw InvalidParameter("gpuhashjoin joins on int, long or date keys");
+    // the key pairs Comparator::init accepts for the join's NotEqual
+    // (comparator.cpp:43-413): int / long / dec with each other (by value,
+    // as doubles when a dec is involved), date with date, char with char
+    {
+        const ColumnType bt = bs.getColumnType(joinattr1), pt = ps.getColumnType(joinattr2);
+        auto num = [](ColumnType t) { return t == CT_INTEGER || t == CT_LONG || t == CT_DECIMAL; };
+        if (!((num(bt) && num(pt)) || (bt == CT_DATE && pt == CT_DATE) || (bt == CT_CHAR && pt == CT_CHAR)))
+            throw UnknownComparisonException();
+    }
     knobs.read(node);
     tables.assign(groupleader.size(), nullptr);
     state.assign(MAX_THREADS, ThreadState());
@@ -171,6 +188,8 @@ void GpuHashJoinOp::threadInit(unsigned short tid) {
             d.probe_row_bytes = ps.getTupleSize();
             d.probe_key_offset = ps.getColumnOffset(joinattr2);
             d.probe_key_bytes = ps.getColumnWidth(joinattr2);
+            d.build_key_type = key_type(bs.getColumnType(joinattr1));
+            d.probe_key_type = key_type(ps.getColumnType(joinattr2));
             d.nproj = (uint32_t)projection.size();
             if (d.nproj > PGPU_MAX_PROJ) throw InvalidParameter("too many projected columns");
             unsigned buildcol = 1;
@@ -439,6 +458,7 @@ bool GpuHashJoinOp::canFuseCountSum(int outcol) const {
     Schema& bs = buildOp->getOutSchema();
     Schema& ps = probeOp->getOutSchema();
     const unsigned kb = bs.getColumnWidth(joinattr1);
+    if (!is_int_key(bs.getColumnType(joinattr1)) || !is_int_key(ps.getColumnType(joinattr2))) return false;
     if ((kb != 4 && kb != 8) || ps.getColumnWidth(joinattr2) != kb) return false;
     uint64_t nb = 0, np = 0;  // the radix plan takes fewer than 2^32 rows a side
     if (!shared_size(buildOp, &nb) || !shared_size(probeOp, &np) || nb >= (1ull << 32) || np >= (1ull << 32))

@@ -51,6 +51,10 @@ class UnknownHashException : public std::runtime_error {
   public:
     UnknownHashException() : std::runtime_error("unknown hash function") {}
 };
+class UnknownComparisonException : public std::runtime_error {
+  public:
+    UnknownComparisonException() : std::runtime_error("no comparison between these column types") {}
+};
 
 class Visitor;
 

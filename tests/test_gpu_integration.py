@@ -45,6 +45,20 @@ def test_join_output_multiset_same_with_gpu_operators(threads):
         (cpu["rows"], cpu["row_bytes"], cpu["sum"], cpu["xor"])
 
 
+@pytest.mark.parametrize("keys,nr,ns", [("dec", 30000, 200000), ("mixed", 3000, 20000), ("char", 30000, 200000),
+                                        ("charw", 3000, 20000)])
+def test_dec_and_char_keys_same_with_gpu_operators(keys, nr, ns):
+    """Join keys the reference compares through Comparator (comparator.cpp:
+    43-413): dec against dec (NaN keys match nothing), an int build key
+    against a dec probe key (as doubles), char(7) against char(7) and
+    char(5) against char(9) (strncmp over the shorter width)."""
+    cpu = run("rows", nr, ns, 3, "cpu", keys)
+    gpu = run("rows", nr, ns, 3, "gpu", keys)
+    assert cpu["rows"] > ns // 2
+    assert (gpu["rows"], gpu["row_bytes"], gpu["sum"], gpu["xor"]) == \
+        (cpu["rows"], cpu["row_bytes"], cpu["sum"], cpu["xor"])
+
+
 @pytest.mark.parametrize("global_", [0, 1])
 def test_join_groupby_same_with_gpu_operators(global_):
     cpu = run("groups", 50000, 400000, 4, "cpu", global_)

@@ -473,3 +473,87 @@ def test_budget_change_while_resident_copies_are_held():
     finally:
         P.clear_tables()
         P.clear_resident()
+
+
+@pytest.mark.parametrize("case", ["dec-dec", "long-dec", "dec-int", "char-char", "char5-char9"])
+def test_dec_and_char_join_keys_match_oracle(case):
+    """Join keys beyond integers, compared as the reference's Comparator does
+    (comparator.cpp:43-413, rawcompfns.cpp): dec keys as doubles (also
+    against int / long keys; a NaN key matches nothing), char keys by strncmp
+    over the shorter width.  Plans that hash the key bytes (`bytes`) or use
+    one bucket (`alwayszero`, for keys whose equal values have different
+    bytes) give the oracle's output multiset byte for byte."""
+    rng = np.random.default_rng(abs(hash(case)) % 1000)
+    nb, np_ = 2000, 6000
+    bkt, pkt = case.split("-")
+    width = {"char": 7, "char5": 5, "char9": 9}
+
+    def col(kind):  # the oracle's char width counts the terminator: char(n) is ("char", n + 1)
+        return ("char", width[kind] + 1) if kind.startswith("char") else (kind,)
+
+    def keys(kind, v):
+        if kind == "dec":
+            out = v.astype(np.float64) * 0.5
+            out[::97] = np.nan
+            return out
+        if kind.startswith("char"):
+            return np.array([b"k%d" % x for x in v], dtype=f"S{width[kind] + 1}")
+        return v.astype(np.int32 if kind == "int" else np.int64)
+
+    bv = rng.integers(0, 700, nb) * (2 if pkt == "dec" and bkt != "dec" else 1)
+    pv = rng.integers(0, 800, np_) * (2 if bkt == "dec" and pkt != "dec" else 1)
+    bs = O.schema(col(bkt), ("long",))
+    ps = O.schema(col(pkt), ("int",))
+    brow = O.rows(bs, keys(bkt, bv), rng.integers(-2**60, 2**60, nb))
+    prow = O.rows(ps, keys(pkt, pv), rng.integers(-2**30, 2**30, np_).astype(np.int32))
+    same_bytes = bkt == pkt and not bkt.startswith("char5")
+    fn, hfn = ("bytes", O.H_BYTES) if same_bytes else ("alwayszero", O.H_ALWAYSZERO)
+    proj = ["P$1", "B$1", "B$0", "P$0"]
+    d = O.join_desc(bs, ps, 0, 0, proj, hashfn=hfn, buckets=256, tuplesperbucket=4, threads=2)
+    exp, _ = O.hashjoin(d, brow, prow, cap=len(prow) * 40)
+    assert len(exp) > np_ // 4
+    names = {"int": '"int"', "long": '"long"', "dec": '"dec"'}
+    sch = lambda k, other: (f'"char({width[k]})"' if k.startswith("char") else names[k]) + ", " + other  # noqa: E731
+    plan = textwrap.dedent(f"""
+        path = "./"; buffsize = 4096;
+        treeroot = {{ name = "merge"; input = {{ name = "join"; build = {{ name = "r"; }}; probe = {{ name = "s"; }}; }}; }};
+        merge = {{ type = "merge"; threads = 2; }};
+        join = {{ type = "gpuhashjoin"; hash = {{ fn = "{fn}"; buckets = 256; }}; threadgroups = ( [ 0, 1 ] );
+                 tuplesperbucket = 4; allocpolicy = "local"; buildjattr = 0; probejattr = 0;
+                 projection = ( "P$1", "B$1", "B$0", "P$0" ); gpu = {{ probebatch = 1000L; }}; }};
+        r = {{ type = "memscan"; table = "dk_r"; schema = ( {sch(bkt, '"long"')} ); }};
+        s = {{ type = "memscan"; table = "dk_s"; schema = ( {sch(pkt, '"int"')} ); }};
+    """)
+    try:
+        P.register_table("dk_r", brow)
+        P.register_table("dk_s", prow)
+        out, meta = P.run(plan)
+        rb = meta["row_bytes"]
+        assert len(out) == len(exp)
+        got = sorted(out.tobytes()[i * rb:(i + 1) * rb] for i in range(len(out)))
+        want = sorted(exp.tobytes()[i * rb:(i + 1) * rb] for i in range(len(exp)))
+        assert got == want
+    finally:
+        P.clear_tables()
+
+
+def test_incomparable_join_keys_are_refused():
+    """A date key against a long key, or a char key against an int key, has
+    no Comparator (comparator.cpp:376-378, :132-135): the plan is refused."""
+    for bt, pt in (('"date(%Y-%m-%d)"', '"long"'), ('"char(4)"', '"int"')):
+        plan = textwrap.dedent(f"""
+            path = "./"; buffsize = 4096;
+            treeroot = {{ name = "join"; build = {{ name = "r"; }}; probe = {{ name = "s"; }}; }};
+            join = {{ type = "gpuhashjoin"; hash = {{ fn = "alwayszero"; buckets = 16; }}; threadgroups = ( [ 0 ] );
+                     tuplesperbucket = 4; allocpolicy = "local"; buildjattr = 0; probejattr = 0;
+                     projection = ( "B$0" ); }};
+            r = {{ type = "memscan"; table = "ic_r"; schema = ( {bt} ); }};
+            s = {{ type = "memscan"; table = "ic_s"; schema = ( {pt} ); }};
+        """)
+        try:
+            P.register_table("ic_r", np.zeros(1, np.dtype((np.void, 5 if "char" in bt else 8))))
+            P.register_table("ic_s", np.zeros(1, np.dtype((np.void, 4 if "int" in pt else 8))))
+            with pytest.raises(P.QueryError, match="comparison"):
+                P.run(plan)
+        finally:
+            P.clear_tables()
