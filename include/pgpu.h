@@ -322,6 +322,18 @@ int pgpu_server_now(pgpu_server* s, double* t);
 int pgpu_server_replay(pgpu_server* s, const uint32_t* tables, const void* const* probes, const uint64_t* ms,
                        const double* offsets, uint64_t n, pgpu_served_result* out);
 int pgpu_server_stats_get(pgpu_server* s, pgpu_server_stats* st);
+/* Persisting resident tables (batched mode): a table's radix-partitioned
+ * (key, payload) rows and their bounds saved as one memory segment at
+ * `path` (/dev/shm/<name> through shm_open, any other path through open)
+ * and attached by a server without the table's columns and without the
+ * partition pass (a warm start: the rows are uploaded and the hash slots
+ * derived from them by one kernel).  Stand in for HashTable::serialize /
+ * deserialize (util/hashtable.cpp:99-201, segments made by
+ * creatememsegment / attachmemsegment) and the segment writer
+ * (operators/memsegmentwriter.cpp:154-462).  Loading follows the rules of
+ * pgpu_server_add_table (before any query; the HBM budget applies). */
+int pgpu_server_save_table(pgpu_server* s, uint32_t table, const char* path);
+int pgpu_server_load_table(pgpu_server* s, const char* path, uint32_t* table_id);
 /* The C5 query trace: query q's table (uniform over ntables), probe rows
  * round(2^U(lo, hi)) and Poisson arrival offset (seconds, rate per second). */
 int pgpu_c5_trace(uint64_t seed, uint64_t count, uint32_t ntables, double lo, double hi, double rate,

@@ -13,6 +13,7 @@
 #include <time.h>
 
 #include "../include/pgpu_gen.h"
+#include <stdio.h>
 
 /* ===================================================================== */
 /* schema (schema.cpp:82-117): packed offsets, no alignment padding       */
@@ -375,6 +376,69 @@ unsigned orc_ht_bucket_tuples(orc_htable* t, unsigned bucket, void** out, unsign
 }
 
 uint64_t orc_ht_spills(const orc_htable* t) { return t->spills; }
+
+/* HashTable::serialize / deserialize (util/hashtable.cpp:99-201) for a table
+ * of one partition: the image of its bucket array, each bucket a 16-byte
+ * BucketHeader (util/hashtable.h:244-267: the 1-byte lock at offset 0, the
+ * uint16 `used` at offset 2, the overflow pointer at offset 8, NULL) followed
+ * by `bucksize` bytes of tuples, buckets (16 + bucksize) bytes apart, written
+ * into a file (the reference's creatememsegment makes /dev/shm segments or
+ * files the same way).  A table that spilled has overflow pointers that
+ * cannot be written: refused, as the reference asserts spills == 0.  Bytes
+ * the reference leaves unwritten (header padding, bucket space past `used`)
+ * are zero here and ignored on reading. */
+#define ORC_SEG_BH 16
+int orc_ht_serialize(orc_htable* t, const char* path)
+{
+    if (t->spills) return -1;
+    FILE* f = fopen(path, "wb");
+    if (!f) return -2;
+    const size_t stride = ORC_SEG_BH + t->bucksize;
+    char* buf = (char*)calloc(1, stride);
+    int rc = 0;
+    for (unsigned b = 0; b < t->nbuckets && !rc; ++b) {
+        orc_bh* bh = header(t, b);
+        memset(buf, 0, stride);
+        uint16_t used = (uint16_t)bh->used;
+        memcpy(buf + 2, &used, 2);
+        memcpy(buf + ORC_SEG_BH, (char*)bh + sizeof(orc_bh), bh->used);
+        if (fwrite(buf, 1, stride, f) != stride) rc = -3;
+    }
+    free(buf);
+    if (fclose(f) != 0 && !rc) rc = -3;
+    return rc;
+}
+
+int orc_ht_deserialize(orc_htable* t, const char* path)
+{
+    FILE* f = fopen(path, "rb");
+    if (!f) return -2;
+    const size_t stride = ORC_SEG_BH + t->bucksize;
+    char* buf = (char*)malloc(stride);
+    int rc = 0;
+    for (unsigned b = 0; b < t->nbuckets && !rc; ++b) {
+        if (fread(buf, 1, stride, f) != stride) {
+            rc = -3;
+            break;
+        }
+        uint16_t used;
+        uint64_t next;
+        memcpy(&used, buf + 2, 2);
+        memcpy(&next, buf + 8, 8);
+        if (used > t->bucksize || next != 0) {
+            rc = -4; /* not an image of a spill-free table of this geometry */
+            break;
+        }
+        orc_bh* bh = header(t, b);
+        bh->lock = 0;
+        bh->used = used;
+        bh->next = NULL;
+        memcpy((char*)bh + sizeof(orc_bh), buf + ORC_SEG_BH, used);
+    }
+    free(buf);
+    fclose(f);
+    return rc;
+}
 
 /* ===================================================================== */
 /* hash join (operators/join.cpp:141-675)                                  */

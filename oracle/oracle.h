@@ -78,6 +78,10 @@ void* orc_ht_allocate(orc_htable* t, unsigned bucket); /* util/hashtable.cpp:230
 /* Visits bucket tuples in chain order; returns the count visited. */
 unsigned orc_ht_bucket_tuples(orc_htable* t, unsigned bucket, void** out, unsigned cap);
 uint64_t orc_ht_spills(const orc_htable* t);
+/* HashTable::serialize / deserialize (util/hashtable.cpp:99-201): the bucket
+ * array image of a spill-free table in the reference's byte layout. */
+int orc_ht_serialize(orc_htable* t, const char* path);
+int orc_ht_deserialize(orc_htable* t, const char* path);
 
 /* ---------------- hash join (operators/join.cpp) ----------------------- */
 typedef struct orc_join_desc {

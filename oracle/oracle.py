@@ -98,6 +98,8 @@ def lib():
             "orc_ht_allocate": (P, [P, C.c_uint]),
             "orc_ht_bucket_tuples": (C.c_uint, [P, C.c_uint, C.POINTER(P), C.c_uint]),
             "orc_ht_spills": (C.c_uint64, [P]),
+            "orc_ht_serialize": (C.c_int, [P, C.c_char_p]),
+            "orc_ht_deserialize": (C.c_int, [P, C.c_char_p]),
             "orc_join_outschema": (None, [C.POINTER(JoinDesc), C.POINTER(Schema)]),
             "orc_hashjoin": (C.c_int64, [C.POINTER(JoinDesc), P, C.c_uint64, P, C.c_uint64, P, C.c_uint64,
                                          C.POINTER(JoinStats)]),

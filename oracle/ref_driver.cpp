@@ -18,6 +18,8 @@
 //          (the batched-join family: one plan per join, buckets = 2^ceil(log2 n))
 //        refjoin hash <int|long|chars> <hash node> <value>...
 //          (hash values through the reference's TupleHasher::create)
+//        refjoin htwrite|htread <buckets> <tuples per bucket> <values> <path>
+//          (HashTable::serialize / deserialize of int tuples)
 //        refjoin c3 <dimension rows> <fact rows> <threads>
 //          (join + group-by: hashjoin(D, F) -> aggregate_sum(field 0) per worker ->
 //           merge; the consumer folds the per-worker groups)
@@ -142,7 +144,8 @@ static void usage() {
             "usage: refjoin <c1|c2> <log2R> <log2S> <threads> <zipf> <buckets> <tpb> <sum|count|probesum>\n"
             "       refjoin c4 <first> <joins> <threads> <tpb> <sum|probesum>\n"
             "       refjoin c3 <dimension rows> <fact rows> <threads>\n"
-            "       refjoin hash <int|long|chars> <hash node> <value>...\n");
+            "       refjoin hash <int|long|chars> <hash node> <value>...\n"
+            "       refjoin htwrite|htread <buckets> <tuples per bucket> <values> <path>\n");
 }
 
 // The batched family: joins first..first+count-1, each its own plan; inputs
@@ -361,8 +364,59 @@ static int run_hash(int argc, char** argv) {
     return 0;
 }
 
+// refjoin htwrite <buckets> <tuples per bucket> <values> <path>
+//   the reference's HashTable (util/hashtable.cpp:57-97) of int tuples, value
+//   v in bucket v % buckets (testhashtable.cpp:87-135 with more than one
+//   tuple per bucket), serialized to <path> (hashtable.cpp:99-121)
+// refjoin htread <buckets> <tuples per bucket> <values> <path>
+//   a table of that geometry deserialized from <path> (hashtable.cpp:178-201)
+//   and iterated: prints how many values appear exactly once, in their bucket
+static int run_ht(int argc, char** argv, bool write) {
+    if (argc < 6) {
+        usage();
+        return 2;
+    }
+    const unsigned nb = (unsigned)atoi(argv[2]), tpb = (unsigned)atoi(argv[3]);
+    const int nv = atoi(argv[4]);
+    HashTable ht;
+    ht.init(nb, tpb * sizeof(int), sizeof(int), std::vector<char>(), 0);
+    ht.bucketclear(0, 1);
+    if (write) {
+        for (int v = 0; v < nv; ++v) *(int*)ht.allocate(v % nb, 0) = v;
+        if (ht.statSpills() != 0) {
+            fprintf(stderr, "spills\n");
+            return 1;
+        }
+        ht.serialize(argv[5], 0);
+        printf("{\"written\": %d}\n", nv);
+    } else {
+        ht.deserialize(argv[5], 0);
+        std::vector<int> seen(nv, 0);
+        int bad = 0;
+        HashTable::Iterator it = ht.createIterator();
+        for (unsigned b = 0; b < nb; ++b) {
+            ht.placeIterator(it, b);
+            void* tup;
+            while ((tup = it.next())) {
+                const int v = *(int*)tup;
+                if (v < 0 || v >= nv || (unsigned)v % nb != b)
+                    ++bad;
+                else
+                    ++seen[v];
+            }
+        }
+        int once = 0;
+        for (int v = 0; v < nv; ++v) once += seen[v] == 1;
+        printf("{\"once\": %d, \"bad\": %d}\n", once, bad);
+    }
+    ht.destroy();
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && std::string(argv[1]) == "hash") return run_hash(argc, argv);
+    if (argc >= 2 && std::string(argv[1]) == "htwrite") return run_ht(argc, argv, true);
+    if (argc >= 2 && std::string(argv[1]) == "htread") return run_ht(argc, argv, false);
     if (argc >= 2 && std::string(argv[1]) == "c4") return run_c4(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "c3") return run_c3(argc, argv);
     if (argc < 9) {

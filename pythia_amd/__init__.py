@@ -368,6 +368,17 @@ class Server:
         call("pgpu_server_add_table", self.handle, C.c_void_p(_addr(key)), C.c_void_p(_addr(pay)), n, C.byref(t))
         return t.value
 
+    def save_table(self, table: int, path: str):
+        """Writes the table's built form to a memory segment (/dev/shm/<name>
+        or a file)."""
+        call("pgpu_server_save_table", self.handle, table, path.encode())
+
+    def load_table(self, path: str) -> int:
+        """Attaches a saved table without rebuilding it; returns its id."""
+        t = C.c_uint32()
+        call("pgpu_server_load_table", self.handle, path.encode(), C.byref(t))
+        return t.value
+
     def prepare_plain(self, max_probe: int):
         call("pgpu_server_prepare_plain", self.handle, max_probe)
 

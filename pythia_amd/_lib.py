@@ -144,6 +144,8 @@ _PROTOS = {
     "pgpu_server_replay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                      C.POINTER(ServedResult)]),
     "pgpu_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "pgpu_server_save_table": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p]),
+    "pgpu_server_load_table": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint32)]),
     "pgpu_c5_trace": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                 C.c_void_p, C.c_void_p, C.c_void_p]),
     "pgpu_part_create": (C.c_int, [C.c_void_p, C.POINTER(PartDesc), C.POINTER(C.c_void_p)]),

@@ -36,7 +36,13 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <string>
 #include <thread>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "batch_internal.h"
 #include "join_common.h"
@@ -95,12 +101,15 @@ __global__ __launch_bounds__(256) void k_tclear(SSlot* __restrict__ t, uint32_t*
 
 // special[0..1]: count / sum of empty-marker keys; special[2]: inserts that
 // found their key already present (duplicates).
+// Row i's key is keys[i * stride], its payload pays[i * stride] (stride 1:
+// columns; 2: interleaved (key, payload) rows).
 __global__ __launch_bounds__(256) void k_tbuild(const long long* __restrict__ keys, const long long* __restrict__ pays,
-                                                uint64_t n, SSlot* __restrict__ t, uint32_t* __restrict__ cnt,
-                                                int shift, uint64_t mask, unsigned long long* __restrict__ special) {
+                                                uint64_t n, uint32_t stride, SSlot* __restrict__ t,
+                                                uint32_t* __restrict__ cnt, int shift, uint64_t mask,
+                                                unsigned long long* __restrict__ special) {
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-        const long long k = keys[i];
-        const unsigned long long v = (unsigned long long)pays[i];
+        const long long k = keys[i * stride];
+        const unsigned long long v = (unsigned long long)pays[i * stride];
         if (k == INT64_MIN) {
             atomicAdd(&special[0], 1ull);
             atomicAdd(&special[1], v);
@@ -758,7 +767,7 @@ int pgpu_server_add_table(pgpu_server* s, const void* d_key, const void* d_pay, 
                            t.slots, t.cnt, t.cap, t.special);
         if (n)
             hipLaunchKernelGGL(k_tbuild, dim3((unsigned)std::min<uint64_t>(ceil_div(n, 256), 4096)), dim3(256), 0, st,
-                               (const long long*)d_key, (const long long*)d_pay, n, t.slots, t.cnt, 64 - logcap,
+                               (const long long*)d_key, (const long long*)d_pay, n, 1u, t.slots, t.cnt, 64 - logcap,
                                t.cap - 1, t.special);
         PGPU_LAUNCH_CHECK();
         PGPU_HIP(hipStreamSynchronize(st));
@@ -890,6 +899,221 @@ int pgpu_server_stats_get(pgpu_server* s, pgpu_server_stats* st) {
     st->queued = s->pending.size();
     st->partitioned = s->part_queries;
     return PGPU_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== persistence
+// A served table saved as one memory segment and attached by a server
+// without its columns and without the partition pass: HashTable::serialize
+// / deserialize (util/hashtable.cpp:99-201), which copy a table's bucket
+// memory into a segment made by creatememsegment and back.  The segment
+// holds the table's radix-partitioned (key, payload) rows and their bounds;
+// the hash slots, about twice as many bytes as the rows and mostly empty,
+// are derived again on attach by one build kernel over the rows.
+// Layout: a 128-byte header, then the rows and the bounds, each at a 4 KiB
+// boundary.
+namespace {
+struct SegHeader {
+    char magic[8];     // "PGPUTBL1"
+    uint32_t version;  // 1
+    uint32_t key_bytes;
+    uint64_t n;        // rows
+    uint32_t bits;     // partition bits (2^bits + 1 bounds)
+    uint32_t pad;
+    uint64_t rows_bytes, bounds_bytes;
+    uint64_t reserved[10];
+};
+static_assert(sizeof(SegHeader) == 128, "segment header");
+constexpr uint64_t kSegAlign = 4096;
+constexpr char kSegMagic[8] = {'P', 'G', 'P', 'U', 'T', 'B', 'L', '1'};
+constexpr int kSegCopyThreads = 8;
+constexpr uint64_t kSegStage = 64ull << 20;  // pinned staging block of an attach
+
+inline uint64_t seg_round(uint64_t b) { return (b + kSegAlign - 1) & ~(kSegAlign - 1); }
+
+// As creatememsegment / attachmemsegment (util/hashtable.cpp:123-176): a
+// /dev/shm/<name> path goes through shm_open, any other through open.
+int seg_fd(const char* path, bool create) {
+    const std::string p(path);
+    const int flags = create ? (O_RDWR | O_CREAT | O_TRUNC) : O_RDONLY;
+    if (p.compare(0, 9, "/dev/shm/") == 0) return shm_open(p.substr(8).c_str(), flags, 0644);
+    return open(path, flags, 0644);
+}
+
+// pread of `bytes` at file offset `off` into dst, split over
+// kSegCopyThreads threads (one thread's copy runs at a fraction of the
+// host's memory bandwidth); false on a short read.
+bool parallel_read(int fd, char* dst, uint64_t off, uint64_t bytes) {
+    auto part = [fd, dst, off](uint64_t a, uint64_t b) {
+        while (a < b) {
+            const ssize_t r = pread(fd, dst + a, b - a, (off_t)(off + a));
+            if (r <= 0) return false;
+            a += (uint64_t)r;
+        }
+        return true;
+    };
+    if (bytes < (8u << 20)) return part(0, bytes);
+    std::thread th[kSegCopyThreads];
+    bool ok[kSegCopyThreads];
+    const uint64_t per = (bytes / kSegCopyThreads + 4095) & ~4095ull;
+    for (int i = 0; i < kSegCopyThreads; ++i) {
+        const uint64_t a = std::min<uint64_t>(bytes, i * per), b = std::min<uint64_t>(bytes, a + per);
+        th[i] = std::thread([&part, &ok, i, a, b] { ok[i] = part(a, b); });
+    }
+    bool all = true;
+    for (int i = 0; i < kSegCopyThreads; ++i) {
+        th[i].join();
+        all = all && ok[i];
+    }
+    return all;
+}
+}  // namespace
+
+extern "C" {
+
+int pgpu_server_save_table(pgpu_server* s, uint32_t table, const char* path) {
+    if (!s || !path) return fail(PGPU_EINVAL, "pgpu_server_save_table: null argument");
+    if (s->cfg.mode != PGPU_SERVE_BATCHED) return fail(PGPU_ESTATE, "only a batched server keeps built tables");
+    if (table >= s->tables.size()) return fail(PGPU_EINVAL, "unknown table %u", table);
+    PGPU_TRY(ctx_activate(s->ctx));
+    const Table& t = s->tables[table];
+    SegHeader h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, kSegMagic, 8);
+    h.version = 1;
+    h.key_bytes = 8;
+    h.n = t.n;
+    h.bits = t.bits;
+    h.rows_bytes = 16 * std::max<uint64_t>(t.n, 1);
+    h.bounds_bytes = 4 * ((1ull << t.bits) + 1);
+    const uint64_t size = kSegAlign + seg_round(h.rows_bytes) + seg_round(h.bounds_bytes);
+    const int fd = seg_fd(path, true);
+    if (fd < 0) return fail(PGPU_EINVAL, "pgpu_server_save_table: cannot create segment %s", path);
+    if (ftruncate(fd, (off_t)size) != 0) {
+        close(fd);
+        return fail(PGPU_ENOMEM, "pgpu_server_save_table: cannot size segment %s to %llu bytes", path,
+                    (unsigned long long)size);
+    }
+    char* m = (char*)mmap(nullptr, size, PROT_READ | PROT_WRITE, MAP_SHARED, fd, 0);
+    close(fd);
+    if (m == MAP_FAILED) return fail(PGPU_ENOMEM, "pgpu_server_save_table: cannot map segment %s", path);
+    memcpy(m, &h, sizeof h);
+    int rc = PGPU_OK;
+    if (hipMemcpy(m + kSegAlign, (const char*)s->R + 16 * t.rdst, h.rows_bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(m + kSegAlign + seg_round(h.rows_bytes), s->RB + t.rbnd, h.bounds_bytes, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        rc = fail(PGPU_EDEVICE, "pgpu_server_save_table: copy of table %u failed", table);
+    munmap(m, size);
+    return rc;
+}
+
+int pgpu_server_load_table(pgpu_server* s, const char* path, uint32_t* table_id) {
+    if (!s || !path || !table_id) return fail(PGPU_EINVAL, "pgpu_server_load_table: null argument");
+    if (s->cfg.mode != PGPU_SERVE_BATCHED) return fail(PGPU_ESTATE, "only a batched server attaches built tables");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->submitted != s->completed || !s->pending.empty())
+            return fail(PGPU_ESTATE, "tables are registered before queries are submitted");
+    }
+    const int fd = seg_fd(path, false);
+    if (fd < 0) return fail(PGPU_EINVAL, "pgpu_server_load_table: no segment %s", path);
+    struct stat sb;
+    SegHeader h;
+    if (fstat(fd, &sb) != 0 || (uint64_t)sb.st_size < kSegAlign || !parallel_read(fd, (char*)&h, 0, sizeof h)) {
+        close(fd);
+        return fail(PGPU_EINVAL, "pgpu_server_load_table: %s is not a table segment", path);
+    }
+    const uint64_t size = (uint64_t)sb.st_size;
+    const bool ok = memcmp(h.magic, kSegMagic, 8) == 0 && h.version == 1 && h.key_bytes == 8 &&
+                    h.n < (1ull << 32) && h.bits <= 16 && h.rows_bytes == 16 * std::max<uint64_t>(h.n, 1) &&
+                    h.bounds_bytes == 4 * ((1ull << h.bits) + 1) &&
+                    kSegAlign + seg_round(h.rows_bytes) + seg_round(h.bounds_bytes) <= size;
+    if (!ok) {
+        close(fd);
+        return fail(PGPU_EINVAL, "pgpu_server_load_table: %s is not a valid table segment", path);
+    }
+    int rc = ctx_activate(s->ctx);
+    Table t;
+    t.n = h.n;
+    t.bits = h.bits;
+    t.cap = 1ull << std::max<uint32_t>(4, ilog2_ceil(2 * std::max<uint64_t>(t.n, 1)));
+    t.bytes = t.cap * (sizeof(SSlot) + 4) + 32;
+    const uint64_t nb = (1ull << t.bits) + 1;
+    if (rc == PGPU_OK && s->resident + t.bytes + h.rows_bytes + h.bounds_bytes > s->cfg.hbm_budget)
+        rc = fail(PGPU_EBUDGET, "resident table of %llu rows exceeds the server's HBM budget", (unsigned long long)t.n);
+    if (rc == PGPU_OK) rc = ctx_alloc(s->ctx, t.cap * sizeof(SSlot), (void**)&t.slots);
+    if (rc == PGPU_OK) rc = ctx_alloc(s->ctx, t.cap * 4, (void**)&t.cnt);
+    if (rc == PGPU_OK) rc = ctx_alloc(s->ctx, 32, (void**)&t.special);
+    if (rc == PGPU_OK) rc = grow(s, &s->R, &s->R_cap, s->R_used, s->R_used + std::max<uint64_t>(t.n, 1), 16);
+    if (rc == PGPU_OK) rc = grow(s, (void**)&s->RB, &s->RB_cap, s->RB_used, s->RB_used + nb, 4);
+    // rows: the segment into two pinned staging blocks in turn (several
+    // threads copy one block while the previous one's DMA runs)
+    char* pin[2] = {nullptr, nullptr};
+    uint64_t pin_cap = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (int i = 0; i < 2 && rc == PGPU_OK; ++i) {
+        rc = ctx_host_alloc_reuse(s->ctx, kSegStage, (void**)&pin[i], &pin_cap);
+        if (rc == PGPU_OK && hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
+            rc = fail(PGPU_EDEVICE, "hipEventCreateWithFlags failed");
+    }
+    hipStream_t st = s->ctx->stream;
+    if (rc == PGPU_OK) {
+        t.rdst = s->R_used;
+        t.rbnd = s->RB_used;
+        char* dst = (char*)s->R + 16 * t.rdst;
+        uint64_t i = 0;
+        for (uint64_t off = 0; off < h.rows_bytes && rc == PGPU_OK; off += kSegStage, ++i) {
+            const uint64_t len = std::min(kSegStage, h.rows_bytes - off);
+            if (i >= 2 && hipEventSynchronize(ev[i & 1]) != hipSuccess) {
+                rc = fail(PGPU_EDEVICE, "pgpu_server_load_table: staging wait failed");
+                break;
+            }
+            if (!parallel_read(fd, pin[i & 1], kSegAlign + off, len)) {
+                rc = fail(PGPU_EINVAL, "pgpu_server_load_table: short read of %s", path);
+                break;
+            }
+            if (hipMemcpyAsync(dst + off, pin[i & 1], len, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipEventRecord(ev[i & 1], st) != hipSuccess)
+                rc = fail(PGPU_EDEVICE, "pgpu_server_load_table: upload of %s failed", path);
+        }
+        // the bounds (at most 256 KiB) through the first block once its DMA is done
+        if (rc == PGPU_OK && (hipStreamSynchronize(st) != hipSuccess ||
+                              !parallel_read(fd, pin[0], kSegAlign + seg_round(h.rows_bytes), h.bounds_bytes) ||
+                              hipMemcpyAsync(s->RB + t.rbnd, pin[0], h.bounds_bytes, hipMemcpyHostToDevice, st) !=
+                                  hipSuccess))
+            rc = fail(PGPU_EDEVICE, "pgpu_server_load_table: upload of %s failed", path);
+    }
+    if (rc == PGPU_OK) {  // the hash slots, derived from the partitioned rows
+        const int logcap = (int)ilog2_ceil(t.cap);
+        const long long* rows = (const long long*)((const char*)s->R + 16 * t.rdst);
+        hipLaunchKernelGGL(k_tclear, dim3((unsigned)std::min<uint64_t>(ceil_div(t.cap, 256), 4096)), dim3(256), 0, st,
+                           t.slots, t.cnt, t.cap, t.special);
+        if (t.n)
+            hipLaunchKernelGGL(k_tbuild, dim3((unsigned)std::min<uint64_t>(ceil_div(t.n, 256), 4096)), dim3(256), 0,
+                               st, rows, rows + 1, t.n, 2u, t.slots, t.cnt, 64 - logcap, t.cap - 1, t.special);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+            rc = fail(PGPU_EDEVICE, "pgpu_server_load_table: table build failed");
+    }
+    if (hipStreamSynchronize(st) != hipSuccess && rc == PGPU_OK)
+        rc = fail(PGPU_EDEVICE, "pgpu_server_load_table: upload of %s failed", path);
+    for (int i = 0; i < 2; ++i) {
+        if (pin[i]) ctx_host_free_reuse(s->ctx, pin[i]);
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    }
+    close(fd);
+    if (rc != PGPU_OK) {
+        if (t.slots) ctx_free(s->ctx, t.slots);
+        if (t.cnt) ctx_free(s->ctx, t.cnt);
+        if (t.special) ctx_free(s->ctx, t.special);
+        return rc;
+    }
+    s->resident += t.bytes;
+    s->R_used += std::max<uint64_t>(t.n, 1);
+    s->RB_used += nb;
+    s->tables.push_back(t);
+    *table_id = (uint32_t)(s->tables.size() - 1);
+    return upload_tabs(s);
 }
 
 }  // extern "C"

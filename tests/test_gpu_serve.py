@@ -228,3 +228,55 @@ def test_c5_full_scale_poisson_2000_alone_and_in_the_mix(ctx, c5_world):
     st = s.stats()
     assert st["partitioned"] >= 2 * nbig  # the large batches took the partitioned path
     s.close()
+
+
+def test_warm_start_from_saved_segments(ctx, world):
+    """Every table of a batched server saved as a memory segment (the
+    HashTable::serialize route, util/hashtable.cpp:99-201) and attached by a
+    fresh server with no build kernel and no columns: every query gives its
+    closed form, alone and in the mix; tables with repeated keys and the
+    empty marker keep their counts and side words; a segment that is not a
+    table is refused."""
+    tabs, qt, qm, probes, exp = world
+    rng = np.random.default_rng(4)
+    dk = rng.integers(-300, 300, 5000).astype(np.int64)
+    dk[:4] = np.iinfo(np.int64).min
+    dp = rng.integers(-2**40, 2**40, 5000).astype(np.int64)
+    pk = rng.integers(-350, 350, 40000).astype(np.int64)
+    pk[:3] = np.iinfo(np.int64).min
+    dup = [ctx.upload(dk), ctx.upload(dp), ctx.upload(pk)]
+    order = np.argsort(dk, kind="stable")
+    ukeys, first = np.unique(dk[order], return_index=True)
+    cnt = np.diff(np.append(first, len(dk)))
+    ssum = np.add.reduceat(dp[order].view(np.uint64), first)
+    pos = np.clip(np.searchsorted(ukeys, pk), 0, len(ukeys) - 1)
+    hit = ukeys[pos] == pk
+    want_dup = (int((cnt[pos] * hit).sum()), int(np.sum(ssum[pos] * hit.astype(np.uint64), dtype=np.uint64)))
+    prefix = f"/dev/shm/pgpu_seg_{os.getpid()}"
+    paths = [f"{prefix}_{t}" for t in range(len(tabs) + 1)]
+    try:
+        s = _server(ctx, tabs)
+        s.add_table(dup[0], dup[1], len(dk))
+        for t, path in enumerate(paths):
+            s.save_table(t, path)
+        s.close()
+        w = pa.Server(ctx)
+        ids = [w.load_table(path) for path in paths]
+        assert ids == list(range(len(paths)))
+        mix = w.replay(qt, probes, qm, np.zeros(len(qt)))
+        assert [(r.matches, r.sum_build) for r in mix] == [(e.count, e.sum_build) for e in exp]
+        w.submit(len(tabs), dup[2], len(pk), tag=7)
+        w.drain()
+        (r,) = w.poll()
+        assert (r.matches, r.sum_build) == want_dup
+        with open(f"{prefix}_bad", "wb") as f:
+            f.write(b"not a table segment" * 500)
+        with pytest.raises(pa.PgpuError):
+            w.load_table(f"{prefix}_bad")
+        w.close()
+    finally:
+        for path in paths + [f"{prefix}_bad"]:
+            if os.path.exists(path):
+                os.unlink(path)
+        for b in dup:
+            b.free()

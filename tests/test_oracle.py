@@ -10,6 +10,7 @@ Sources of truth, none of them the oracle's own formulas:
 import collections
 import ctypes as C
 import math
+import os
 
 import numpy as np
 import pytest
@@ -380,3 +381,49 @@ def test_c5_closed_form_is_c2_family_with_offsets():
     tail = sk[1 << 13:]
     c = O.c5_closed(3, n, 1 << 13, m, threads=3)
     assert c.count == m and c.sum_build == int(sum(int(rp[pos[int(k)]]) for k in tail)) % 2**64
+
+
+def _ht_fill(nbuckets, tpb, nvalues):
+    t = O.lib().orc_ht_create(nbuckets, 4 * tpb, 4)
+    for v in range(nvalues):
+        C.cast(O.lib().orc_ht_allocate(t, v % nbuckets), C.POINTER(C.c_int32))[0] = v
+    return t
+
+
+def _ht_values_once(t, nbuckets, nvalues):
+    seen = np.zeros(nvalues, np.int64)
+    out = (C.c_void_p * 64)()
+    for b in range(nbuckets):
+        n = O.lib().orc_ht_bucket_tuples(t, b, out, 64)
+        for i in range(min(n, 64)):
+            v = C.cast(out[i], C.POINTER(C.c_int32))[0]
+            assert 0 <= v < nvalues and v % nbuckets == b
+            seen[v] += 1
+    return seen
+
+
+@pytest.mark.parametrize("tpb", [1, 8])
+def test_hashtable_serialize_round_trip(tmp_path, tpb):
+    """unit_tests/testhashtable.cpp:87-182: a table of int tuples (value v in
+    bucket v % buckets) serialized to a segment file (util/hashtable.cpp:
+    99-121) and deserialized into a fresh table of the same geometry
+    (:178-201): every value appears exactly once, in its bucket.  A table
+    that spilled into overflow chunks is refused (the reference asserts
+    spills == 0)."""
+    rng = np.random.default_rng(9 + tpb)
+    nbuckets = int(rng.integers(100, 10000)) if tpb == 1 else 256
+    nvalues = nbuckets * tpb - (0 if tpb == 1 else 37)
+    path = str(tmp_path / "ht.seg").encode()
+    t = _ht_fill(nbuckets, tpb, nvalues)
+    assert O.lib().orc_ht_spills(t) == 0
+    assert O.lib().orc_ht_serialize(t, path) == 0
+    O.lib().orc_ht_destroy(t)
+    assert os.path.getsize(path) == nbuckets * (16 + 4 * tpb)  # BucketHeader + bucket space, per bucket
+    u = O.lib().orc_ht_create(nbuckets, 4 * tpb, 4)
+    assert O.lib().orc_ht_deserialize(u, path) == 0
+    assert np.all(_ht_values_once(u, nbuckets, nvalues) == 1)
+    O.lib().orc_ht_destroy(u)
+    spilled = _ht_fill(16, 1, 40)
+    assert O.lib().orc_ht_spills(spilled) > 0
+    assert O.lib().orc_ht_serialize(spilled, path) == -1
+    O.lib().orc_ht_destroy(spilled)

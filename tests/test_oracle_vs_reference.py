@@ -156,3 +156,35 @@ def test_tpchq1magic_equals_the_reference():
     buf = np.array(vals, np.uint32)
     got = [O.lib().orc_hash_tuple(ctypes.byref(h), buf.ctypes.data + 4 * i) for i in range(len(vals))]
     assert got == ref["hashes"]
+
+
+@pytest.mark.parametrize("nb,tpb,nv", [(5000, 1, 5000), (256, 8, 2000), (1024, 3, 2900)])
+def test_hashtable_segments_interchange_with_the_reference(tmp_path, nb, tpb, nv):
+    """HashTable::serialize / deserialize (util/hashtable.cpp:99-201): a
+    segment the reference writes is read by the oracle with every value
+    once in its bucket, and one the oracle writes is read by the reference
+    the same way; both are the same size."""
+    import ctypes
+
+    ref_path, orc_path = str(tmp_path / "ref.seg"), str(tmp_path / "orc.seg")
+    out = subprocess.run([REF, "htwrite", str(nb), str(tpb), str(nv), ref_path], stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    u = O.lib().orc_ht_create(nb, 4 * tpb, 4)
+    assert O.lib().orc_ht_deserialize(u, ref_path.encode()) == 0
+    seen = np.zeros(nv, np.int64)
+    ptrs = (ctypes.c_void_p * 64)()
+    for b in range(nb):
+        n = O.lib().orc_ht_bucket_tuples(u, b, ptrs, 64)
+        for i in range(n):
+            v = ctypes.cast(ptrs[i], ctypes.POINTER(ctypes.c_int32))[0]
+            assert v % nb == b
+            seen[v] += 1
+    assert np.all(seen == 1)
+    assert O.lib().orc_ht_serialize(u, orc_path.encode()) == 0
+    O.lib().orc_ht_destroy(u)
+    assert os.path.getsize(orc_path) == os.path.getsize(ref_path)
+    out = subprocess.run([REF, "htread", str(nb), str(tpb), str(nv), orc_path], stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert json.loads(out.stdout) == {"once": nv, "bad": 0}
