@@ -40,8 +40,29 @@ METRIC = "join tuples/s + % of 8 TB/s HBM roofline; served p50/p99 vs plain per-
 REFJOIN = os.path.join(ROOT, "oracle", "_ref", "refjoin")
 
 
+CPU_RUNS = 5          # the reference plan is run this many times; the median is reported
+CPU_SHARE = 16        # host cores that one GPU's share of the box grants to its worker threads
+
+
 def cpu_cores():
-    return int(os.environ.get("PYTHIA_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    """Worker threads for the CPU baseline: the cores this process may run on
+    (nproc), capped at one GPU's host share of the machine."""
+    env = int(os.environ.get("PYTHIA_CPU_THREADS", "0"))
+    if env:
+        return env
+    return min(len(os.sched_getaffinity(0)), CPU_SHARE)
+
+
+def cpu_threads_note(cores):
+    """How the thread count was chosen, for the cpu_baseline object."""
+    return {"nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count(),
+            "threads_rule": f"T = min(nproc, {CPU_SHARE}): {CPU_SHARE} host cores are one GPU's share of the box",
+            "runs": CPU_RUNS, "statistic": "median"}
+
+
+def ref_runs(r):
+    """Formats the per-run plan times of a refjoin result."""
+    return ", ".join(f"{x:.3f}" for x in r.get("runs", [r["seconds"]]))
 
 
 def cpu_model():
@@ -69,16 +90,26 @@ def kernel_bytes_c2(nR: int, nS: int, esR: int = 16, esS: int = 8, kb: int = 8) 
 
 
 def roofline(per_kernel: dict, kb: dict, traffic_file: str | None):
+    """Roofline of the dominant kernel.  `traffic` (HBM bytes per launch) is
+    not counted in this run: it is read from a committed summary of two
+    rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) named in
+    `traffic_source`.  `achieved` divides min(algorithmic, measured) bytes by
+    the kernel's average launch time from HIP events in this run."""
     dom = max((k for k in per_kernel if k in kb), key=lambda k: per_kernel[k]["avg_ms"])
     avg_s = per_kernel[dom]["avg_ms"] / 1e3
-    achieved = kb[dom] / avg_s
-    traffic = None
+    traffic, source = None, None
     if traffic_file and os.path.exists(traffic_file):
         with open(traffic_file) as f:
             traffic = json.load(f).get(dom)
+        if traffic is not None:
+            rel = os.path.relpath(os.path.abspath(traffic_file), ROOT)
+            source = (f"{rel} (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload, scaled by a "
+                      f"copy kernel of known size; {rel[:-5]}.md gives the commands); not measured in this run")
+    moved = min(kb[dom], traffic) if traffic else kb[dom]
+    achieved = moved / avg_s
     return {
         "bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK, "traffic": traffic,
+        "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": source,
         "algorithmic_bytes_per_launch": kb[dom], "avg_launch_ms": per_kernel[dom]["avg_ms"],
     }
 
@@ -95,15 +126,16 @@ def cpu_baseline_c2(args, nR, nS, gpu_result):
     cores = cpu_cores()
     if os.path.exists(REFJOIN):
         out = subprocess.run([REFJOIN, "c2", str(args.log_build), str(args.log_probe), str(cores), str(args.zipf),
-                              str(1 << 24), "4", "sum"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
-                             timeout=900, cwd="/tmp")
+                              str(1 << 24), "4", "sum", str(CPU_RUNS)], stdout=subprocess.PIPE,
+                             stderr=subprocess.PIPE, text=True, timeout=900, cwd="/tmp")
         if out.returncode == 0:
             r = json.loads(out.stdout)
             return {
                 "value": (nR + nS) / r["seconds"], "unit": "tuples/s", "cores": cores, "kind": "reference",
                 "sample": f"the full workload through the reference's hashjoin -> aggregate_sum -> merge({cores}) "
-                          f"plan, one run of {r['seconds']:.2f} s (inputs generated outside the clock)",
+                          f"plan, median of {CPU_RUNS} runs ({ref_runs(r)} s; inputs generated outside the clock)",
                 "result_matches_gpu": bool(r["value"] % 2**64 == gpu_result.sum_build),
+                **cpu_threads_note(cores),
             }
         print(f"# reference baseline failed: {out.stderr[-300:]}", file=sys.stderr)
     from oracle import oracle as O
@@ -239,16 +271,16 @@ def cpu_baseline_c4(args, sizes, res):
     cum = np.cumsum(9 * sizes)
     count = max(1, min(int(np.searchsorted(cum, 2.5e8)) + 1, len(sizes)))
     if os.path.exists(REFJOIN):
-        out = subprocess.run([REFJOIN, "c4", "0", str(count), str(cores), "4", "sum"], stdout=subprocess.PIPE,
-                             stderr=subprocess.PIPE, text=True, timeout=900, cwd="/tmp")
+        out = subprocess.run([REFJOIN, "c4", "0", str(count), str(cores), "4", "sum", str(CPU_RUNS)],
+                             stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=900, cwd="/tmp")
         if out.returncode == 0:
             r = json.loads(out.stdout)
             gpu32 = [int(np.array([x.sum_build & 0xFFFFFFFF], np.uint32).view(np.int32)[0]) for x in res[:count]]
             return {"value": r["tuples"] / r["seconds"], "unit": "tuples/s", "cores": cores, "kind": "reference",
                     "sample": f"joins 0..{count - 1} of the batch ({r['tuples']} tuples), each through the "
                               f"reference's hashjoin -> aggregate_sum -> merge({cores}) plan with 2^ceil(log2 n) "
-                              f"modulo buckets; {r['seconds']:.2f} s summed plan time",
-                    "result_matches_gpu": r["values"] == gpu32}
+                              f"modulo buckets; summed plan time, median of {CPU_RUNS} runs ({ref_runs(r)} s)",
+                    "result_matches_gpu": r["values"] == gpu32, **cpu_threads_note(cores)}
         print(f"# reference baseline failed: {out.stderr[-300:]}", file=sys.stderr)
     from oracle import oracle as O
 
@@ -475,16 +507,16 @@ def run_c1(args):
                                                sum32 == int(np.int64(dres.sum_build % 2**32).astype(np.int32)))},
     }
     if not args.no_cpu_baseline and os.path.exists(REFJOIN):
-        out = subprocess.run([REFJOIN, "c1", "20", "24", str(T), "0", "65536", "16", "sum"], stdout=subprocess.PIPE,
-                             stderr=subprocess.PIPE, text=True, timeout=900, cwd="/tmp")
+        out = subprocess.run([REFJOIN, "c1", "20", "24", str(T), "0", "65536", "16", "sum", str(CPU_RUNS)],
+                             stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=900, cwd="/tmp")
         if out.returncode == 0:
             r = json.loads(out.stdout)
             line["cpu_baseline"] = {"value": (nR + nS) / r["seconds"], "unit": "tuples/s", "cores": T,
                                     "kind": "reference",
                                     "sample": f"the full workload through the reference's hashjoin -> aggregate_sum "
-                                              f"-> merge({T}) plan (modulo 2^16 buckets, 16 per bucket), "
-                                              f"{r['seconds']:.3f} s",
-                                    "result_matches_gpu": bool(r["value"] == sum32)}
+                                              f"-> merge({T}) plan (modulo 2^16 buckets, 16 per bucket), median of "
+                                              f"{CPU_RUNS} runs ({ref_runs(r)} s)",
+                                    "result_matches_gpu": bool(r["value"] == sum32), **cpu_threads_note(T)}
     return line
 
 
@@ -561,16 +593,17 @@ def run_c3(args):
         cores = cpu_cores()
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         if os.path.exists(REFJOIN):
-            out = subprocess.run([REFJOIN, "c3", str(nD), str(nF), str(cores)], stdout=subprocess.PIPE,
-                                 stderr=subprocess.PIPE, text=True, timeout=900, cwd="/tmp")
+            out = subprocess.run([REFJOIN, "c3", str(nD), str(nF), str(cores), str(CPU_RUNS)],
+                                 stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=900, cwd="/tmp")
             if out.returncode == 0:
                 r = json.loads(out.stdout)
                 line["cpu_baseline"] = {
                     "value": (nD + nF) / r["seconds"], "unit": "tuples/s", "cores": cores, "kind": "reference",
                     "sample": f"the full workload through the reference's hashjoin -> aggregate_sum(field 0) -> "
-                              f"merge({cores}) plan, one run of {r['seconds']:.2f} s (inputs generated outside "
-                              f"the clock)",
+                              f"merge({cores}) plan, median of {CPU_RUNS} runs ({ref_runs(r)} s; inputs generated "
+                              f"outside the clock)",
                     "result_matches_gpu": bool(r["groups"] == ng and abs(r["total"] - total) <= 1e-9 * abs(total)),
+                    **cpu_threads_note(cores),
                 }
             else:
                 print(f"# reference baseline failed: {out.stderr[-300:]}", file=sys.stderr)
@@ -725,6 +758,7 @@ def run_c5(args):
                    "tables": ntab, "table_rows": int(tsz.sum()), "query_pool": pool,
                    "probe_rows_pool": int(qm.sum())},
         "served_vs_plain": per_rate,
+        "batching": c5_batching(per_rate, sat),
         "saturated": sat,
         "same_results": same,
         "resident_build_s": build_s,
@@ -734,6 +768,24 @@ def run_c5(args):
     line["warm_start"] = c5_warm_start(ctx, served, tabs, qt, probes, qm, rs)
     served.close()
     return line
+
+
+def c5_batching(per_rate, sat):
+    """Mean queries per device batch at the configured arrival rates (200 to
+    2000 queries/s) and above them.  A batch forms only from queries that
+    wait: while one query's device time is far below the mean gap between
+    arrivals, nearly every batch holds one query."""
+    conf = {f"{e['rate_per_s']:g}": round(e["served"]["mean_batch"], 3)
+            for e in per_rate if e["rate_per_s"] <= 2000}
+    over = {f"{e['rate_per_s']:g}": round(e["served"]["mean_batch"], 3)
+            for e in per_rate if e["rate_per_s"] > 2000}
+    over["all queries at once"] = round(sat["served"]["mean_batch"], 3)
+    p50 = [e["served"]["p50_ms"] for e in per_rate if e["rate_per_s"] <= 2000]
+    note = ("At the configured 200-2000 queries/s the median served query completes "
+            f"{min(p50):.2f}-{max(p50):.2f} ms after its arrival, well under the 0.5-5 ms mean gap between "
+            "arrivals, so queries seldom wait and a batch almost always holds one query: batching engages only "
+            "above about 2000 queries/s.") if p50 else ""
+    return {"mean_batch_configured_rates": conf, "mean_batch_above": over, "note": note}
 
 
 def c5_warm_start(ctx, served, tabs, qt, probes, qm, ref_results):
@@ -822,7 +874,8 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X, so --gpus must be 1 "
-              "(the path shards by radix partition; see DESIGN.md)", file=sys.stderr)
+              "(the join would shard by a pass-0 radix digit per GPU; nothing here runs a second GPU)",
+              file=sys.stderr)
         sys.exit(2)
     line = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
     if isinstance(line.get("cpu_baseline"), dict):

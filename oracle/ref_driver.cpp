@@ -13,17 +13,20 @@
 // scanStart, so a final aggregate above it would skip its input.)  Timed from scanStart to scanStop with CLOCK_MONOTONIC as
 // drivers/executequery.cpp:38-59 does (with rdtsc there).
 //
-// usage: refjoin <c1|c2> <log2 R> <log2 S> <threads> <zipf s> <buckets> <tuplesperbucket> <sum|count|probesum>
-//        refjoin c4 <first join> <joins> <threads> <tuplesperbucket> <sum|probesum>
+// usage: refjoin <c1|c2> <log2 R> <log2 S> <threads> <zipf s> <buckets> <tuplesperbucket> <sum|count|probesum> [runs]
+//        refjoin c4 <first join> <joins> <threads> <tuplesperbucket> <sum|probesum> [runs]
 //          (the batched-join family: one plan per join, buckets = 2^ceil(log2 n))
 //        refjoin hash <int|long|chars> <hash node> <value>...
 //          (hash values through the reference's TupleHasher::create)
 //        refjoin htwrite|htread <buckets> <tuples per bucket> <values> <path>
 //          (HashTable::serialize / deserialize of int tuples)
-//        refjoin c3 <dimension rows> <fact rows> <threads>
+//        refjoin c3 <dimension rows> <fact rows> <threads> [runs]
 //          (join + group-by: hashjoin(D, F) -> aggregate_sum(field 0) per worker ->
 //           merge; the consumer folds the per-worker groups)
-// prints one JSON line {"value": ..., "seconds": ..., ...}
+// prints one JSON line {"value": ..., "seconds": ..., ...}.  With runs > 1 the
+// plan is built and run that many times over the same inputs (a fresh plan
+// each time, as one executequery process per run would); "seconds" is the
+// median and "runs" lists every run.
 #include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -31,6 +34,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -139,11 +143,32 @@ static int run_plan(char* R, uint64_t nR, char* S, uint64_t nS, bool is32, bool 
     return 0;
 }
 
+static double median(std::vector<double> v) {
+    std::sort(v.begin(), v.end());
+    const size_t n = v.size();
+    return n % 2 ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
+}
+
+static std::string runs_json(const std::vector<double>& v) {
+    std::string out;
+    char buf[32];
+    for (size_t i = 0; i < v.size(); ++i) {
+        snprintf(buf, sizeof buf, "%s%.6f", i ? ", " : "", v[i]);
+        out += buf;
+    }
+    return out;
+}
+
+static int parse_runs(int argc, char** argv, int at) {
+    const int r = argc > at ? atoi(argv[at]) : 1;
+    return r < 1 ? 1 : r;
+}
+
 static void usage() {
     fprintf(stderr,
-            "usage: refjoin <c1|c2> <log2R> <log2S> <threads> <zipf> <buckets> <tpb> <sum|count|probesum>\n"
-            "       refjoin c4 <first> <joins> <threads> <tpb> <sum|probesum>\n"
-            "       refjoin c3 <dimension rows> <fact rows> <threads>\n"
+            "usage: refjoin <c1|c2> <log2R> <log2S> <threads> <zipf> <buckets> <tpb> <sum|count|probesum> [runs]\n"
+            "       refjoin c4 <first> <joins> <threads> <tpb> <sum|probesum> [runs]\n"
+            "       refjoin c3 <dimension rows> <fact rows> <threads> [runs]\n"
             "       refjoin hash <int|long|chars> <hash node> <value>...\n"
             "       refjoin htwrite|htread <buckets> <tuples per bucket> <values> <path>\n");
 }
@@ -159,8 +184,9 @@ static int run_c4(int argc, char** argv) {
     const uint64_t first = strtoull(argv[2], NULL, 10), count = strtoull(argv[3], NULL, 10);
     const int T = atoi(argv[4]), tpb = atoi(argv[5]);
     const std::string agg = argv[6];
+    const int runs = parse_runs(argc, argv, 7);
     const uint64_t seed = PGPU_DEFAULT_SEED;
-    double total = 0;
+    std::vector<double> total(runs, 0.0);
     uint64_t tuples = 0;
     std::string values;
     for (uint64_t j = first; j < first + count; ++j) {
@@ -179,20 +205,25 @@ static int run_c4(int argc, char** argv) {
         int buckets = 1;
         while ((uint64_t)buckets < n) buckets <<= 1;
         long long v = 0;
-        double secs = 0;
-        if (run_plan(&R[0], n, &S[0], m, true, true, T, buckets, tpb, agg, &v, &secs)) return 1;
-        total += secs;
+        for (int r = 0; r < runs; ++r) {
+            double secs = 0;
+            if (run_plan(&R[0], n, &S[0], m, true, true, T, buckets, tpb, agg, &v, &secs)) return 1;
+            total[r] += secs;
+        }
         tuples += n + m;
         char buf[32];
         snprintf(buf, sizeof buf, "%s%lld", values.empty() ? "" : ", ", v);
         values += buf;
     }
     printf("{\"workload\": \"c4\", \"first\": %llu, \"joins\": %llu, \"threads\": %d, \"agg\": \"%s\", "
-           "\"tuples\": %llu, \"seconds\": %.6f, \"values\": [%s]}\n",
-           (unsigned long long)first, (unsigned long long)count, T, agg.c_str(), (unsigned long long)tuples, total,
-           values.c_str());
+           "\"tuples\": %llu, \"seconds\": %.6f, \"runs\": [%s], \"values\": [%s]}\n",
+           (unsigned long long)first, (unsigned long long)count, T, agg.c_str(), (unsigned long long)tuples,
+           median(total), runs_json(total).c_str(), values.c_str());
     return 0;
 }
+
+static int run_c3_plan(char* D, uint64_t nD, char* F, uint64_t nF, int T, uint64_t* groups_out, double* total_out,
+                       double* secs);
 
 // C3: SUM(F.price) GROUP BY D.custkey over F JOIN D ON orderkey, with the
 // packed rows the reference would scan (D: int orderkey, int custkey;
@@ -204,6 +235,7 @@ static int run_c3(int argc, char** argv) {
     }
     const uint64_t nD = strtoull(argv[2], NULL, 10), nF = strtoull(argv[3], NULL, 10);
     const int T = atoi(argv[4]);
+    const int runs = parse_runs(argc, argv, 5);
     const uint64_t seed = PGPU_DEFAULT_SEED, norders = nD;
     std::vector<char> D(nD * 8), F(nF * 20);
     for (uint64_t i = 0; i < nD; ++i) {
@@ -220,6 +252,22 @@ static int run_c3(int argc, char** argv) {
         memcpy(&F[j * 20 + 8], &q, 4);
         memcpy(&F[j * 20 + 12], &pr, 8);
     }
+    std::vector<double> secs(runs);
+    uint64_t groups = 0;
+    double total = 0;
+    for (int r = 0; r < runs; ++r)
+        if (run_c3_plan(&D[0], nD, &F[0], nF, T, &groups, &total, &secs[r])) return 1;
+    printf("{\"workload\": \"c3\", \"build\": %llu, \"probe\": %llu, \"threads\": %d, \"groups\": %llu, "
+           "\"total\": %.17g, \"seconds\": %.6f, \"runs\": [%s]}\n",
+           (unsigned long long)nD, (unsigned long long)nF, T, (unsigned long long)groups, total, median(secs),
+           runs_json(secs).c_str());
+    return 0;
+}
+
+// One C3 plan over the packed rows: D (int orderkey, int custkey) and F
+// (int orderkey, int partkey, int quantity, dec price).
+static int run_c3_plan(char* D, uint64_t nD, char* F, uint64_t nF, int T, uint64_t* groups_out, double* total_out,
+                       double* secs) {
     Schema ds, fs;
     ds.add(CT_INTEGER);
     ds.add(CT_INTEGER);
@@ -259,7 +307,7 @@ static int run_c3(int argc, char** argv) {
     mn.add("threads", Setting::TypeInt) = T;
     Setting& sc = cfg.getRoot().add("scan", Setting::TypeGroup);
 
-    MemScan scanD(&D[0], nD, ds), scanF(&F[0], nF, fs);
+    MemScan scanD(D, nD, ds), scanF(F, nF, fs);
     HashJoinOp join;
     AggregateSum partial;
     MergeOp merge;
@@ -310,9 +358,9 @@ static int run_c3(int argc, char** argv) {
         groups += seen[c];
         total += sums[c];
     }
-    printf("{\"workload\": \"c3\", \"build\": %llu, \"probe\": %llu, \"threads\": %d, \"groups\": %llu, "
-           "\"total\": %.17g, \"seconds\": %.6f}\n",
-           (unsigned long long)nD, (unsigned long long)nF, T, (unsigned long long)groups, total, t1 - t0);
+    *groups_out = groups;
+    *total_out = total;
+    *secs = t1 - t0;
     return 0;
 }
 
@@ -429,6 +477,7 @@ int main(int argc, char** argv) {
     const double zipf = atof(argv[5]);
     const int buckets = atoi(argv[6]), tpb = atoi(argv[7]);
     const std::string agg = argv[8];
+    const int runs = parse_runs(argc, argv, 9);
     const uint64_t seed = PGPU_DEFAULT_SEED;
     const bool is32 = wl == "c1";
     const unsigned kb = is32 ? 4 : 8;
@@ -464,10 +513,12 @@ int main(int argc, char** argv) {
         }
     }
     long long value = 0;
-    double secs = 0;
-    if (run_plan(&R[0], nR, &S[0], nS, is32, is32, T, buckets, tpb, agg, &value, &secs)) return 1;
+    std::vector<double> secs(runs);
+    for (int r = 0; r < runs; ++r)
+        if (run_plan(&R[0], nR, &S[0], nS, is32, is32, T, buckets, tpb, agg, &value, &secs[r])) return 1;
     printf("{\"workload\": \"%s\", \"build\": %llu, \"probe\": %llu, \"threads\": %d, \"agg\": \"%s\", "
-           "\"value\": %lld, \"seconds\": %.6f}\n",
-           wl.c_str(), (unsigned long long)nR, (unsigned long long)nS, T, agg.c_str(), value, secs);
+           "\"value\": %lld, \"seconds\": %.6f, \"runs\": [%s]}\n",
+           wl.c_str(), (unsigned long long)nR, (unsigned long long)nS, T, agg.c_str(), value, median(secs),
+           runs_json(secs).c_str());
     return 0;
 }
