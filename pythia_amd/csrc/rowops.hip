@@ -442,6 +442,93 @@ __global__ __launch_bounds__(256) void k_agg_append(AggDev a, const char* __rest
     }
 }
 
+// GROUP BY with few groups: a workgroup first folds its kLocalRows rows in
+// an LDS table (kLocalSlots slots, linear probing, the same state protocol
+// as agg_insert), then adds each of its groups' partials to the HBM table
+// once.  Without it every row is an HBM atomic on its group's accumulator, and
+// with a handful of groups (GROUP BY flags, as in TPC-H Q1) those atomics hit a
+// few addresses and serialise in L2.  A row whose key finds no LDS slot within
+// kLocalProbes steps is folded into the HBM table directly, so any key count is
+// correct; the host picks this kernel only while the groups found so far fit
+// the LDS table.  Keys of at most KW 64-bit words.
+constexpr int kLocalSlots = 1024;
+constexpr int kLocalRows = 1024;
+constexpr int kLocalProbes = 16;
+constexpr uint32_t kLocalMaxKeyWords = 4;
+
+__device__ __forceinline__ void fold_local(const AggDev& a, unsigned long long* acc, const char* row, bool a4) {
+    switch (a.kind) {
+        case PGPU_AGG_SUM_I32:
+            atomicAdd((unsigned int*)acc, ld_u32(row + a.val_offset, a4));  // low word: the sum wraps mod 2^32
+            break;
+        case PGPU_AGG_SUM_I64:
+            atomicAdd(acc, (unsigned long long)ld_u64(row + a.val_offset, a4));
+            break;
+        case PGPU_AGG_SUM_F64:
+            atomicAdd((double*)acc, __longlong_as_double((long long)ld_u64(row + a.val_offset, a4)));
+            break;
+        default:
+            atomicAdd(acc, 1ull);
+    }
+}
+
+template <int KW>
+__global__ __launch_bounds__(256) void k_agg_local(AggDev a, const char* __restrict__ rows, uint64_t n,
+                                                   unsigned long long* __restrict__ t, uint64_t mask,
+                                                   unsigned long long* __restrict__ ngroups) {
+    __shared__ uint64_t lkey[KW][kLocalSlots];
+    __shared__ unsigned long long lacc[kLocalSlots];
+    __shared__ uint32_t lst[kLocalSlots];
+    for (int i = threadIdx.x; i < kLocalSlots; i += 256) {
+        lst[i] = 0;
+        lacc[i] = 0;
+    }
+    __syncthreads();
+    const bool a4 = a.layout_a4 && aligned4((uintptr_t)rows);
+    const uint64_t r0 = (uint64_t)blockIdx.x * kLocalRows;
+    const uint64_t r1 = min(n, r0 + kLocalRows);
+    for (uint64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+        const char* row = rows + i * a.row_bytes;
+        uint64_t kw[kMaxKeyWords];
+        gather_key(a, row, kw, a4);  // words past key_words are zero
+        uint64_t h = 0x9E3779B97F4A7C15ull;
+        for (uint32_t w = 0; w < a.key_words; ++w) h = mix64(h ^ kw[w]);
+        uint32_t sl = (uint32_t)(h >> 40) & (kLocalSlots - 1);
+        bool done = false;
+        for (int probe = 0; probe < kLocalProbes && !done;) {
+            uint32_t st = __hip_atomic_load(&lst[sl], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (st == 0) {
+                if (atomicCAS(&lst[sl], 0u, 1u) != 0u) continue;  // lost the race: re-read the slot
+#pragma unroll
+                for (int w = 0; w < KW; ++w) lkey[w][sl] = kw[w];
+                __hip_atomic_store(&lst[sl], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else if (st == 1) {
+                continue;  // the winner stores the key in the same step
+            }
+            bool eq = true;
+#pragma unroll
+            for (int w = 0; w < KW; ++w) eq = eq && lkey[w][sl] == kw[w];
+            if (eq) {
+                fold_local(a, &lacc[sl], row, a4);
+                done = true;
+            } else {
+                sl = (sl + 1) & (kLocalSlots - 1);
+                ++probe;
+            }
+        }
+        if (!done) agg_insert(a, t, mask, kw, row, nullptr, ngroups, a4);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kLocalSlots; i += 256) {
+        if (lst[i] != 2) continue;
+        uint64_t kw[kMaxKeyWords] = {};
+#pragma unroll
+        for (int w = 0; w < KW; ++w) kw[w] = lkey[w][i];
+        const unsigned long long part = lacc[i];
+        agg_insert(a, t, mask, kw, nullptr, &part, ngroups);
+    }
+}
+
 // No GROUP BY: one group, folded in two launches with no same-address
 // atomics.  k_agg_partial: each workgroup sums its rows (in the accumulator's
 // type, so an int32 sum still wraps mod 2^32; several independent loads per
@@ -1095,8 +1182,24 @@ namespace {
 // that much headroom) so that few groups over many rows take few launches.
 constexpr uint64_t kAggMinPiece = 1ull << 18;
 
-int agg_fold_piece(pgpu_agg* g, const char* src, uint64_t n) {
+// ng: groups in the HBM table before this piece.
+int agg_fold_piece(pgpu_agg* g, const char* src, uint64_t n, uint64_t ng) {
     hipStream_t st = g->st;
+    if (g->a.nkeys && g->a.key_words <= kLocalMaxKeyWords && ng <= (uint64_t)kLocalSlots / 2) {
+        const dim3 grid((unsigned)ceil_div(n, kLocalRows));
+        const uint32_t kw = g->a.key_words;
+        if (kw == 1)
+            hipLaunchKernelGGL(k_agg_local<1>, grid, dim3(256), 0, st, g->a, src, n, g->d_table, g->cap - 1,
+                               g->d_ngroups);
+        else if (kw == 2)
+            hipLaunchKernelGGL(k_agg_local<2>, grid, dim3(256), 0, st, g->a, src, n, g->d_table, g->cap - 1,
+                               g->d_ngroups);
+        else
+            hipLaunchKernelGGL(k_agg_local<4>, grid, dim3(256), 0, st, g->a, src, n, g->d_table, g->cap - 1,
+                               g->d_ngroups);
+        PGPU_LAUNCH_CHECK();
+        return PGPU_OK;
+    }
     if (g->a.nkeys == 0) {
         const unsigned np = (unsigned)std::min<uint64_t>(ceil_div(n, 256 * kAggUnroll), kAggPartials);
         hipLaunchKernelGGL(k_agg_partial, dim3(np), dim3(256), 0, st, g->a, src, n, g->d_partial);
@@ -1123,7 +1226,7 @@ int agg_fold(pgpu_agg* g, const void* rows, uint64_t n, bool upload) {
         PGPU_HIP(hipStreamWaitEvent(st, g->ctx_done, 0));
     }
     if (g->a.nkeys == 0) {  // one group: the slot exists after the first fold
-        PGPU_TRY(agg_fold_piece(g, src, n));
+        PGPU_TRY(agg_fold_piece(g, src, n, 1));
         PGPU_HIP(hipStreamSynchronize(st));
         return PGPU_OK;
     }
@@ -1136,7 +1239,7 @@ int agg_fold(pgpu_agg* g, const void* rows, uint64_t n, bool upload) {
         while (ng > nc / 4 || nc / 2 < want) nc *= 2;
         if (nc != g->cap) PGPU_TRY(agg_grow(g, nc));
         const uint64_t piece = std::min(n - done, g->cap / 2);
-        PGPU_TRY(agg_fold_piece(g, src + done * g->a.row_bytes, piece));
+        PGPU_TRY(agg_fold_piece(g, src + done * g->a.row_bytes, piece, ng));
         done += piece;
     }
     PGPU_HIP(hipStreamSynchronize(st));
