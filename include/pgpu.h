@@ -529,8 +529,11 @@ int pgpu_groupjoin_path(pgpu_groupjoin* p, int* path);
  * accumulator, genericaggregate.cpp:96-104): the int32 group key followed by
  * the FP64 sum (PGPU_GJ_ROW_SUM, AggregateSum over a dec column,
  * aggregatesum.cpp:46-49) or the int64 count (PGPU_GJ_ROW_COUNT,
- * aggregatecount.cpp:37-42), packed into 12-byte rows, copied into host
- * memory dst (by direct DMA when dst comes from pgpu_host_alloc). */
+ * aggregatecount.cpp:37-42), packed into 12-byte rows, into host memory
+ * dst.  When dst lies in a block from pgpu_host_alloc and is 16-byte
+ * aligned the rows are written there by the kernel that packs them;
+ * otherwise they are packed in HBM and copied.  Returns when they are in
+ * place. */
 #define PGPU_GJ_ROW_SUM 1
 #define PGPU_GJ_ROW_COUNT 2
 int pgpu_groupjoin_rows(pgpu_groupjoin* p, int acc, uint64_t first, uint64_t n, void* dst);

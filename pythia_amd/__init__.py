@@ -99,6 +99,15 @@ class Context:
     def sync(self):
         call("pgpu_sync", self.handle)
 
+    def host_alloc(self, nbytes: int) -> int:
+        """Pinned host memory (pgpu_host_alloc); returns its address."""
+        p = C.c_void_p()
+        call("pgpu_host_alloc", self.handle, nbytes, C.byref(p))
+        return p.value
+
+    def host_free(self, addr: int):
+        call("pgpu_host_free", self.handle, C.c_void_p(addr))
+
     def stream(self) -> int:
         s = C.c_void_p()
         call("pgpu_ctx_stream", self.handle, C.byref(s))
@@ -176,6 +185,9 @@ def _times(fn, handle):
     return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(min(n.value, 64))}
 
 
+GJ_ROW_SUM, GJ_ROW_COUNT = 1, 2
+
+
 class GroupJoin:
     """SUM(F.val) GROUP BY D.grp over F JOIN D ON key (int32 keys / groups, FP64 sums)."""
 
@@ -201,6 +213,11 @@ class GroupJoin:
         call("pgpu_groupjoin_fetch", self.handle, k.ctypes.data_as(C.c_void_p), s.ctypes.data_as(C.c_void_p),
              c.ctypes.data_as(C.c_void_p), n)
         return k, s, c
+
+    def rows(self, acc: int, first: int, n: int, dst: int):
+        """Result rows [first, first + n) as 12-byte (int32 key, FP64 sum or
+        int64 count) rows into host memory at address dst."""
+        call("pgpu_groupjoin_rows", self.handle, acc, first, n, C.c_void_p(dst))
 
     def path(self) -> int:
         """GJ_PARTITIONED or GJ_GENERAL: the algorithm of the last run."""

@@ -159,6 +159,9 @@ _PROTOS = {
     "pgpu_groupjoin_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                                      C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "pgpu_groupjoin_fetch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "pgpu_groupjoin_rows": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, C.c_void_p]),
+    "pgpu_host_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pgpu_host_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pgpu_groupjoin_path": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "pgpu_groupjoin_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "pgpu_groupjoin_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.c_int, C.POINTER(C.c_int)]),

@@ -840,6 +840,34 @@ __global__ __launch_bounds__(256) void k_gj_rows(const int32_t* __restrict__ oke
     }
 }
 
+// As k_gj_rows, written straight into pinned host memory (dst 16-byte
+// aligned): a block's 256 rows are packed in LDS and leave as 16-byte stores,
+// so every write that crosses the host link is a whole aligned chunk.
+__global__ __launch_bounds__(256) void k_gj_rows_host(const int32_t* __restrict__ okey,
+                                                      const double* __restrict__ osum,
+                                                      const uint64_t* __restrict__ ocnt, int count, uint64_t first,
+                                                      uint64_t n, uint4* __restrict__ dst) {
+    __shared__ __align__(16) uint32_t buf[256 * 3];
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t b0 = (uint64_t)blockIdx.x * 256; b0 < n; b0 += stride) {
+        const uint64_t i = b0 + threadIdx.x;
+        if (i < n) {
+            const uint64_t a = count ? ocnt[first + i] : (uint64_t)__double_as_longlong(osum[first + i]);
+            buf[3 * threadIdx.x] = (uint32_t)okey[first + i];
+            buf[3 * threadIdx.x + 1] = (uint32_t)a;
+            buf[3 * threadIdx.x + 2] = (uint32_t)(a >> 32);
+        }
+        __syncthreads();
+        const uint32_t words = (uint32_t)min<uint64_t>(256, n - b0) * 3;
+        const uint32_t vecs = words / 4;
+        uint4* o = dst + b0 * 3 / 4;  // 768 words per block: b0 * 3 is a multiple of 4
+        if (threadIdx.x < vecs) o[threadIdx.x] = reinterpret_cast<const uint4*>(buf)[threadIdx.x];
+        if (threadIdx.x < words - vecs * 4)
+            reinterpret_cast<uint32_t*>(o + vecs)[threadIdx.x] = buf[vecs * 4 + threadIdx.x];
+        __syncthreads();
+    }
+}
+
 struct pgpu_groupjoin {
     pgpu_ctx* ctx = nullptr;
     uint64_t maxD = 0, maxF = 0, maxG = 0;
@@ -1166,10 +1194,19 @@ int pgpu_groupjoin_rows(pgpu_groupjoin* p, int acc, uint64_t first, uint64_t n, 
     if (!n) return PGPU_OK;
     PGPU_TRY(ctx_activate(p->ctx));
     hipStream_t st = p->ctx->stream;
-    hipLaunchKernelGGL(k_gj_rows, dim3(grid_for(n)), dim3(256), 0, st, p->okey, p->osum, p->ocnt,
-                       acc == PGPU_GJ_ROW_COUNT ? 1 : 0, first, n, p->orow);
-    PGPU_LAUNCH_CHECK();
-    PGPU_HIP(hipMemcpyAsync(dst, p->orow, n * 12, hipMemcpyDeviceToHost, st));
+    const int count = acc == PGPU_GJ_ROW_COUNT ? 1 : 0;
+    if (((uintptr_t)dst & 15) == 0 && host_alloc_holds(dst, n * 12)) {
+        // pinned memory from pgpu_host_alloc: the kernel writes it directly
+        // (one launch, no staging copy, no DMA set-up)
+        hipLaunchKernelGGL(k_gj_rows_host, dim3(grid_for(n)), dim3(256), 0, st, p->okey, p->osum, p->ocnt, count,
+                           first, n, (uint4*)dst);
+        PGPU_LAUNCH_CHECK();
+    } else {
+        hipLaunchKernelGGL(k_gj_rows, dim3(grid_for(n)), dim3(256), 0, st, p->okey, p->osum, p->ocnt, count, first,
+                           n, p->orow);
+        PGPU_LAUNCH_CHECK();
+        PGPU_HIP(hipMemcpyAsync(dst, p->orow, n * 12, hipMemcpyDeviceToHost, st));
+    }
     PGPU_HIP(hipStreamSynchronize(st));
     return PGPU_OK;
 }

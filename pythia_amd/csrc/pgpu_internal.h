@@ -125,4 +125,7 @@ void ctx_host_free_reuse(pgpu_ctx* ctx, void* p);
 // One pinned host word (8 bytes) from the context's pool.
 int ctx_host_word(pgpu_ctx* ctx, unsigned long long** w);
 void ctx_host_word_put(pgpu_ctx* ctx, unsigned long long* w);
+// True when [p, p + bytes) lies in one block from pgpu_host_alloc (pinned
+// host memory that kernels may write directly).
+bool host_alloc_holds(const void* p, uint64_t bytes);
 }  // namespace pgpu

@@ -7,6 +7,8 @@
 // and an allocation that would exceed it fails with PGPU_EBUDGET instead of
 // blocking, so admission control can act on it.
 #include <algorithm>
+#include <mutex>
+#include <map>
 #include <cstdio>
 #include <cstring>
 
@@ -280,6 +282,21 @@ int KernelTimer::harvest() {
 
 using namespace pgpu;
 
+namespace {
+// Blocks from pgpu_host_alloc: start -> bytes.
+std::mutex g_host_allocs_mu;
+std::map<uintptr_t, uint64_t> g_host_allocs;
+}  // namespace
+
+bool pgpu::host_alloc_holds(const void* p, uint64_t bytes) {
+    const uintptr_t a = (uintptr_t)p;
+    std::lock_guard<std::mutex> lk(g_host_allocs_mu);
+    auto it = g_host_allocs.upper_bound(a);
+    if (it == g_host_allocs.begin()) return false;
+    --it;
+    return a >= it->first && a - it->first <= it->second && bytes <= it->second - (a - it->first);
+}
+
 extern "C" {
 
 const char* pgpu_version(void) { return "pgpu 0.1 gfx950"; }
@@ -380,7 +397,10 @@ int pgpu_host_alloc(pgpu_ctx* ctx, uint64_t bytes, void** p) {
     if (!ctx || !p) return fail(PGPU_EINVAL, "pgpu_host_alloc: null argument");
     *p = nullptr;
     PGPU_TRY(ctx_activate(ctx));
-    PGPU_HIP(hipHostMalloc(p, std::max<uint64_t>(bytes, 1), hipHostMallocDefault));
+    bytes = std::max<uint64_t>(bytes, 1);
+    PGPU_HIP(hipHostMalloc(p, bytes, hipHostMallocDefault));
+    std::lock_guard<std::mutex> lk(g_host_allocs_mu);
+    g_host_allocs[(uintptr_t)*p] = bytes;
     return PGPU_OK;
 }
 
@@ -388,6 +408,10 @@ int pgpu_host_free(pgpu_ctx* ctx, void* p) {
     if (!ctx) return fail(PGPU_EINVAL, "pgpu_host_free: null ctx");
     if (!p) return PGPU_OK;
     PGPU_TRY(ctx_activate(ctx));
+    {
+        std::lock_guard<std::mutex> lk(g_host_allocs_mu);
+        g_host_allocs.erase((uintptr_t)p);
+    }
     PGPU_HIP(hipHostFree(p));
     return PGPU_OK;
 }

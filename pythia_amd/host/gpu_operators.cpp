@@ -569,9 +569,23 @@ const char* GpuHashJoinOp::groupRows(unsigned short tid, int groupcol, int sumco
         *n = published_rows_;
         return fuse_rows_;
     }
-    // the claimer alone touches the inputs and the pinned rows until it publishes
+    // The claimer alone touches the inputs and the pinned rows.  It publishes
+    // the group count as soon as the join has run, then writes the rows out
+    // in order, ~3 MiB at a time, so the first threads' shares are emitted
+    // (and consumed) while the later ones cross the host link.
     uint64_t ng = 0;
-    std::string err;
+    bool published = false;
+    auto publish = [&](const std::string& err, uint64_t rows, uint64_t copied) {
+        {
+            std::lock_guard<std::mutex> lk(fuse_mu_);
+            run_error_ = err;
+            published_rows_ = rows;
+            copied_rows_ = copied;
+            published_run_ = run;
+        }
+        fuse_cv_.notify_all();
+        published = true;
+    };
     try {
         residentInputs();
         if (nbuild_rows_ && nprobe_rows_) {
@@ -592,25 +606,46 @@ const char* GpuHashJoinOp::groupRows(unsigned short tid, int groupcol, int sumco
                                                                                      ng * 12);
                                            fuse_rows_cap_ = ng * 12;
                                        }
-                                       if (ng)
-                                           check(pgpu_groupjoin_rows(gj, PGPU_GJ_ROW_SUM, 0, ng, fuse_rows_),
-                                                 "pgpu_groupjoin_rows");
+                                       publish(std::string(), ng, 0);
+                                       // chunks of ~3 MiB starting on 4-row (48-byte) boundaries, so
+                                       // each lands 16-byte aligned in the pinned block
+                                       const uint64_t C = std::max<uint64_t>(1, std::min<uint64_t>(32, ng / 262144));
+                                       for (uint64_t c = 0; c < C; ++c) {
+                                           const uint64_t a = c * ng / C & ~3ull;
+                                           const uint64_t b = c + 1 == C ? ng : (c + 1) * ng / C & ~3ull;
+                                           if (b > a)
+                                               check(pgpu_groupjoin_rows(gj, PGPU_GJ_ROW_SUM, a, b - a,
+                                                                         fuse_rows_ + a * 12),
+                                                     "pgpu_groupjoin_rows");
+                                           {
+                                               std::lock_guard<std::mutex> lk(fuse_mu_);
+                                               copied_rows_ = b;
+                                           }
+                                           fuse_cv_.notify_all();
+                                       }
                                    });
         }
+        if (!published) publish(std::string(), 0, 0);
     } catch (const std::exception& e) {
-        err = e.what();
-        ng = 0;
+        if (published) {
+            {
+                std::lock_guard<std::mutex> lk(fuse_mu_);
+                run_error_ = e.what();
+            }
+            fuse_cv_.notify_all();
+        } else {
+            publish(e.what(), 0, 0);
+        }
+        throw QueryExecutionError(e.what());
     }
-    {
-        std::lock_guard<std::mutex> lk(fuse_mu_);
-        run_error_ = err;
-        published_rows_ = ng;
-        published_run_ = run;
-    }
-    fuse_cv_.notify_all();
-    if (!err.empty()) throw QueryExecutionError(err);
     *n = ng;
     return fuse_rows_;
+}
+
+void GpuHashJoinOp::waitGroupRows(uint64_t upto) {
+    std::unique_lock<std::mutex> lk(fuse_mu_);
+    fuse_cv_.wait(lk, [&] { return copied_rows_ >= upto || !run_error_.empty(); });
+    if (!run_error_.empty()) throw QueryExecutionError(run_error_);
 }
 
 void GpuHashJoinOp::threadClose(unsigned short tid) {
@@ -857,6 +892,7 @@ Operator::GetNextResultT GpuAggregate::getNext(unsigned short tid) {
         if (st.pos >= st.end) return {Finished, out};
         const unsigned ts = schema.getTupleSize();
         const uint64_t k = std::min<uint64_t>(out->capacity() / ts, st.end - st.pos);
+        join_->waitGroupRows(st.pos + k);
         st.view.reset(new Page((void*)(st.rows + st.pos * ts), k * ts, nullptr, ts));
         st.pos += k;
         return {st.pos >= st.end ? Finished : Ready, st.view.get()};

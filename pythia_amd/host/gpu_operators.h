@@ -89,8 +89,11 @@ class GpuHashJoinOp : public DualInputOp {
     // The group join's result rows of this run (int group key, FP64 sum: 12
     // bytes each) in pinned host memory owned by the join until the next run
     // or destroy(): the thread that claims the run computes them, the others
-    // wait for them (every thread then emits its share).
+    // wait for the count (every thread then emits its share).  The rows are
+    // written out in order after the count is published; waitGroupRows(upto)
+    // returns once rows [0, upto) are in place.
     const char* groupRows(unsigned short tid, int groupcol, int sumcol, uint64_t* n);
+    void waitGroupRows(uint64_t upto);
     // Threads of the (single, in resident mode) thread group and tid's place in it.
     unsigned short groupThreads() const { return groupsize.empty() ? 1 : groupsize[0]; }
     unsigned short groupPosition(unsigned short tid) const { return threadposingrp[tid]; }
@@ -141,6 +144,7 @@ class GpuHashJoinOp : public DualInputOp {
     std::condition_variable fuse_cv_;        // signals a published run (under fuse_mu_)
     uint64_t published_run_ = 0;             // the last run whose group rows are out
     uint64_t published_rows_ = 0;
+    uint64_t copied_rows_ = 0;               // rows of the published run already in fuse_rows_
     std::string run_error_;                  // the claimer's failure, rethrown by the waiters
     std::atomic<uint64_t> pulled_bytes_{0};
 };

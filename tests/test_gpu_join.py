@@ -256,6 +256,42 @@ def test_c3_groupjoin_matches_exact_cents(ctx, nD, nF):
         b.free()
 
 
+def test_groupjoin_rows_pinned_and_pageable(ctx):
+    """The group join's result as the aggregate's 12-byte (int32 key, FP64
+    sum or int64 count) rows: written by the packing kernel straight into
+    pinned memory from pgpu_host_alloc (16-byte aligned places), packed in
+    HBM and copied into pinned memory at an unaligned place and into
+    pageable memory; every row range gives the fetched columns' bytes,
+    including ranges whose length leaves a partial 16-byte tail."""
+    import ctypes as C
+
+    nD, nF = 20000, 100000
+    bufs = _c3_device(ctx, nD, nF, nD)
+    gj = pa.GroupJoin(ctx, nD, nF, 1_500_000)
+    ng, _ = gj.run(*bufs[:2], nD, *bufs[2:], nF)
+    k, s, c = gj.fetch(ng)
+    row = np.dtype({"names": ["k", "a"], "formats": [np.int32, np.uint64], "offsets": [0, 4], "itemsize": 12})
+    pinned = ctx.host_alloc(ng * 12 + 64)
+    try:
+        for acc, a in ((pa.GJ_ROW_SUM, s.view(np.uint64)), (pa.GJ_ROW_COUNT, c)):
+            want = np.zeros(ng, row)
+            want["k"], want["a"] = k, a
+            want = want.tobytes()
+            for first, n, off in ((0, ng, 0), (0, 1, 0), (0, 2, 16), (0, 3, 32), (1, 255, 0), (7, 257, 48),
+                                  (4, ng - 4, 0), (5, ng - 5, 4), (3, 300, 20)):
+                gj.rows(acc, first, n, pinned + off)
+                got = bytes((C.c_char * (n * 12)).from_address(pinned + off))
+                assert got == want[first * 12:(first + n) * 12], (acc, first, n, off)
+            page = np.zeros(ng * 12, np.uint8)
+            gj.rows(acc, 0, ng, page.ctypes.data)
+            assert page.tobytes() == want
+    finally:
+        ctx.host_free(pinned)
+        gj.close()
+        for b in bufs:
+            b.free()
+
+
 def test_groupjoin_duplicates_and_sentinel_keys(ctx):
     """Duplicate dimension keys (each match contributes), the INT32_MIN
     marker as a real key and group, unmatched fact rows."""
