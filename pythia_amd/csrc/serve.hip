@@ -310,10 +310,12 @@ void set_error(pgpu_server* s, int rc) {
     }
 }
 
-// Chunk size of a probe batch on the partitioned path: about 128 blocks per
-// batch, so one query alone still spreads over the device.
+// Chunk size of a probe batch on the partitioned path: about 256 blocks per
+// batch (one per CU), so one query alone still spreads over the device (a
+// 2^24-row probe of an 8M-row table: 0.52 ms at 128 blocks, 0.44 ms at 256,
+// 0.455 ms at 512).
 uint32_t chunk_rows_for(uint64_t m) {
-    const uint64_t c = ceil_div(std::max<uint64_t>(m, 1), 128);
+    const uint64_t c = ceil_div(std::max<uint64_t>(m, 1), 256);
     return (uint32_t)std::min<uint64_t>(kChunkRows, std::max<uint64_t>(4096, ceil_div(c, 4096) * 4096));
 }
 
