@@ -443,7 +443,7 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
             const uint32_t pn = min(PIECE, nR - p0);
 #pragma unroll
             for (int b = 0; b < RB; ++b) {
-                if (tid + b * T >= pn) continue;
+                if ((uint32_t)(tid + b * T) >= pn) continue;
                 const Row<KT, true> r = br[b];
                 if (r.k == EMPTY) {
                     atomicAdd(&sp_cnt, 1ull);

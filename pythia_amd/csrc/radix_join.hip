@@ -48,7 +48,6 @@ using pgpu::dev::nt_store;
 constexpr uint64_t kFib = 0x9E3779B97F4A7C15ull;
 constexpr int kMaxBits = 14;
 constexpr int kScatterThreads = 512;
-constexpr int kJoinThreads = 512;
 constexpr int kJoinCap = 4096;            // LDS table slots
 constexpr int kJoinPiece = 2048;          // build rows per LDS piece (load <= 0.5)
 constexpr uint32_t kItemRows = 32768;     // probe rows per join item
@@ -576,7 +575,7 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
         const uint32_t pn = min<uint32_t>(kJoinPiece, nR - p0);
 #pragma unroll
         for (int b = 0; b < RB; ++b) {
-            if (tid + b * kJoinThreads >= pn) continue;
+            if ((uint32_t)(tid + b * kJoinThreads) >= pn) continue;
             const Elem<KT, true> r = br[b];
             const long long k = (long long)r.k;
             if (k == EMPTY) {
@@ -696,10 +695,6 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
         const unsigned long long pieces = (nR + kJoinPiece - 1) / kJoinPiece;
         if (pieces > 1) atomicMax(&slot[3], pieces);
     }
-}
-
-__global__ void k_zero_u64(unsigned long long* p, int n) {
-    if (threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------- global table

@@ -790,7 +790,7 @@ int pgpu_server_prepare_plain(pgpu_server* s, uint64_t max_probe) {
     PGPU_TRY(ctx_activate(s->ctx));
     if (s->plain) pgpu_join_plan_destroy(s->plain);
     s->plain = nullptr;
-    pgpu_join_cfg c{8, 0, 0, PGPU_ALGO_AUTO};
+    pgpu_join_cfg c{8, 0, 0, PGPU_ALGO_AUTO, 0};
     s->plain_maxS = max_probe;
     return pgpu_join_plan_create(s->ctx, &c, std::max<uint64_t>(s->plain_maxR, 1), std::max<uint64_t>(max_probe, 1),
                                  &s->plain);
