@@ -128,4 +128,20 @@ void ctx_host_word_put(pgpu_ctx* ctx, unsigned long long* w);
 // True when [p, p + bytes) lies in one block from pgpu_host_alloc (pinned
 // host memory that kernels may write directly).
 bool host_alloc_holds(const void* p, uint64_t bytes);
+
+// A build side radix-partitioned once (int64 keys and payloads, 2-pass split
+// of `bits` bits, as pgpu_join_count_sum partitions it) and joined by many
+// probes, all on the caller's stream.  p: a radix plan with 8-byte keys (no
+// probe payload) whose build / probe capacities hold n / m rows; a plan's
+// build-side and probe-side scratch are separate, so one build and one probe
+// may use p at the same time, but not two of either.
+//   radix_build_resident  rows (n 16-byte rows), base (2^bits + 1 words),
+//                         tot (2^bits words) receive the partitioned side
+//   radix_probe_resident  count and build-payload sum of the join of m probe
+//                         keys with it into res[0], res[1] (res[2] = 0,
+//                         res[3] = LDS passes)
+int radix_build_resident(pgpu_join_plan* p, hipStream_t st, const int64_t* key, const int64_t* pay, uint64_t n,
+                         int bits, void* rows, uint64_t* base, uint32_t* tot);
+int radix_probe_resident(pgpu_join_plan* p, hipStream_t st, const int64_t* probe, uint64_t m, int bits,
+                         const void* rows, const uint64_t* base, const uint32_t* tot, unsigned long long* res);
 }  // namespace pgpu

@@ -881,10 +881,9 @@ struct Timed {  // t may be null (untimed launches)
 };
 
 template <typename KT, bool HV>
-int run_partition(pgpu_join_plan* p, const KT* keys, const KT* vals, uint64_t n, int B, uint32_t* H, uint32_t* tot,
-                  uint64_t* base, uint64_t* off1, Elem<KT, HV>* P1, Elem<KT, HV>* P2, const char* tag) {
+int run_partition(pgpu_join_plan* p, hipStream_t st, const KT* keys, const KT* vals, uint64_t n, int B, uint32_t* H,
+                  uint32_t* tot, uint64_t* base, uint64_t* off1, Elem<KT, HV>* P1, Elem<KT, HV>* P2, const char* tag) {
     using E = Elem<KT, HV>;
-    hipStream_t st = p->ctx->stream;
     const int B2 = B - B / 2;  // generate(2): pass 1 takes floor(B/2) high bits
     const uint32_t nb = 1u << B, nb1 = 1u << (B - B2);
     const uint64_t blk = block_rows(n, sizeof(E));
@@ -930,24 +929,21 @@ int run_partition(pgpu_join_plan* p, const KT* keys, const KT* vals, uint64_t n,
     return PGPU_OK;
 }
 
+// Items and the LDS join over partitions already in place: build partition
+// d at R2[baseR[d], + totR[d]), probe partitions from p's probe side.
 template <typename KT, bool HVS>
-int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const KT* sk, const KT* sv, uint64_t nS,
-              int B) {
-    hipStream_t st = p->ctx->stream;
+int run_lds_join(pgpu_join_plan* p, hipStream_t st, const Elem<KT, true>* R2, const uint64_t* baseR,
+                 const uint32_t* totR, uint64_t nS, int B, uint32_t variant) {
     const uint32_t nb = 1u << B;
-    PGPU_TRY((run_partition<KT, true>(p, rk, rv, nR, B, p->HR, p->totR, p->baseR, p->off1R,
-                                      (Elem<KT, true>*)p->R1, (Elem<KT, true>*)p->R2, "build")));
-    PGPU_TRY((run_partition<KT, HVS>(p, sk, sv, nS, B, p->HS, p->totS, p->baseS, p->off1S,
-                                     (Elem<KT, HVS>*)p->S1, (Elem<KT, HVS>*)p->S2, "probe")));
     {
         Timed tm(p->timer, "join_plan");
-        hipLaunchKernelGGL(k_icount, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, p->totS, p->totR, nb,
+        hipLaunchKernelGGL(k_icount, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, p->totS, totR, nb,
                            kItemRows, p->icount);
         PGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_binscan, dim3(1), dim3(1024), 0, st, p->icount, nb, p->ioff);
         PGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_items, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, p->icount, p->ioff,
-                           p->baseS, p->totS, p->baseR, p->totR, nb, kItemRows, p->items);
+                           p->baseS, p->totS, baseR, totR, nb, kItemRows, p->items);
         PGPU_LAUNCH_CHECK();
     }
     const uint64_t max_items = nb + ceil_div(nS, kItemRows) + 1;
@@ -956,9 +952,9 @@ int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const 
         Timed tm(p->timer, "join_lds");
 #define PGPU_JOIN(CAP, T, PIECE, UB, WPE)                                                                     \
     hipLaunchKernelGGL((k_join_cs<KT, HVS, CAP, T, PIECE, UB, WPE>), dim3((unsigned)max_items), dim3(T), 0, st, \
-                       (const Elem<KT, true>*)p->R2, p->baseR, p->totR, (const Elem<KT, HVS>*)p->S2, p->baseS,    \
-                       p->totS, p->ioff, p->items, nb, B, p->part)
-        switch (p->cfg.join_variant) {
+                       R2, baseR, totR, (const Elem<KT, HVS>*)p->S2, p->baseS, p->totS, p->ioff, p->items, nb, B, \
+                       p->part)
+        switch (variant) {
             case 1: PGPU_JOIN(4096, 512, 2048, 8, 4); break;
             case 2: PGPU_JOIN(4096, 256, 2048, 32, 2); break;
             case 3: PGPU_JOIN(2048, 256, 1280, 16, 4); break;
@@ -972,6 +968,17 @@ int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const 
         PGPU_LAUNCH_CHECK();
     }
     return PGPU_OK;
+}
+
+template <typename KT, bool HVS>
+int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const KT* sk, const KT* sv, uint64_t nS,
+              int B) {
+    hipStream_t st = p->ctx->stream;
+    PGPU_TRY((run_partition<KT, true>(p, st, rk, rv, nR, B, p->HR, p->totR, p->baseR, p->off1R,
+                                      (Elem<KT, true>*)p->R1, (Elem<KT, true>*)p->R2, "build")));
+    PGPU_TRY((run_partition<KT, HVS>(p, st, sk, sv, nS, B, p->HS, p->totS, p->baseS, p->off1S,
+                                     (Elem<KT, HVS>*)p->S1, (Elem<KT, HVS>*)p->S2, "probe")));
+    return run_lds_join<KT, HVS>(p, st, (const Elem<KT, true>*)p->R2, p->baseR, p->totR, nS, B, p->cfg.join_variant);
 }
 
 // The global-table join over explicit buffers: `table` of at least
@@ -1016,6 +1023,44 @@ int run_global(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const
 }  // namespace
 
 using namespace pgpu;
+
+// A build side partitioned once and probed by many queries (the serving
+// engine's large queries): rows / base / tot in the layout of a plan's
+// partitioned build side.
+int pgpu::radix_build_resident(pgpu_join_plan* p, hipStream_t st, const int64_t* key, const int64_t* pay,
+                               uint64_t n, int bits, void* rows, uint64_t* base, uint32_t* tot) {
+    if (!p || !p->R1 || p->cfg.key_bytes != 8)
+        return fail(PGPU_EINVAL, "radix_build_resident: a plan with 8-byte radix buffers is required");
+    if (!n || n > p->maxR)
+        return fail(PGPU_EINVAL, "radix_build_resident: %llu build rows (the plan holds %llu)", (unsigned long long)n,
+                    (unsigned long long)p->maxR);
+    if (bits < 2 || bits > kMaxBits || (1u << bits) > p->nbmax)
+        return fail(PGPU_EINVAL, "radix_build_resident: %d radix bits", bits);
+    PGPU_TRY(ctx_activate(p->ctx));
+    return run_partition<int64_t, true>(p, st, key, pay, n, bits, p->HR, tot, base, p->off1R,
+                                        (Elem<int64_t, true>*)p->R1, (Elem<int64_t, true>*)rows, "build");
+}
+
+int pgpu::radix_probe_resident(pgpu_join_plan* p, hipStream_t st, const int64_t* probe, uint64_t m, int bits,
+                               const void* rows, const uint64_t* base, const uint32_t* tot,
+                               unsigned long long* res) {
+    if (!p || !p->S1 || p->cfg.key_bytes != 8 || p->cfg.probe_payload)
+        return fail(PGPU_EINVAL, "radix_probe_resident: a plan with 8-byte keys and no probe payload is required");
+    if (!m || m > p->maxS)
+        return fail(PGPU_EINVAL, "radix_probe_resident: %llu probe rows (the plan holds %llu)", (unsigned long long)m,
+                    (unsigned long long)p->maxS);
+    if (bits < 2 || bits > kMaxBits || (1u << bits) > p->nbmax)
+        return fail(PGPU_EINVAL, "radix_probe_resident: %d radix bits", bits);
+    PGPU_TRY(ctx_activate(p->ctx));
+    hipLaunchKernelGGL(k_fill64, dim3(8), dim3(256), 0, st, p->part, (uint64_t)4 * kPartSlots);
+    PGPU_LAUNCH_CHECK();
+    PGPU_TRY((run_partition<int64_t, false>(p, st, probe, nullptr, m, bits, p->HS, p->totS, p->baseS, p->off1S,
+                                            (Elem<int64_t, false>*)p->S1, (Elem<int64_t, false>*)p->S2, "probe")));
+    PGPU_TRY((run_lds_join<int64_t, false>(p, st, (const Elem<int64_t, true>*)rows, base, tot, m, bits, 0)));
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, p->part, kPartSlots, res);
+    PGPU_LAUNCH_CHECK();
+    return PGPU_OK;
+}
 
 extern "C" {
 

@@ -236,6 +236,15 @@ struct Table {
     uint32_t bits = 0;      // partition bits of the resident partitioned rows
     uint64_t rdst = 0;      // first row in the resident partitioned-row arena
     uint64_t rbnd = 0;      // first bound word in the resident bounds arena
+    // The same rows in the 2-pass radix layout (radix_build_resident): a lone
+    // large query probes them with the 2-pass radix partitioning and the LDS
+    // join.  Absent for tables attached from segments or when the budget has
+    // no room.
+    int rbits = 0;
+    void* rrows = nullptr;
+    uint64_t* rbase = nullptr;
+    uint32_t* rtot = nullptr;
+    uint64_t rbytes = 0;
 };
 
 // Per-slot capacities of the partitioned path.
@@ -268,6 +277,7 @@ struct Slot {
     void* SP = nullptr;
     std::vector<Query*> qs;
     bool busy = false;
+    bool radix = false;     // its batch holds the server's radix plan
 };
 
 }  // namespace
@@ -288,6 +298,12 @@ struct pgpu_server {
     std::vector<Slot> slots;
     pgpu_join_plan* plain = nullptr;
     uint64_t plain_maxR = 0, plain_maxS = 0;
+    // The radix route of lone large queries: one plan (its build-side scratch
+    // sized for the largest table, its probe side for caps.rows), used by one
+    // slot at a time.
+    pgpu_join_plan* rplan = nullptr;
+    uint64_t rplan_maxR = 0, rplan_bytes = 0;
+    bool rplan_busy = false;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Query*> pending;
@@ -358,6 +374,8 @@ void reap(pgpu_server* s, Slot& sl) {
     }
     sl.qs.clear();
     sl.busy = false;
+    if (sl.radix) s->rplan_busy = false;
+    sl.radix = false;
     s->cv.notify_all();
 }
 
@@ -400,6 +418,36 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
     if (qs.empty()) return false;
     ctx_activate(s->ctx);
     const uint32_t bid = (uint32_t)s->batches++;
+    // A lone large query on a table with the radix layout: the 2-pass radix
+    // partitioning of its probe keys and the LDS join against the table's
+    // partitions (one pass over 2048- to 4096-way runs costs more than two 32-
+    // to 128-way passes, and the LDS join runs at ~1000 build rows per
+    // partition instead of ~3000).
+    if (qs.size() == 1 && partitioned(s, qs[0]) && s->tables[qs[0]->table].rrows && s->rplan && !s->rplan_busy &&
+        qs[0]->m <= s->caps.rows) {
+        Query* q = qs[0];
+        const Table& tb = s->tables[q->table];
+        q->batch = bid;
+        q->batch_size = 1;
+        s->rplan_busy = true;
+        sl.radix = true;
+        uint32_t* queue = (uint32_t*)(sl.d_res + 4ull * s->cfg.max_batch);
+        int rc = radix_probe_resident(s->rplan, sl.st, (const int64_t*)q->probe, q->m, tb.rbits, tb.rrows, tb.rbase,
+                                      tb.rtot, sl.d_res);
+        if (rc == PGPU_OK) {
+            hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, sl.st, sl.d_res, 1u, sl.d.res, queue);
+            if (hipGetLastError() != hipSuccess) rc = fail(PGPU_EDEVICE, "finish launch failed");
+        }
+        if (rc == PGPU_OK && hipEventRecord(sl.ev, sl.st) != hipSuccess) rc = fail(PGPU_EDEVICE, "event record failed");
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            ++s->part_queries;
+        }
+        sl.qs = std::move(qs);
+        sl.busy = true;
+        if (rc != PGPU_OK) set_error(s, rc);
+        return true;
+    }
     uint32_t ni = 0, nseg = 0, nchunk = 0, nitem = 0, maxbits = 0;
     uint64_t prow = 0, phist = 0, pbnd = 0;
     // Admission counted kItemRows items; smaller ones are used when they fit.
@@ -615,6 +663,63 @@ int partition_table(pgpu_server* s, Table& t) {
     return PGPU_OK;
 }
 
+// Bytes of a radix plan for builds of maxR rows and probes of maxS rows (its
+// radix buffers, as pgpu_join_plan_create sizes them, rounded up).
+uint64_t radix_plan_bytes(uint64_t maxR, uint64_t maxS) {
+    const uint64_t nb = 1ull << 14, blocks = 512;
+    return 2 * 16 * std::max<uint64_t>(maxR, 1) + 2 * 8 * std::max<uint64_t>(maxS, 1) + 2 * blocks * nb * 4 +
+           (2 * blocks + 2) * 256 * 8 + 64 * nb + 32 * (nb + ceil_div(maxS, 32768) + 1) + (1 << 20);
+}
+
+// The table's rows in the radix layout (and the plan that probes it, grown to
+// the largest table), for tables whose segmented layout has more than 1024
+// partitions, when the budget has room; other tables keep only the segmented
+// layout.  Below that fan-out the one-pass segmented partitioning of a probe
+// beats two radix passes (a 2^25-row probe of a 2^20-row table: 0.39 ms
+// segmented, 0.46 ms radix; of a 2^23-row table: 0.63 / 0.55 ms).  Runs while
+// no query is in flight.
+constexpr uint32_t kRadixMinBits = 11;
+int radix_table(pgpu_server* s, Table& t) {
+    if (!t.n || t.bits < kRadixMinBits) return PGPU_OK;
+    int bits = (int)ilog2_ceil(ceil_div(t.n, 1024));
+    bits = std::max(2, std::min(14, bits));
+    const uint64_t nb = 1ull << bits;
+    const uint64_t cache = 16 * t.n + 8 * (nb + 1) + 4 * nb;
+    const uint64_t maxR = std::max(s->rplan_maxR, t.n);
+    const uint64_t plan = maxR > s->rplan_maxR ? radix_plan_bytes(maxR, s->caps.rows) : 0;
+    if (s->resident - s->rplan_bytes + plan + cache > s->cfg.hbm_budget) return PGPU_OK;
+    if (plan) {
+        pgpu_join_cfg c{8, 0, 0, PGPU_ALGO_RADIX, 0};
+        pgpu_join_plan* np = nullptr;
+        if (pgpu_join_plan_create(s->ctx, &c, maxR, s->caps.rows, &np) != PGPU_OK) return PGPU_OK;  // no room
+        if (s->rplan) pgpu_join_plan_destroy(s->rplan);
+        s->rplan = np;
+        s->resident = s->resident - s->rplan_bytes + plan;
+        s->rplan_bytes = plan;
+        s->rplan_maxR = maxR;
+    }
+    int rc = ctx_alloc(s->ctx, 16 * t.n, &t.rrows);
+    if (rc == PGPU_OK) rc = ctx_alloc(s->ctx, 8 * (nb + 1), (void**)&t.rbase);
+    if (rc == PGPU_OK) rc = ctx_alloc(s->ctx, 4 * nb, (void**)&t.rtot);
+    if (rc == PGPU_OK)
+        rc = radix_build_resident(s->rplan, s->ctx->stream, (const int64_t*)t.key, (const int64_t*)t.pay, t.n, bits,
+                                  t.rrows, t.rbase, t.rtot);
+    if (rc == PGPU_OK && hipStreamSynchronize(s->ctx->stream) != hipSuccess)
+        rc = fail(PGPU_EDEVICE, "radix layout of a table failed");
+    if (rc != PGPU_OK) {
+        for (void* p : {t.rrows, (void*)t.rbase, (void*)t.rtot})
+            if (p) (void)ctx_free(s->ctx, p);
+        t.rrows = nullptr;
+        t.rbase = nullptr;
+        t.rtot = nullptr;
+        return rc;
+    }
+    t.rbits = bits;
+    t.rbytes = cache;
+    s->resident += cache;
+    return PGPU_OK;
+}
+
 template <typename T>
 T* carve(char*& p, uint64_t n) {
     T* r = (T*)p;
@@ -723,7 +828,10 @@ int pgpu_server_destroy(pgpu_server* s) {
         if (t.slots) (void)ctx_free(s->ctx, t.slots);
         if (t.cnt) (void)ctx_free(s->ctx, t.cnt);
         if (t.special) (void)ctx_free(s->ctx, t.special);
+        for (void* p : {t.rrows, (void*)t.rbase, (void*)t.rtot})
+            if (p) (void)ctx_free(s->ctx, p);
     }
+    if (s->rplan) pgpu_join_plan_destroy(s->rplan);
     if (s->d_tabs) (void)ctx_free(s->ctx, s->d_tabs);
     if (s->R) (void)ctx_free(s->ctx, s->R);
     if (s->RB) (void)ctx_free(s->ctx, s->RB);
@@ -775,6 +883,7 @@ int pgpu_server_add_table(pgpu_server* s, const void* d_key, const void* d_pay, 
         PGPU_HIP(hipStreamSynchronize(st));
         s->resident += t.bytes;
         PGPU_TRY(partition_table(s, t));
+        PGPU_TRY(radix_table(s, t));
     } else {
         s->plain_maxR = std::max(s->plain_maxR, n);  // plain mode keeps only the columns
     }

@@ -280,3 +280,58 @@ def test_warm_start_from_saved_segments(ctx, world):
                 os.unlink(path)
         for b in dup:
             b.free()
+
+
+def test_lone_queries_on_large_tables_take_the_radix_route(ctx):
+    """Tables of more than 1024 x 3584 rows also keep their rows in the radix
+    layout; a query that runs alone is partitioned in two radix passes and
+    joined in LDS against it.  Every probe size, a table with repeated keys
+    and the minimum integer as a real key, and the same queries batched
+    together (the segmented path) give the exact result."""
+    n = (1 << 22) + 12345
+    k, p = ctx.alloc(8 * n), ctx.alloc(8 * n)
+    ctx.gen("C2_RKEY", n, k, sub=0)
+    ctx.gen("C2_RPAY", n, p, sub=0)
+    rng = np.random.default_rng(21)
+    nd = 1 << 22
+    dk = (rng.integers(0, 1 << 20, nd) * 7919 - 5).astype(np.int64)
+    dk[:5] = np.iinfo(np.int64).min
+    dp = rng.integers(-2**40, 2**40, nd).astype(np.int64)
+    m_dup = 300_000
+    pk = np.concatenate([dk[rng.integers(0, nd, m_dup - 1000)], rng.integers(-2**40, 2**40, 997),
+                         np.full(3, np.iinfo(np.int64).min)]).astype(np.int64)
+    order = np.argsort(dk, kind="stable")
+    ukeys, first = np.unique(dk[order], return_index=True)
+    cnt = np.diff(np.append(first, nd))
+    ssum = np.add.reduceat(dp[order].view(np.uint64), first)
+    pos = np.clip(np.searchsorted(ukeys, pk), 0, len(ukeys) - 1)
+    hit = ukeys[pos] == pk
+    want_dup = (int((cnt[pos] * hit).sum()), int(np.sum(ssum[pos] * hit.astype(np.uint64), dtype=np.uint64)))
+    bufs = [k, p, ctx.upload(dk), ctx.upload(dp), ctx.upload(pk)]
+    sizes = [1, 999, (1 << 20) + 7, 1 << 22]
+    probes = []
+    for q, m in enumerate(sizes):
+        b = ctx.alloc(8 * m)
+        ctx.gen("C2_SFK_UNIFORM", m, b, sub=0, n_build=n, offset=q << 32)
+        probes.append(b)
+    ctx.sync()
+    exp = [O.c5_closed(0, n, q << 32, m, threads=THREADS) for q, m in enumerate(sizes)]
+    try:
+        for part_min in (1, 0):
+            s = pa.Server(ctx, part_min_rows=part_min)
+            assert s.add_table(k, p, n) == 0 and s.add_table(bufs[2], bufs[3], nd) == 1
+            for q, m in enumerate(sizes):  # alone
+                s.submit(0, probes[q], m, tag=q)
+                s.drain()
+                (r,) = s.poll()
+                assert (r.matches, r.sum_build) == (exp[q].count, exp[q].sum_build), (part_min, m)
+            s.submit(1, bufs[4], m_dup, tag=9)
+            s.drain()
+            (r,) = s.poll()
+            assert (r.matches, r.sum_build) == want_dup
+            mix = s.replay([0] * len(sizes) + [1], probes + [bufs[4]], sizes + [m_dup], np.zeros(len(sizes) + 1))
+            assert [(r.matches, r.sum_build) for r in mix] == [(e.count, e.sum_build) for e in exp] + [want_dup]
+            s.close()
+    finally:
+        for b in bufs + probes:
+            b.free()
