@@ -418,38 +418,23 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
     if (qs.empty()) return false;
     ctx_activate(s->ctx);
     const uint32_t bid = (uint32_t)s->batches++;
-    // A lone large query on a table with the radix layout: the 2-pass radix
-    // partitioning of its probe keys and the LDS join against the table's
-    // partitions (one pass over 2048- to 4096-way runs costs more than two 32-
-    // to 128-way passes, and the LDS join runs at ~1000 build rows per
-    // partition instead of ~3000).
-    if (qs.size() == 1 && partitioned(s, qs[0]) && s->tables[qs[0]->table].rrows && s->rplan && !s->rplan_busy &&
-        qs[0]->m <= s->caps.rows) {
-        Query* q = qs[0];
-        const Table& tb = s->tables[q->table];
-        q->batch = bid;
-        q->batch_size = 1;
-        s->rplan_busy = true;
-        sl.radix = true;
-        uint32_t* queue = (uint32_t*)(sl.d_res + 4ull * s->cfg.max_batch);
-        int rc = radix_probe_resident(s->rplan, sl.st, (const int64_t*)q->probe, q->m, tb.rbits, tb.rrows, tb.rbase,
-                                      tb.rtot, sl.d_res);
-        if (rc == PGPU_OK) {
-            hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, sl.st, sl.d_res, 1u, sl.d.res, queue);
-            if (hipGetLastError() != hipSuccess) rc = fail(PGPU_EDEVICE, "finish launch failed");
-        }
-        if (rc == PGPU_OK && hipEventRecord(sl.ev, sl.st) != hipSuccess) rc = fail(PGPU_EDEVICE, "event record failed");
-        {
-            std::lock_guard<std::mutex> g(s->mu);
-            ++s->part_queries;
-        }
-        sl.qs = std::move(qs);
-        sl.busy = true;
-        if (rc != PGPU_OK) set_error(s, rc);
-        return true;
-    }
     uint32_t ni = 0, nseg = 0, nchunk = 0, nitem = 0, maxbits = 0;
     uint64_t prow = 0, phist = 0, pbnd = 0;
+    // The one large query of a batch (if it holds exactly one) on a table with
+    // the radix layout goes through the 2-pass radix partitioning of its probe
+    // keys and the LDS join against that layout: one pass over 2048- to
+    // 4096-way runs costs more than two 32- to 128-way passes, and the LDS
+    // join runs at ~1000 build rows per partition instead of ~3000.  Several
+    // large queries share the segmented pass instead (one launch for all).
+    int radix_q = -1;
+    {
+        int nlarge = 0, last = -1;
+        for (uint32_t i = 0; i < qs.size(); ++i)
+            if (partitioned(s, qs[i])) ++nlarge, last = (int)i;
+        if (nlarge == 1 && s->rplan && !s->rplan_busy && s->tables[qs[last]->table].rrows &&
+            qs[last]->m <= s->caps.rows)
+            radix_q = last;
+    }
     // Admission counted kItemRows items; smaller ones are used when they fit.
     uint64_t irows = item_rows_for(rrows), nsmall = 0;
     for (Query* q : qs)
@@ -464,6 +449,7 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
                 sl.h.items[ni++] = QItem{q->probe, r, std::min(q->m, r + irows), i, q->table};
             continue;
         }
+        if ((int)i == radix_q) continue;
         const Table& tb = s->tables[q->table];
         Seg sg{};
         sg.key = q->probe;
@@ -507,6 +493,15 @@ bool launch_batch(pgpu_server* s, Slot& sl) {
         if (rc == PGPU_OK)
             rc = batch::launch_join(sl.st, 8, false, kVariant, std::min<int>(s->join_grid, (int)nitem), sl.d.jobs,
                                     sl.d.item_job, nitem, s->R, sl.SP, s->RB, sl.bnd, queue, sl.d_res);
+    }
+    if (rc == PGPU_OK && radix_q >= 0) {
+        const Query* q = qs[radix_q];
+        const Table& tb = s->tables[q->table];
+        s->rplan_busy = true;
+        sl.radix = true;
+        ++nseg;
+        rc = radix_probe_resident(s->rplan, sl.st, (const int64_t*)q->probe, q->m, tb.rbits, tb.rrows, tb.rbase,
+                                  tb.rtot, sl.d_res + 4ull * radix_q);
     }
     if (rc == PGPU_OK) {
         hipLaunchKernelGGL(k_sfinish, dim3(1), dim3(256), 0, sl.st, sl.d_res, (uint32_t)qs.size(), sl.d.res, queue);
