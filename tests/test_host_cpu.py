@@ -40,6 +40,39 @@ def test_scan_merge_protocol_returns_every_row_once(tmp_path, threads):
     assert np.all(out["c1"] == out["c0"] + 0.25)
 
 
+@pytest.mark.parametrize("threads,buffsize", [(1, 1 << 16), (4, 1 << 20), (3, (1 << 20) + 48)])
+def test_large_pages_reach_the_driver_whole(threads, buffsize):
+    """Pages of 256 KB and more are copied into the driver's result buffer by
+    several threads at once; pages below that by the driver alone.  Every
+    row arrives once and unchanged, query after query (the helpers are
+    reused), and a small query after a large one still returns only its own
+    rows."""
+    n = 700_001
+    dt = np.dtype({"names": ["k", "v"], "formats": [np.int64, np.int64], "offsets": [0, 8], "itemsize": 16})
+    T = np.empty(n, dt)
+    T["k"] = np.arange(n)
+    T["v"] = (np.arange(n, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)).view(np.int64)
+    P.register_table("big_pages", T)
+    try:
+        plan = textwrap.dedent(f"""
+            path = "./"; buffsize = {buffsize};
+            treeroot = {{ name = "merge"; input = {{ name = "s"; }}; }};
+            merge = {{ type = "merge"; threads = {threads}; }};
+            s = {{ type = "memscan"; table = "big_pages"; schema = ( "long", "long" ); }};
+        """)
+        for _ in range(3):
+            out, meta = P.run(plan)
+            assert len(out) == n
+            order = np.argsort(out["c0"], kind="stable")
+            assert np.array_equal(out["c0"][order], T["k"])
+            assert np.array_equal(out["c1"][order], T["v"])
+        P.register_table("small_pages", T[:1000])
+        out, _ = P.run(plan.replace("big_pages", "small_pages"))
+        assert sorted(out["c0"].tolist()) == list(range(1000))
+    finally:
+        P.clear_tables()
+
+
 @pytest.mark.parametrize("threads,buffsize", [(1, 20), (3, 20), (5, 4096)])
 def test_threadidprepend(tmp_path, threads, buffsize):
     """threadidprepend (operators/threadidprepend.cpp:42-110): output =
