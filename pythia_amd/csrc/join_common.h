@@ -86,6 +86,63 @@ __device__ __forceinline__ void nt_store(T* p, const T& v) {
     }
 }
 
+// Workgroups of a launch are dispatched to the 8 XCDs (each with its own L2)
+// in turn: workgroup i runs on XCD i % 8.  xcd_remap(i, n) renumbers them so
+// that XCD x takes one contiguous range of logical blocks [0, n): blocks whose
+// output runs meet in one 128-byte line then complete it in the same L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t n) {
+    const uint32_t q = n / kXcds, r = n % kXcds;
+    const uint32_t x = i % kXcds, k = i / kXcds;  // XCD x runs q + (x < r) of the blocks
+    return x * q + min(x, r) + k;
+}
+
+// As xcd_remap, in chunks of K consecutive logical blocks: chunk c runs on
+// XCD c % 8, so neighbours within a chunk share an L2 while a range of
+// unusually heavy blocks (a hot partition's units) still spreads over all
+// XCDs.  Blocks past the last whole round of 8 chunks keep their index.
+template <uint32_t K>
+__device__ __forceinline__ uint32_t xcd_remap_chunked(uint32_t i, uint32_t n) {
+    const uint32_t full = n / (kXcds * K) * (kXcds * K);
+    if (i >= full) return i;
+    const uint32_t x = i % kXcds, k = i / kXcds;  // k-th block on XCD x
+    return ((k / K) * kXcds + x) * K + k % K;
+}
+
+// Reads through a buffer resource: one 32-bit offset register per load in
+// flight instead of a 64-bit address, so more loads stay in registers.  The
+// base must be the same for the whole wavefront; offsets at or past `bytes`
+// read zero.  Word 3 = 0x00020000 selects the untyped 32-bit data format of
+// gfx9 buffer resources.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+    const uint64_t a = (uint64_t)base;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), (short)0, (int)bytes, 0x00020000);
+}
+
+// NT: the non-temporal cache policy (cache-policy operand bit 1, the same
+// `nt` bit nt_load sets on a global load).
+template <typename T, bool NT = false>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "buf_load width");
+    constexpr int kPol = NT ? 2 : 0;
+    T v;
+    if constexpr (sizeof(T) == 16) {
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        const v4i x = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, kPol);
+        __builtin_memcpy(&v, &x, 16);
+    } else if constexpr (sizeof(T) == 8) {
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        const v2i x = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, kPol);
+        __builtin_memcpy(&v, &x, 8);
+    } else {
+        const int x = __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, kPol);
+        __builtin_memcpy(&v, &x, 4);
+    }
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

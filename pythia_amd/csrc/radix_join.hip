@@ -44,14 +44,22 @@ namespace {
 
 using pgpu::dev::nt_load;
 using pgpu::dev::nt_store;
+using pgpu::dev::xcd_remap;
+using pgpu::dev::xcd_remap_chunked;
+using pgpu::dev::buf_rsrc;
+using pgpu::dev::buf_load;
 
 constexpr uint64_t kFib = 0x9E3779B97F4A7C15ull;
 constexpr int kMaxBits = 14;
+static_assert((kMaxBits + 1) / 2 <= 7, "one pass takes at most 7 bits (kDigits)");
 constexpr int kScatterThreads = 512;
 constexpr int kJoinCap = 4096;            // LDS table slots
 constexpr int kJoinPiece = 2048;          // build rows per LDS piece (load <= 0.5)
 constexpr uint32_t kItemRows = 32768;     // probe rows per join item
-constexpr int kTileBytes = 65536;         // LDS staging tile of the pass-2 scatter (two workgroups per CU)
+constexpr int kTileBytes = 77824;         // LDS staging tile of the pass-2 scatter (two workgroups per CU):
+                                          // 9728 8-byte rows hold a whole unit of ~8K rows (with 64 KB,
+                                          // half the units spilled a few rows into a second tile)
+constexpr int kDigits = 128;              // digits of one pass: at most 7 of the <= 14 radix bits
 constexpr int kTileBytes1 = 131072;       // pass 1: one workgroup per CU; twice the rows per partition run
                                           // per tile halves the partial lines written per byte
 constexpr int kPartSlots = 2048;          // partial-result slots of the join kernels
@@ -247,30 +255,38 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
     using E = Elem<KT, HV>;
     constexpr int T = kScatterThreads * IPT;
     __shared__ E stage[T];
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t lofs[257];
-    __shared__ uint64_t run[256];
+    __shared__ uint32_t hist[kDigits];
+    __shared__ uint32_t lofs[kDigits + 1];
+    __shared__ uint64_t run[kDigits];
     const int tid = threadIdx.x;
     const int nb1 = 1 << (B - B2);
+    // block b's runs for p1 end where block b+1's begin: neighbours share an XCD
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
     for (int d = tid; d < nb1; d += kScatterThreads) {
-        run[d] = off1[(uint64_t)blockIdx.x * nb1 + d];
+        run[d] = off1[(uint64_t)bid * nb1 + d];
         hist[d] = 0;
     }
     __syncthreads();
-    const uint64_t s = (uint64_t)blockIdx.x * blk;
+    const uint64_t s = (uint64_t)bid * blk;
     const uint64_t e = min(n, s + blk);
+    // the block's columns through buffer resources (a block holds at most
+    // 2^24 rows, so offsets fit 32 bits)
+    const uint32_t nrow = (uint32_t)(e > s ? e - s : 0);
+    const __amdgpu_buffer_rsrc_t rk = buf_rsrc(keys + s, nrow * (uint32_t)sizeof(KT));
+    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(HV ? vals + s : keys + s, nrow * (uint32_t)sizeof(KT));
     E el[IPT];
     auto load = [&](uint64_t t0, uint32_t cnt) {
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
             const uint32_t idx = tid + k * kScatterThreads;
             if (idx < cnt) {
-                el[k].k = keys[t0 + idx];
-                if constexpr (HV) el[k].v = vals[t0 + idx];
+                const uint32_t off = ((uint32_t)(t0 - s) + idx) * (uint32_t)sizeof(KT);
+                el[k].k = buf_load<KT>(rk, off);
+                if constexpr (HV) el[k].v = buf_load<KT>(rv, off);
             }
         }
     };
-    uint32_t cnt = (uint32_t)min<uint64_t>(T, e > s ? e - s : 0);
+    uint32_t cnt = min<uint32_t>(T, nrow);
     if (s < e) load(s, cnt);
     for (uint64_t t0 = s; t0 < e;) {
         uint32_t dg[IPT], rk[IPT];
@@ -290,6 +306,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
             const uint32_t idx = tid + k * kScatterThreads;
             if (idx < cnt) stage[lofs[dg[k]] + rk[k]] = el[k];
         }
+        if (tid < nb1) run[tid] -= lofs[tid];  // write base: staged row j of digit d goes to run[d] + j
         __syncthreads();
         // The tile now lives in LDS: start loading the next one into the
         // registers before writing this one out.
@@ -299,11 +316,11 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
         for (uint32_t j = tid; j < cnt; j += kScatterThreads) {
             const E x = stage[j];
             const uint32_t d = (uint32_t)(hmix(x.k) >> (64 - B)) >> B2;
-            out[run[d] + (j - lofs[d])] = x;
+            out[run[d] + j] = x;
         }
         __syncthreads();
         for (int d = tid; d < nb1; d += kScatterThreads) {
-            run[d] += hist[d];
+            run[d] += lofs[d + 1];  // = old run + hist
             hist[d] = 0;
         }
         __syncthreads();
@@ -320,7 +337,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
 // Two workgroups per CU where the registers allow it (pass-2 units are short:
 // the second workgroup hides one unit's start-up and tail).
 template <typename KT, bool HV, int IPT>
-__global__ __launch_bounds__(kScatterThreads, IPT <= 16 ? 4 : 1) void k_scatter2(const Elem<KT, HV>* __restrict__ in, uint32_t nblk,
+__global__ __launch_bounds__(kScatterThreads, IPT <= 20 ? 4 : 1) void k_scatter2(const Elem<KT, HV>* __restrict__ in, uint32_t nblk,
                                                               uint32_t G, int B, int B2,
                                                               const uint64_t* __restrict__ off1,
                                                               const uint32_t* __restrict__ Hs,
@@ -329,18 +346,22 @@ __global__ __launch_bounds__(kScatterThreads, IPT <= 16 ? 4 : 1) void k_scatter2
     using E = Elem<KT, HV>;
     constexpr int T = kScatterThreads * IPT;
     __shared__ E stage[T];
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t lofs[257];
-    __shared__ uint64_t run[256];
+    __shared__ uint32_t hist[kDigits];
+    __shared__ uint32_t lofs[kDigits + 1];
+    __shared__ uint64_t run[kDigits];
     const int tid = threadIdx.x;
     const uint32_t nb1 = 1u << (B - B2), nb2 = 1u << B2, nb = 1u << B;
     const uint32_t ngrp = (nblk + G - 1) / G;
-    const uint32_t p1 = blockIdx.x / ngrp, b0 = (blockIdx.x % ngrp) * G, b1 = min(nblk, b0 + G);
-    const uint64_t s = off1[(uint64_t)b0 * nb1 + p1];
-    const uint64_t e = off1[(uint64_t)b1 * nb1 + p1];
-    if (s >= e) return;
+    // units (p1, b0) and (p1, b0 + G) write adjacent runs of every final
+    // partition: neighbours share an XCD, so the L2 merges the line they share
+    // (in chunks of 32 units: a hot partition's units still cover every XCD)
+    const uint32_t u = xcd_remap_chunked<32>(blockIdx.x, gridDim.x);
+    const uint32_t p1 = u / ngrp, b0 = (u % ngrp) * G, b1 = min(nblk, b0 + G);
+    const uint64_t s0 = off1[(uint64_t)b0 * nb1 + p1];
+    const uint64_t e = off1[(uint64_t)b1 * nb1 + p1] - s0;  // the unit's rows: [s0, s0 + e)
+    if (e == 0) return;
     const uint32_t mask2 = nb2 - 1;
-    static_assert(kScatterThreads >= 128, "one thread per pass-2 digit");
+    static_assert(kScatterThreads >= kDigits, "one thread per pass-2 digit");
     // The run starts and the unit's first tile are read together: the tile
     // does not wait for the offsets.
     uint64_t rbase = 0;
@@ -348,26 +369,29 @@ __global__ __launch_bounds__(kScatterThreads, IPT <= 16 ? 4 : 1) void k_scatter2
         const uint64_t pp = ((uint64_t)p1 << B2) + tid;
         rbase = base[pp] + Hs[(uint64_t)b0 * nb + pp];
     }
+    // the unit spans at most max(block rows, T * 2^7) rows: far below 4 GB
+    const __amdgpu_buffer_rsrc_t src = buf_rsrc(in + s0, (uint32_t)(e * sizeof(E)));
     E el[IPT];
     auto load = [&](uint64_t t0, uint32_t cnt) {
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
             const uint32_t idx = tid + k * kScatterThreads;
-            if (idx < cnt) el[k] = in[t0 + idx];
+            if (idx < cnt) el[k] = buf_load<E>(src, (uint32_t)(t0 + idx) * (uint32_t)sizeof(E));
         }
     };
-    uint32_t cnt = (uint32_t)min<uint64_t>(T, e - s);
-    load(s, cnt);
+    uint32_t cnt = (uint32_t)min<uint64_t>(T, e);
+    load(0, cnt);
     if ((uint32_t)tid < nb2) {
         run[tid] = rbase;
         hist[tid] = 0;
     }
     __syncthreads();
-    for (uint64_t t0 = s; t0 < e;) {
+    for (uint64_t t0 = 0; t0 < e;) {
         uint32_t dg[IPT], rk[IPT];
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
             const uint32_t idx = tid + k * kScatterThreads;
+            dg[k] = ~0u;  // rows past the tile's end
             if (idx < cnt) {
                 dg[k] = (uint32_t)(hmix(el[k].k) >> (64 - B)) & mask2;
                 rk[k] = atomicAdd(&hist[dg[k]], 1u);
@@ -377,10 +401,9 @@ __global__ __launch_bounds__(kScatterThreads, IPT <= 16 ? 4 : 1) void k_scatter2
         if (tid < 64) wave_excl_scan256(hist, lofs, (int)nb2);
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            const uint32_t idx = tid + k * kScatterThreads;
-            if (idx < cnt) stage[lofs[dg[k]] + rk[k]] = el[k];
-        }
+        for (int k = 0; k < IPT; ++k)
+            if (dg[k] != ~0u) stage[lofs[dg[k]] + rk[k]] = el[k];
+        if ((uint32_t)tid < nb2) run[tid] -= lofs[tid];  // write base: staged row j of digit d goes to run[d] + j
         __syncthreads();
         const uint64_t next = t0 + cnt;
         const uint32_t ncnt = next < e ? (uint32_t)min<uint64_t>(T, e - next) : 0u;
@@ -388,11 +411,11 @@ __global__ __launch_bounds__(kScatterThreads, IPT <= 16 ? 4 : 1) void k_scatter2
         for (uint32_t j = tid; j < cnt; j += kScatterThreads) {
             const E x = stage[j];
             const uint32_t d = (uint32_t)(hmix(x.k) >> (64 - B)) & mask2;
-            out[run[d] + (j - lofs[d])] = x;
+            out[run[d] + j] = x;
         }
         __syncthreads();
         for (uint32_t d = tid; d < nb2; d += kScatterThreads) {
-            run[d] += hist[d];
+            run[d] += lofs[d + 1];  // = old run + hist
             hist[d] = 0;
         }
         __syncthreads();
@@ -533,17 +556,21 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
         const uint32_t r0 = rb + tid;
         return r0 < sN ? min<uint32_t>(U, (sN - r0 + kJoinThreads - 1) / kJoinThreads) : 0u;
     };
+    // The item's probe rows (at most kItemRows) through a buffer resource:
+    // 32-bit offsets, and rows past the item's end read as zero.
+    using PE = Elem<KT, HVS>;
+    const __amdgpu_buffer_rsrc_t srs = buf_rsrc(S, sN * (uint32_t)sizeof(PE));
     auto fetch = [&](uint32_t rb) {
         const uint32_t cnt = rows_in(rb);
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if ((uint32_t)u < cnt) x[u] = nt_load(&S[rb + tid + u * kJoinThreads]);
+            if ((uint32_t)u < cnt) x[u] = buf_load<PE, true>(srs, (rb + tid + u * kJoinThreads) * (uint32_t)sizeof(PE));
     };
     // Unpredicated fetch of the batch at `rb` into z: rows past the item's
-    // end read its last row (sN > 0) and are masked by the probe.
-    auto fetch_clamped = [&](Elem<KT, HVS>(&z)[U], uint32_t rb) {
+    // end read zero and are masked by the probe.
+    auto fetch_clamped = [&](PE(&z)[U], uint32_t rb) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) z[u] = nt_load(&S[min(rb + tid + u * kJoinThreads, sN - 1)]);
+        for (int u = 0; u < U; ++u) z[u] = buf_load<PE, true>(srs, (rb + tid + u * kJoinThreads) * (uint32_t)sizeof(PE));
     };
     constexpr int RB = (kJoinPiece + kJoinThreads - 1) / kJoinThreads;  // build rows per thread per piece
     Elem<KT, true> br[RB];
@@ -598,6 +625,7 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
         __syncthreads();
         if (p0) fetch(0);  // later pieces re-read the item's probe rows
         const unsigned long long spc = sp_cnt, sps = sp_sum;
+        unsigned long long pe_n = 0, pe_v = 0;  // this piece's probe rows keyed by the empty marker (and payloads)
         // One batch: all U slot reads in flight at once, collision chains
         // walked only by the lanes that need them, selects instead of
         // branches.  FULL batches (every thread holds U rows) skip the
@@ -626,30 +654,35 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
                     }
                 }
             }
-            uint32_t bc = 0;  // fits 32 bits: U rows x PIECE matches
-            unsigned long long bs = 0, bp = 0;
+            // Each walk ended on the key's slot or on an empty one, whose
+            // count and sum are 0: no hit / miss select.  A probe key equal to
+            // the empty marker also ends on an empty slot; those rows are
+            // tallied and take the side accumulator once per piece.
+            uint32_t bc = 0, ne = 0;  // fits 32 bits: U rows x PIECE matches
+            unsigned long long bs = 0, bp = 0, ve = 0;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const long long k = (long long)x[u].k;
-                const uint32_t raw = tc16[slot[u]];
-                const bool hit = t[u].key == k;
-                uint32_t c = hit ? raw : 0u;
-                unsigned long long sb = hit ? t[u].sum : 0ull;
-                if (k == EMPTY) {  // the empty marker as a real key: side accumulator
-                    c = (uint32_t)spc;
-                    sb = sps;
-                }
-                if (!FULL && (uint32_t)u >= cnt) {
+                const bool live = FULL || (uint32_t)u < cnt;
+                uint32_t c = tc16[slot[u]];
+                unsigned long long sb = t[u].sum;
+                if (!live) {
                     c = 0;
                     sb = 0;
                 }
                 bc += c;
                 bs += sb;
                 if constexpr (HVS) bp += (unsigned long long)c * (unsigned long long)(int64_t)x[u].v;
+                if constexpr (sizeof(KT) == 8) {  // sign-extended int32 keys never equal it
+                    const bool em = live && (long long)x[u].k == EMPTY;
+                    ne += em ? 1u : 0u;
+                    if constexpr (HVS) ve += em ? (unsigned long long)(int64_t)x[u].v : 0ull;
+                }
             }
             acc_c += bc;
             acc_b += bs;
             acc_p += bp;
+            pe_n += ne;
+            pe_v += ve;
         };
         auto probe = [&](const Elem<KT, HVS>(&z)[U], uint32_t rb) {
             if (rb + BATCH <= sN)
@@ -666,6 +699,9 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
             fetch_clamped(x, rb + 2 * BATCH);
             if (rb + BATCH < sN) probe(y, rb + BATCH);
         }
+        acc_c += pe_n * spc;
+        acc_b += pe_n * sps;
+        acc_p += pe_v * spc;
         __syncthreads();
     }
     acc_c = wave_sum(acc_c);
