@@ -13,12 +13,16 @@
 // ParameterizedModuloValueHasher::generate(2), hash.cpp:237-267: pass 1 takes
 // the high B1 = B/2 bits of the B-bit radix value, pass 2 the low B2 = B-B1):
 //   k_hist      one read of each key column; per block of `blk` rows an LDS
-//               histogram over all 2^B final partitions, written to H[blk][2^B]
-//   k_colscan   exclusive scan of H down the blocks; per-bin totals
+//               histogram over all 2^B final partitions, written to H[blk][2^B],
+//               and per pass-1 tile of the block the rows of each p1 in the
+//               block's tiles before it (toff)
+//   k_colscan   exclusive scan of H down the blocks; per-bin totals; per
+//               (block, p1) the sum of the block's prefixes over p2, which
+//               with base[] gives pass 1's write offsets (pass1_off)
 //   k_binscan   exclusive scan of the totals -> base[] (partition starts)
-//   k_off1      pass-1 write offsets off1[b][p1] = base(p1) + sum over p2 of H
-//   k_scatter1  block b streams its rows, reorders each tile by p1 in LDS and
-//               writes contiguous digit runs at its exact offsets (no atomics)
+//   k_scatter1  one workgroup per tile: reorders the tile by p1 in LDS and
+//               writes contiguous digit runs at its exact offsets (no atomics);
+//               consecutive tiles run side by side on one XCD
 //   k_scatter2  pass-2 block (p1, b) = the rows block b wrote to p1; it
 //               reorders them by p2 and writes runs at base(p1,p2)+H[b][p1,p2]
 //   k_icount /  probe items: each final partition's probe rows are cut into
@@ -60,8 +64,8 @@ constexpr int kTileBytes = 77824;         // LDS staging tile of the pass-2 scat
                                           // 9728 8-byte rows hold a whole unit of ~8K rows (with 64 KB,
                                           // half the units spilled a few rows into a second tile)
 constexpr int kDigits = 128;              // digits of one pass: at most 7 of the <= 14 radix bits
-constexpr int kTileBytes1 = 131072;       // pass 1: one workgroup per CU; twice the rows per partition run
-                                          // per tile halves the partial lines written per byte
+constexpr int kTileBytes1 = 65536;        // pass-1 tile (one workgroup each, two per CU); a multiple of the
+                                          // histogram's 4096-row step for rows of at most 16 bytes
 constexpr int kPartSlots = 2048;          // partial-result slots of the join kernels
 static_assert(kJoinPiece < 65536, "LDS match counters are 16 bits wide");
 static_assert(2 * kJoinPiece <= kJoinCap, "LDS table load factor above 0.5");
@@ -150,108 +154,163 @@ __device__ __forceinline__ uint64_t block_excl_scan1024(uint64_t v, uint64_t* to
 }
 
 // ---------------------------------------------------------------- radix kernels
+// Block b's rows are pass-1 tiles of T1 rows (blk is a multiple of T1).
+// Besides H[b], toff[t][p1] receives, for each tile t of the block, the
+// block's rows of partition p1 in its tiles before t.  Each wavefront counts
+// whole tiles (tiles w, w + 8, ... of the block) in its own LDS row, so the
+// sweep needs no barrier; the counts go to toff as the wavefront finishes a
+// tile and become in-block prefixes after the sweep.  Dynamic LDS: 2^B words.
 template <typename KT>
-__global__ __launch_bounds__(512) void k_hist(const KT* __restrict__ keys, uint64_t n, uint64_t blk, int B,
-                                              uint32_t* __restrict__ H) {
+__global__ __launch_bounds__(512) void k_hist(const KT* __restrict__ keys, uint64_t n, uint64_t blk, uint32_t T1,
+                                              int B, int B2, uint32_t* __restrict__ H, uint32_t* __restrict__ toff) {
     extern __shared__ uint32_t lh[];
-    const uint32_t nb = 1u << B;
-    for (uint32_t i = threadIdx.x; i < nb; i += 512) lh[i] = 0;
+    __shared__ uint32_t lt[8 * kDigits];  // [wavefront][p1] counts of the tile being swept
+    const uint32_t nb = 1u << B, nb1 = 1u << (B - B2);
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    for (uint32_t i = tid; i < nb; i += 512) lh[i] = 0;
+    for (uint32_t i = tid; i < 8 * kDigits; i += 512) lt[i] = 0;
     __syncthreads();
     const uint64_t s = (uint64_t)blockIdx.x * blk;
     const uint64_t e = min(n, s + blk);
     constexpr int U = 8;
-    uint64_t i = s + threadIdx.x;
-    for (; i + (U - 1) * 512 < e; i += U * 512) {
-        KT k[U];
+    uint32_t* lw = lt + w * kDigits;
+    for (uint64_t t0 = s + (uint64_t)w * T1; t0 < e; t0 += 8ull * T1) {
+        const uint64_t te = min(e, t0 + T1);
+        uint64_t i = t0 + lane;
+        for (; i + (U - 1) * 64 < te; i += U * 64) {
+            KT k[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) k[u] = nt_load(&keys[i + u * 512]);
+            for (int u = 0; u < U; ++u) k[u] = nt_load(&keys[i + u * 64]);
 #pragma unroll
-        for (int u = 0; u < U; ++u) atomicAdd(&lh[(uint32_t)(hmix(k[u]) >> (64 - B))], 1u);
+            for (int u = 0; u < U; ++u) {
+                const uint32_t d = (uint32_t)(hmix(k[u]) >> (64 - B));
+                atomicAdd(&lh[d], 1u);
+                atomicAdd(&lw[d >> B2], 1u);
+            }
+        }
+        for (; i < te; i += 64) {
+            const uint32_t d = (uint32_t)(hmix(keys[i]) >> (64 - B));
+            atomicAdd(&lh[d], 1u);
+            atomicAdd(&lw[d >> B2], 1u);
+        }
+        // one wavefront's LDS operations complete in order: its counts are final
+        uint32_t* tc = toff + (t0 / T1) * kDigits;
+        for (uint32_t d = lane; d < nb1; d += 64) {
+            tc[d] = lw[d];
+            lw[d] = 0;
+        }
     }
-    for (; i < e; i += 512) atomicAdd(&lh[(uint32_t)(hmix(keys[i]) >> (64 - B))], 1u);
-    __syncthreads();
+    __syncthreads();  // every tile's counts are in toff
+    if ((uint32_t)tid < nb1 && e > s) {
+        const uint64_t t1 = (e + T1 - 1) / T1;
+        uint32_t run = 0;
+#pragma unroll 8
+        for (uint64_t t = s / T1; t < t1; ++t) {
+            const uint32_t c = toff[t * kDigits + tid];
+            toff[t * kDigits + tid] = run;
+            run += c;
+        }
+    }
     uint32_t* Hb = H + (uint64_t)blockIdx.x * nb;
-    for (uint32_t j = threadIdx.x; j < nb; j += 512) Hb[j] = lh[j];
+    for (uint32_t j = tid; j < nb; j += 512) Hb[j] = lh[j];
 }
 
 // One workgroup per 64 bins; its 8 wavefronts scan 8 contiguous segments of
 // blocks, then each bin's column becomes its exclusive prefix over blocks.
-__global__ __launch_bounds__(512) void k_colscan(uint32_t* __restrict__ H, uint32_t nblk, uint32_t nb,
-                                                 uint32_t* __restrict__ tot) {
+// Each lane reads its segment's counts once, all loads in flight together
+// (at most kMaxBlocks / 8 rows), and writes the prefixes from registers.
+// px[sub][b][p1] receives the sum of block b's prefixes over this
+// workgroup's bins of partition p1 (sub = which 64-bin half of p1 when p1 has
+// 128 bins): the rows of p1 in blocks before b, so pass 1 needs no offset
+// kernel of its own.
+constexpr uint32_t kMaxBlocks = 512;  // histogram blocks of one side
+__global__ __launch_bounds__(512) void k_colscan(uint32_t* __restrict__ H, uint32_t nblk, uint32_t nb, int B2,
+                                                 uint32_t* __restrict__ tot, uint32_t* __restrict__ px) {
+    constexpr int kSeg = kMaxBlocks / 8;
     __shared__ uint32_t part[8][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t bin = blockIdx.x * 64 + lane;
     const uint32_t seg = (nblk + 7) / 8;
-    const uint32_t b0 = min(nblk, w * seg), b1 = min(nblk, b0 + seg);
+    const uint32_t b0 = min(nblk, w * seg), nseg = min(nblk, b0 + seg) - b0;
+    const bool live = bin < nb;
+    const uint32_t nb2 = 1u << B2, seg2 = min(nb2, 64u);  // bins of one p1 within these 64
+    const uint32_t p1 = bin >> B2, sub = nb2 > 64 ? (bin >> 6) & 1 : 0;
+    uint32_t h[kSeg];
     uint32_t s = 0;
-    if (bin < nb)
-        for (uint32_t b = b0; b < b1; ++b) s += H[(uint64_t)b * nb + bin];
+#pragma unroll
+    for (int j = 0; j < kSeg; ++j) {
+        h[j] = live && (uint32_t)j < nseg ? H[(uint64_t)(b0 + j) * nb + bin] : 0u;
+        s += h[j];
+    }
     part[w][lane] = s;
     __syncthreads();
     uint32_t run = 0;
     for (int k = 0; k < w; ++k) run += part[k][lane];
-    if (bin < nb) {
-        for (uint32_t b = b0; b < b1; ++b) {
-            const uint32_t h = H[(uint64_t)b * nb + bin];
-            H[(uint64_t)b * nb + bin] = run;
-            run += h;
+#pragma unroll
+    for (int j = 0; j < kSeg; ++j) {
+        if ((uint32_t)j < nseg) {  // uniform across the wavefront: the shuffles see every lane
+            if (live) H[(uint64_t)(b0 + j) * nb + bin] = run;
+            uint32_t ps = live ? run : 0u;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1)  // unrolled: the rows' shuffles overlap
+                if ((uint32_t)d < seg2) ps += __shfl_xor(ps, d, 64);
+            if (live && (lane & (seg2 - 1)) == 0) px[((uint64_t)sub * kMaxBlocks + b0 + j) * kDigits + p1] = ps;
         }
-        if (w == 7) tot[bin] = run;
+        run += h[j];
     }
+    if (live && w == 7) tot[bin] = run;
 }
 
-// base[i] = sum of in[0..i) for i <= n (n <= 16384), single workgroup: each
-// thread holds its 16 consecutive counts in registers (one load batch).
+// Pass-1 write offset of block b's rows of partition p1 (b == nblk: the
+// partition's end): base(p1, 0) plus the rows of p1 in blocks before b.
+__device__ __forceinline__ uint64_t pass1_off(const uint64_t* __restrict__ base, const uint32_t* __restrict__ px,
+                                              uint32_t b, uint32_t nblk, uint32_t p1, int B2) {
+    if (b == nblk) return base[(uint64_t)(p1 + 1) << B2];
+    uint64_t o = base[(uint64_t)p1 << B2] + px[(uint64_t)b * kDigits + p1];
+    if (B2 > 6) o += px[((uint64_t)kMaxBlocks + b) * kDigits + p1];
+    return o;
+}
+
+// base[i] = sum of in[0..i) for i <= n (n <= 16384, sums below 2^32), one
+// workgroup: the counts are staged in LDS with coalesced loads, each thread
+// scans 16 consecutive ones, and the prefixes go out with coalesced stores.
 __global__ __launch_bounds__(1024) void k_binscan(const uint32_t* __restrict__ in, uint32_t n,
                                                   uint64_t* __restrict__ base) {
-    constexpr int PER = 16;
-    const uint32_t t0 = threadIdx.x * PER;
-    uint32_t v[PER];
-    uint64_t s = 0;
+    constexpr int PER = 16, N = 1024 * PER;
+    __shared__ uint32_t v[N + N / 32];  // element i at i + i / 32: the 16-element strips read without conflicts
+    const int t = threadIdx.x;
+    for (int i = t; i < N; i += 1024) v[i + (i >> 5)] = (uint32_t)i < n ? in[i] : 0u;
+    __syncthreads();
+    uint32_t a[PER];
+    uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        v[i] = t0 + i < n ? in[t0 + i] : 0u;
-        s += v[i];
+    for (int k = 0; k < PER; ++k) {
+        const int i = t * PER + k;
+        a[k] = v[i + (i >> 5)];
+        s += a[k];
     }
     __shared__ uint64_t tot_sh;
-    uint64_t ex = block_excl_scan1024(s, &tot_sh);
+    uint32_t ex = (uint32_t)block_excl_scan1024(s, &tot_sh);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        if (t0 + i < n) base[t0 + i] = ex;
-        ex += v[i];
+    for (int k = 0; k < PER; ++k) {
+        const int i = t * PER + k;
+        v[i + (i >> 5)] = ex;
+        ex += a[k];
     }
     __syncthreads();
-    if (threadIdx.x == 0) base[n] = tot_sh;
+    for (int i = t; (uint32_t)i < n; i += 1024) base[i] = v[i + (i >> 5)];
+    if (t == 0) base[n] = tot_sh;
 }
 
-// off1[b][p1] = base[p1 << B2] + sum_p2 Hs[b][p1, p2] for b < nblk;
-// off1[nblk][p1] = base[(p1 + 1) << B2] (end of partition p1).  One
-// wavefront per (b, p1): its 2^B2 (<= 128) counts are read coalesced and
-// reduced with shuffles.
-__global__ __launch_bounds__(256) void k_off1(const uint32_t* __restrict__ Hs, const uint64_t* __restrict__ base,
-                                              uint32_t nblk, int B, int B2, uint64_t* __restrict__ off1) {
-    const uint32_t nb1 = 1u << (B - B2), nb2 = 1u << B2, nb = 1u << B;
-    const int lane = threadIdx.x & 63;
-    const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (t >= (uint64_t)(nblk + 1) * nb1) return;
-    const uint32_t b = (uint32_t)(t / nb1), p1 = (uint32_t)(t % nb1);
-    if (b == nblk) {
-        if (lane == 0) off1[t] = base[(uint64_t)(p1 + 1) << B2];
-        return;
-    }
-    const uint32_t* row = Hs + (uint64_t)b * nb + ((uint64_t)p1 << B2);
-    uint64_t s = 0;  // column prefixes: their sum over 128 bins can pass 2^32
-    for (uint32_t j = lane; j < nb2; j += 64) s += row[j];
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (lane == 0) off1[t] = base[(uint64_t)p1 << B2] + s;
-}
-
+// Tile t (rows [t*T, t*T + T), inside block b = t / ntb) goes to
+// pass1_off(b, p1) + toff[t][p1] for each of its digit runs.
 template <typename KT, bool HV, int IPT>
-__global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restrict__ keys, const KT* __restrict__ vals,
-                                                              uint64_t n, uint64_t blk, int B, int B2,
-                                                              const uint64_t* __restrict__ off1,
-                                                              Elem<KT, HV>* __restrict__ out) {
+__global__ __launch_bounds__(kScatterThreads, 4) void k_scatter1(const KT* __restrict__ keys, const KT* __restrict__ vals,
+                                                                 uint64_t n, uint32_t ntb, uint32_t nblk, int B, int B2,
+                                                                 const uint64_t* __restrict__ base,
+                                                                 const uint32_t* __restrict__ px,
+                                                                 const uint32_t* __restrict__ toff,
+                                                                 Elem<KT, HV>* __restrict__ out) {
     using E = Elem<KT, HV>;
     constexpr int T = kScatterThreads * IPT;
     __shared__ E stage[T];
@@ -260,77 +319,55 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
     __shared__ uint64_t run[kDigits];
     const int tid = threadIdx.x;
     const int nb1 = 1 << (B - B2);
-    // block b's runs for p1 end where block b+1's begin: neighbours share an XCD
-    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    // tiles t and t+1 write adjacent runs of every p1: neighbours share an XCD
+    // (in chunks of 32 tiles), so the L2 merges the line they share
+    const uint32_t t = xcd_remap_chunked<32>(blockIdx.x, gridDim.x);
+    const uint64_t s = (uint64_t)t * T;
+    const uint32_t cnt = (uint32_t)min<uint64_t>(T, n - s);
+    // the tile's columns through buffer resources (offsets below T rows)
+    const __amdgpu_buffer_rsrc_t rk = buf_rsrc(keys + s, cnt * (uint32_t)sizeof(KT));
+    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(HV ? vals + s : keys + s, cnt * (uint32_t)sizeof(KT));
+    E el[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const uint32_t idx = tid + k * kScatterThreads;
+        if (idx < cnt) {
+            el[k].k = buf_load<KT>(rk, idx * (uint32_t)sizeof(KT));
+            if constexpr (HV) el[k].v = buf_load<KT>(rv, idx * (uint32_t)sizeof(KT));
+        }
+    }
     for (int d = tid; d < nb1; d += kScatterThreads) {
-        run[d] = off1[(uint64_t)bid * nb1 + d];
+        run[d] = pass1_off(base, px, t / ntb, nblk, (uint32_t)d, B2) + toff[(uint64_t)t * kDigits + d];
         hist[d] = 0;
     }
     __syncthreads();
-    const uint64_t s = (uint64_t)bid * blk;
-    const uint64_t e = min(n, s + blk);
-    // the block's columns through buffer resources (a block holds at most
-    // 2^24 rows, so offsets fit 32 bits)
-    const uint32_t nrow = (uint32_t)(e > s ? e - s : 0);
-    const __amdgpu_buffer_rsrc_t rk = buf_rsrc(keys + s, nrow * (uint32_t)sizeof(KT));
-    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(HV ? vals + s : keys + s, nrow * (uint32_t)sizeof(KT));
-    E el[IPT];
-    auto load = [&](uint64_t t0, uint32_t cnt) {
+    uint32_t dg[IPT], rk_[IPT];
 #pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            const uint32_t idx = tid + k * kScatterThreads;
-            if (idx < cnt) {
-                const uint32_t off = ((uint32_t)(t0 - s) + idx) * (uint32_t)sizeof(KT);
-                el[k].k = buf_load<KT>(rk, off);
-                if constexpr (HV) el[k].v = buf_load<KT>(rv, off);
-            }
+    for (int k = 0; k < IPT; ++k) {
+        const uint32_t idx = tid + k * kScatterThreads;
+        dg[k] = ~0u;  // rows past the tile's end
+        if (idx < cnt) {
+            dg[k] = (uint32_t)(hmix(el[k].k) >> (64 - B)) >> B2;
+            rk_[k] = atomicAdd(&hist[dg[k]], 1u);
         }
-    };
-    uint32_t cnt = min<uint32_t>(T, nrow);
-    if (s < e) load(s, cnt);
-    for (uint64_t t0 = s; t0 < e;) {
-        uint32_t dg[IPT], rk[IPT];
+    }
+    __syncthreads();
+    if (tid < 64) wave_excl_scan256(hist, lofs, nb1);
+    __syncthreads();
 #pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            const uint32_t idx = tid + k * kScatterThreads;
-            if (idx < cnt) {
-                dg[k] = (uint32_t)(hmix(el[k].k) >> (64 - B)) >> B2;
-                rk[k] = atomicAdd(&hist[dg[k]], 1u);
-            }
-        }
-        __syncthreads();
-        if (tid < 64) wave_excl_scan256(hist, lofs, nb1);
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            const uint32_t idx = tid + k * kScatterThreads;
-            if (idx < cnt) stage[lofs[dg[k]] + rk[k]] = el[k];
-        }
-        if (tid < nb1) run[tid] -= lofs[tid];  // write base: staged row j of digit d goes to run[d] + j
-        __syncthreads();
-        // The tile now lives in LDS: start loading the next one into the
-        // registers before writing this one out.
-        const uint64_t next = t0 + cnt;
-        const uint32_t ncnt = next < e ? (uint32_t)min<uint64_t>(T, e - next) : 0u;
-        if (ncnt) load(next, ncnt);
-        for (uint32_t j = tid; j < cnt; j += kScatterThreads) {
-            const E x = stage[j];
-            const uint32_t d = (uint32_t)(hmix(x.k) >> (64 - B)) >> B2;
-            out[run[d] + j] = x;
-        }
-        __syncthreads();
-        for (int d = tid; d < nb1; d += kScatterThreads) {
-            run[d] += lofs[d + 1];  // = old run + hist
-            hist[d] = 0;
-        }
-        __syncthreads();
-        t0 = next;
-        cnt = ncnt;
+    for (int k = 0; k < IPT; ++k)
+        if (dg[k] != ~0u) stage[lofs[dg[k]] + rk_[k]] = el[k];
+    if (tid < nb1) run[tid] -= lofs[tid];  // write base: staged row j of digit d goes to run[d] + j
+    __syncthreads();
+    for (uint32_t j = tid; j < cnt; j += kScatterThreads) {
+        const E x = stage[j];
+        const uint32_t d = (uint32_t)(hmix(x.k) >> (64 - B)) >> B2;
+        out[run[d] + j] = x;
     }
 }
 
 // A pass-2 unit is partition p1's rows from pass-1 blocks [b0, b0+G): they
-// are contiguous in the pass-1 output (block b wrote p1's rows at off1[b][p1]),
+// are contiguous in the pass-1 output (block b wrote p1's rows at pass1_off(b, p1)),
 // and the units' output runs for (p1, p2) are contiguous too, starting at
 // base(p1,p2) + H[b0][p1,p2].  G > 1 keeps units near a tile when blocks
 // are small.
@@ -339,7 +376,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter1(const KT* __restri
 template <typename KT, bool HV, int IPT>
 __global__ __launch_bounds__(kScatterThreads, IPT <= 20 ? 4 : 1) void k_scatter2(const Elem<KT, HV>* __restrict__ in, uint32_t nblk,
                                                               uint32_t G, int B, int B2,
-                                                              const uint64_t* __restrict__ off1,
+                                                              const uint32_t* __restrict__ px,
                                                               const uint32_t* __restrict__ Hs,
                                                               const uint64_t* __restrict__ base,
                                                               Elem<KT, HV>* __restrict__ out) {
@@ -357,8 +394,8 @@ __global__ __launch_bounds__(kScatterThreads, IPT <= 20 ? 4 : 1) void k_scatter2
     // (in chunks of 32 units: a hot partition's units still cover every XCD)
     const uint32_t u = xcd_remap_chunked<32>(blockIdx.x, gridDim.x);
     const uint32_t p1 = u / ngrp, b0 = (u % ngrp) * G, b1 = min(nblk, b0 + G);
-    const uint64_t s0 = off1[(uint64_t)b0 * nb1 + p1];
-    const uint64_t e = off1[(uint64_t)b1 * nb1 + p1] - s0;  // the unit's rows: [s0, s0 + e)
+    const uint64_t s0 = pass1_off(base, px, b0, nblk, p1, B2);
+    const uint64_t e = pass1_off(base, px, b1, nblk, p1, B2) - s0;  // the unit's rows: [s0, s0 + e)
     if (e == 0) return;
     const uint32_t mask2 = nb2 - 1;
     static_assert(kScatterThreads >= kDigits, "one thread per pass-2 digit");
@@ -858,7 +895,9 @@ struct pgpu_join_plan {
     uint32_t nblkR = 0, nblkS = 0;  // allocated block rows
     void *R1 = nullptr, *R2 = nullptr, *S1 = nullptr, *S2 = nullptr;
     uint32_t *HR = nullptr, *HS = nullptr, *totR = nullptr, *totS = nullptr, *icount = nullptr;
-    uint64_t *baseR = nullptr, *baseS = nullptr, *off1R = nullptr, *off1S = nullptr, *ioff = nullptr;
+    uint64_t *baseR = nullptr, *baseS = nullptr, *ioff = nullptr;
+    uint32_t *pxR = nullptr, *pxS = nullptr;  // per-(block, p1) prefixes of k_colscan (2 x kMaxBlocks x kDigits)
+    uint32_t *toffR = nullptr, *toffS = nullptr;  // per-(pass-1 tile, p1) offsets within the tile's block
     unsigned long long* result = nullptr;  // [4]: count, sum_build, sum_probe, max pieces
     unsigned long long* part = nullptr;    // [kPartSlots][4] per-workgroup partials
     JoinItem* items = nullptr;             // one record per join work item
@@ -888,9 +927,9 @@ int plan_alloc(pgpu_join_plan* p, uint64_t bytes, void** out) {
 
 // Elements per LDS staging tile for an element of `esz` bytes.
 inline uint64_t tile_elems(uint64_t esz) { return kTileBytes / esz; }
-// pass-1 rows per thread: the 128 KB tile, at most 32 rows (4-byte rows
-// would otherwise spill registers; 16K of them already fill 64 KB)
-constexpr int pass1_ipt(int esz) { return kTileBytes1 / esz / kScatterThreads < 32 ? kTileBytes1 / esz / kScatterThreads : 32; }
+// pass-1 rows per thread: the 64 KB tile, at most 16 rows (two workgroups
+// per CU hold at most 128 registers per lane)
+constexpr int pass1_ipt(int esz) { return kTileBytes1 / esz / kScatterThreads < 16 ? kTileBytes1 / esz / kScatterThreads : 16; }
 inline uint64_t tile1_elems(uint64_t esz) { return (uint64_t)pass1_ipt((int)esz) * kScatterThreads; }
 
 // Rows per histogram/scatter block: at most 256 blocks (one per CU), tile-aligned.
@@ -918,38 +957,42 @@ struct Timed {  // t may be null (untimed launches)
 
 template <typename KT, bool HV>
 int run_partition(pgpu_join_plan* p, hipStream_t st, const KT* keys, const KT* vals, uint64_t n, int B, uint32_t* H,
-                  uint32_t* tot, uint64_t* base, uint64_t* off1, Elem<KT, HV>* P1, Elem<KT, HV>* P2, const char* tag) {
+                  uint32_t* tot, uint64_t* base, uint32_t* px, uint32_t* toff, Elem<KT, HV>* P1, Elem<KT, HV>* P2,
+                  const char* tag) {
     using E = Elem<KT, HV>;
     const int B2 = B - B / 2;  // generate(2): pass 1 takes floor(B/2) high bits
     const uint32_t nb = 1u << B, nb1 = 1u << (B - B2);
     const uint64_t blk = block_rows(n, sizeof(E));
     const uint32_t nblk = (uint32_t)ceil_div(std::max<uint64_t>(n, 1), blk);
+    if (nblk > kMaxBlocks) return pgpu::fail(PGPU_EINVAL, "radix partition: %u blocks (at most %u)", nblk, kMaxBlocks);
     constexpr int IPT = kTileBytes / (int)sizeof(E) / kScatterThreads;    // pass 2
     constexpr int IPT1 = pass1_ipt((int)sizeof(E));                       // pass 1
+    const uint64_t T1 = tile1_elems(sizeof(E));
+    const uint64_t ntiles = ceil_div(n, T1);  // blocks are whole tiles
     char name[48];
     {
         snprintf(name, sizeof name, "hist_%s", tag);
         Timed tm(p->timer, name);
-        hipLaunchKernelGGL(k_hist<KT>, dim3(nblk), dim3(512), nb * sizeof(uint32_t), st, keys, n, blk, B, H);
+        hipLaunchKernelGGL(k_hist<KT>, dim3(nblk), dim3(512), nb * sizeof(uint32_t), st, keys, n, blk, (uint32_t)T1, B,
+                           B2, H, toff);
         PGPU_LAUNCH_CHECK();
     }
     {
         snprintf(name, sizeof name, "scan_%s", tag);
         Timed tm(p->timer, name);
-        hipLaunchKernelGGL(k_colscan, dim3((unsigned)ceil_div(nb, 64)), dim3(512), 0, st, H, nblk, nb, tot);
+        hipLaunchKernelGGL(k_colscan, dim3((unsigned)ceil_div(nb, 64)), dim3(512), 0, st, H, nblk, nb, B2, tot, px);
         PGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_binscan, dim3(1), dim3(1024), 0, st, tot, nb, base);
-        PGPU_LAUNCH_CHECK();
-        const uint64_t nt = (uint64_t)(nblk + 1) * nb1;
-        hipLaunchKernelGGL(k_off1, dim3((unsigned)ceil_div(nt, 4)), dim3(256), 0, st, H, base, nblk, B, B2, off1);
         PGPU_LAUNCH_CHECK();
     }
     {
         snprintf(name, sizeof name, "scatter1_%s", tag);
         Timed tm(p->timer, name);
-        hipLaunchKernelGGL((k_scatter1<KT, HV, IPT1>), dim3(nblk), dim3(kScatterThreads), 0, st, keys, vals, n, blk,
-                           B, B2, off1, P1);
-        PGPU_LAUNCH_CHECK();
+        if (ntiles) {
+            hipLaunchKernelGGL((k_scatter1<KT, HV, IPT1>), dim3((unsigned)ntiles), dim3(kScatterThreads), 0, st, keys,
+                               vals, n, (uint32_t)(blk / T1), nblk, B, B2, base, px, toff, P1);
+            PGPU_LAUNCH_CHECK();
+        }
     }
     {
         snprintf(name, sizeof name, "scatter2_%s", tag);
@@ -959,7 +1002,7 @@ int run_partition(pgpu_join_plan* p, hipStream_t st, const KT* keys, const KT* v
         const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nblk, T * nb1 / blk));
         const uint32_t ngrp = (nblk + G - 1) / G;
         hipLaunchKernelGGL((k_scatter2<KT, HV, IPT>), dim3(ngrp * nb1), dim3(kScatterThreads), 0, st, P1, nblk, G, B,
-                           B2, off1, H, base, P2);
+                           B2, px, H, base, P2);
         PGPU_LAUNCH_CHECK();
     }
     return PGPU_OK;
@@ -1010,9 +1053,9 @@ template <typename KT, bool HVS>
 int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const KT* sk, const KT* sv, uint64_t nS,
               int B) {
     hipStream_t st = p->ctx->stream;
-    PGPU_TRY((run_partition<KT, true>(p, st, rk, rv, nR, B, p->HR, p->totR, p->baseR, p->off1R,
+    PGPU_TRY((run_partition<KT, true>(p, st, rk, rv, nR, B, p->HR, p->totR, p->baseR, p->pxR, p->toffR,
                                       (Elem<KT, true>*)p->R1, (Elem<KT, true>*)p->R2, "build")));
-    PGPU_TRY((run_partition<KT, HVS>(p, st, sk, sv, nS, B, p->HS, p->totS, p->baseS, p->off1S,
+    PGPU_TRY((run_partition<KT, HVS>(p, st, sk, sv, nS, B, p->HS, p->totS, p->baseS, p->pxS, p->toffS,
                                      (Elem<KT, HVS>*)p->S1, (Elem<KT, HVS>*)p->S2, "probe")));
     return run_lds_join<KT, HVS>(p, st, (const Elem<KT, true>*)p->R2, p->baseR, p->totR, nS, B, p->cfg.join_variant);
 }
@@ -1073,7 +1116,7 @@ int pgpu::radix_build_resident(pgpu_join_plan* p, hipStream_t st, const int64_t*
     if (bits < 2 || bits > kMaxBits || (1u << bits) > p->nbmax)
         return fail(PGPU_EINVAL, "radix_build_resident: %d radix bits", bits);
     PGPU_TRY(ctx_activate(p->ctx));
-    return run_partition<int64_t, true>(p, st, key, pay, n, bits, p->HR, tot, base, p->off1R,
+    return run_partition<int64_t, true>(p, st, key, pay, n, bits, p->HR, tot, base, p->pxR, p->toffR,
                                         (Elem<int64_t, true>*)p->R1, (Elem<int64_t, true>*)rows, "build");
 }
 
@@ -1090,7 +1133,7 @@ int pgpu::radix_probe_resident(pgpu_join_plan* p, hipStream_t st, const int64_t*
     PGPU_TRY(ctx_activate(p->ctx));
     hipLaunchKernelGGL(k_fill64, dim3(8), dim3(256), 0, st, p->part, (uint64_t)4 * kPartSlots);
     PGPU_LAUNCH_CHECK();
-    PGPU_TRY((run_partition<int64_t, false>(p, st, probe, nullptr, m, bits, p->HS, p->totS, p->baseS, p->off1S,
+    PGPU_TRY((run_partition<int64_t, false>(p, st, probe, nullptr, m, bits, p->HS, p->totS, p->baseS, p->pxS, p->toffS,
                                             (Elem<int64_t, false>*)p->S1, (Elem<int64_t, false>*)p->S2, "probe")));
     PGPU_TRY((run_lds_join<int64_t, false>(p, st, (const Elem<int64_t, true>*)rows, base, tot, m, bits, 0)));
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, p->part, kPartSlots, res);
@@ -1168,8 +1211,10 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
             {(void**)&p->baseS, (nb + 1) * 8},
             {(void**)&p->ioff, (nb + 1) * 8},
             {(void**)&p->items, (nb + ceil_div(std::max<uint64_t>(max_probe, 1), kItemRows) + 1) * sizeof(JoinItem)},
-            {(void**)&p->off1R, (uint64_t)(p->nblkR + 1) * 256 * 8},
-            {(void**)&p->off1S, (uint64_t)(p->nblkS + 1) * 256 * 8},
+            {(void**)&p->pxR, 2ull * kMaxBlocks * kDigits * 4},
+            {(void**)&p->pxS, 2ull * kMaxBlocks * kDigits * 4},
+            {(void**)&p->toffR, (ceil_div(std::max<uint64_t>(max_build, 1), tile1_elems(esR)) + 1) * kDigits * 4},
+            {(void**)&p->toffS, (ceil_div(std::max<uint64_t>(max_probe, 1), tile1_elems(esS)) + 1) * kDigits * 4},
         };
         for (auto& x : a)
             if ((rc = plan_alloc(p, x.bytes, x.ptr)) != PGPU_OK) return cleanup(rc);
