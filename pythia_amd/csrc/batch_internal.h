@@ -69,6 +69,10 @@ int launch_hist(hipStream_t st, int key_bytes, const Seg* segs, const Chunk* chu
 // max_bits: the largest partition bits of the launched segments.
 int launch_scan(hipStream_t st, uint32_t max_bits, const Seg* segs, uint32_t seg_base, uint32_t nsegs, uint32_t* H,
                 uint32_t* bnd);
+// Rows of one scatter tile for rows of key_bytes (x 2 with a payload) when
+// the launch's largest partition bits are max_bits: segments whose chunks
+// are one tile long get exact offsets for every tile.
+uint32_t scatter_tile_rows(int key_bytes, bool hv, uint32_t max_bits);
 // max_bits: the largest partition bits of the launched chunks' segments.
 int launch_scatter(hipStream_t st, int key_bytes, bool hv, uint32_t max_bits, const Seg* segs, const Chunk* chunks,
                    uint32_t chunk_base, uint32_t nchunks, const uint32_t* H, void* out);

@@ -85,11 +85,12 @@ __global__ __launch_bounds__(512) void k_bhist(const Seg* __restrict__ segs, con
 // large segment, e.g. a single served query of 2^12 partitions x 256 chunks,
 // is 4 MB of histogram):
 //   k_bcol   block (g, s): column totals of bins [64 g, 64 g + 64) of
-//            segment s over its chunks (4 waves, a quarter of the chunks
+//            segment s over its chunks (16 waves, a sixteenth of the chunks
 //            each), into bnd[d]
 //   k_bbins  block s: exclusive scan of the totals in place; bnd[nd] = n
 //   k_bapply block (g, s): h[c][d] = bnd[d] + sum of h[c'][d] for c' < c
 constexpr int kColBins = 64;
+constexpr int kColWaves = 16;
 
 __device__ __forceinline__ uint32_t col_sum(const uint32_t* h, uint32_t nd, uint32_t d, uint32_t c0, uint32_t c1) {
     uint32_t t = 0;
@@ -98,18 +99,23 @@ __device__ __forceinline__ uint32_t col_sum(const uint32_t* h, uint32_t nd, uint
     return t;
 }
 
-__global__ __launch_bounds__(256) void k_bcol(const Seg* __restrict__ segs, uint32_t seg_base,
-                                              const uint32_t* __restrict__ H, uint32_t* __restrict__ bnd) {
-    __shared__ uint32_t part[4][kColBins];
+__global__ __launch_bounds__(64 * kColWaves) void k_bcol(const Seg* __restrict__ segs, uint32_t seg_base,
+                                                          const uint32_t* __restrict__ H, uint32_t* __restrict__ bnd) {
+    __shared__ uint32_t part[kColWaves][kColBins];
     const Seg s = segs[seg_base + blockIdx.y];
     const uint32_t nd = 1u << s.bits, nc = s.nchunks;
     if (blockIdx.x * kColBins >= nd) return;  // whole block: fewer bins than groups
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t d = blockIdx.x * kColBins + lane;
-    const uint32_t c0 = (uint32_t)((uint64_t)nc * w / 4), c1 = (uint32_t)((uint64_t)nc * (w + 1) / 4);
+    const uint32_t c0 = (uint32_t)((uint64_t)nc * w / kColWaves), c1 = (uint32_t)((uint64_t)nc * (w + 1) / kColWaves);
     part[w][lane] = d < nd ? col_sum(H + s.hist, nd, d, c0, c1) : 0u;
     __syncthreads();
-    if (w == 0 && d < nd) bnd[s.bnd + d] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    if (w == 0 && d < nd) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int q = 0; q < kColWaves; ++q) t += part[q][lane];
+        bnd[s.bnd + d] = t;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_bbins(const Seg* __restrict__ segs, uint32_t seg_base,
@@ -137,15 +143,15 @@ __global__ __launch_bounds__(1024) void k_bbins(const Seg* __restrict__ segs, ui
     if (threadIdx.x == 0) b[nd] = (uint32_t)s.n;
 }
 
-__global__ __launch_bounds__(256) void k_bapply(const Seg* __restrict__ segs, uint32_t seg_base,
-                                                uint32_t* __restrict__ H, const uint32_t* __restrict__ bnd) {
-    __shared__ uint32_t part[4][kColBins];
+__global__ __launch_bounds__(64 * kColWaves) void k_bapply(const Seg* __restrict__ segs, uint32_t seg_base,
+                                                            uint32_t* __restrict__ H, const uint32_t* __restrict__ bnd) {
+    __shared__ uint32_t part[kColWaves][kColBins];
     const Seg s = segs[seg_base + blockIdx.y];
     const uint32_t nd = 1u << s.bits, nc = s.nchunks;
     if (blockIdx.x * kColBins >= nd) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t d = blockIdx.x * kColBins + lane;
-    const uint32_t c0 = (uint32_t)((uint64_t)nc * w / 4), c1 = (uint32_t)((uint64_t)nc * (w + 1) / 4);
+    const uint32_t c0 = (uint32_t)((uint64_t)nc * w / kColWaves), c1 = (uint32_t)((uint64_t)nc * (w + 1) / kColWaves);
     uint32_t* h = H + s.hist;
     part[w][lane] = d < nd ? col_sum(h, nd, d, c0, c1) : 0u;
     __syncthreads();
@@ -188,26 +194,32 @@ __device__ __forceinline__ void load_group(const KT* key, const KT* pay, bool ve
 }
 
 // TB bytes of LDS staging per tile, ND partitions at most (the launch's
-// largest 2^bits): fan-outs up to 1024 take 128 KB tiles, twice the rows per
-// partition run of the 64 KB tile that 4096 partitions leave room for, so
-// fewer partial cache lines reach HBM.
-template <typename KT, bool HV, int TB, int ND>
-__global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs, const Chunk* __restrict__ chunks,
-                                                   uint32_t chunk_base, const uint32_t* __restrict__ H,
-                                                   Row<KT, HV>* __restrict__ out) {
+// largest 2^bits), T threads: fan-outs up to 1024 take 128 KB tiles, twice
+// the rows per partition run of the 64 KB tile that 4096 partitions leave
+// room for (64 KB tiles at two workgroups per CU leave runs of 8 rows, half a
+// line: slower).
+template <typename KT, bool HV, int TB, int ND, int T>
+__global__ __launch_bounds__(T, T == 512 ? 4 : 1) void k_bscatter(const Seg* __restrict__ segs, const Chunk* __restrict__ chunks,
+                                                uint32_t chunk_base, const uint32_t* __restrict__ H,
+                                                Row<KT, HV>* __restrict__ out) {
     using E = Row<KT, HV>;
-    constexpr int T = 1024;
     constexpr int G = 16 / sizeof(KT);            // rows per 16-byte key load
     constexpr int TR = TB / sizeof(E);            // rows per tile
     constexpr int NG = TR / (T * G);              // row groups per thread per tile
     static_assert(NG >= 1 && TR % (T * G) == 0, "tile shape");
     static_assert(ND >= T && ND <= kND, "partition table");
     __shared__ E stage[TR];
-    // cur: next output row of each partition (segment-relative); delta: the
-    // output row of stage row j of partition d is delta[d] + j (mod 2^32)
-    __shared__ uint32_t cnt[ND], start[ND], cur[ND], delta[ND];
+    // cnt: the tile's rows per partition, then (after the scan) each
+    // partition's first stage row; cur: next output row of each partition
+    // (segment-relative); delta: the output row of stage row j of partition d
+    // is delta[d] + j (mod 2^32)
+    __shared__ uint32_t cnt[ND], cur[ND], delta[ND];
+    static_assert(ND <= 4096 && TR < (1 << 20), "digit (12 bits) and rank (20 bits) share a register");
     __shared__ uint32_t ws[T / 64 + 1];
-    const uint32_t ci = chunk_base + blockIdx.x;
+    // consecutive chunks of a segment write adjacent runs of every partition:
+    // neighbours run side by side on one XCD (chunks of 32), so the L2 merges
+    // the lines they share
+    const uint32_t ci = chunk_base + xcd_remap_chunked<32>(blockIdx.x, gridDim.x);
     const Chunk c = chunks[ci];
     const Seg s = segs[c.seg];
     const uint64_t r0 = c.row0, r1 = min<uint64_t>(s.n, r0 + s.chunk_rows);
@@ -245,14 +257,17 @@ __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs,
         const uint64_t te = min<uint64_t>(r1, t0 + TR);
         const uint32_t tn = (uint32_t)(te - t0);
         KT k[NG][G], v[NG][G];
-        uint32_t rk[NG][G];
+        uint32_t dr[NG][G];  // partition | rank in the tile << 12
 #pragma unroll
         for (int q = 0; q < NG; ++q) {
             const uint64_t row = t0 + ((uint64_t)q * T + threadIdx.x) * G;
             load_group<KT, HV>(key, pay, vec, row, te, k[q], v[q]);
 #pragma unroll
             for (int u = 0; u < G; ++u)
-                if (row + u < te) rk[q][u] = atomicAdd(&cnt[hmix(k[q][u]) >> sh], 1u);
+                if (row + u < te) {
+                    const uint32_t d = (uint32_t)(hmix(k[q][u]) >> sh);
+                    dr[q][u] = d | (atomicAdd(&cnt[d], 1u) << 12);
+                }
         }
         __syncthreads();
         uint32_t loc[PER], sum = 0;
@@ -264,16 +279,15 @@ __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs,
         }
         uint32_t total;
         uint32_t ex = block_excl_scan<T>(sum, ws, &total);
-        // each thread owns bins [tid * PER, tid * PER + PER): their counters
-        // are consumed here and reset for the next tile
+        // each thread owns bins [tid * PER, tid * PER + PER): their counts
+        // become the partitions' first stage rows
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const uint32_t idx = threadIdx.x * PER + u;
             if (idx < nd) {
-                start[idx] = ex;
+                cnt[idx] = ex;
                 delta[idx] = cur[idx] - ex;
                 cur[idx] += loc[u];
-                cnt[idx] = 0;
             }
             ex += loc[u];
         }
@@ -287,15 +301,20 @@ __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs,
                     E e;
                     e.k = k[q][u];
                     if constexpr (HV) e.v = v[q][u];
-                    stage[start[hmix(e.k) >> sh] + rk[q][u]] = e;
+                    stage[cnt[dr[q][u] & 0xFFF] + (dr[q][u] >> 12)] = e;
                 }
             }
         }
         __syncthreads();
+        // every stage write is done: the owned counters restart at 0 (the
+        // next tile ranks after the barrier at the end of this one)
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const uint32_t idx = threadIdx.x * PER + u;
+            if (idx < nd) cnt[idx] = 0;
+        }
         // runs of each partition at its exact offsets (plain stores: the L2
-        // merges the partial lines at run ends).  The next tile rewrites
-        // stage and delta only after its first barrier, which every thread
-        // reaches after these stores.
+        // merges the partial lines at run ends).
         if (tn == (uint32_t)TR) {
             constexpr int SB = 4;  // rows whose LDS reads are issued together
 #pragma unroll
@@ -315,6 +334,7 @@ __global__ __launch_bounds__(1024) void k_bscatter(const Seg* __restrict__ segs,
                 dst[delta[(uint32_t)(hmix(e.k) >> sh)] + pos] = e;
             }
         }
+        if (te < r1) __syncthreads();  // chunks of several tiles: counters reset, stage read out
     }
 }
 
@@ -644,13 +664,19 @@ int launch_scan(hipStream_t st, uint32_t max_bits, const Seg* segs, uint32_t seg
     if (!nsegs) return PGPU_OK;
     if (max_bits > (uint32_t)kMaxBits) return fail(PGPU_EINVAL, "scan: %u partition bits (at most %d)", max_bits, kMaxBits);
     const uint32_t groups = (uint32_t)ceil_div(1u << max_bits, kColBins);
-    hipLaunchKernelGGL(k_bcol, dim3(groups, nsegs), dim3(256), 0, st, segs, seg_base, H, bnd);
+    hipLaunchKernelGGL(k_bcol, dim3(groups, nsegs), dim3(64 * kColWaves), 0, st, segs, seg_base, H, bnd);
     PGPU_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_bbins, dim3(nsegs), dim3(1024), 0, st, segs, seg_base, bnd);
     PGPU_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bapply, dim3(groups, nsegs), dim3(256), 0, st, segs, seg_base, H, bnd);
+    hipLaunchKernelGGL(k_bapply, dim3(groups, nsegs), dim3(64 * kColWaves), 0, st, segs, seg_base, H, bnd);
     PGPU_LAUNCH_CHECK();
     return PGPU_OK;
+}
+
+uint32_t scatter_tile_rows(int kb, bool hv, uint32_t max_bits) {
+    const uint32_t esz = (uint32_t)kb * (hv ? 2u : 1u);
+    const uint32_t tb = esz >= 8 && max_bits <= 10 ? 2 * kScatterTileBytes : kScatterTileBytes;
+    return tb / esz;
 }
 
 template <typename KT, bool HV>
@@ -659,12 +685,12 @@ void launch_scatter_t(hipStream_t st, uint32_t max_bits, const Seg* segs, const 
     // 4-byte rows already hold 16K rows in the 64 KB tile
     if constexpr (sizeof(Row<KT, HV>) >= 8) {
         if (max_bits <= 10) {
-            hipLaunchKernelGGL((k_bscatter<KT, HV, 2 * kScatterTileBytes, 1024>), dim3(n), dim3(1024), 0, st, segs,
-                               chunks, chunk_base, H, (Row<KT, HV>*)out);
+            hipLaunchKernelGGL((k_bscatter<KT, HV, 2 * kScatterTileBytes, 1024, 1024>), dim3(n), dim3(1024), 0, st,
+                               segs, chunks, chunk_base, H, (Row<KT, HV>*)out);
             return;
         }
     }
-    hipLaunchKernelGGL((k_bscatter<KT, HV, kScatterTileBytes, kND>), dim3(n), dim3(1024), 0, st, segs, chunks,
+    hipLaunchKernelGGL((k_bscatter<KT, HV, kScatterTileBytes, kND, 1024>), dim3(n), dim3(1024), 0, st, segs, chunks,
                        chunk_base, H, (Row<KT, HV>*)out);
 }
 

@@ -123,10 +123,16 @@ int pgpu_batch_plan_create(pgpu_ctx* ctx, const pgpu_batch_cfg* cfg, const pgpu_
     });
     p->bits.resize(n);
     for (uint32_t i = 0; i < n; ++i) p->bits[i] = bits_for(joins[i].n_build, c.part_rows);
-    for (uint32_t i = 0; i < n; ++i) p->lr.add(joins[i].d_rkey, joins[i].d_rpay, joins[i].n_build, p->bits[i], 0);
+    // chunks of one scatter tile: every tile writes its runs at offsets of its own
+    const uint32_t mb = *std::max_element(p->bits.begin(), p->bits.end());
+    const uint32_t crR = scatter_tile_rows((int)c.key_bytes, true, mb);
+    const uint32_t crS = scatter_tile_rows((int)c.key_bytes, c.probe_payload != 0, mb);
+    for (uint32_t i = 0; i < n; ++i)
+        p->lr.add(joins[i].d_rkey, joins[i].d_rpay, joins[i].n_build, p->bits[i], 0, crR);
     const uint32_t nbc = (uint32_t)p->lr.chunks.size();
     for (uint32_t i = 0; i < n; ++i)
-        p->ls.add(joins[i].d_skey, c.probe_payload ? joins[i].d_spay : nullptr, joins[i].n_probe, p->bits[i], nbc);
+        p->ls.add(joins[i].d_skey, c.probe_payload ? joins[i].d_spay : nullptr, joins[i].n_probe, p->bits[i], nbc,
+                  crS);
     // probe segments index the combined segment table after the build ones,
     // and their histograms / bounds follow the build side's
     for (auto& ch : p->ls.chunks) ch.seg += n;
