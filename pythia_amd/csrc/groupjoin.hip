@@ -16,7 +16,9 @@
 //   k_tscan / k_rscatter   p = (key - kmin) >> 15 holds 32 K consecutive keys;
 //                          rows are stored as (16-bit local key, payload) with
 //                          D's payload turned into a dense group id
-//                          gid = grp - gmin
+//                          gid = grp - gmin; the histogram is per chunk of
+//                          16 tiles, with each tile's offsets in its chunk,
+//                          and the scatter runs one workgroup per tile
 //   k_items                per partition, work items of <= 64 K rows
 //   k_phist                per item: the partition's 32 K-entry slice
 //                          local key -> gid is built in LDS from D's rows,
@@ -257,12 +259,14 @@ constexpr uint32_t kP1Max = 1024;                              // key-range part
 constexpr uint32_t kP2Max = 512;                               // group-range partitions
 constexpr uint64_t kMaxKeyRange = (uint64_t)kP1Max << kSK;     // 2^25
 constexpr uint64_t kMaxGroupRange = (uint64_t)kP2Max << kSG;   // 2^22
-constexpr uint32_t kCR = 65536;                                // rows per histogram / scatter chunk
+constexpr uint32_t kCR = 65536;                                // rows per histogram chunk (16 scatter tiles)
 constexpr uint32_t kIR1 = 196608;                              // rows per probe item
 constexpr uint32_t kIR2 = 98304;                               // rows per aggregation item
 constexpr int kT = 1024;                                       // threads per workgroup (scatter, probe, agg)
 constexpr int kMinmaxBlocks = 256;
 constexpr int kRangeTile = 4096;                               // rows per LDS staging tile: key-range scatter
+constexpr uint32_t kTilesPerChunk = kCR / kRangeTile;
+constexpr int kRT = 512;                                       // key-range scatter threads (two workgroups per CU)
 constexpr int kGroupTile = 12288;                              // rows per LDS staging tile: group-range scatter
 
 struct Item {
@@ -329,35 +333,61 @@ __global__ __launch_bounds__(kT) void k_minmax2(const int32_t* __restrict__ part
 }
 
 // Key-range histogram of one chunk of kCR rows: H[chunk][p] counts the rows
-// whose key lies in [kmin, kmin + range), p = (key - kmin) >> kSK.
+// whose key lies in [kmin, kmin + range), p = (key - kmin) >> kSK, and
+// toff[tile][p] the chunk's rows of partition p in its tiles (kRangeTile
+// rows each) before `tile`.  Each wavefront counts whole tiles (tiles w and
+// w + 8 of the chunk) in its own LDS row: no barrier during the sweep.
 __global__ __launch_bounds__(512) void k_rhist(const int32_t* __restrict__ key, uint64_t n, int64_t kmin,
-                                               uint64_t range, uint32_t Ppad, uint32_t* __restrict__ H) {
-    __shared__ uint32_t h[kP1Max];
-    for (uint32_t i = threadIdx.x; i < Ppad; i += 512) h[i] = 0;
-    __syncthreads();
+                                               uint64_t range, uint32_t Ppad, uint32_t* __restrict__ H,
+                                               uint16_t* __restrict__ toff) {
+    static_assert(kTilesPerChunk == 16 && kCR < 65536 + 1, "two tiles per wavefront; in-chunk offsets fit 16 bits");
+    __shared__ uint32_t lt[8][kP1Max];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t i = lane; i < Ppad; i += 64) lt[w][i] = 0;
     const uint64_t r0 = (uint64_t)blockIdx.x * kCR, r1 = min<uint64_t>(n, r0 + kCR);
+    const bool vec = ((uintptr_t)key & 15) == 0;  // tile starts are multiples of 4 rows
     auto add = [&](int32_t k) {
         const uint64_t off = (uint64_t)((int64_t)k - kmin);
-        if (off < range) atomicAdd(&h[off >> kSK], 1u);
+        if (off < range) atomicAdd(&lt[w][off >> kSK], 1u);
     };
-    uint64_t r = r0;
-    if (((uintptr_t)key & 15) == 0) {  // r0 is a multiple of 4 rows
-        const uint64_t ng = (r1 - r0) / 4;
-        const int4* kv = (const int4*)(key + r0);
+    for (uint32_t t = w; t < kTilesPerChunk; t += 8) {
+        const uint64_t t0 = r0 + (uint64_t)t * kRangeTile;
+        if (t0 >= r1) break;
+        const uint64_t te = min<uint64_t>(r1, t0 + kRangeTile);
+        uint64_t r = t0;
+        if (vec) {
+            const uint64_t ng = (te - t0) / 4;
+            const int4* kv = (const int4*)(key + t0);
 #pragma unroll 4
-        for (uint64_t g = threadIdx.x; g < ng; g += 512) {
-            const int4 q = nt_load(&kv[g]);
-            add(q.x);
-            add(q.y);
-            add(q.z);
-            add(q.w);
+            for (uint64_t g = lane; g < ng; g += 64) {
+                const int4 q = nt_load(&kv[g]);
+                add(q.x);
+                add(q.y);
+                add(q.z);
+                add(q.w);
+            }
+            r = t0 + ng * 4;
         }
-        r = r0 + ng * 4;
+        for (r += lane; r < te; r += 64) add(key[r]);
+        // one wavefront's LDS operations complete in order: its counts are final
+        uint16_t* tc = toff + (uint64_t)(r0 / kRangeTile + t) * Ppad;
+        for (uint32_t i = lane; i < Ppad; i += 64) {
+            tc[i] = (uint16_t)lt[w][i];
+            lt[w][i] = 0;
+        }
     }
-    for (r += threadIdx.x; r < r1; r += 512) add(key[r]);
-    __syncthreads();
-    uint32_t* out = H + (uint64_t)blockIdx.x * Ppad;
-    for (uint32_t i = threadIdx.x; i < Ppad; i += 512) out[i] = h[i];
+    __syncthreads();  // every tile's counts are in toff
+    const uint64_t t1 = (r1 - r0 + kRangeTile - 1) / kRangeTile;
+    uint16_t* tc = toff + (uint64_t)(r0 / kRangeTile) * Ppad;
+    for (uint32_t i = threadIdx.x; i < Ppad; i += 512) {
+        uint32_t run = 0;
+        for (uint64_t t = 0; t < t1; ++t) {
+            const uint32_t c = tc[t * Ppad + i];
+            tc[t * Ppad + i] = (uint16_t)run;
+            run += c;
+        }
+        H[(uint64_t)blockIdx.x * Ppad + i] = run;
+    }
 }
 
 // Exclusive scan of every column of H[rows][Ppad] in place; tot[d] receives
@@ -486,102 +516,85 @@ __device__ __forceinline__ void load4(const V* __restrict__ p, uint64_t row, uin
     for (int u = 0; u < 4; ++u) out[u] = row + u < end ? p[row + u] : V();
 }
 
-// Scatter of one chunk of kCR rows into key-range partitions: rows
-// (local key, conv(payload)) at exact offsets base[p] + H[chunk][p] (H
-// column-scanned).  Each LDS tile of 4096 rows is reordered by partition and
-// written as contiguous runs.  Rows whose key is out of range are dropped.
+// Scatter of one tile of kRangeTile rows into key-range partitions: rows
+// (local key, conv(payload)) at exact offsets base[p] + H[chunk][p] +
+// toff[tile][p] (H column-scanned), reordered by partition in LDS and written
+// as contiguous runs.  Rows whose key is out of range are dropped.  Tiles t
+// and t + 1 write adjacent runs of every partition: neighbours run side by
+// side on one XCD (chunks of 32 tiles), so the L2 merges the lines they share.
 template <typename VIn, typename VOut, int TR>
-__global__ __launch_bounds__(kT) void k_rscatter(const int32_t* __restrict__ key, const VIn* __restrict__ pay,
-                                                 uint64_t n, int64_t kmin, uint64_t range, int64_t gmin,
-                                                 uint32_t Ppad, const uint32_t* __restrict__ H,
-                                                 const uint32_t* __restrict__ base, uint16_t* __restrict__ olk,
-                                                 VOut* __restrict__ oval) {
-    constexpr int PR = TR / kT, PS = kP1Max / kT;
+__global__ __launch_bounds__(kRT, 4) void k_rscatter(const int32_t* __restrict__ key, const VIn* __restrict__ pay,
+                                                      uint64_t n, int64_t kmin, uint64_t range, int64_t gmin,
+                                                      uint32_t Ppad, const uint32_t* __restrict__ H,
+                                                      const uint16_t* __restrict__ toff,
+                                                      const uint32_t* __restrict__ base, uint16_t* __restrict__ olk,
+                                                      VOut* __restrict__ oval) {
+    constexpr int PR = TR / kRT, PS = kP1Max / kRT;
+    static_assert(TR == kRangeTile && PR % 4 == 0, "tile shape");
     __shared__ uint16_t slk[TR], spart[TR];
     __shared__ VOut sval[TR];
     __shared__ uint32_t cnt[kP1Max], start[kP1Max], cur[kP1Max];
-    __shared__ uint32_t ws[kT / 64 + 1];
-    const uint64_t r0 = (uint64_t)blockIdx.x * kCR, r1 = min<uint64_t>(n, r0 + kCR);
-    const uint32_t* h = H + (uint64_t)blockIdx.x * Ppad;
-    for (uint32_t d = threadIdx.x; d < Ppad; d += kT) {
-        cur[d] = base[d] + h[d];
+    __shared__ uint32_t ws[kRT / 64 + 1];
+    const uint32_t t = pgpu::dev::xcd_remap_chunked<32>(blockIdx.x, gridDim.x);
+    const uint64_t t0 = (uint64_t)t * TR, te = min<uint64_t>(n, t0 + TR);
+    const bool vec = ((uintptr_t)key & 15) == 0 && ((uintptr_t)pay & 15) == 0;
+    int32_t k[PR];
+    VIn v[PR];
+    const uint64_t row = t0 + (uint64_t)threadIdx.x * PR;
+#pragma unroll
+    for (int j = 0; j < PR; j += 4) {
+        load4(key, row + j, te, vec, k + j);
+        load4(pay, row + j, te, vec, v + j);
+    }
+    const uint32_t* h = H + (uint64_t)(t / kTilesPerChunk) * Ppad;
+    const uint16_t* to = toff + (uint64_t)t * Ppad;
+    for (uint32_t d = threadIdx.x; d < Ppad; d += kRT) {
+        cur[d] = base[d] + h[d] + to[d];
         cnt[d] = 0;
     }
     __syncthreads();
-    const bool vec = ((uintptr_t)key & 15) == 0 && ((uintptr_t)pay & 15) == 0;
-    // a tile's rows are loaded while the previous tile is ranked and written
-    auto fetch = [&](uint64_t t0, int32_t* k, VIn* v) {
-        const uint64_t te = min<uint64_t>(r1, t0 + TR);
-        const uint64_t row = t0 + (uint64_t)threadIdx.x * PR;
+    uint32_t part[PR], rk[PR];
+    bool ok[PR];
 #pragma unroll
-        for (int j = 0; j < PR; j += 4) {
-            load4(key, row + j, te, vec, k + j);
-            load4(pay, row + j, te, vec, v + j);
-        }
-    };
-    int32_t k[PR];
-    VIn v[PR];
-    fetch(r0, k, v);
-    for (uint64_t t0 = r0; t0 < r1; t0 += TR) {
-        const uint64_t te = min<uint64_t>(r1, t0 + TR);
-        const uint64_t row = t0 + (uint64_t)threadIdx.x * PR;
-        uint32_t part[PR], rk[PR];
-        bool ok[PR];
+    for (int u = 0; u < PR; ++u) {
+        const uint64_t off = (uint64_t)((int64_t)k[u] - kmin);
+        ok[u] = row + u < te && off < range;
+        part[u] = (uint32_t)(off >> kSK);
+        k[u] = (int32_t)(off & ((1u << kSK) - 1));
+        if (ok[u]) rk[u] = atomicAdd(&cnt[part[u]], 1u);
+    }
+    __syncthreads();
+    uint32_t loc[PS], sum = 0;
 #pragma unroll
-        for (int u = 0; u < PR; ++u) {
-            const uint64_t off = (uint64_t)((int64_t)k[u] - kmin);
-            ok[u] = row + u < te && off < range;
-            part[u] = (uint32_t)(off >> kSK);
-            k[u] = (int32_t)(off & ((1u << kSK) - 1));
-            if (ok[u]) rk[u] = atomicAdd(&cnt[part[u]], 1u);
-        }
-        int32_t k2[PR];
-        VIn v2[PR];
-        if (t0 + TR < r1) fetch(t0 + TR, k2, v2);
-        __syncthreads();
-        uint32_t loc[PS], sum = 0;
+    for (int u = 0; u < PS; ++u) {
+        const uint32_t idx = threadIdx.x * PS + u;
+        loc[u] = idx < Ppad ? cnt[idx] : 0u;
+        sum += loc[u];
+    }
+    uint32_t total;
+    uint32_t ex = block_excl_scan<kRT>(sum, ws, &total);
 #pragma unroll
-        for (int u = 0; u < PS; ++u) {
-            const uint32_t idx = threadIdx.x * PS + u;
-            loc[u] = idx < Ppad ? cnt[idx] : 0u;
-            sum += loc[u];
-        }
-        uint32_t total;
-        uint32_t ex = block_excl_scan<kT>(sum, ws, &total);
+    for (int u = 0; u < PS; ++u) {
+        const uint32_t idx = threadIdx.x * PS + u;
+        if (idx < Ppad) start[idx] = ex;
+        ex += loc[u];
+    }
+    __syncthreads();
 #pragma unroll
-        for (int u = 0; u < PS; ++u) {
-            const uint32_t idx = threadIdx.x * PS + u;
-            if (idx < Ppad) start[idx] = ex;
-            ex += loc[u];
+    for (int u = 0; u < PR; ++u) {
+        if (ok[u]) {
+            const uint32_t pos = start[part[u]] + rk[u];
+            slk[pos] = (uint16_t)k[u];
+            spart[pos] = (uint16_t)part[u];
+            sval[pos] = conv<VIn, VOut>(v[u], gmin);
         }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < PR; ++u) {
-            if (ok[u]) {
-                const uint32_t pos = start[part[u]] + rk[u];
-                slk[pos] = (uint16_t)k[u];
-                spart[pos] = (uint16_t)part[u];
-                sval[pos] = conv<VIn, VOut>(v[u], gmin);
-            }
-        }
-        __syncthreads();
-        for (uint32_t pos = threadIdx.x; pos < total; pos += kT) {
-            const uint32_t p = spart[pos];
-            const uint32_t idx = cur[p] + (pos - start[p]);
-            olk[idx] = slk[pos];
-            oval[idx] = sval[pos];
-        }
-        __syncthreads();
-        for (uint32_t d = threadIdx.x; d < Ppad; d += kT) {
-            cur[d] += cnt[d];
-            cnt[d] = 0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < PR; ++u) {
-            k[u] = k2[u];
-            v[u] = v2[u];
-        }
+    }
+    __syncthreads();
+    for (uint32_t pos = threadIdx.x; pos < total; pos += kRT) {
+        const uint32_t p = spart[pos];
+        const uint32_t idx = cur[p] + (pos - start[p]);
+        olk[idx] = slk[pos];
+        oval[idx] = sval[pos];
     }
 }
 
@@ -890,6 +903,7 @@ struct pgpu_groupjoin {
     int32_t* mmpart = nullptr;
     int32_t* h_mm = nullptr;
     uint32_t* H = nullptr;
+    uint16_t* toff = nullptr;  // per scatter tile and key-range partition: rows in the chunk's earlier tiles
     uint32_t *tot = nullptr, *baseD = nullptr, *baseF = nullptr, *base2 = nullptr;
     uint32_t *first1 = nullptr, *first2 = nullptr;
     part::Item *items1 = nullptr, *items2 = nullptr;
@@ -930,6 +944,7 @@ int alloc_partitioned(pgpu_groupjoin* p) {
     PGPU_TRY(gj_alloc(p, kMmWords * sizeof(int32_t), (void**)&p->mm));
     PGPU_TRY(gj_alloc(p, kMinmaxBlocks * 4 * sizeof(int32_t), (void**)&p->mmpart));
     PGPU_TRY(gj_alloc(p, hwords * 4, (void**)&p->H));
+    PGPU_TRY(gj_alloc(p, (std::max(chunksD, chunksF) + 1) * kTilesPerChunk * kP1Max * 2, (void**)&p->toff));
     PGPU_TRY(gj_alloc(p, kP1Max * 4, (void**)&p->tot));
     PGPU_TRY(gj_alloc(p, (kP1Max + 1) * 4, (void**)&p->baseD));
     PGPU_TRY(gj_alloc(p, (kP1Max + 1) * 4, (void**)&p->baseF));
@@ -965,7 +980,7 @@ int partition_side(pgpu_groupjoin* p, const char* name, const int32_t* key, cons
     {
         snprintf(nm, sizeof nm, "%s_hist", name);
         Timed t(p->timer, nm);
-        hipLaunchKernelGGL(k_rhist, dim3(chunks), dim3(512), 0, st, key, n, kmin, range, Ppad, p->H);
+        hipLaunchKernelGGL(k_rhist, dim3(chunks), dim3(512), 0, st, key, n, kmin, range, Ppad, p->H, p->toff);
         PGPU_LAUNCH_CHECK();
     }
     {
@@ -978,9 +993,12 @@ int partition_side(pgpu_groupjoin* p, const char* name, const int32_t* key, cons
     {
         snprintf(nm, sizeof nm, "%s_scatter", name);
         Timed t(p->timer, nm);
-        hipLaunchKernelGGL((k_rscatter<VIn, VOut, TR>), dim3(chunks), dim3(kT), 0, st, key, pay, n, kmin, range, gmin,
-                           Ppad, p->H, base, olk, oval);
-        PGPU_LAUNCH_CHECK();
+        const uint32_t tiles = (uint32_t)ceil_div(n, (uint64_t)TR);
+        if (tiles) {
+            hipLaunchKernelGGL((k_rscatter<VIn, VOut, TR>), dim3(tiles), dim3(kRT), 0, st, key, pay, n, kmin, range,
+                               gmin, Ppad, p->H, p->toff, base, olk, oval);
+            PGPU_LAUNCH_CHECK();
+        }
     }
     return PGPU_OK;
 }
