@@ -419,17 +419,25 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
             const uint64_t r0 = b0 + tid;
             return r0 < se ? (uint32_t)min<uint64_t>(U, (se - r0 + T - 1) / T) : 0u;
         };
+        // A batch's probe rows through a buffer resource based at the batch:
+        // 32-bit offsets (fewer registers in flight) and rows past the item's
+        // end read as zero (masked by the probe).
+        auto batch_rsrc = [&](uint64_t b0) {
+            const uint64_t left = b0 < se ? (se - b0) * sizeof(SR) : 0;
+            return buf_rsrc(S + b0, (uint32_t)min<uint64_t>(left, 0xFFFFFFF0ull));
+        };
         auto fetch = [&](uint64_t b0) {
             const uint32_t cnt = rows_in(b0);
+            const __amdgpu_buffer_rsrc_t rs = batch_rsrc(b0);
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if ((uint32_t)u < cnt) x[u] = nt_load(&S[b0 + tid + (uint64_t)u * T]);
+                if ((uint32_t)u < cnt) x[u] = buf_load<SR, true>(rs, (uint32_t)(tid + u * T) * (uint32_t)sizeof(SR));
         };
-        // Unpredicated fetch of the batch at b0 into z: rows past the item's
-        // end read its last row (se > sb) and are masked by the probe.
+        // Unpredicated fetch of the batch at b0 into z.
         auto fetch_clamped = [&](SR(&z)[U], uint64_t b0) {
+            const __amdgpu_buffer_rsrc_t rs = batch_rsrc(b0);
 #pragma unroll
-            for (int u = 0; u < U; ++u) z[u] = nt_load(&S[min(b0 + tid + (uint64_t)u * T, se - 1)]);
+            for (int u = 0; u < U; ++u) z[u] = buf_load<SR, true>(rs, (uint32_t)(tid + u * T) * (uint32_t)sizeof(SR));
         };
         Row<KT, true> br[RB];
         auto fetch_build = [&](uint32_t p0) {
@@ -492,6 +500,7 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
             __syncthreads();
             if (p0) fetch(sb);  // later pieces re-read the item's probe rows
             const unsigned long long spc = sp_cnt, sps = sp_sum;
+            unsigned long long pe_n = 0, pe_v = 0;  // this piece's probe rows keyed by the empty marker (and payloads)
             // One batch: all U slot reads in flight at once, collision chains
             // walked only by the lanes that need them, selects instead of
             // branches; FULL batches (every thread holds U rows) skip the
@@ -519,33 +528,37 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
                         }
                     }
                 }
-                uint32_t bc = 0;  // fits 32 bits: U rows x PIECE matches
-                unsigned long long bs = 0, bp = 0;
+                // Each walk ended on the key's slot or on an empty one, whose
+                // count and sum are 0: no hit / miss select.  A probe key
+                // equal to the empty marker also ends on an empty slot; those
+                // rows are tallied and take the side accumulator per piece.
+                uint32_t bc = 0, ne = 0;  // fits 32 bits: U rows x PIECE matches
+                unsigned long long bs = 0, bp = 0, ve = 0;
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const bool hit = t[u].key == x[u].k;
-                    uint32_t raw;
+                    const bool live = FULL || (uint32_t)u < cnt;
+                    uint32_t c;
                     if constexpr (W)
-                        raw = tc16[slot[u]];
+                        c = tc16[slot[u]];
                     else
-                        raw = t[u].cnt;
-                    uint32_t c = hit ? raw : 0u;
-                    unsigned long long sbv = hit ? t[u].sum : 0ull;
-                    if (x[u].k == EMPTY) {  // the empty marker as a real key: side accumulator
-                        c = (uint32_t)spc;
-                        sbv = sps;
-                    }
-                    if (!FULL && (uint32_t)u >= cnt) {
+                        c = t[u].cnt;
+                    unsigned long long sbv = t[u].sum;
+                    if (!live) {
                         c = 0;
                         sbv = 0;
                     }
                     bc += c;
                     bs += sbv;
                     if constexpr (HVS) bp += (unsigned long long)c * (unsigned long long)(int64_t)x[u].v;
+                    const bool em = live && x[u].k == EMPTY;
+                    ne += em ? 1u : 0u;
+                    if constexpr (HVS) ve += em ? (unsigned long long)(int64_t)x[u].v : 0ull;
                 }
                 acc_c += bc;
                 acc_b += bs;
                 acc_p += bp;
+                pe_n += ne;
+                pe_v += ve;
             };
             auto probe = [&](const SR(&z)[U], uint64_t b0) {
                 if (b0 + (uint64_t)U * T <= se)
@@ -562,6 +575,9 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
                 fetch_clamped(x, b0 + 2ull * U * T);
                 if (b0 + (uint64_t)U * T < se) probe(y, b0 + (uint64_t)U * T);
             }
+            acc_c += pe_n * spc;
+            acc_b += pe_n * sps;
+            acc_p += pe_v * spc;
             __syncthreads();
         }
         acc_c = wave_sum(acc_c);
