@@ -225,6 +225,53 @@ def test_c2_full_shape(ctx, zipf):
         b.free()
 
 
+@pytest.mark.parametrize("nS,bits,zipf", [((1 << 23) + 12345, 14, 0.0), ((5 << 20) + 1, 6, 1.0), (3 << 22, 9, 0.5)])
+def test_pass1_tiles_ragged(ctx, nS, bits, zipf):
+    """Pass 1 runs one workgroup per tile at offsets of its own (block prefix
+    + the tile's offset in its block): probe sides whose blocks hold several
+    tiles, a partial last tile and a partial last block, uniform and skewed
+    keys.  Count and sum bit-exact against the closed form."""
+    nR = 1 << 16
+    rk, rp, sk = _c2_device(ctx, nR, nS, zipf)
+    plan = pa.JoinPlan(ctx, 8, nR, nS, radix_bits=bits, algo=pa.ALGO_RADIX)
+    r = plan.count_sum(rk, rp, nR, sk, None, nS)
+    ref = O.c2_closed(nR, nS, zipf, threads=THREADS)
+    assert (r.matches, r.sum_build) == (ref.count, ref.sum_build)
+    plan.close()
+    for b in (rk, rp, sk):
+        b.free()
+
+
+def test_pass1_tiles_with_payloads(ctx):
+    """Per-tile pass 1 for 8-byte (int32 key + payload) and 16-byte (int64
+    key + payload) probe rows: 4096- and 8192-row tiles, several per block."""
+    nR, nS = 1 << 20, (1 << 24) + 777  # the C1 shape, a ragged probe side
+    bufs = [ctx.alloc(nR * 4), ctx.alloc(nR * 4), ctx.alloc(nS * 4), ctx.alloc(nS * 4)]
+    for b, col, n in zip(bufs, ["C1_RKEY", "C1_RPAY", "C1_SFK", "C1_SPAY"], [nR, nR, nS, nS]):
+        ctx.gen(col, n, b, n_build=nR)
+    plan = pa.JoinPlan(ctx, 4, nR, nS, probe_payload=True, radix_bits=8, algo=pa.ALGO_RADIX)
+    r = plan.count_sum(bufs[0], bufs[1], nR, bufs[2], bufs[3], nS)
+    ref = O.c1_closed(nR, nS, threads=THREADS)
+    assert (r.matches, r.sum_build, r.sum_probe) == (ref.count, ref.sum_build, ref.sum_probe)
+    plan.close()
+    for b in bufs:
+        b.free()
+    # int64 keys with a probe payload: every probe key is a build key (the
+    # build keys are a permutation of 0..nR-1), so sum_probe is the payload sum
+    nR, nS = 1 << 16, (1 << 23) + 77
+    rk, rp, sk = _c2_device(ctx, nR, nS)
+    pv = np.random.default_rng(5).integers(-(1 << 40), 1 << 40, nS, dtype=np.int64)
+    sp = _dev(ctx, pv)
+    plan = pa.JoinPlan(ctx, 8, nR, nS, probe_payload=True, radix_bits=5, algo=pa.ALGO_RADIX)
+    r = plan.count_sum(rk, rp, nR, sk, sp, nS)
+    ref = O.c2_closed(nR, nS, 0.0, threads=THREADS)
+    assert (r.matches, r.sum_build) == (ref.count, ref.sum_build) and r.matches == nS
+    assert r.sum_probe == int(pv.astype(object).sum()) % 2**64
+    plan.close()
+    for b in (rk, rp, sk, sp):
+        b.free()
+
+
 def _c3_device(ctx, nD, nF, norders):
     dk, dc, fk, fp = ctx.alloc(nD * 4), ctx.alloc(nD * 4), ctx.alloc(nF * 4), ctx.alloc(nF * 8)
     ctx.gen("C3_D_ORDERKEY", nD, dk, n_build=nD)
