@@ -25,8 +25,10 @@
 //                          every fact row looks its group up there; writes
 //                          the row's gid, counts rows per group range
 //                          (gid >> 13) and flags duplicate dimension keys
-//   k_pscatter             per item: (gid & 8191, value) rows into group-range
-//                          partitions
+//   k_pscatter             one workgroup per 6144-row tile of a probe item:
+//                          (gid & 8191, value) rows into group-range
+//                          partitions at offsets of its own (k_phist counts
+//                          each tile's group ranges per wavefront)
 //   k_gagg                 per item of a group range: LDS FP64 / count tables
 //                          of 8192 groups, one partial table per item
 //   k_gcompact             partial tables summed per group, non-empty groups
@@ -267,15 +269,20 @@ constexpr int kMinmaxBlocks = 256;
 constexpr int kRangeTile = 4096;                               // rows per LDS staging tile: key-range scatter
 constexpr uint32_t kTilesPerChunk = kCR / kRangeTile;
 constexpr int kRT = 512;                                       // key-range scatter threads (two workgroups per CU)
-constexpr int kGroupTile = 12288;                              // rows per LDS staging tile: group-range scatter
+constexpr int kGroupTile = 6144;                               // rows per LDS staging tile: group-range scatter
+constexpr uint32_t kTilesPerItem1 = kIR1 / kGroupTile;         // group-scatter tiles per probe item
+constexpr int kPST = 512;                                      // group-range scatter threads (two workgroups per CU)
+static_assert(kIR1 % kGroupTile == 0 && kTilesPerItem1 <= 2 * (kT / 64) && kTilesPerItem1 <= 256,
+              "two tiles per probe-pass wavefront; tile numbers fit 8 bits");
 
 struct Item {
     uint32_t part, r0, r1, pad;
 };
 
 // mm words: [0] min key, [1] max key, [2] min group, [3] max group,
-// [4] duplicate dimension key seen, [5] / [6] item counts (probe, aggregation)
-enum { kMmKMin = 0, kMmKMax, kMmGMin, kMmGMax, kMmDup, kMmItems1, kMmItems2, kMmWords = 8 };
+// [4] duplicate dimension key seen, [5] / [6] item counts (probe, aggregation),
+// [7] group-scatter tile count
+enum { kMmKMin = 0, kMmKMax, kMmGMin, kMmGMax, kMmDup, kMmItems1, kMmItems2, kMmTiles1, kMmWords = 8 };
 
 __global__ void k_mm_init(int32_t* mm) {
     if (threadIdx.x < kMmWords)
@@ -449,21 +456,30 @@ __global__ __launch_bounds__(1024) void k_tscan(const uint32_t* __restrict__ tot
 
 // Work items of at most `ir` rows over the partitions' row ranges
 // [base[p], base[p+1]), p < P (P <= 2048).  first[p] = partition p's first
-// item, first[P] = *nitems = the item count.  One block.
+// item, first[P] = *nitems = the item count.  With tile_rows (ir a multiple
+// of it), the items are also cut into tiles of tile_rows rows: tile0[i] =
+// item i's first tile, titem[t] = (item << 8) | tile-in-item, *ntiles = the
+// tile count.  One block.
 __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ base, uint32_t P, uint32_t ir,
                                                 Item* __restrict__ items, uint32_t* __restrict__ first,
-                                                int32_t* __restrict__ nitems) {
+                                                int32_t* __restrict__ nitems, uint32_t tile_rows,
+                                                uint32_t* __restrict__ tile0, uint32_t* __restrict__ titem,
+                                                int32_t* __restrict__ ntiles) {
     __shared__ uint32_t ws[1024 / 64 + 1];
     constexpr int PER = kP1Max / 1024;
-    uint32_t cnt[PER], sum = 0;
+    uint32_t cnt[PER], tcnt[PER], sum = 0, tsum = 0;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const uint32_t p = threadIdx.x * PER + u;
-        cnt[u] = p < P ? (base[p + 1] - base[p] + ir - 1) / ir : 0u;
+        const uint32_t rows = p < P ? base[p + 1] - base[p] : 0u;
+        cnt[u] = (rows + ir - 1) / ir;
+        tcnt[u] = tile_rows ? rows / ir * (ir / tile_rows) + (rows % ir + tile_rows - 1) / tile_rows : 0u;
         sum += cnt[u];
+        tsum += tcnt[u];
     }
-    uint32_t total;
+    uint32_t total, ttotal;
     uint32_t ex = block_excl_scan<1024>(sum, ws, &total);
+    uint32_t tex = block_excl_scan<1024>(tsum, ws, &ttotal);
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const uint32_t p = threadIdx.x * PER + u;
@@ -477,13 +493,21 @@ __global__ __launch_bounds__(1024) void k_items(const uint32_t* __restrict__ bas
                 it.r1 = min(b1, it.r0 + ir);
                 it.pad = 0;
                 items[ex + j] = it;
+                if (tile_rows) {
+                    const uint32_t tb = tex + j * (ir / tile_rows);
+                    tile0[ex + j] = tb;
+                    const uint32_t nt = (it.r1 - it.r0 + tile_rows - 1) / tile_rows;
+                    for (uint32_t k = 0; k < nt; ++k) titem[tb + k] = ((ex + j) << 8) | k;
+                }
             }
         }
         ex += cnt[u];
+        tex += tcnt[u];
     }
     if (threadIdx.x == 0) {
         first[P] = total;
         *nitems = (int32_t)total;
+        if (tile_rows) *ntiles = (int32_t)ttotal;
     }
 }
 
@@ -614,121 +638,135 @@ __device__ __forceinline__ void build_slice(uint32_t* tab, const uint16_t* __res
 }
 
 // Pass 1 over one probe item: every row's gid (kNone: no match) into fgid,
-// and H2[item][q] = rows of the item whose group lies in group-range
-// partition q = gid >> kSG.
+// H2[item][q] = rows of the item whose group lies in group-range partition
+// q = gid >> kSG, and toff2[t][q] = the item's rows of range q in its tiles
+// (kGroupTile rows each) before tile t.  Each wavefront looks up whole tiles
+// (tiles w and w + 16 of the item) and counts them in its own LDS row (two
+// 16-bit counters per word: a tile holds at most 6144 rows).
 __global__ __launch_bounds__(kT) void k_phist(const Item* __restrict__ items, const int32_t* __restrict__ mm,
                                               const uint32_t* __restrict__ bD, const uint16_t* __restrict__ dlk,
                                               const uint32_t* __restrict__ dgid, const uint16_t* __restrict__ flk,
                                               uint32_t P2pad, uint32_t* __restrict__ H2, uint32_t* __restrict__ fgid,
-                                              int32_t* __restrict__ dup) {
+                                              int32_t* __restrict__ dup, const uint32_t* __restrict__ tile0,
+                                              uint32_t* __restrict__ toff2) {
     __shared__ uint32_t tab[1u << kSK];
-    __shared__ uint32_t h[kP2Max];
+    __shared__ uint32_t lt[kT / 64][kP2Max / 2];
     uint32_t* out = H2 + (uint64_t)blockIdx.x * P2pad;
     if ((int32_t)blockIdx.x >= mm[kMmItems1]) {  // past the item count: a zero row for the column scan
         for (uint32_t i = threadIdx.x; i < P2pad; i += kT) out[i] = 0;
         return;
     }
     const Item it = items[blockIdx.x];
-    for (uint32_t i = threadIdx.x; i < P2pad; i += kT) h[i] = 0;
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t i = lane; i < kP2Max / 2; i += 64) lt[w][i] = 0;
     build_slice(tab, dlk, dgid, bD[it.part], bD[it.part + 1], dup);
-    constexpr int U = 16;  // rows per thread per step, all loads issued before the lookups
-    for (uint32_t t0 = it.r0; t0 < it.r1; t0 += U * kT) {
-        uint32_t lk[U];
+    const uint32_t nt = (it.r1 - it.r0 + kGroupTile - 1) / kGroupTile;
+    const uint32_t tb = tile0[blockIdx.x];
+    constexpr int U = 16;  // rows per lane per step, all loads issued before the lookups
+    for (uint32_t k = w; k < nt; k += kT / 64) {
+        const uint32_t rs = it.r0 + k * kGroupTile, re = min(it.r1, rs + kGroupTile);
+        for (uint32_t t0 = rs; t0 < re; t0 += U * 64) {
+            uint32_t lk[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t r = t0 + u * kT + threadIdx.x;
-            lk[u] = r < it.r1 ? flk[r] : kNone;
-        }
+            for (int u = 0; u < U; ++u) {
+                const uint32_t r = t0 + u * 64 + lane;
+                lk[u] = r < re ? flk[r] : kNone;
+            }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (lk[u] != kNone) {
-                const uint32_t g = tab[lk[u]];
-                if (g != kNone) atomicAdd(&h[g >> kSG], 1u);
-                fgid[t0 + u * kT + threadIdx.x] = g;
+            for (int u = 0; u < U; ++u) {
+                if (lk[u] != kNone) {
+                    const uint32_t g = tab[lk[u]];
+                    if (g != kNone) {
+                        const uint32_t q = g >> kSG;
+                        atomicAdd(&lt[w][q >> 1], 1u << (16 * (q & 1)));
+                    }
+                    fgid[t0 + u * 64 + lane] = g;
+                }
             }
         }
+        // one wavefront's LDS operations complete in order: its counts are final
+        uint32_t* tc = toff2 + (uint64_t)(tb + k) * P2pad;
+        for (uint32_t q = lane; q < P2pad; q += 64) tc[q] = (lt[w][q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
+        for (uint32_t i = lane; i < kP2Max / 2; i += 64) lt[w][i] = 0;
     }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < P2pad; i += kT) out[i] = h[i];
+    __syncthreads();  // every tile's counts are in toff2
+    for (uint32_t q = threadIdx.x; q < P2pad; q += kT) {
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < nt; ++k) {
+            uint32_t* pk = toff2 + (uint64_t)(tb + k) * P2pad + q;
+            const uint32_t c = *pk;
+            *pk = run;
+            run += c;
+        }
+        out[q] = run;
+    }
 }
 
-// Pass 2 over one probe item: (gid & 8191, value) rows into group-range
-// partitions at base2[q] + H2[item][q] (H2 column-scanned), tile by tile.
-// A tile's rows are loaded before they are ranked, and the next tile's rows
-// are fetched while this one is ranked, staged and written.
+// Pass 2 over one tile of a probe item: (gid & 8191, value) rows into
+// group-range partitions at base2[q] + H2[item][q] + toff2[tile][q] (H2
+// column-scanned).  Tiles t and t + 1 write adjacent runs of every range:
+// neighbours run side by side on one XCD (chunks of 32 tiles).
 template <int TR>
-__global__ __launch_bounds__(kT) void k_pscatter(const Item* __restrict__ items, const int32_t* __restrict__ mm,
-                                                 const uint32_t* __restrict__ fgid, const double* __restrict__ fval,
-                                                 uint32_t P2pad, const uint32_t* __restrict__ H2,
-                                                 const uint32_t* __restrict__ base2, uint16_t* __restrict__ opg,
-                                                 double* __restrict__ opv) {
-    constexpr int PR = TR / kT;
+__global__ __launch_bounds__(kPST, 4) void k_pscatter(const Item* __restrict__ items, const int32_t* __restrict__ mm,
+                                                      const uint32_t* __restrict__ titem,
+                                                      const uint32_t* __restrict__ fgid, const double* __restrict__ fval,
+                                                      uint32_t P2pad, const uint32_t* __restrict__ H2,
+                                                      const uint32_t* __restrict__ toff2,
+                                                      const uint32_t* __restrict__ base2, uint16_t* __restrict__ opg,
+                                                      double* __restrict__ opv) {
+    constexpr int PR = TR / kPST;
+    static_assert(TR % kPST == 0 && kP2Max <= kPST, "tile shape; one thread per range in the scan");
     __shared__ uint16_t sg[TR], sq[TR];
     __shared__ double sv[TR];
     __shared__ uint32_t cnt[kP2Max], start[kP2Max], cur[kP2Max];
-    __shared__ uint32_t ws[kT / 64 + 1];
-    if ((int32_t)blockIdx.x >= mm[kMmItems1]) return;
-    const Item it = items[blockIdx.x];
-    const uint32_t* h = H2 + (uint64_t)blockIdx.x * P2pad;
+    __shared__ uint32_t ws[kPST / 64 + 1];
+    const uint32_t t = pgpu::dev::xcd_remap_chunked<32>(blockIdx.x, gridDim.x);
+    if ((int32_t)t >= mm[kMmTiles1]) return;
+    const uint32_t ik = titem[t], item = ik >> 8;
+    const Item it = items[item];
+    const uint32_t rs = it.r0 + (ik & 0xFFu) * TR, re = min(it.r1, rs + TR);
     uint32_t g[PR];
     double v[PR];
-    auto fetch = [&](uint32_t t0, uint32_t* k, double* x) {
 #pragma unroll
-        for (int u = 0; u < PR; ++u) {
-            const uint32_t r = t0 + u * kT + threadIdx.x;
-            const bool ok = r < it.r1;
-            k[u] = ok ? fgid[r] : kNone;
-            x[u] = ok ? fval[r] : 0.0;
-        }
-    };
-    fetch(it.r0, g, v);
-    for (uint32_t d = threadIdx.x; d < P2pad; d += kT) {
-        cur[d] = base2[d] + h[d];
+    for (int u = 0; u < PR; ++u) {
+        const uint32_t r = rs + u * kPST + threadIdx.x;
+        const bool ok = r < re;
+        g[u] = ok ? fgid[r] : kNone;
+        v[u] = ok ? fval[r] : 0.0;
+    }
+    const uint32_t* h = H2 + (uint64_t)item * P2pad;
+    const uint32_t* to = toff2 + (uint64_t)t * P2pad;
+    for (uint32_t d = threadIdx.x; d < P2pad; d += kPST) {
+        cur[d] = base2[d] + h[d] + to[d];
         cnt[d] = 0;
     }
     __syncthreads();
-    for (uint32_t t0 = it.r0; t0 < it.r1; t0 += TR) {
-        uint32_t rk[PR];
+    uint32_t rk[PR];
 #pragma unroll
-        for (int u = 0; u < PR; ++u)
-            if (g[u] != kNone) rk[u] = atomicAdd(&cnt[g[u] >> kSG], 1u);
-        uint32_t g2[PR];
-        double v2[PR];
-        fetch(t0 + TR, g2, v2);
-        __syncthreads();
-        const uint32_t loc = threadIdx.x < P2pad ? cnt[threadIdx.x] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_excl_scan<kT>(loc, ws, &total);
-        if (threadIdx.x < P2pad) start[threadIdx.x] = ex;
-        __syncthreads();
+    for (int u = 0; u < PR; ++u)
+        if (g[u] != kNone) rk[u] = atomicAdd(&cnt[g[u] >> kSG], 1u);
+    __syncthreads();
+    const uint32_t loc = threadIdx.x < P2pad ? cnt[threadIdx.x] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan<kPST>(loc, ws, &total);
+    if (threadIdx.x < P2pad) start[threadIdx.x] = ex;
+    __syncthreads();
 #pragma unroll
-        for (int u = 0; u < PR; ++u) {
-            if (g[u] != kNone) {
-                const uint32_t q = g[u] >> kSG;
-                const uint32_t pos = start[q] + rk[u];
-                sg[pos] = (uint16_t)(g[u] & ((1u << kSG) - 1));
-                sq[pos] = (uint16_t)q;
-                sv[pos] = v[u];
-            }
+    for (int u = 0; u < PR; ++u) {
+        if (g[u] != kNone) {
+            const uint32_t q = g[u] >> kSG;
+            const uint32_t pos = start[q] + rk[u];
+            sg[pos] = (uint16_t)(g[u] & ((1u << kSG) - 1));
+            sq[pos] = (uint16_t)q;
+            sv[pos] = v[u];
         }
-        __syncthreads();
-        for (uint32_t pos = threadIdx.x; pos < total; pos += kT) {
-            const uint32_t q = sq[pos];
-            const uint32_t idx = cur[q] + (pos - start[q]);
-            opg[idx] = sg[pos];
-            opv[idx] = sv[pos];
-        }
-        __syncthreads();
-        for (uint32_t d = threadIdx.x; d < P2pad; d += kT) {
-            cur[d] += cnt[d];
-            cnt[d] = 0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < PR; ++u) {
-            g[u] = g2[u];
-            v[u] = v2[u];
-        }
+    }
+    __syncthreads();
+    for (uint32_t pos = threadIdx.x; pos < total; pos += kPST) {
+        const uint32_t q = sq[pos];
+        const uint32_t idx = cur[q] + (pos - start[q]);
+        opg[idx] = sg[pos];
+        opv[idx] = sv[pos];
     }
 }
 
@@ -907,6 +945,8 @@ struct pgpu_groupjoin {
     uint32_t *tot = nullptr, *baseD = nullptr, *baseF = nullptr, *base2 = nullptr;
     uint32_t *first1 = nullptr, *first2 = nullptr;
     part::Item *items1 = nullptr, *items2 = nullptr;
+    uint32_t *tile0 = nullptr, *titem = nullptr, *toff2 = nullptr;  // group-range scatter tiles of the probe items
+    uint64_t tiles1_cap = 0;
     uint64_t items1_cap = 0, items2_cap = 0;
     uint16_t *dlk = nullptr, *flk = nullptr, *pg = nullptr;
     uint32_t *dgid = nullptr, *fgid = nullptr;
@@ -953,6 +993,10 @@ int alloc_partitioned(pgpu_groupjoin* p) {
     PGPU_TRY(gj_alloc(p, (kP2Max + 1) * 4, (void**)&p->first2));
     PGPU_TRY(gj_alloc(p, p->items1_cap * sizeof(Item), (void**)&p->items1));
     PGPU_TRY(gj_alloc(p, p->items2_cap * sizeof(Item), (void**)&p->items2));
+    p->tiles1_cap = ceil_div(p->maxF, kGroupTile) + kP1Max + 1;  // whole tiles + one partial tile per partition
+    PGPU_TRY(gj_alloc(p, p->items1_cap * 4, (void**)&p->tile0));
+    PGPU_TRY(gj_alloc(p, p->tiles1_cap * 4, (void**)&p->titem));
+    PGPU_TRY(gj_alloc(p, p->tiles1_cap * kP2Max * 4, (void**)&p->toff2));
     PGPU_TRY(gj_alloc(p, p->maxD * 2, (void**)&p->dlk));
     PGPU_TRY(gj_alloc(p, p->maxD * 4, (void**)&p->dgid));
     PGPU_TRY(gj_alloc(p, p->maxF * 2, (void**)&p->flk));
@@ -1022,9 +1066,9 @@ int run_partitioned(pgpu_groupjoin* p, const int32_t* dkey, const int32_t* dgrp,
     {
         Timed t(p->timer, "gj_probe_hist");
         hipLaunchKernelGGL(k_items, dim3(1), dim3(1024), 0, st, p->baseF, P1, kIR1, p->items1, p->first1,
-                           p->mm + kMmItems1);
+                           p->mm + kMmItems1, (uint32_t)kGroupTile, p->tile0, p->titem, p->mm + kMmTiles1);
         hipLaunchKernelGGL(k_phist, dim3(it1), dim3(kT), 0, st, p->items1, p->mm, p->baseD, p->dlk, p->dgid, p->flk,
-                           P2pad, p->H, p->fgid, p->mm + kMmDup);
+                           P2pad, p->H, p->fgid, p->mm + kMmDup, p->tile0, p->toff2);
         PGPU_LAUNCH_CHECK();
     }
     {
@@ -1032,13 +1076,16 @@ int run_partitioned(pgpu_groupjoin* p, const int32_t* dkey, const int32_t* dgrp,
         hipLaunchKernelGGL(k_cscan, dim3(P2pad / 64), dim3(1024), 0, st, p->H, it1, P2pad, p->tot);
         hipLaunchKernelGGL(k_tscan, dim3(1), dim3(1024), 0, st, p->tot, P2pad, p->base2);
         hipLaunchKernelGGL(k_items, dim3(1), dim3(1024), 0, st, p->base2, P2, kIR2, p->items2, p->first2,
-                           p->mm + kMmItems2);
+                           p->mm + kMmItems2, 0u, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr);
         PGPU_LAUNCH_CHECK();
     }
     {
         Timed t(p->timer, "gj_probe_scatter");
-        hipLaunchKernelGGL(k_pscatter<kGroupTile>, dim3(it1), dim3(kT), 0, st, p->items1, p->mm, p->fgid, p->fv, P2pad, p->H,
-                           p->base2, p->pg, p->pv);
+        // tiles: at most the whole tiles of the fact rows plus one partial tile per partition
+        const uint32_t tiles = (uint32_t)(ceil_div(nF, (uint64_t)kGroupTile) + P1);
+        if (tiles > p->tiles1_cap) return fail(PGPU_EOVERFLOW, "groupjoin: tiles exceed the allocation");
+        hipLaunchKernelGGL(k_pscatter<kGroupTile>, dim3(tiles), dim3(kPST), 0, st, p->items1, p->mm, p->titem,
+                           p->fgid, p->fv, P2pad, p->H, p->toff2, p->base2, p->pg, p->pv);
         PGPU_LAUNCH_CHECK();
     }
     {
