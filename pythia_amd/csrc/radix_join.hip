@@ -911,6 +911,12 @@ struct pgpu_join_plan {
     int last_algo = 0;
     bool pending = false;
     pgpu::KernelTimer* timer = nullptr;
+    // The build side is partitioned on a second stream (default priority)
+    // beside the probe side: its kernels share CUs with the probe side's
+    // histogram, whose LDS counters, not HBM, bound it.
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_build = nullptr;
+    pgpu::KernelTimer* timer2 = nullptr;  // kernels on st2
     std::vector<void*> owned;
     int cus = 0;  // compute units of the device (persistent grid sizing)
 };
@@ -958,7 +964,8 @@ struct Timed {  // t may be null (untimed launches)
 template <typename KT, bool HV>
 int run_partition(pgpu_join_plan* p, hipStream_t st, const KT* keys, const KT* vals, uint64_t n, int B, uint32_t* H,
                   uint32_t* tot, uint64_t* base, uint32_t* px, uint32_t* toff, Elem<KT, HV>* P1, Elem<KT, HV>* P2,
-                  const char* tag) {
+                  const char* tag, pgpu::KernelTimer* timer = nullptr) {
+    if (!timer) timer = p->timer;
     using E = Elem<KT, HV>;
     const int B2 = B - B / 2;  // generate(2): pass 1 takes floor(B/2) high bits
     const uint32_t nb = 1u << B, nb1 = 1u << (B - B2);
@@ -972,14 +979,14 @@ int run_partition(pgpu_join_plan* p, hipStream_t st, const KT* keys, const KT* v
     char name[48];
     {
         snprintf(name, sizeof name, "hist_%s", tag);
-        Timed tm(p->timer, name);
+        Timed tm(timer, name);
         hipLaunchKernelGGL(k_hist<KT>, dim3(nblk), dim3(512), nb * sizeof(uint32_t), st, keys, n, blk, (uint32_t)T1, B,
                            B2, H, toff);
         PGPU_LAUNCH_CHECK();
     }
     {
         snprintf(name, sizeof name, "scan_%s", tag);
-        Timed tm(p->timer, name);
+        Timed tm(timer, name);
         hipLaunchKernelGGL(k_colscan, dim3((unsigned)ceil_div(nb, 64)), dim3(512), 0, st, H, nblk, nb, B2, tot, px);
         PGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_binscan, dim3(1), dim3(1024), 0, st, tot, nb, base);
@@ -987,7 +994,7 @@ int run_partition(pgpu_join_plan* p, hipStream_t st, const KT* keys, const KT* v
     }
     {
         snprintf(name, sizeof name, "scatter1_%s", tag);
-        Timed tm(p->timer, name);
+        Timed tm(timer, name);
         if (ntiles) {
             hipLaunchKernelGGL((k_scatter1<KT, HV, IPT1>), dim3((unsigned)ntiles), dim3(kScatterThreads), 0, st, keys,
                                vals, n, (uint32_t)(blk / T1), nblk, B, B2, base, px, toff, P1);
@@ -996,7 +1003,7 @@ int run_partition(pgpu_join_plan* p, hipStream_t st, const KT* keys, const KT* v
     }
     {
         snprintf(name, sizeof name, "scatter2_%s", tag);
-        Timed tm(p->timer, name);
+        Timed tm(timer, name);
         // group pass-1 blocks so a pass-2 unit averages about one tile
         const uint64_t T = tile_elems(sizeof(E));
         const uint32_t G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nblk, T * nb1 / blk));
@@ -1053,10 +1060,15 @@ template <typename KT, bool HVS>
 int run_radix(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const KT* sk, const KT* sv, uint64_t nS,
               int B) {
     hipStream_t st = p->ctx->stream;
-    PGPU_TRY((run_partition<KT, true>(p, st, rk, rv, nR, B, p->HR, p->totR, p->baseR, p->pxR, p->toffR,
-                                      (Elem<KT, true>*)p->R1, (Elem<KT, true>*)p->R2, "build")));
+    // build side on st2 beside the probe side on st; the join waits for both
+    PGPU_HIP(hipEventRecord(p->ev_fork, st));
+    PGPU_HIP(hipStreamWaitEvent(p->st2, p->ev_fork, 0));
+    PGPU_TRY((run_partition<KT, true>(p, p->st2, rk, rv, nR, B, p->HR, p->totR, p->baseR, p->pxR, p->toffR,
+                                      (Elem<KT, true>*)p->R1, (Elem<KT, true>*)p->R2, "build", p->timer2)));
+    PGPU_HIP(hipEventRecord(p->ev_build, p->st2));
     PGPU_TRY((run_partition<KT, HVS>(p, st, sk, sv, nS, B, p->HS, p->totS, p->baseS, p->pxS, p->toffS,
                                      (Elem<KT, HVS>*)p->S1, (Elem<KT, HVS>*)p->S2, "probe")));
+    PGPU_HIP(hipStreamWaitEvent(st, p->ev_build, 0));
     return run_lds_join<KT, HVS>(p, st, (const Elem<KT, true>*)p->R2, p->baseR, p->totR, nS, B, p->cfg.join_variant);
 }
 
@@ -1170,6 +1182,11 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
         pgpu_join_plan_destroy(p);
         return rc;
     };
+    if (hipStreamCreateWithFlags(&p->st2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&p->ev_build, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(PGPU_EDEVICE, "cannot create the plan's build-side stream"));
+    p->timer2 = new KernelTimer(p->st2);
     if (hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
         p->cus < 1)
         return cleanup(fail(PGPU_EDEVICE, "cannot read the device's compute unit count"));
@@ -1236,8 +1253,13 @@ int pgpu_join_plan_destroy(pgpu_join_plan* p) {
     if (p->ctx) {
         (void)hipSetDevice(p->ctx->device);
         (void)hipStreamSynchronize(p->ctx->stream);
+        if (p->st2) (void)hipStreamSynchronize(p->st2);
     }
     delete p->timer;
+    delete p->timer2;
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_build) (void)hipEventDestroy(p->ev_build);
+    if (p->st2) (void)hipStreamDestroy(p->st2);
     for (void* x : p->owned) (void)ctx_free(p->ctx, x);
     if (p->h_result) (void)hipHostFree(p->h_result);
     delete p;
@@ -1325,6 +1347,7 @@ int pgpu_join_plan_timing(pgpu_join_plan* p, int enable) {
     if (!p) return fail(PGPU_EINVAL, "pgpu_join_plan_timing: null plan");
     PGPU_TRY(ctx_activate(p->ctx));
     p->timer->enable(enable != 0);
+    if (p->timer2) p->timer2->enable(enable != 0);
     return PGPU_OK;
 }
 
@@ -1332,7 +1355,11 @@ int pgpu_join_plan_times(pgpu_join_plan* p, pgpu_kernel_time* out, int cap, int*
     if (!p || !n || (cap > 0 && !out)) return fail(PGPU_EINVAL, "pgpu_join_plan_times: null argument");
     PGPU_TRY(ctx_activate(p->ctx));
     p->timer->harvest();
-    const auto& t = p->timer->totals();
+    std::vector<KernelTimer::Total> t = p->timer->totals();  // the build side's kernels (st2) first
+    if (p->timer2) {
+        p->timer2->harvest();
+        t.insert(t.begin(), p->timer2->totals().begin(), p->timer2->totals().end());
+    }
     *n = (int)t.size();
     for (int i = 0; i < (int)t.size() && i < cap; ++i) {
         memset(&out[i], 0, sizeof out[i]);
