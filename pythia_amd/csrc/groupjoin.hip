@@ -522,22 +522,25 @@ __device__ __forceinline__ uint32_t conv<int32_t, uint32_t>(int32_t v, int64_t g
     return (uint32_t)((int64_t)v - gmin);
 }
 
-// Four consecutive rows (vector loads when aligned and complete).
+// Two consecutive rows (one vector load when aligned and complete).  Lane l
+// of a wavefront loads rows 2l, 2l + 1 of a 128-row stretch, so each load
+// instruction covers whole 128-byte lines: with four or eight consecutive
+// rows per lane, each instruction touched every line of the stretch in part,
+// and the streaming (non-temporal) loads fetched some lines twice.
 template <typename V>
-__device__ __forceinline__ void load4(const V* __restrict__ p, uint64_t row, uint64_t end, bool vec, V* out) {
-    if (vec && row + 4 <= end) {
+__device__ __forceinline__ void load2(const V* __restrict__ p, uint64_t row, uint64_t end, bool vec, V* out) {
+    if (vec && row + 2 <= end) {
         if constexpr (sizeof(V) == 8) {
-            const uint4 a = nt_load((const uint4*)(p + row)), b = nt_load((const uint4*)(p + row + 2));
-            memcpy(out, &a, 16);
-            memcpy(out + 2, &b, 16);
-        } else {
             const uint4 a = nt_load((const uint4*)(p + row));
             memcpy(out, &a, 16);
+        } else {
+            const uint2 a = nt_load((const uint2*)(p + row));
+            memcpy(out, &a, 8);
         }
         return;
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) out[u] = row + u < end ? p[row + u] : V();
+    out[0] = row < end ? p[row] : V();
+    out[1] = row + 1 < end ? p[row + 1] : V();
 }
 
 // Scatter of one tile of kRangeTile rows into key-range partitions: rows
@@ -554,21 +557,22 @@ __global__ __launch_bounds__(kRT, 4) void k_rscatter(const int32_t* __restrict__
                                                       const uint32_t* __restrict__ base, uint16_t* __restrict__ olk,
                                                       VOut* __restrict__ oval) {
     constexpr int PR = TR / kRT, PS = kP1Max / kRT;
-    static_assert(TR == kRangeTile && PR % 4 == 0, "tile shape");
+    static_assert(TR == kRangeTile && PR % 2 == 0, "tile shape");
     __shared__ uint16_t slk[TR], spart[TR];
     __shared__ VOut sval[TR];
     __shared__ uint32_t cnt[kP1Max], start[kP1Max], cur[kP1Max];
     __shared__ uint32_t ws[kRT / 64 + 1];
     const uint32_t t = pgpu::dev::xcd_remap_chunked<32>(blockIdx.x, gridDim.x);
     const uint64_t t0 = (uint64_t)t * TR, te = min<uint64_t>(n, t0 + TR);
-    const bool vec = ((uintptr_t)key & 15) == 0 && ((uintptr_t)pay & 15) == 0;
+    const bool vec = ((uintptr_t)key & 7) == 0 && ((uintptr_t)pay & (2 * sizeof(VIn) - 1)) == 0;
     int32_t k[PR];
     VIn v[PR];
-    const uint64_t row = t0 + (uint64_t)threadIdx.x * PR;
+    uint64_t rows[PR / 2];
 #pragma unroll
-    for (int j = 0; j < PR; j += 4) {
-        load4(key, row + j, te, vec, k + j);
-        load4(pay, row + j, te, vec, v + j);
+    for (int j = 0; j < PR / 2; ++j) {
+        rows[j] = t0 + ((uint64_t)j * kRT + threadIdx.x) * 2;
+        load2(key, rows[j], te, vec, k + 2 * j);
+        load2(pay, rows[j], te, vec, v + 2 * j);
     }
     const uint32_t* h = H + (uint64_t)(t / kTilesPerChunk) * Ppad;
     const uint16_t* to = toff + (uint64_t)t * Ppad;
@@ -582,7 +586,7 @@ __global__ __launch_bounds__(kRT, 4) void k_rscatter(const int32_t* __restrict__
 #pragma unroll
     for (int u = 0; u < PR; ++u) {
         const uint64_t off = (uint64_t)((int64_t)k[u] - kmin);
-        ok[u] = row + u < te && off < range;
+        ok[u] = rows[u / 2] + (u & 1) < te && off < range;
         part[u] = (uint32_t)(off >> kSK);
         k[u] = (int32_t)(off & ((1u << kSK) - 1));
         if (ok[u]) rk[u] = atomicAdd(&cnt[part[u]], 1u);
