@@ -265,7 +265,7 @@ constexpr uint32_t kCR = 65536;                                // rows per histo
 constexpr uint32_t kIR1 = 196608;                              // rows per probe item
 constexpr uint32_t kIR2 = 98304;                               // rows per aggregation item
 constexpr int kT = 1024;                                       // threads per workgroup (scatter, probe, agg)
-constexpr int kMinmaxBlocks = 256;
+constexpr int kMinmaxBlocks = 1024;
 constexpr int kRangeTile = 4096;                               // rows per LDS staging tile: key-range scatter
 constexpr uint32_t kTilesPerChunk = kCR / kRangeTile;
 constexpr int kRT = 512;                                       // key-range scatter threads (two workgroups per CU)
@@ -319,16 +319,48 @@ __device__ __forceinline__ void mm_block(int32_t* v, int32_t* __restrict__ out) 
     }
 }
 
+// Each lane folds groups of four rows (one 16-byte load per column, whole
+// lines per load instruction), four groups in flight per step.
 __global__ __launch_bounds__(kT) void k_minmax(const int32_t* __restrict__ dkey, const int32_t* __restrict__ dgrp,
                                                uint64_t n, int32_t* __restrict__ part) {
     int32_t v[4] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
-    const uint64_t stride = (uint64_t)gridDim.x * kT;
-#pragma unroll 4
-    for (uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride) {
-        const int32_t k = dkey[i], g = dgrp[i];
+    auto fold = [&](int32_t k, int32_t g) {
         const int32_t o[4] = {k, k, g, g};
         mm_fold(v, o);
+    };
+    const uint64_t nthr = (uint64_t)gridDim.x * kT, tid = (uint64_t)blockIdx.x * kT + threadIdx.x;
+    uint64_t done = 0;  // rows [0, done) are covered by the vector loop
+    if (((uintptr_t)dkey & 15) == 0 && ((uintptr_t)dgrp & 15) == 0) {
+        constexpr int U = 4;
+        const uint64_t ng = n / 4;  // whole groups of four rows
+        const int4* kv = (const int4*)dkey;
+        const int4* gv = (const int4*)dgrp;
+        for (uint64_t g0 = tid; g0 < ng; g0 += U * nthr) {
+            int4 a[U], b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t g = g0 + (uint64_t)u * nthr;
+                if (g < ng) {
+                    a[u] = kv[g];  // cached: the histogram pass reads the keys next
+                    b[u] = gv[g];
+                } else {
+                    a[u] = make_int4(INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX);
+                    b[u] = make_int4(INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (g0 + (uint64_t)u * nthr < ng) {
+                    fold(a[u].x, b[u].x);
+                    fold(a[u].y, b[u].y);
+                    fold(a[u].z, b[u].z);
+                    fold(a[u].w, b[u].w);
+                }
+            }
+        }
+        done = ng * 4;
     }
+    for (uint64_t i = done + tid; i < n; i += nthr) fold(dkey[i], dgrp[i]);
     mm_block(v, part + 4 * blockIdx.x);
 }
 
@@ -636,23 +668,40 @@ __device__ __forceinline__ void build_slice(uint32_t* tab, const uint16_t* __res
     for (uint32_t i = threadIdx.x; i < (1u << kSK); i += kT) tab[i] = kNone;
     __syncthreads();
     bool seen = false;
-    for (uint32_t r = b0 + threadIdx.x; r < b1; r += kT) seen |= atomicMin(&tab[dlk[r]], dgid[r]) != kNone;
+    constexpr int U = 8;  // rows per thread per step, all loads issued before the LDS updates
+    for (uint32_t r0 = b0; r0 < b1; r0 += U * kT) {
+        uint32_t lk[U], g[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t r = r0 + u * kT + threadIdx.x;
+            lk[u] = r < b1 ? dlk[r] : 0u;
+            g[u] = r < b1 ? dgid[r] : kNone;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (r0 + u * kT + threadIdx.x < b1) seen |= atomicMin(&tab[lk[u]], g[u]) != kNone;
+    }
     if (dup && seen) *dup = 1;
     __syncthreads();
 }
 
 // Pass 1 over one probe item: every row's gid (kNone: no match) into fgid,
 // H2[item][q] = rows of the item whose group lies in group-range partition
-// q = gid >> kSG, and toff2[t][q] = the item's rows of range q in its tiles
-// (kGroupTile rows each) before tile t.  Each wavefront looks up whole tiles
-// (tiles w and w + 16 of the item) and counts them in its own LDS row (two
-// 16-bit counters per word: a tile holds at most 6144 rows).
+// q = gid >> kSG, and toff2[s][q] = the item's rows of range q before its
+// half-tile s (kGroupTile / 2 rows; half-tiles 2t and 2t + 1 make up tile t,
+// counted globally, so toff2[2t] holds tile t's offsets).  Each wavefront
+// looks up whole half-tiles (w, w + 16, ... of the item) and counts them in
+// its own LDS row (two 16-bit counters per word: a half-tile holds at most
+// 3072 rows).  Half-tiles spread an item over the waves more evenly than
+// tiles: an item of 22 tiles kept 6 waves busy with a second tile while the
+// other 10 waited.
 __global__ __launch_bounds__(kT) void k_phist(const Item* __restrict__ items, const int32_t* __restrict__ mm,
                                               const uint32_t* __restrict__ bD, const uint16_t* __restrict__ dlk,
                                               const uint32_t* __restrict__ dgid, const uint16_t* __restrict__ flk,
                                               uint32_t P2pad, uint32_t* __restrict__ H2, uint32_t* __restrict__ fgid,
                                               int32_t* __restrict__ dup, const uint32_t* __restrict__ tile0,
                                               uint32_t* __restrict__ toff2) {
+    constexpr uint32_t kHalf = kGroupTile / 2;
     __shared__ uint32_t tab[1u << kSK];
     __shared__ uint32_t lt[kT / 64][kP2Max / 2];
     uint32_t* out = H2 + (uint64_t)blockIdx.x * P2pad;
@@ -664,11 +713,11 @@ __global__ __launch_bounds__(kT) void k_phist(const Item* __restrict__ items, co
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (uint32_t i = lane; i < kP2Max / 2; i += 64) lt[w][i] = 0;
     build_slice(tab, dlk, dgid, bD[it.part], bD[it.part + 1], dup);
-    const uint32_t nt = (it.r1 - it.r0 + kGroupTile - 1) / kGroupTile;
-    const uint32_t tb = tile0[blockIdx.x];
+    const uint32_t ns = 2 * ((it.r1 - it.r0 + kGroupTile - 1) / kGroupTile);  // half-tiles, the last may be empty
+    uint32_t* const tcb = toff2 + (uint64_t)(2 * tile0[blockIdx.x]) * P2pad;
     constexpr int U = 16;  // rows per lane per step, all loads issued before the lookups
-    for (uint32_t k = w; k < nt; k += kT / 64) {
-        const uint32_t rs = it.r0 + k * kGroupTile, re = min(it.r1, rs + kGroupTile);
+    for (uint32_t k = w; k < ns; k += kT / 64) {
+        const uint32_t rs = it.r0 + k * kHalf, re = min(it.r1, rs + kHalf);
         for (uint32_t t0 = rs; t0 < re; t0 += U * 64) {
             uint32_t lk[U];
 #pragma unroll
@@ -689,25 +738,30 @@ __global__ __launch_bounds__(kT) void k_phist(const Item* __restrict__ items, co
             }
         }
         // one wavefront's LDS operations complete in order: its counts are final
-        uint32_t* tc = toff2 + (uint64_t)(tb + k) * P2pad;
+        uint32_t* tc = tcb + (uint64_t)k * P2pad;
         for (uint32_t q = lane; q < P2pad; q += 64) tc[q] = (lt[w][q >> 1] >> (16 * (q & 1))) & 0xFFFFu;
         for (uint32_t i = lane; i < kP2Max / 2; i += 64) lt[w][i] = 0;
     }
-    __syncthreads();  // every tile's counts are in toff2
+    __syncthreads();  // every half-tile's counts are in toff2
+    constexpr int B = 8;  // half-tiles per step: their counts are loaded together
     for (uint32_t q = threadIdx.x; q < P2pad; q += kT) {
         uint32_t run = 0;
-        for (uint32_t k = 0; k < nt; ++k) {
-            uint32_t* pk = toff2 + (uint64_t)(tb + k) * P2pad + q;
-            const uint32_t c = *pk;
-            *pk = run;
-            run += c;
+        for (uint32_t k0 = 0; k0 < ns; k0 += B) {
+            uint32_t c[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) c[b] = k0 + b < ns ? tcb[(uint64_t)(k0 + b) * P2pad + q] : 0u;
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                if (k0 + b < ns) tcb[(uint64_t)(k0 + b) * P2pad + q] = run;
+                run += c[b];
+            }
         }
         out[q] = run;
     }
 }
 
 // Pass 2 over one tile of a probe item: (gid & 8191, value) rows into
-// group-range partitions at base2[q] + H2[item][q] + toff2[tile][q] (H2
+// group-range partitions at base2[q] + H2[item][q] + toff2[2 tile][q] (H2
 // column-scanned).  Tiles t and t + 1 write adjacent runs of every range:
 // neighbours run side by side on one XCD (chunks of 32 tiles).
 template <int TR>
@@ -739,7 +793,7 @@ __global__ __launch_bounds__(kPST, 4) void k_pscatter(const Item* __restrict__ i
         v[u] = ok ? fval[r] : 0.0;
     }
     const uint32_t* h = H2 + (uint64_t)item * P2pad;
-    const uint32_t* to = toff2 + (uint64_t)t * P2pad;
+    const uint32_t* to = toff2 + (uint64_t)(2 * t) * P2pad;  // the tile's first half-tile
     for (uint32_t d = threadIdx.x; d < P2pad; d += kPST) {
         cur[d] = base2[d] + h[d] + to[d];
         cnt[d] = 0;
@@ -817,35 +871,57 @@ __global__ __launch_bounds__(kT) void k_gagg(const Item* __restrict__ items, con
 }
 
 // Group gid = sum of its partition's partial tables; non-empty groups are
-// written out (positions past out_cap counted, not written).  One block per
-// group-range partition: one cursor atomic and one joined-count atomic each.
+// written out (positions past out_cap counted, not written).  kCompactSplit
+// blocks per group-range partition, each over a slice of its groups: one
+// cursor atomic and one joined-count atomic each.
+constexpr uint32_t kCompactSplit = 4;
 __global__ __launch_bounds__(kT) void k_gcompact(const uint32_t* __restrict__ first2, uint64_t Rg, int64_t gmin,
                                                  const double* __restrict__ psum, const uint32_t* __restrict__ pcnt,
                                                  int32_t* __restrict__ okey, double* __restrict__ osum,
                                                  uint64_t* __restrict__ ocnt, uint64_t out_cap,
                                                  unsigned long long* __restrict__ ct) {
-    constexpr uint32_t NG = 1u << kSG;
-    constexpr int U = NG / kT;
+    constexpr uint32_t NG = 1u << kSG, NS = NG / kCompactSplit;
+    constexpr int U = NS / kT;
+    static_assert(NS % kT == 0, "whole groups per thread");
     __shared__ unsigned long long base;
     __shared__ uint32_t ws[kT / 64 + 1];
     __shared__ unsigned long long wj[kT / 64];
-    const uint32_t q = blockIdx.x;
+    const uint32_t q = blockIdx.x / kCompactSplit;
     const uint32_t i0 = first2[q], i1 = first2[q + 1];
-    const uint64_t g0 = (uint64_t)q << kSG;
+    const uint64_t g0 = ((uint64_t)q << kSG) + (uint64_t)(blockIdx.x % kCompactSplit) * NS;
+    const uint32_t s0 = (blockIdx.x % kCompactSplit) * NS;  // first slot of this block in a partial table
     uint32_t c[U];
     double sm[U];
     uint32_t live = 0;
     unsigned long long joined = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint32_t sl = u * kT + threadIdx.x;
         c[u] = 0;
         sm[u] = 0.0;
-        if (g0 + sl < Rg)
-            for (uint32_t i = i0; i < i1; ++i) {
-                c[u] += pcnt[(uint64_t)i * NG + sl];
-                sm[u] += psum[(uint64_t)i * NG + sl];
+    }
+    constexpr int B = 4;  // partial tables read per step, all loads issued before the adds
+    for (uint32_t i = i0; i < i1; i += B) {
+        uint32_t cc[B][U];
+        double ss[B][U];
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t sl = s0 + u * kT + threadIdx.x;
+                const bool ok = i + b < i1 && g0 + u * kT + threadIdx.x < Rg;
+                cc[b][u] = ok ? pcnt[(uint64_t)(i + b) * NG + sl] : 0u;
+                ss[b][u] = ok ? psum[(uint64_t)(i + b) * NG + sl] : 0.0;
             }
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c[u] += cc[b][u];
+                sm[u] += ss[b][u];
+            }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
         live += c[u] > 0;
         joined += c[u];
     }
@@ -1000,7 +1076,7 @@ int alloc_partitioned(pgpu_groupjoin* p) {
     p->tiles1_cap = ceil_div(p->maxF, kGroupTile) + kP1Max + 1;  // whole tiles + one partial tile per partition
     PGPU_TRY(gj_alloc(p, p->items1_cap * 4, (void**)&p->tile0));
     PGPU_TRY(gj_alloc(p, p->tiles1_cap * 4, (void**)&p->titem));
-    PGPU_TRY(gj_alloc(p, p->tiles1_cap * kP2Max * 4, (void**)&p->toff2));
+    PGPU_TRY(gj_alloc(p, 2 * p->tiles1_cap * kP2Max * 4, (void**)&p->toff2));  // two half-tiles per tile
     PGPU_TRY(gj_alloc(p, p->maxD * 2, (void**)&p->dlk));
     PGPU_TRY(gj_alloc(p, p->maxD * 4, (void**)&p->dgid));
     PGPU_TRY(gj_alloc(p, p->maxF * 2, (void**)&p->flk));
@@ -1099,8 +1175,8 @@ int run_partitioned(pgpu_groupjoin* p, const int32_t* dkey, const int32_t* dgrp,
     }
     {
         Timed t(p->timer, "gj_compact");
-        hipLaunchKernelGGL(k_gcompact, dim3(P2), dim3(kT), 0, st, p->first2, Rg, gmin, p->psum, p->pcnt, p->okey,
-                           p->osum, p->ocnt, p->maxG + 1, p->counters);
+        hipLaunchKernelGGL(k_gcompact, dim3(P2 * kCompactSplit), dim3(kT), 0, st, p->first2, Rg, gmin, p->psum,
+                           p->pcnt, p->okey, p->osum, p->ocnt, p->maxG + 1, p->counters);
         PGPU_LAUNCH_CHECK();
     }
     PGPU_HIP(hipMemcpyAsync(p->h_counters, p->counters, kCtWords * 8, hipMemcpyDeviceToHost, st));
