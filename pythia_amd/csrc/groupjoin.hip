@@ -419,47 +419,63 @@ __global__ __launch_bounds__(512) void k_rhist(const int32_t* __restrict__ key, 
     const uint64_t t1 = (r1 - r0 + kRangeTile - 1) / kRangeTile;
     uint16_t* tc = toff + (uint64_t)(r0 / kRangeTile) * Ppad;
     for (uint32_t i = threadIdx.x; i < Ppad; i += 512) {
+        uint32_t c[kTilesPerChunk];  // all of the column's counts loaded before the first store
+#pragma unroll
+        for (uint32_t t = 0; t < kTilesPerChunk; ++t) c[t] = t < t1 ? tc[t * Ppad + i] : 0u;
         uint32_t run = 0;
-        for (uint64_t t = 0; t < t1; ++t) {
-            const uint32_t c = tc[t * Ppad + i];
-            tc[t * Ppad + i] = (uint16_t)run;
-            run += c;
+#pragma unroll
+        for (uint32_t t = 0; t < kTilesPerChunk; ++t) {
+            if (t < t1) tc[t * Ppad + i] = (uint16_t)run;
+            run += c[t];
         }
         H[(uint64_t)blockIdx.x * Ppad + i] = run;
     }
 }
 
 // Exclusive scan of every column of H[rows][Ppad] in place; tot[d] receives
-// column d's sum.  One block per 64 columns, 16 row segments per column.
+// column d's sum.  One block per kScanCols columns, 64 row segments per
+// column (Ppad a multiple of kScanCols).
+constexpr uint32_t kScanCols = 16;
 __global__ __launch_bounds__(1024) void k_cscan(uint32_t* __restrict__ H, uint32_t rows, uint32_t Ppad,
                                                 uint32_t* __restrict__ tot) {
-    __shared__ uint32_t s[16][64];
-    const uint32_t c = threadIdx.x & 63, seg = threadIdx.x >> 6;
-    const uint32_t d = blockIdx.x * 64 + c;
-    const uint32_t per = (rows + 15) / 16;
+    constexpr uint32_t SG = 1024 / kScanCols;
+    static_assert(SG == 64, "one wavefront scans one column's segments");
+    __shared__ uint32_t s[SG][kScanCols + 1];
+    const uint32_t c = threadIdx.x % kScanCols, seg = threadIdx.x / kScanCols;
+    const uint32_t d = blockIdx.x * kScanCols + c;
+    const uint32_t per = (rows + SG - 1) / SG;
     const uint32_t a = min(rows, seg * per), b = min(rows, a + per);
+    constexpr int B = 8;  // rows per step: their counts are loaded together
     uint32_t sum = 0;
-#pragma unroll 8
-    for (uint32_t r = a; r < b; ++r) sum += H[(uint64_t)r * Ppad + d];
+    for (uint32_t r0 = a; r0 < b; r0 += B) {
+#pragma unroll
+        for (int j = 0; j < B; ++j) sum += r0 + j < b ? H[(uint64_t)(r0 + j) * Ppad + d] : 0u;
+    }
     s[seg][c] = sum;
     __syncthreads();
-    if (seg == 0) {
-        uint32_t run = 0;
-        for (int i = 0; i < 16; ++i) {
-            const uint32_t t = s[i][c];
-            s[i][c] = run;
-            run += t;
+    {
+        const uint32_t col = threadIdx.x >> 6, lane = threadIdx.x & 63;  // wave col scans column col
+        const uint32_t v = s[lane][col];
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= (uint32_t)o) x += y;
         }
-        tot[d] = run;
+        s[lane][col] = x - v;
+        if (lane == 63) tot[blockIdx.x * kScanCols + col] = x;
     }
     __syncthreads();
     uint32_t run = s[seg][c];
-#pragma unroll 8
-    for (uint32_t r = a; r < b; ++r) {
-        uint32_t* p = &H[(uint64_t)r * Ppad + d];
-        const uint32_t v = *p;
-        *p = run;
-        run += v;
+    for (uint32_t r0 = a; r0 < b; r0 += B) {
+        uint32_t v[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) v[j] = r0 + j < b ? H[(uint64_t)(r0 + j) * Ppad + d] : 0u;
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (r0 + j < b) H[(uint64_t)(r0 + j) * Ppad + d] = run;
+            run += v[j];
+        }
     }
 }
 
@@ -1110,7 +1126,7 @@ int partition_side(pgpu_groupjoin* p, const char* name, const int32_t* key, cons
     {
         snprintf(nm, sizeof nm, "%s_scan", name);
         Timed t(p->timer, nm);
-        hipLaunchKernelGGL(k_cscan, dim3(Ppad / 64), dim3(1024), 0, st, p->H, chunks, Ppad, p->tot);
+        hipLaunchKernelGGL(k_cscan, dim3(Ppad / kScanCols), dim3(1024), 0, st, p->H, chunks, Ppad, p->tot);
         hipLaunchKernelGGL(k_tscan, dim3(1), dim3(1024), 0, st, p->tot, Ppad, base);
         PGPU_LAUNCH_CHECK();
     }
@@ -1153,7 +1169,7 @@ int run_partitioned(pgpu_groupjoin* p, const int32_t* dkey, const int32_t* dgrp,
     }
     {
         Timed t(p->timer, "gj_probe_scan");
-        hipLaunchKernelGGL(k_cscan, dim3(P2pad / 64), dim3(1024), 0, st, p->H, it1, P2pad, p->tot);
+        hipLaunchKernelGGL(k_cscan, dim3(P2pad / kScanCols), dim3(1024), 0, st, p->H, it1, P2pad, p->tot);
         hipLaunchKernelGGL(k_tscan, dim3(1), dim3(1024), 0, st, p->tot, P2pad, p->base2);
         hipLaunchKernelGGL(k_items, dim3(1), dim3(1024), 0, st, p->base2, P2, kIR2, p->items2, p->first2,
                            p->mm + kMmItems2, 0u, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr);
