@@ -158,10 +158,16 @@ __global__ __launch_bounds__(64 * kColWaves) void k_bapply(const Seg* __restrict
     if (d >= nd) return;
     uint32_t run = bnd[s.bnd + d];
     for (int q = 0; q < w; ++q) run += part[q][lane];
-    for (uint32_t c = c0; c < c1; ++c) {
-        const uint32_t v = h[(uint64_t)c * nd + d];
-        h[(uint64_t)c * nd + d] = run;
-        run += v;
+    constexpr int B = 8;  // chunks per step: their counts are loaded before the first store
+    for (uint32_t cb = c0; cb < c1; cb += B) {
+        uint32_t v[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) v[j] = cb + j < c1 ? h[(uint64_t)(cb + j) * nd + d] : 0u;
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (cb + j < c1) h[(uint64_t)(cb + j) * nd + d] = run;
+            run += v[j];
+        }
     }
 }
 
