@@ -430,7 +430,7 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
         // end read as zero (masked by the probe).
         auto batch_rsrc = [&](uint64_t b0) {
             const uint64_t left = b0 < se ? (se - b0) * sizeof(SR) : 0;
-            return buf_rsrc(S + b0, (uint32_t)min<uint64_t>(left, 0xFFFFFFF0ull));
+            return buf_rsrc(S + b0, left < 0xFFFFFFF0ull ? (uint32_t)left : 0xFFFFFFF0u);
         };
         auto fetch = [&](uint64_t b0) {
             const uint32_t cnt = rows_in(b0);
@@ -538,8 +538,9 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
                 // count and sum are 0: no hit / miss select.  A probe key
                 // equal to the empty marker also ends on an empty slot; those
                 // rows are tallied and take the side accumulator per piece.
-                uint32_t bc = 0, ne = 0;  // fits 32 bits: U rows x PIECE matches
-                unsigned long long bs = 0, bp = 0, ve = 0;
+                uint32_t bc = 0;  // fits 32 bits: U rows x PIECE matches
+                unsigned long long bs = 0, bp = 0;
+                bool em = false;
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const bool live = FULL || (uint32_t)u < cnt;
@@ -555,16 +556,26 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
                     }
                     bc += c;
                     bs += sbv;
-                    if constexpr (HVS) bp += (unsigned long long)c * (unsigned long long)(int64_t)x[u].v;
-                    const bool em = live && x[u].k == EMPTY;
-                    ne += em ? 1u : 0u;
-                    if constexpr (HVS) ve += em ? (unsigned long long)(int64_t)x[u].v : 0ull;
+                    if constexpr (HVS) {
+                        if constexpr (W)
+                            bp += (unsigned long long)c * (unsigned long long)(int64_t)x[u].v;
+                        else  // c <= PIECE < 2^31: one signed 32 x 32 -> 64-bit multiply-add
+                            bp += (unsigned long long)((int64_t)(int32_t)c * (int64_t)x[u].v);
+                    }
+                    em |= live && x[u].k == EMPTY;
                 }
                 acc_c += bc;
                 acc_b += bs;
                 acc_p += bp;
-                pe_n += ne;
-                pe_v += ve;
+                if (__any(em)) {  // rare: a wavefront holds a probe key equal to the empty marker
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        if ((FULL || (uint32_t)u < cnt) && x[u].k == EMPTY) {
+                            pe_n += 1;
+                            if constexpr (HVS) pe_v += (unsigned long long)(int64_t)x[u].v;
+                        }
+                    }
+                }
             };
             auto probe = [&](const SR(&z)[U], uint64_t b0) {
                 if (b0 + (uint64_t)U * T <= se)

@@ -111,14 +111,18 @@ __device__ __forceinline__ uint32_t xcd_remap_chunked(uint32_t i, uint32_t n) {
 
 // Reads through a buffer resource: one 32-bit offset register per load in
 // flight instead of a 64-bit address, so more loads stay in registers.  The
-// base must be the same for the whole wavefront; offsets at or past `bytes`
-// read zero.  Word 3 = 0x00020000 selects the untyped 32-bit data format of
-// gfx9 buffer resources.
+// base and `bytes` must be the same for the whole wavefront (both are read
+// from its first lane, so the resource lives in scalar registers even where
+// the compiler cannot prove it uniform: otherwise every load is wrapped in a
+// loop over the distinct resources of the wavefront); offsets at or past
+// `bytes` read zero.  Word 3 = 0x00020000 selects the untyped 32-bit data
+// format of gfx9 buffer resources.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
     const uint64_t a = (uint64_t)base;
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-    return __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), (short)0, (int)bytes, 0x00020000);
+    const uint32_t nb = __builtin_amdgcn_readfirstlane(bytes);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), (short)0, (int)nb, 0x00020000);
 }
 
 // NT: the non-temporal cache policy (cache-policy operand bit 1, the same

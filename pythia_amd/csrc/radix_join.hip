@@ -696,8 +696,9 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
             // count and sum are 0: no hit / miss select.  A probe key equal to
             // the empty marker also ends on an empty slot; those rows are
             // tallied and take the side accumulator once per piece.
-            uint32_t bc = 0, ne = 0;  // fits 32 bits: U rows x PIECE matches
-            unsigned long long bs = 0, bp = 0, ve = 0;
+            uint32_t bc = 0;  // fits 32 bits: U rows x PIECE matches
+            unsigned long long bs = 0, bp = 0;
+            bool em = false;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const bool live = FULL || (uint32_t)u < cnt;
@@ -709,18 +710,27 @@ __global__ __launch_bounds__(T, WPE) void k_join_cs(
                 }
                 bc += c;
                 bs += sb;
-                if constexpr (HVS) bp += (unsigned long long)c * (unsigned long long)(int64_t)x[u].v;
-                if constexpr (sizeof(KT) == 8) {  // sign-extended int32 keys never equal it
-                    const bool em = live && (long long)x[u].k == EMPTY;
-                    ne += em ? 1u : 0u;
-                    if constexpr (HVS) ve += em ? (unsigned long long)(int64_t)x[u].v : 0ull;
+                if constexpr (HVS) {
+                    if constexpr (sizeof(KT) == 8)
+                        bp += (unsigned long long)c * (unsigned long long)(int64_t)x[u].v;
+                    else  // c <= PIECE < 2^31: one signed 32 x 32 -> 64-bit multiply-add
+                        bp += (unsigned long long)((int64_t)(int32_t)c * (int64_t)x[u].v);
                 }
+                if constexpr (sizeof(KT) == 8)  // sign-extended int32 keys never equal it
+                    em |= live && (long long)x[u].k == EMPTY;
             }
             acc_c += bc;
             acc_b += bs;
             acc_p += bp;
-            pe_n += ne;
-            pe_v += ve;
+            if (sizeof(KT) == 8 && __any(em)) {  // rare: a wavefront holds a probe key equal to the empty marker
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if ((FULL || (uint32_t)u < cnt) && (long long)x[u].k == EMPTY) {
+                        pe_n += 1;
+                        if constexpr (HVS) pe_v += (unsigned long long)(int64_t)x[u].v;
+                    }
+                }
+            }
         };
         auto probe = [&](const Elem<KT, HVS>(&z)[U], uint32_t rb) {
             if (rb + BATCH <= sN)
