@@ -215,7 +215,11 @@ typedef struct pgpu_batch_cfg {
     uint32_t key_bytes;     /* 4 or 8                                               */
     uint32_t probe_payload; /* 1: sum probe payloads of matches too                 */
     uint32_t variant;       /* join workgroup shape: 1 = 4096-slot LDS table x 512   */
-                            /* threads, 2 = 8192 slots x 1024 threads; 0 = 2         */
+                            /* threads, 2 = 8192 slots x 1024 threads, 3 (4-byte    */
+                            /* keys) = 8192 8-byte (key, payload) slots x 512       */
+                            /* threads, two per CU, for partitions of unique build  */
+                            /* keys, the others handed to variant 2's kernel; 0 = 3 */
+                            /* for 4-byte keys, else 2                              */
     uint32_t part_rows;     /* target build rows per partition; 0 = variant default */
     uint32_t graph;         /* 1: capture the launch sequence into a hipGraph       */
 } pgpu_batch_cfg;

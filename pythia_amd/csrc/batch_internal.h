@@ -59,6 +59,9 @@ struct Layout {
 
 // Rows per partition the join kernel variant holds in LDS in one piece.
 int variant_piece(int variant);
+// Variants 1 and 2 take any keys; variant 3 (unique-key 8-byte slots, with
+// variant 2 for the partitions it hands back) takes 4-byte keys.
+bool variant_ok(int variant, int key_bytes);
 // Default target build rows per partition for a variant.
 uint32_t variant_target(int variant);
 // Partition bits for n build rows at `target` rows per partition.
@@ -76,10 +79,12 @@ uint32_t scatter_tile_rows(int key_bytes, bool hv, uint32_t max_bits);
 // max_bits: the largest partition bits of the launched chunks' segments.
 int launch_scatter(hipStream_t st, int key_bytes, bool hv, uint32_t max_bits, const Seg* segs, const Chunk* chunks,
                    uint32_t chunk_base, uint32_t nchunks, const uint32_t* H, void* out);
-// Persistent join over n_items work items; `grid` from join_grid().
+// Persistent join over n_items work items; `grid` from join_grid().  queue:
+// zeroed words (one; three for variant 3); redo: n_items words of scratch
+// (variant 3 only: the partitions handed to its second kernel).
 int launch_join(hipStream_t st, int key_bytes, bool hvs, int variant, int grid, const Job* jobs,
                 const uint32_t* item_job, uint32_t n_items, const void* R, const void* S, const uint32_t* bndR,
-                const uint32_t* bndS, uint32_t* queue, unsigned long long* res);
+                const uint32_t* bndS, uint32_t* queue, unsigned long long* res, uint32_t* redo = nullptr);
 int join_grid(int key_bytes, bool hvs, int variant, int* grid);
 // Zeroes n 64-bit words with a kernel (usable inside graph capture).
 int launch_zero(hipStream_t st, unsigned long long* p, uint64_t n);

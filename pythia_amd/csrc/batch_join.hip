@@ -371,12 +371,15 @@ struct Log2<8192> {
     static constexpr int v = 13;
 };
 
+// ilist (optional): the work items to run are ilist[0, *ilist_n) instead of
+// 0 .. n_items - 1 (the partitions the unique-key kernel handed back).
 template <typename KT, bool HVS, int CAP, int T>
 __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const uint32_t* __restrict__ item_job,
                                              uint32_t n_items, const Row<KT, true>* __restrict__ R,
                                              const Row<KT, HVS>* __restrict__ S, const uint32_t* __restrict__ bndR,
                                              const uint32_t* __restrict__ bndS, uint32_t* __restrict__ queue,
-                                             unsigned long long* __restrict__ res) {
+                                             unsigned long long* __restrict__ res,
+                                             const uint32_t* __restrict__ ilist, const uint32_t* __restrict__ ilist_n) {
     constexpr int L = Log2<CAP>::v;
     constexpr uint32_t PIECE = CAP / 2;  // load factor <= 0.5
     constexpr bool W = sizeof(KT) == 8;
@@ -392,11 +395,12 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
     __shared__ uint32_t s_item;
     const int tid = threadIdx.x;
     const unsigned short* tc16 = (const unsigned short*)tc;  // little-endian halves of tc[]
+    const uint32_t n_run = ilist ? *ilist_n : n_items;
     while (true) {
         if (tid == 0) s_item = atomicAdd(queue, 1u);
         __syncthreads();
-        const uint32_t item = s_item;
-        if (item >= n_items) break;
+        if (s_item >= n_run) break;
+        const uint32_t item = ilist ? ilist[s_item] : s_item;
         const uint32_t j = item_job[item];
         const Job jb = jobs[j];
         const uint32_t K = jb.split > 1 ? jb.split : 1u;
@@ -626,6 +630,223 @@ __global__ __launch_bounds__(T) void k_bjoin(const Job* __restrict__ jobs, const
     }
 }
 
+// Unique-key LDS join for 4-byte keys (variant 3): slots of (key, build
+// payload) in 8 bytes, kCap8 of them in 64 KB, so two 512-thread workgroups
+// share a CU and one's table build overlaps the other's probe loads.  A
+// partition whose build rows repeat a key, or number more than kPiece8, is
+// appended to `redo` (count in queue[1]) and left to k_bjoin, which runs
+// after this kernel over exactly those items; nothing of it is counted here.
+// Results per job as k_bjoin: res[4 j + 0..2] = matches, sum of matched
+// build payloads, sum of probe payload x matches.
+constexpr int kCap8 = 8192;
+constexpr uint32_t kPiece8 = 6144;  // load factor <= 0.75
+constexpr int kT8 = 512;
+
+template <bool HVS>
+__global__ __launch_bounds__(kT8, 4) void k_bjoin8(const Job* __restrict__ jobs, const uint32_t* __restrict__ item_job,
+                                                uint32_t n_items, const Row<int32_t, true>* __restrict__ R,
+                                                const Row<int32_t, HVS>* __restrict__ S,
+                                                const uint32_t* __restrict__ bndR, const uint32_t* __restrict__ bndS,
+                                                uint32_t* __restrict__ queue, uint32_t* __restrict__ redo,
+                                                unsigned long long* __restrict__ res) {
+    constexpr int T = kT8, CAP = kCap8, L = Log2<kCap8>::v;
+    constexpr int32_t EMPTY = INT32_MIN;
+    using SR = Row<int32_t, HVS>;
+    constexpr int U = 3;                                // probe rows per thread per batch (two batches in flight)
+    constexpr int RB = (kPiece8 + T - 1) / T;           // build rows per thread
+    __shared__ int2 tab[CAP];                           // (key, build payload); empty: (EMPTY, 0)
+    __shared__ unsigned long long sp_cnt, sp_sum;       // build rows keyed by the empty marker
+    __shared__ unsigned long long red[3][T / 64];
+    __shared__ uint32_t s_item;
+    __shared__ int s_dup;
+    const int tid = threadIdx.x;
+    while (true) {
+        if (tid == 0) {
+            s_item = atomicAdd(&queue[0], 1u);
+            s_dup = 0;
+            sp_cnt = sp_sum = 0;
+        }
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= n_items) break;
+        const uint32_t j = item_job[item];
+        const Job jb = jobs[j];
+        const uint32_t K = jb.split > 1 ? jb.split : 1u;
+        const uint32_t d = (item - jb.item0) / K, ks = (item - jb.item0) % K;
+        const uint64_t rb = jb.rdst + bndR[jb.rbnd + d];
+        const uint32_t nR = bndR[jb.rbnd + d + 1] - bndR[jb.rbnd + d];
+        const uint64_t s0 = bndS[jb.sbnd + d], sl = bndS[jb.sbnd + d + 1] - s0;
+        const uint64_t sb = jb.sdst + s0 + sl * ks / K;
+        const uint64_t se = jb.sdst + s0 + sl * (ks + 1) / K;
+        if (!nR || se <= sb) {  // nothing matches
+            __syncthreads();
+            continue;
+        }
+        if (nR > kPiece8) {  // more than one table's worth of build rows: the general kernel
+            if (tid == 0) redo[atomicAdd(&queue[1], 1u)] = item;
+            __syncthreads();
+            continue;
+        }
+        const int sh = 32 - (int)jb.bits - L;
+        auto slot_of = [&](int32_t key) -> uint32_t {
+            const uint64_t kk = (uint64_t)(int64_t)key;
+            const uint32_t lo = (uint32_t)kk, hi = (uint32_t)(kk >> 32);
+            const uint32_t h = __umulhi(lo, (uint32_t)kFib) + lo * (uint32_t)(kFib >> 32) + hi * (uint32_t)kFib;
+            return (h >> sh) & (CAP - 1);
+        };
+        auto rows_in = [&](uint64_t b0) -> uint32_t {
+            const uint64_t r0 = b0 + tid;
+            return r0 < se ? (uint32_t)min<uint64_t>(U, (se - r0 + T - 1) / T) : 0u;
+        };
+        auto batch_rsrc = [&](uint64_t b0) {
+            const uint64_t left = b0 < se ? (se - b0) * sizeof(SR) : 0;
+            return buf_rsrc(S + b0, left < 0xFFFFFFF0ull ? (uint32_t)left : 0xFFFFFFF0u);
+        };
+        auto fetch_clamped = [&](SR(&z)[U], uint64_t b0) {
+            const __amdgpu_buffer_rsrc_t rs = batch_rsrc(b0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) z[u] = buf_load<SR, true>(rs, (uint32_t)(tid + u * T) * (uint32_t)sizeof(SR));
+        };
+        // build rows and the first probe batch are in flight while the table is cleared
+        Row<int32_t, true> br[RB];
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+            if ((uint32_t)(tid + b * T) < nR) br[b] = nt_load(&R[rb + tid + b * T]);
+        SR x[U];
+        fetch_clamped(x, sb);
+        for (int i = tid; i < CAP; i += T) tab[i] = make_int2(EMPTY, 0);
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            if ((uint32_t)(tid + b * T) >= nR) continue;
+            const Row<int32_t, true> r = br[b];
+            if (r.k == EMPTY) {
+                atomicAdd(&sp_cnt, 1ull);
+                atomicAdd(&sp_sum, (unsigned long long)(int64_t)r.v);
+                continue;
+            }
+            uint32_t slot = slot_of(r.k);
+            while (true) {
+                const int prev = atomicCAS(&tab[slot].x, EMPTY, r.k);
+                if (prev == EMPTY) {
+                    tab[slot].y = r.v;  // the slot is this row's alone
+                    break;
+                }
+                if (prev == r.k) {  // a repeated key: the general kernel takes the partition
+                    s_dup = 1;
+                    break;
+                }
+                slot = (slot + 1) & (CAP - 1);
+            }
+        }
+        __syncthreads();
+        if (s_dup) {
+            if (tid == 0) redo[atomicAdd(&queue[1], 1u)] = item;
+            __syncthreads();
+            continue;
+        }
+        unsigned long long acc_c = 0, acc_b = 0, acc_p = 0, pe_n = 0, pe_v = 0;
+        // One batch: every walk ends on the key's slot or on an empty one
+        // (payload 0); a hit is key equality.  A probe key equal to the empty
+        // marker "hits" an empty slot: such rows are taken back out and
+        // matched against the build rows of that key (sp_cnt, sp_sum).
+        auto probe_batch = [&](auto full_tag, const SR(&z)[U], uint32_t cnt) {
+            constexpr bool FULL = decltype(full_tag)::value;
+            uint32_t slot[U];
+            int2 t[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                slot[u] = slot_of(z[u].k);
+                t[u] = tab[slot[u]];
+            }
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                any |= (FULL || (uint32_t)u < cnt) && t[u].x != z[u].k && t[u].x != EMPTY;
+            if (any) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (!FULL && (uint32_t)u >= cnt) continue;
+                    while (t[u].x != z[u].k && t[u].x != EMPTY) {
+                        slot[u] = (slot[u] + 1) & (CAP - 1);
+                        t[u] = tab[slot[u]];
+                    }
+                }
+            }
+            uint32_t bc = 0;
+            long long bs = 0, bp = 0;
+            bool em = false;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool live = FULL || (uint32_t)u < cnt;
+                const bool hit = live && t[u].x == z[u].k;
+                bc += hit ? 1u : 0u;
+                bs += live ? (long long)t[u].y : 0ll;
+                if constexpr (HVS) bp += hit ? (long long)z[u].v : 0ll;
+                em |= live && z[u].k == EMPTY;
+            }
+            acc_c += bc;
+            acc_b += (unsigned long long)bs;
+            acc_p += (unsigned long long)bp;
+            if (__any(em)) {  // rare: a wavefront holds a probe key equal to the empty marker
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if ((FULL || (uint32_t)u < cnt) && z[u].k == EMPTY) {
+                        acc_c -= 1;  // its "hit" on an empty slot (payload 0)
+                        pe_n += 1;
+                        if constexpr (HVS) {
+                            acc_p -= (unsigned long long)(int64_t)z[u].v;
+                            pe_v += (unsigned long long)(int64_t)z[u].v;
+                        }
+                    }
+                }
+            }
+        };
+        auto probe = [&](const SR(&z)[U], uint64_t b0) {
+            if (b0 + (uint64_t)U * T <= se)
+                probe_batch(std::true_type{}, z, (uint32_t)U);
+            else
+                probe_batch(std::false_type{}, z, rows_in(b0));
+        };
+        SR y[U];
+        for (uint64_t b0 = sb; b0 < se; b0 += 2ull * U * T) {
+            fetch_clamped(y, b0 + (uint64_t)U * T);
+            probe(x, b0);
+            fetch_clamped(x, b0 + 2ull * U * T);
+            if (b0 + (uint64_t)U * T < se) probe(y, b0 + (uint64_t)U * T);
+        }
+        const unsigned long long spc = sp_cnt, sps = sp_sum;
+        acc_c += pe_n * spc;
+        acc_b += pe_n * sps;
+        acc_p += pe_v * spc;
+        acc_c = wave_sum(acc_c);
+        acc_b = wave_sum(acc_b);
+        acc_p = wave_sum(acc_p);
+        const int lane = tid & 63, w = tid >> 6;
+        if (lane == 0) {
+            red[0][w] = acc_c;
+            red[1][w] = acc_b;
+            red[2][w] = acc_p;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long cc = 0, bb = 0, pp = 0;
+            for (int k = 0; k < T / 64; ++k) {
+                cc += red[0][k];
+                bb += red[1][k];
+                pp += red[2][k];
+            }
+            unsigned long long* o = res + 4 * (uint64_t)j;
+            if (cc) {
+                atomicAdd(&o[0], cc);
+                atomicAdd(&o[1], bb);
+                atomicAdd(&o[2], pp);
+            }
+        }
+        __syncthreads();  // red and the shared item words are free for the next item
+    }
+}
+
 __global__ __launch_bounds__(256) void k_bzero(unsigned long long* __restrict__ p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = 0;
 }
@@ -635,7 +856,27 @@ int launch_join_t(hipStream_t st, int grid, const Job* jobs, const uint32_t* ite
                   const void* R, const void* S, const uint32_t* bndR, const uint32_t* bndS, uint32_t* queue,
                   unsigned long long* res) {
     hipLaunchKernelGGL((k_bjoin<KT, HVS, CAP, T>), dim3(grid), dim3(T), 0, st, jobs, item_job, n_items,
-                       (const Row<KT, true>*)R, (const Row<KT, HVS>*)S, bndR, bndS, queue, res);
+                       (const Row<KT, true>*)R, (const Row<KT, HVS>*)S, bndR, bndS, queue, res, nullptr, nullptr);
+    PGPU_LAUNCH_CHECK();
+    return PGPU_OK;
+}
+
+// Variant 3: the unique-key kernel over every item, then variant 2's kernel
+// over the items it handed back (one workgroup per CU; its count is read on
+// the device).  queue[0..2]: item queue, handed-back count, their queue.
+template <bool HVS>
+int launch_join8_t(hipStream_t st, int grid, const Job* jobs, const uint32_t* item_job, uint32_t n_items,
+                   const void* R, const void* S, const uint32_t* bndR, const uint32_t* bndS, uint32_t* queue,
+                   uint32_t* redo, unsigned long long* res) {
+    int dev = 0, cus = 0;
+    PGPU_HIP(hipGetDevice(&dev));
+    PGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL(k_bjoin8<HVS>, dim3(grid), dim3(kT8), 0, st, jobs, item_job, n_items,
+                       (const Row<int32_t, true>*)R, (const Row<int32_t, HVS>*)S, bndR, bndS, queue, redo, res);
+    PGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_bjoin<int32_t, HVS, 8192, 1024>), dim3(std::min<uint32_t>(cus, n_items)), dim3(1024), 0, st,
+                       jobs, item_job, n_items, (const Row<int32_t, true>*)R, (const Row<int32_t, HVS>*)S, bndR, bndS,
+                       queue + 2, res, redo, queue + 1);
     PGPU_LAUNCH_CHECK();
     return PGPU_OK;
 }
@@ -675,6 +916,7 @@ void Layout::add(const void* key, const void* pay, uint64_t n, uint32_t bits, ui
 
 int variant_piece(int variant) { return variant == 1 ? 2048 : 4096; }
 uint32_t variant_target(int variant) { return variant == 1 ? 1536 : 3072; }
+bool variant_ok(int variant, int key_bytes) { return variant == 1 || variant == 2 || (variant == 3 && key_bytes == 4); }
 
 uint32_t bits_for(uint64_t n, uint32_t target) {
     if (n <= target) return 0;
@@ -754,8 +996,13 @@ int launch_scatter(hipStream_t st, int kb, bool hv, uint32_t max_bits, const Seg
 
 int launch_join(hipStream_t st, int kb, bool hvs, int variant, int grid, const Job* jobs, const uint32_t* item_job,
                 uint32_t n_items, const void* R, const void* S, const uint32_t* bndR, const uint32_t* bndS,
-                uint32_t* queue, unsigned long long* res) {
+                uint32_t* queue, unsigned long long* res, uint32_t* redo) {
     if (!n_items) return PGPU_OK;
+    if (variant == 3) {
+        if (kb != 4 || !redo) return fail(PGPU_EINVAL, "join variant 3 needs 4-byte keys and a hand-back list");
+        return hvs ? launch_join8_t<true>(st, grid, jobs, item_job, n_items, R, S, bndR, bndS, queue, redo, res)
+                   : launch_join8_t<false>(st, grid, jobs, item_job, n_items, R, S, bndR, bndS, queue, redo, res);
+    }
 #define X(KT, HV, V, CAP, T)                                                                                      \
     if ((kb == 8) == (sizeof(KT) == 8) && hvs == HV && variant == V)                                              \
         return launch_join_t<KT, HV, CAP, T>(st, grid, jobs, item_job, n_items, R, S, bndR, bndS, queue, res);
@@ -768,6 +1015,13 @@ int join_grid(int kb, bool hvs, int variant, int* grid) {
     int dev = 0, cus = 0, blocks = 0;
     PGPU_HIP(hipGetDevice(&dev));
     PGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (variant == 3 && kb == 4) {
+        const void* f = hvs ? (const void*)k_bjoin8<true> : (const void*)k_bjoin8<false>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, kT8, 0) != hipSuccess)
+            return fail(PGPU_EDEVICE, "occupancy query failed");
+        *grid = cus * std::max(blocks, 1);
+        return PGPU_OK;
+    }
 #define X(KT, HV, V, CAP, T)                                             \
     if ((kb == 8) == (sizeof(KT) == 8) && hvs == HV && variant == V) { \
         PGPU_TRY((occupancy_t<KT, HV, CAP, T>(&blocks)));                \

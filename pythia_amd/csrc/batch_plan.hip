@@ -26,7 +26,8 @@ struct pgpu_batch_plan {
     void* SP = nullptr;
     Job* jobs = nullptr;
     uint32_t* item_job = nullptr;
-    unsigned long long* res = nullptr;  // [n][4] then the queue word
+    unsigned long long* res = nullptr;  // [n][4] then two words of queue counters (three used)
+    uint32_t* redo = nullptr;           // variant 3: partitions handed to the general join kernel
     unsigned long long* h_res = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -56,7 +57,7 @@ int enqueue(pgpu_batch_plan* p, bool timed) {
     const int kb = (int)p->cfg.key_bytes;
     const bool hvs = p->cfg.probe_payload != 0;
     const uint32_t nbc = (uint32_t)p->lr.chunks.size(), nsc = (uint32_t)p->ls.chunks.size();
-    const uint64_t words = 4ull * p->n + 1;
+    const uint64_t words = 4ull * p->n + 2;
     auto timed_run = [&](const char* name, auto&& fn) -> int {
         if (!tm) return fn();
         Timed t(tm, name);
@@ -72,7 +73,7 @@ int enqueue(pgpu_batch_plan* p, bool timed) {
                        [&] { return launch_scatter(st, kb, hvs, mb, p->segs, p->chunks, nbc, nsc, p->H, p->SP); }));
     PGPU_TRY(timed_run("batch_join", [&] {
         return launch_join(st, kb, hvs, (int)p->cfg.variant, p->grid, p->jobs, p->item_job, p->n_items, p->RP, p->SP,
-                           p->bnd, p->bnd + p->lr.bnd, (uint32_t*)(p->res + 4ull * p->n), p->res);
+                           p->bnd, p->bnd + p->lr.bnd, (uint32_t*)(p->res + 4ull * p->n), p->res, p->redo);
     }));
     PGPU_HIP(hipMemcpyAsync(p->h_res, p->res, 4ull * p->n * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     return PGPU_OK;
@@ -90,8 +91,10 @@ int pgpu_batch_plan_create(pgpu_ctx* ctx, const pgpu_batch_cfg* cfg, const pgpu_
         return fail(PGPU_EUNSUPPORTED, "batched join keys must be 4 or 8 bytes (got %u)", cfg->key_bytes);
     if (n == 0) return fail(PGPU_EINVAL, "pgpu_batch_plan_create: empty batch");
     pgpu_batch_cfg c = *cfg;
-    if (c.variant == 0) c.variant = 2;
-    if (c.variant != 1 && c.variant != 2) return fail(PGPU_EINVAL, "unknown join variant %u", c.variant);
+    if (c.variant == 0) c.variant = c.key_bytes == 4 ? 3 : 2;
+    if (!variant_ok((int)c.variant, (int)c.key_bytes))
+        return fail(PGPU_EINVAL, "join variant %u with %u-byte keys: unknown (variants 1 and 2 take any keys, 3 "
+                    "takes 4-byte keys)", c.variant, c.key_bytes);
     if (c.part_rows == 0) c.part_rows = variant_target((int)c.variant);
     if (c.part_rows > (uint32_t)variant_piece((int)c.variant))
         return fail(PGPU_EINVAL, "part_rows %u exceeds the LDS piece of variant %u (%d rows)", c.part_rows,
@@ -175,7 +178,8 @@ int pgpu_batch_plan_create(pgpu_ctx* ctx, const pgpu_batch_cfg* cfg, const pgpu_
         {&p->SP, p->ls.rows * esS},
         {(void**)&p->jobs, n * sizeof(Job)},
         {(void**)&p->item_job, item_job.size() * 4},
-        {(void**)&p->res, (4ull * n + 1) * 8},
+        {(void**)&p->res, (4ull * n + 2) * 8},
+        {(void**)&p->redo, c.variant == 3 ? (uint64_t)p->n_items * 4 : 0},
     };
     for (auto& x : a)
         if ((rc = balloc(p, x.bytes, x.ptr)) != PGPU_OK) return cleanup(rc);
@@ -289,7 +293,10 @@ int pgpu_batch_autotune(pgpu_ctx* ctx, const pgpu_batch_cfg* base, const pgpu_ba
     struct Cand {
         uint32_t variant, part_rows;
     };
-    const Cand cands[] = {{1, 1024}, {1, 1536}, {2, 2048}, {2, 3072}, {2, 4096}};
+    const Cand all[] = {{1, 1024}, {1, 1536}, {2, 2048}, {2, 3072}, {2, 4096}, {3, 4096}};
+    std::vector<Cand> cands;
+    for (const Cand& cd : all)
+        if (variant_ok((int)cd.variant, (int)base->key_bytes)) cands.push_back(cd);
     PGPU_TRY(ctx_activate(ctx));
     hipEvent_t e0, e1;
     PGPU_HIP(hipEventCreate(&e0));

@@ -44,7 +44,8 @@ def _check(res, ref):
     assert got == exp
 
 
-@pytest.mark.parametrize("variant,part_rows,graph", [(1, 0, True), (2, 0, True), (2, 1024, False), (1, 2048, True)])
+@pytest.mark.parametrize("variant,part_rows,graph", [(1, 0, True), (2, 0, True), (2, 1024, False), (1, 2048, True),
+                                                     (3, 0, True), (3, 4096, False)])
 def test_batch_c4_family_vs_closed_form(ctx, variant, part_rows, graph):
     sizes = pa.sizes(4, 40, 10, 17)
     joins, bufs = _c4_batch(ctx, 0, sizes)
@@ -123,12 +124,73 @@ def test_batch_random_keys_vs_oracle(ctx, kb):
             mask = (1 << (8 * kb)) - 1
             assert csb.count == exp[-1][0]
             assert csb.isum & mask == exp[-1][1] & mask and csp.isum & mask == exp[-1][2] & mask
-    for variant in (1, 2):
+    # variant 3 (4-byte keys) hands the partitions with repeated build keys
+    # (all of them here) and the hot key's oversized one to variant 2's kernel
+    for variant in (1, 2, 3) if kb == 4 else (1, 2):
         plan = pa.BatchPlan(ctx, joins, key_bytes=kb, probe_payload=True, variant=variant)
         res = plan.run()
         assert [(r.matches, r.sum_build, r.sum_probe) for r in res] == exp
         assert res[4].lds_passes > 1
         plan.close()
+    _free(bufs)
+
+
+def test_batch_unique_keys_variant3_vs_exact(ctx):
+    """Unique build keys (variant 3's 8-byte slots) with the minimum integer
+    as a real key on both sides, probe keys that miss, a partition over
+    6144 rows (handed to variant 2's kernel), an empty probe side and a join
+    small enough for one partition."""
+    rng = np.random.default_rng(33)
+    lo = np.iinfo(np.int32).min
+    specs = []
+    for nR, nS in [(200000, 900000), (50000, 0), (3000, 40000), (7000, 7000)]:
+        bk = rng.permutation(np.arange(-nR, nR, 2, dtype=np.int64))[:nR].astype(np.int32)
+        if nR > 10:
+            bk[3] = lo
+        pk = rng.choice(bk, nS) if nS else np.zeros(0, np.int32)
+        if nS > 100:
+            pk[:50] = lo
+            pk[50:90] += 1  # odd keys: no match
+        bp = rng.integers(-2 ** 30, 2 ** 30, nR).astype(np.int32)
+        pp = rng.integers(-2 ** 30, 2 ** 30, nS).astype(np.int32)
+        specs.append((bk, bp, pk.astype(np.int32), pp))
+    joins, bufs, exp = [], [], []
+    for bk, bp, pk, pp in specs:
+        arrs = []
+        for a in (bk, bp, pk, pp):
+            buf = ctx.alloc(max(a.nbytes, 16))
+            if len(a):
+                buf.upload(a)
+            arrs.append(buf)
+            bufs.append(buf)
+        joins.append((arrs[0], arrs[1], len(bk), arrs[2], arrs[3], len(pk)))
+        exp.append(_exact(bk, bp, pk, pp))
+    for variant, part_rows in ((3, 0), (3, 4096), (2, 0)):
+        plan = pa.BatchPlan(ctx, joins, key_bytes=4, probe_payload=True, variant=variant, part_rows=part_rows)
+        res = plan.run()
+        assert [(r.matches, r.sum_build, r.sum_probe) for r in res] == exp
+        plan.close()
+    # one partition of more than 6144 rows: 8000 build rows take 1 bit at 4096 rows per partition, and these
+    # keys all hash to partition 0 (the top bit of key x 0x9E3779B97F4A7C15 is clear)
+    cand = np.arange(0, 40000, dtype=np.int64)
+    with np.errstate(over="ignore"):
+        top = (cand.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) >> np.uint64(63)
+    bk = rng.permutation(cand[top == 0][:8000]).astype(np.int32)
+    assert len(bk) == 8000
+    pk = rng.choice(bk, 30000).astype(np.int32)
+    bp = rng.integers(-1000, 1000, 8000).astype(np.int32)
+    pp = rng.integers(-1000, 1000, 30000).astype(np.int32)
+    arrs = []
+    for a in (bk, bp, pk, pp):
+        buf = ctx.alloc(a.nbytes)
+        buf.upload(a)
+        arrs.append(buf)
+        bufs.append(buf)
+    plan = pa.BatchPlan(ctx, [(arrs[0], arrs[1], 8000, arrs[2], arrs[3], 30000)], key_bytes=4, probe_payload=True,
+                        variant=3, part_rows=4096)
+    res = plan.run()
+    assert [(r.matches, r.sum_build, r.sum_probe) for r in res] == [_exact(bk, bp, pk, pp)]
+    plan.close()
     _free(bufs)
 
 
@@ -156,7 +218,7 @@ def test_batch_autotune_picks_a_candidate_with_same_results(ctx):
     sizes = pa.sizes(4, 24, 10, 18, first=200)
     joins, bufs = _c4_batch(ctx, 200, sizes)
     best, table = pa.batch_autotune(ctx, joins, key_bytes=4, probe_payload=True, reps=1)
-    assert len(table) == 5 and best in [(v, r) for v, r, _ in table]
+    assert len(table) == 6 and best in [(v, r) for v, r, _ in table]
     assert min(table, key=lambda e: e[2])[:2] == best
     plan = pa.BatchPlan(ctx, joins, variant=best[0], part_rows=best[1])
     _check(plan.run(), O.c4_closed_many(200, sizes, 8 * sizes, threads=THREADS))
@@ -171,6 +233,8 @@ def test_batch_rejects_bad_arguments(ctx):
         pa.BatchPlan(ctx, j, key_bytes=2)
     with pytest.raises(pa.PgpuError):
         pa.BatchPlan(ctx, j, variant=1, part_rows=4096)  # larger than that variant's LDS piece
+    with pytest.raises(pa.PgpuError):
+        pa.BatchPlan(ctx, j, key_bytes=8, variant=3)  # the 8-byte slots hold 4-byte keys
     with pytest.raises(pa.PgpuError):
         pa.BatchPlan(ctx, [(None, b, 10, b, b, 10)])
     with pytest.raises(pa.PgpuError):
