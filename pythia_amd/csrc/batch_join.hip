@@ -92,10 +92,18 @@ __global__ __launch_bounds__(512) void k_bhist(const Seg* __restrict__ segs, con
 constexpr int kColBins = 64;
 constexpr int kColWaves = 16;
 
+// Sum of column d over chunks [c0, c1): 32 loads in flight per step (a
+// large segment's column is ~128 chunks per wave).
 __device__ __forceinline__ uint32_t col_sum(const uint32_t* h, uint32_t nd, uint32_t d, uint32_t c0, uint32_t c1) {
+    constexpr int B = 32;
     uint32_t t = 0;
-#pragma unroll 8
-    for (uint32_t c = c0; c < c1; ++c) t += h[(uint64_t)c * nd + d];
+    for (uint32_t cb = c0; cb < c1; cb += B) {
+        uint32_t v[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) v[j] = cb + j < c1 ? h[(uint64_t)(cb + j) * nd + d] : 0u;
+#pragma unroll
+        for (int j = 0; j < B; ++j) t += v[j];
+    }
     return t;
 }
 
@@ -158,7 +166,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_bapply(const Seg* __restrict
     if (d >= nd) return;
     uint32_t run = bnd[s.bnd + d];
     for (int q = 0; q < w; ++q) run += part[q][lane];
-    constexpr int B = 8;  // chunks per step: their counts are loaded before the first store
+    constexpr int B = 16;  // chunks per step: their counts are loaded before the first store
     for (uint32_t cb = c0; cb < c1; cb += B) {
         uint32_t v[B];
 #pragma unroll
