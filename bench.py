@@ -89,6 +89,17 @@ def kernel_bytes_c2(nR: int, nS: int, esR: int = 16, esS: int = 8, kb: int = 8) 
     }
 
 
+def kernel_bytes_onepass(nR: int, nS: int, esR: int, esS: int, kb: int) -> dict:
+    """Algorithmic HBM bytes of one launch of each kernel of the one-pass join
+    (one radix pass, the batched-join kernels)."""
+    return {
+        "hist": (nR + nS) * kb,                # one launch over both sides' key columns
+        "scatter_build": 2 * nR * esR,
+        "scatter_probe": 2 * nS * esS,
+        "join_lds": nR * esR + nS * esS + 16,
+    }
+
+
 def roofline(per_kernel: dict, kb: dict, traffic_file: str | None):
     """Roofline of the dominant kernel.  `traffic` (HBM bytes per launch) is
     not counted in this run: it is read from a committed summary of two
@@ -408,8 +419,8 @@ def run_c1(args):
         ctx.gen(name, n, b, seed=args.seed, n_build=nR)
         cols[name] = b
     ctx.sync()
-    # device-resident columns: the fused count + sum
-    plan = pa.JoinPlan(ctx, 4, nR, nS, probe_payload=True, algo=pa.ALGO_RADIX)
+    # device-resident columns: the fused count + sum, the algorithm the operators' plan takes (one radix pass)
+    plan = pa.JoinPlan(ctx, 4, nR, nS, probe_payload=True, algo=pa.ALGO_AUTO)
     for _ in range(max(args.warmup, 1)):
         dres = plan.count_sum(cols["C1_RKEY"], cols["C1_RPAY"], nR, cols["C1_SFK"], cols["C1_SPAY"], nS)
     t0 = time.perf_counter()
@@ -496,7 +507,8 @@ def run_c1(args):
         "device_columns": {"ms_per_join": dev_ms, "tuples_per_s": (nR + nS) / (dev_ms / 1e3),
                            "matches": dres.matches, "sum_build": dres.sum_build, "kernels": dev_kernels},
         # the device-column join's kernels (the operator path's time is host-side page traffic)
-        "roofline": dict(roofline(dev_kernels, kernel_bytes_c2(nR, nS, esR=8, esS=8, kb=4), args.traffic),
+        "roofline": dict(roofline(dev_kernels, {**kernel_bytes_c2(nR, nS, esR=8, esS=8, kb=4),
+                                                **kernel_bytes_onepass(nR, nS, esR=8, esS=8, kb=4)}, args.traffic),
                          of="device_columns join"),
         "job_roofline": {"compulsory_bytes": nR * 8 + nS * 8 + 16,
                          "achieved_GBps": (nR * 8 + nS * 8 + 16) / (dev_ms / 1e3) / 1e9,

@@ -121,9 +121,13 @@ int pgpu_sizes(uint64_t seed, uint64_t sub, uint64_t first, uint64_t count, doub
 /* 37-107) with no group-by (`fields = ()`).                            */
 /* ------------------------------------------------------------------ */
 enum pgpu_join_algo {
-    PGPU_ALGO_AUTO = 0,
+    PGPU_ALGO_AUTO = 0,   /* RADIX1 for builds of 2^16 to 2^22 rows (plans    */
+                          /* of at most 2^22 build rows), GLOBAL below, else  */
+                          /* RADIX                                            */
     PGPU_ALGO_RADIX = 1,  /* 2-pass radix partitioning + LDS tables         */
-    PGPU_ALGO_GLOBAL = 2  /* one open-addressing table in HBM (no partitions) */
+    PGPU_ALGO_GLOBAL = 2, /* one open-addressing table in HBM (no partitions) */
+    PGPU_ALGO_RADIX1 = 3  /* one radix pass of <= 10 bits (partitions of     */
+                          /* ~4096 build rows) + the batched-join LDS kernels */
 };
 
 typedef struct pgpu_join_cfg {

@@ -13,10 +13,10 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import ALGO_AUTO, ALGO_GLOBAL, ALGO_RADIX, GEN, PgpuError, call, lib  # noqa: F401
+from ._lib import ALGO_AUTO, ALGO_GLOBAL, ALGO_RADIX, ALGO_RADIX1, GEN, PgpuError, call, lib  # noqa: F401
 
 __all__ = ["Context", "DeviceBuffer", "JoinPlan", "GroupJoin", "Partition", "BatchPlan", "batch_autotune", "Server",
-           "c5_trace", "device_count", "PgpuError", "GEN", "ALGO_AUTO", "ALGO_RADIX", "ALGO_GLOBAL",
+           "c5_trace", "device_count", "PgpuError", "GEN", "ALGO_AUTO", "ALGO_RADIX", "ALGO_GLOBAL", "ALGO_RADIX1",
            "SERVE_BATCHED", "SERVE_PLAIN"]
 
 SERVE_BATCHED, SERVE_PLAIN = 0, 1

@@ -22,7 +22,7 @@ GEN = {name: i for i, name in enumerate([
     "C3_D_ORDERKEY", "C3_D_CUSTKEY", "C3_F_ORDERKEY", "C3_F_PARTKEY", "C3_F_QUANTITY", "C3_F_PRICE",
     "C4_RKEY", "C4_RPAY", "C4_SFK", "C4_SPAY"], start=1)}
 
-ALGO_AUTO, ALGO_RADIX, ALGO_GLOBAL = 0, 1, 2
+ALGO_AUTO, ALGO_RADIX, ALGO_GLOBAL, ALGO_RADIX1 = 0, 1, 2, 3
 
 
 class GenSpec(C.Structure):

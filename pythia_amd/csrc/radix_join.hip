@@ -41,6 +41,7 @@
 #include <type_traits>
 #include <cstring>
 
+#include "batch_internal.h"
 #include "join_common.h"
 #include "pgpu_internal.h"
 
@@ -65,6 +66,12 @@ constexpr int kTileBytes = 77824;         // LDS staging tile of the pass-2 scat
                                           // half the units spilled a few rows into a second tile)
 constexpr int kDigits = 128;              // digits of one pass: at most 7 of the <= 14 radix bits
 constexpr uint64_t kForkMinProbe = 1ull << 26;  // probe rows from which the build side runs on a second stream
+// One-pass route: one radix pass of at most 10 bits (partitions of ~4096
+// build rows, 128 KB scatter tiles) and the batched-join LDS kernels.
+constexpr uint32_t kOnePassPartRows = 4096;
+constexpr uint64_t kOnePassMaxBuild = 1ull << 22;  // 1024 partitions
+constexpr uint32_t kOnePassMaxItems = 1u << 10;
+constexpr uint64_t kOpSeg = 0, kOpChunk = 256;     // table offsets in the one-pass device / staging block
 constexpr int kTileBytes1 = 65536;        // pass-1 tile (one workgroup each, two per CU); a multiple of the
                                           // histogram's 4096-row step for rows of at most 16 bytes
 constexpr int kPartSlots = 2048;          // partial-result slots of the join kernels
@@ -910,6 +917,16 @@ struct pgpu_join_plan {
     uint32_t *pxR = nullptr, *pxS = nullptr;  // per-(block, p1) prefixes of k_colscan (2 x kMaxBlocks x kDigits)
     uint32_t *toffR = nullptr, *toffS = nullptr;  // per-(pass-1 tile, p1) offsets within the tile's block
     unsigned long long* result = nullptr;  // [4]: count, sum_build, sum_probe, max pieces
+    // one-pass route (PGPU_ALGO_RADIX1, plans of at most kOnePassMaxBuild build
+    // rows): the batched-join kernels over one build and one probe segment
+    void* opdev = nullptr;                 // segs, chunks, job, item -> job, at kOnePass* offsets
+    char* ophost = nullptr;                // pinned staging of the same tables
+    uint32_t* opH = nullptr;               // per-chunk histograms
+    uint32_t* opbnd = nullptr;             // partition bounds of both sides
+    unsigned long long* opres = nullptr;   // [4] result words, [2] queue words
+    uint32_t* opredo = nullptr;            // partitions handed back by the unique-key join
+    uint64_t opchunks = 0;                 // chunk capacity
+    int opgrid = 0;                        // persistent join workgroups
     unsigned long long* part = nullptr;    // [kPartSlots][4] per-workgroup partials
     JoinItem* items = nullptr;             // one record per join work item
     uint64_t max_items = 0;
@@ -1130,6 +1147,72 @@ int run_global(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const
     return launch_global<KT, HVS>(p->ctx->stream, p->timer, p->gtable, p->gspecial, p->part, rk, rv, nR, sk, sv, nS);
 }
 
+uint64_t onepass_job_off(uint64_t chunks) { return kOpChunk + ((chunks * sizeof(pgpu::batch::Chunk) + 255) & ~255ull); }
+
+// PGPU_ALGO_RADIX1: both sides as one segment each of the batched plans'
+// one-pass partitioning (chunks of one scatter tile, exact per-tile
+// offsets), then the persistent LDS join (variant 3: unique-key 8-byte slots
+// with the general kernel for the partitions it hands back; variant 2 for
+// 8-byte keys).  Results land in opres[0..3].
+template <typename KT, bool HVS>
+int run_onepass(pgpu_join_plan* p, const KT* rk, const KT* rv, uint64_t nR, const KT* sk, const KT* sv, uint64_t nS) {
+    using namespace pgpu::batch;
+    hipStream_t st = p->ctx->stream;
+    constexpr int kb = (int)sizeof(KT);
+    const uint32_t bits = bits_for(nR, kOnePassPartRows);
+    if (nR > kOnePassMaxBuild || bits > 10) return pgpu::fail(PGPU_EINVAL, "one-pass join: build side too large");
+    Layout lr, ls;
+    lr.add(rk, rv, nR, bits, 0, scatter_tile_rows(kb, true, bits));
+    const uint32_t nbc = (uint32_t)lr.chunks.size();
+    ls.add(sk, HVS ? sv : nullptr, nS, bits, nbc, scatter_tile_rows(kb, HVS, bits));
+    const uint32_t nsc = (uint32_t)ls.chunks.size();
+    for (auto& ch : ls.chunks) ch.seg = 1;
+    ls.segs[0].hist += lr.hist;
+    ls.segs[0].bnd += lr.bnd;
+    if ((uint64_t)nbc + nsc > p->opchunks) return pgpu::fail(PGPU_EINVAL, "one-pass join: chunks exceed the plan");
+    Job jb{};
+    jb.bits = bits;  // both sides' rows and bounds start at 0; one item per partition
+    const uint32_t nitems = 1u << bits;
+    // the staging block may still feed the previous call's copy
+    if (p->pending) PGPU_HIP(hipStreamSynchronize(st));
+    char* h = p->ophost;
+    const uint64_t job_off = onepass_job_off(p->opchunks), item_off = job_off + 256;
+    memcpy(h + kOpSeg, &lr.segs[0], sizeof(Seg));
+    memcpy(h + kOpSeg + sizeof(Seg), &ls.segs[0], sizeof(Seg));
+    memcpy(h + kOpChunk, lr.chunks.data(), nbc * sizeof(Chunk));
+    memcpy(h + kOpChunk + nbc * sizeof(Chunk), ls.chunks.data(), nsc * sizeof(Chunk));
+    memcpy(h + job_off, &jb, sizeof(Job));
+    memset(h + item_off, 0, nitems * sizeof(uint32_t));  // every item belongs to job 0
+    char* d = (char*)p->opdev;
+    PGPU_HIP(hipMemcpyAsync(d, h, item_off + nitems * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    const Seg* segs = (const Seg*)(d + kOpSeg);
+    const Chunk* chunks = (const Chunk*)(d + kOpChunk);
+    PGPU_TRY(launch_zero(st, p->opres, 6));
+    {
+        Timed tm(p->timer, "hist");
+        PGPU_TRY(launch_hist(st, kb, segs, chunks, 0, nbc + nsc, p->opH));
+    }
+    {
+        Timed tm(p->timer, "scan");
+        PGPU_TRY(launch_scan(st, bits, segs, 0, 2, p->opH, p->opbnd));
+    }
+    {
+        Timed tm(p->timer, "scatter_build");
+        PGPU_TRY(launch_scatter(st, kb, true, bits, segs, chunks, 0, nbc, p->opH, p->R2));
+    }
+    {
+        Timed tm(p->timer, "scatter_probe");
+        PGPU_TRY(launch_scatter(st, kb, HVS, bits, segs, chunks, nbc, nsc, p->opH, p->S2));
+    }
+    {
+        Timed tm(p->timer, "join_lds");
+        PGPU_TRY(launch_join(st, kb, HVS, kb == 4 ? 3 : 2, std::min<int>(p->opgrid, (int)nitems),
+                             (const Job*)(d + job_off), (const uint32_t*)(d + item_off), nitems, p->R2, p->S2,
+                             p->opbnd, p->opbnd + lr.bnd, (uint32_t*)(p->opres + 4), p->opres, p->opredo));
+    }
+    return PGPU_OK;
+}
+
 }  // namespace
 
 using namespace pgpu;
@@ -1182,7 +1265,10 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
         return fail(PGPU_EUNSUPPORTED, "join keys must be 4 or 8 bytes (got %u)", cfg->key_bytes);
     if (cfg->radix_bits != 0 && (cfg->radix_bits < 2 || cfg->radix_bits > kMaxBits))
         return fail(PGPU_EINVAL, "radix_bits must be 0 or in [2, %d] (got %u)", kMaxBits, cfg->radix_bits);
-    if (cfg->algo > PGPU_ALGO_GLOBAL) return fail(PGPU_EINVAL, "unknown join algorithm %u", cfg->algo);
+    if (cfg->algo > PGPU_ALGO_RADIX1) return fail(PGPU_EINVAL, "unknown join algorithm %u", cfg->algo);
+    if (cfg->algo == PGPU_ALGO_RADIX1 && max_build > kOnePassMaxBuild)
+        return fail(PGPU_EINVAL, "the one-pass join takes at most %llu build rows",
+                    (unsigned long long)kOnePassMaxBuild);
     if (cfg->join_variant > 8) return fail(PGPU_EINVAL, "unknown LDS join variant %u", cfg->join_variant);
     // Histogram columns, per-bin totals and the per-(block, bin) prefixes of
     // k_colscan are 32-bit: one final partition may hold every row of a side,
@@ -1255,6 +1341,28 @@ int pgpu_join_plan_create(pgpu_ctx* ctx, const pgpu_join_cfg* cfg, uint64_t max_
         for (auto& x : a)
             if ((rc = plan_alloc(p, x.bytes, x.ptr)) != PGPU_OK) return cleanup(rc);
     }
+    if (want_radix && max_build <= kOnePassMaxBuild) {
+        const uint32_t crR = batch::scatter_tile_rows((int)kb, true, 10);
+        const uint32_t crS = batch::scatter_tile_rows((int)kb, cfg->probe_payload != 0, 10);
+        p->opchunks = ceil_div(std::max<uint64_t>(max_build, 1), crR) + ceil_div(std::max<uint64_t>(max_probe, 1), crS);
+        const uint64_t block = onepass_job_off(p->opchunks) + 256 + kOnePassMaxItems * 4;
+        struct {
+            void** ptr;
+            uint64_t bytes;
+        } a[] = {
+            {&p->opdev, block},
+            {(void**)&p->opH, p->opchunks * kOnePassMaxItems * 4},
+            {(void**)&p->opbnd, 2 * (kOnePassMaxItems + 1) * 4},
+            {(void**)&p->opres, 6 * 8},
+            {(void**)&p->opredo, kOnePassMaxItems * 4},
+        };
+        for (auto& x : a)
+            if ((rc = plan_alloc(p, x.bytes, x.ptr)) != PGPU_OK) return cleanup(rc);
+        if (hipHostMalloc((void**)&p->ophost, block, hipHostMallocDefault) != hipSuccess)
+            return cleanup(fail(PGPU_ENOMEM, "hipHostMalloc for the one-pass tables failed"));
+        if ((rc = batch::join_grid((int)kb, cfg->probe_payload != 0, kb == 4 ? 3 : 2, &p->opgrid)) != PGPU_OK)
+            return cleanup(rc);
+    }
     if (want_global) {
         p->glog = (int)ilog2_ceil(std::max<uint64_t>(2 * max_build, 2));
         p->gcap = 1ull << p->glog;
@@ -1281,6 +1389,7 @@ int pgpu_join_plan_destroy(pgpu_join_plan* p) {
     if (p->st2) (void)hipStreamDestroy(p->st2);
     for (void* x : p->owned) (void)ctx_free(p->ctx, x);
     if (p->h_result) (void)hipHostFree(p->h_result);
+    if (p->ophost) (void)hipHostFree(p->ophost);
     delete p;
     return PGPU_OK;
 }
@@ -1299,12 +1408,34 @@ int pgpu_join_count_sum_async(pgpu_join_plan* p, const void* d_rkey, const void*
     hipLaunchKernelGGL(k_fill64, dim3(8), dim3(256), 0, st, p->part, (uint64_t)4 * kPartSlots);
     PGPU_LAUNCH_CHECK();
     int algo = (int)p->cfg.algo;
-    if (algo == PGPU_ALGO_AUTO) algo = (nR <= (1u << 16) || !p->R1) ? PGPU_ALGO_GLOBAL : PGPU_ALGO_RADIX;
+    if (algo == PGPU_ALGO_AUTO)
+        algo = (nR <= (1u << 16) || !p->R1)            ? PGPU_ALGO_GLOBAL
+               : (p->opdev && !p->bits && nR <= kOnePassMaxBuild) ? PGPU_ALGO_RADIX1
+                                                                 : PGPU_ALGO_RADIX;
     if (algo == PGPU_ALGO_GLOBAL && !p->gtable) algo = PGPU_ALGO_RADIX;
     if (algo == PGPU_ALGO_RADIX && !p->R1) algo = PGPU_ALGO_GLOBAL;
     p->last_algo = algo;
     int rc = PGPU_OK;
     const bool hv = p->cfg.probe_payload != 0;
+    const bool onepass = algo == PGPU_ALGO_RADIX1 && nR && nS;
+    if (onepass) {
+        p->last_bits = batch::bits_for(nR, kOnePassPartRows);
+        if (p->cfg.key_bytes == 8)
+            rc = hv ? run_onepass<int64_t, true>(p, (const int64_t*)d_rkey, (const int64_t*)d_rpay, nR,
+                                                 (const int64_t*)d_skey, (const int64_t*)d_spay, nS)
+                    : run_onepass<int64_t, false>(p, (const int64_t*)d_rkey, (const int64_t*)d_rpay, nR,
+                                                  (const int64_t*)d_skey, nullptr, nS);
+        else
+            rc = hv ? run_onepass<int32_t, true>(p, (const int32_t*)d_rkey, (const int32_t*)d_rpay, nR,
+                                                 (const int32_t*)d_skey, (const int32_t*)d_spay, nS)
+                    : run_onepass<int32_t, false>(p, (const int32_t*)d_rkey, (const int32_t*)d_rpay, nR,
+                                                  (const int32_t*)d_skey, nullptr, nS);
+        if (rc != PGPU_OK) return rc;
+        PGPU_HIP(hipMemcpyAsync(p->h_result, p->opres, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        p->pending = true;
+        return PGPU_OK;
+    }
+    if (algo == PGPU_ALGO_RADIX1) algo = PGPU_ALGO_RADIX;  // an empty side: every count is 0
     if (nR && nS) {
         if (algo == PGPU_ALGO_RADIX) {
             const int B = p->bits ? p->bits : std::min(choose_bits(nR), (int)ilog2_ceil(p->nbmax));
@@ -1352,7 +1483,7 @@ int pgpu_join_plan_result(pgpu_join_plan* p, pgpu_join_result* out) {
     out->sum_build = p->h_result[1];
     out->sum_probe = p->h_result[2];
     out->radix_bits = p->last_bits;
-    out->lds_passes = (uint32_t)p->h_result[3];
+    out->lds_passes = std::max<uint32_t>(1, (uint32_t)p->h_result[3]);
     return PGPU_OK;
 }
 

@@ -112,7 +112,7 @@ def test_c2_global_matches_closed_form(ctx):
         b.free()
 
 
-@pytest.mark.parametrize("algo", [pa.ALGO_RADIX, pa.ALGO_GLOBAL])
+@pytest.mark.parametrize("algo", [pa.ALGO_RADIX, pa.ALGO_GLOBAL, pa.ALGO_RADIX1, pa.ALGO_AUTO])
 def test_c1_int32_with_probe_payload(ctx, algo):
     nR, nS = 1 << 20, 1 << 24  # the C1 shape
     bufs = [ctx.alloc(nR * 4), ctx.alloc(nR * 4), ctx.alloc(nS * 4), ctx.alloc(nS * 4)]
@@ -128,7 +128,8 @@ def test_c1_int32_with_probe_payload(ctx, algo):
 
 
 @pytest.mark.parametrize("kb", [4, 8])
-@pytest.mark.parametrize("algo,bits", [(pa.ALGO_RADIX, 3), (pa.ALGO_RADIX, 9), (pa.ALGO_GLOBAL, 0)])
+@pytest.mark.parametrize("algo,bits", [(pa.ALGO_RADIX, 3), (pa.ALGO_RADIX, 9), (pa.ALGO_GLOBAL, 0),
+                                       (pa.ALGO_RADIX1, 0)])
 def test_random_keys_vs_oracle_join(ctx, kb, algo, bits):
     """Duplicates on both sides, negative keys, the minimum integer (the
     table's empty marker) as a real key, and non-matching probes."""
@@ -165,6 +166,37 @@ def test_random_keys_vs_oracle_join(ctx, kb, algo, bits):
     plan.close()
     for b in bufs:
         b.free()
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+def test_one_pass_unique_keys_vs_exact(ctx, kb):
+    """The one-pass route (ALGO_RADIX1; ALGO_AUTO takes it for builds of
+    2^16 to 2^22 rows) with unique build keys, the minimum integer as a real
+    key on both sides and probe keys that miss, called twice with different
+    sizes on one plan."""
+    rng = np.random.default_rng(77 + kb)
+    dt = np.int32 if kb == 4 else np.int64
+    lo = np.iinfo(dt).min
+    plan = pa.JoinPlan(ctx, kb, 1 << 18, 1 << 21, probe_payload=True, algo=pa.ALGO_AUTO)
+    for nR, nS in ((1 << 18, 1 << 21), (100000, 700001)):
+        bk = rng.permutation(np.arange(-2 * nR, 2 * nR, 2, dtype=np.int64))[:nR].astype(dt)
+        bk[7] = lo
+        pk = rng.choice(bk, nS).astype(dt)
+        pk[:33] = lo
+        pk[33:333] += 1  # odd: no match
+        bp = rng.integers(np.iinfo(dt).min // 4, np.iinfo(dt).max // 4, nR).astype(dt)
+        pp = rng.integers(np.iinfo(dt).min // 4, np.iinfo(dt).max // 4, nS).astype(dt)
+        bsum = dict(zip(bk.tolist(), bp.tolist()))
+        hits = [(bsum[k], v) for k, v in zip(pk.tolist(), pp.tolist()) if k in bsum]
+        bufs = [_dev(ctx, a) for a in (bk, bp, pk, pp)]
+        r = plan.count_sum(bufs[0], bufs[1], nR, bufs[2], bufs[3], nS)
+        assert r.matches == len(hits)
+        assert r.sum_build == sum(b for b, _ in hits) % 2**64
+        assert r.sum_probe == sum(v for _, v in hits) % 2**64
+        assert r.radix_bits == int(np.ceil(np.log2(nR / 4096)))  # one pass at ~4096 build rows per partition
+        for b in bufs:
+            b.free()
+    plan.close()
 
 
 @pytest.mark.parametrize("nR,nS", [(0, 1000), (1000, 0), (0, 0), (1, 1)])
